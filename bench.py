@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the Dornic / Rietkerk step on one MI355X.
+
+Headline (`value`): the batched FP32 DP quench over 256 mixed-size instances
+(128 x 128^2 + 64 x 256^2 + 48 x 512^2 + 16 x 1024^2 = 35,651,584 cells/step;
+a ramped 0.72 -> 0.70 over 10^4 steps; b = 2, D = 1, sigma = sqrt 2, dt = 0.1;
+rho(t) every step in FP64), timed over the full 10^4-step run with every
+rho(t) row copied back to the host.  Reported beside it, each also timed over
+its full run: the same sweep in FP64, the plain FP64 single-instance Rietkerk
+step (256^2, 2x10^4 steps), and the batched FP32 Rietkerk sweep (256 x 512^2,
+R = linspace(0.5, 1.5, 16) x 16 seeds, 5x10^3 steps).  Every FP32 run is
+compared with the FP64 run of the same seeds (rho(t) ensemble error).
+
+Roofline: achieved HBM GB/s of the step kernel = algorithmic bytes per launch
+(8 B/cell FP32 DP, 24 B/cell FP32 Rietkerk, 16 / 48 in FP64: one read and one
+write of every field) / the kernel's average duration, measured with HIP
+events on the context stream over a replay of the same trajectory.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--with-c5] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s + % HBM peak: batched FP32 vs plain FP64, ρ(t) err ≤1e-3"
+HBM_PEAK_GBS = 8000.0
+SEED = 20251202
+
+
+def c4_batch(tm, precision):
+    sizes = [128] * 128 + [256] * 64 + [512] * 48 + [1024] * 16
+    p = tm.DPParams(a=0.72, a_end=0.70, ramp_steps=10_000, b=2.0, D=1.0, sigma=math.sqrt(2.0),
+                    dt=0.1, dx=1.0, scheme=tm.DP_EULER, sampler=tm.SAMPLER_EXACT)
+    inst = [tm.Instance(L, 0, SEED, k) for k, L in enumerate(sizes)]
+    return dict(model=tm.MODEL_DP, precision=precision, params=[p], inst=inst, lo=[0.0], hi=[1.0],
+                steps=10_000, nf=1)
+
+
+def c3_batch(tm, precision):
+    Rs = np.linspace(0.5, 1.5, 16)
+    ps = [tm.RietkerkParams(R=float(R)) for R in Rs]
+    inst = [tm.Instance(512, k // 16, SEED, k) for k in range(256)]
+    return dict(model=tm.MODEL_RIETKERK, precision=precision, params=ps, inst=inst,
+                lo=[0.0, 0.0, 0.0], hi=[90.0, 1.0, 1.0], steps=5_000, nf=3)
+
+
+def c2_batch(tm, precision):
+    return dict(model=tm.MODEL_RIETKERK, precision=precision, params=[tm.RietkerkParams(R=1.0)],
+                inst=[tm.Instance(256, 0, SEED, 0)], lo=[0.0, 0.0, 0.0], hi=[90.0, 1.0, 1.0],
+                steps=20_000, nf=3)
+
+
+def c5_batch(tm, precision):
+    return dict(model=tm.MODEL_RIETKERK, precision=precision, params=[tm.RietkerkParams(R=0.75)],
+                inst=[tm.Instance(8192, 0, SEED, 0)], lo=[0.0, 0.0, 0.0], hi=[90.0, 1.0, 1.0],
+                steps=100_000, nf=3)
+
+
+def cells_of(b):
+    return sum(i.L * i.L for i in b["inst"])
+
+
+def bytes_per_cell(b):
+    return b["nf"] * 2 * (8 if b["precision"] == 0 else 4)
+
+
+def run_workload(tm, b, steps, warmup, want_rows=True, tune=True):
+    """Warm up, then time `steps` steps of the real trajectory from t = 0 and replay
+    it with HIP events around the step kernels only."""
+    with tm.Context(0) as ctx:
+        ctx.configure(b["model"], b["precision"], b["params"], b["inst"])
+        variant = None
+        if tune:
+            variant = ctx.autotune(os.environ.get("TM_TUNE_CACHE"))
+        ctx.init_uniform(b["lo"], b["hi"])
+        if warmup:
+            ctx.run(warmup, rows=want_rows)
+        ctx.reset()
+        ctx.init_uniform(b["lo"], b["hi"])
+        t0 = time.perf_counter()
+        rows = ctx.run(steps, rows=want_rows)
+        t1 = time.perf_counter()
+        final = [ctx.download(k) for k in range(min(len(b["inst"]), 4))]
+        # replay of the same trajectory: step-kernel time only
+        ctx.reset()
+        ctx.init_uniform(b["lo"], b["hi"])
+        kms = ctx.time_steps(steps)
+        names = tm.variant_names(b["model"], b["precision"])
+        vname = names[variant[0]] if variant else None
+    cells = cells_of(b)
+    wall = t1 - t0
+    avg_kernel_s = kms / 1e3 / steps
+    achieved = cells * bytes_per_cell(b) / avg_kernel_s / 1e9
+    return dict(value=cells * steps / wall, wall_s=wall, ms_per_step=wall / steps * 1e3,
+                kernel_ms=avg_kernel_s * 1e3, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
+                cells=cells, steps=steps, variant=vname, graph_steps=variant[1] if variant else None,
+                rows=rows, final=final)
+
+
+def rho_error(r32, r64, nf):
+    """Errors of FP32 rho(t) against the FP64 run of the same seeds.  Ensemble: the
+    mean over instances of the first field's spatial mean, per step."""
+    m32 = r32[:, :, 1].mean(axis=0)
+    m64 = r64[:, :, 1].mean(axis=0)
+    live = np.abs(m64) > 0
+    ens = np.abs(m32[live] - m64[live]) / np.abs(m64[live])
+    per = np.abs(r32[:, :, 1::2] - r64[:, :, 1::2]) / np.maximum(np.abs(r64[:, :, 1::2]), 1e-300)
+    per = per[np.abs(r64[:, :, 1::2]) > 0]
+    return dict(ensemble_rel_max=float(ens.max()) if ens.size else 0.0,
+                ensemble_rel_mean=float(ens.mean()) if ens.size else 0.0,
+                instance_rel_max=float(per.max()) if per.size else 0.0,
+                instance_rel_mean=float(per.mean()) if per.size else 0.0)
+
+
+def field_error(f32, f64):
+    a = np.concatenate([x.ravel() for x in f32])
+    b = np.concatenate([x.ravel() for x in f64])
+    d = np.abs(a - b)
+    return dict(max_abs=float(d.max()), mean_abs=float(d.mean()))
+
+
+def cpu_baseline(max_threads=16):
+    """The reference's own DP-Dornic integrator (percolationDornic_2D, built from
+    its source into oracle/_ref) on the host cores: one independent call per
+    thread (the reference's thread-per-parameter OpenMP pattern), L = 128,
+    dt = 0.1, a = 0.72, b = 2, D = 1, sigma = sqrt 2, 2 replicates x 301 steps per
+    thread.  Falls back to the oracle port when the reference library is absent."""
+    from oracle import oracle
+    threads = max(1, min(os.cpu_count() or 1, max_threads, 8))
+    L, r, t_max, dt = 128, 2, 30.0, 0.1
+    steps = int(round(t_max / dt)) + 1
+    rho0 = oracle.init_uniform(L, [0.0], [1.0], SEED, 0).reshape(-1).copy()
+    ref = oracle.ref_lib()
+    if ref is not None:
+        tmeas = np.array([2.0, 5.0, 10.0, 20.0, 30.0])
+        fd = os.dup(1)
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(devnull, 1)
+        try:
+            t0 = time.perf_counter()
+            ref.ref_ssp_parallel(threads, L, 0.72, 2.0, 1.0, math.sqrt(2.0), dt, 1.0, r, rho0, tmeas,
+                                 len(tmeas), t_max)
+            wall = time.perf_counter() - t0
+        finally:
+            os.dup2(fd, 1)
+            os.close(fd)
+            os.close(devnull)
+        work = threads * r * steps * L * L
+        return dict(value=work / wall, unit="cell-updates/s", cores=threads, kind="reference",
+                    sample=f"percolationDornic_2D (stochasticSPDP.cpp:940) x {threads} threads, "
+                           f"L={L}, {r} replicates x {steps} steps each, a=0.72 b=2 D=1 sigma=sqrt2 dt=0.1",
+                    wall_s=wall)
+    p = oracle.dp_params(scheme=0)
+    t0 = time.perf_counter()
+    oracle.dp_run(p, rho0, L, SEED, 0, 0, steps, rows=True)
+    wall = time.perf_counter() - t0
+    return dict(value=L * L * steps / wall, unit="cell-updates/s", cores=1, kind="port",
+                sample=f"CPU restatement, L={L}, {steps} steps, 1 thread", wall_s=wall)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps of the headline sweep (default: full 10^4)")
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--with-c5", action="store_true", help="also run the 8192^2 Rietkerk run (10^5 steps)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-side", action="store_true", help="headline only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); instances are independent, "
+              "so a batch shards across GPUs with no communication.", file=sys.stderr)
+        sys.exit(2)
+
+    import trophic_maelstorm_amd as tm
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    c4 = c4_batch(tm, tm.F32)
+    K = args.steps or c4["steps"]
+    log(f"[bench] C4 FP32 batched DP quench: {cells_of(c4)} cells/step x {K} steps")
+    h = run_workload(tm, c4, K, args.warmup)
+    log(f"[bench]   {h['value']:.4g} cell-updates/s, kernel {h['kernel_ms']:.4f} ms, "
+        f"{h['achieved_gbs']:.0f} GB/s ({100 * h['frac']:.1f}% of peak)")
+    c4d = c4_batch(tm, tm.F64)
+    log("[bench] C4 FP64 (same seeds)")
+    h64 = run_workload(tm, c4d, K, args.warmup)
+    errors = {"c4": dict(rho=rho_error(h["rows"], h64["rows"], 1), fields=field_error(h["final"], h64["final"]))}
+    side = {"c4_fp64": {k: h64[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}}
+
+    if not args.no_side:
+        c2 = c2_batch(tm, tm.F64)
+        log("[bench] C2 plain FP64 Rietkerk 256^2")
+        r2 = run_workload(tm, c2, c2["steps"], args.warmup)
+        side["c2_plain_fp64"] = {k: r2[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
+        c3 = c3_batch(tm, tm.F32)
+        log("[bench] C3 FP32 batched Rietkerk 256 x 512^2")
+        r3 = run_workload(tm, c3, c3["steps"], args.warmup)
+        side["c3_batched_fp32"] = {k: r3[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
+        log(f"[bench]   {r3['value']:.4g} cell-updates/s, {r3['achieved_gbs']:.0f} GB/s")
+        c3d = c3_batch(tm, tm.F64)
+        log("[bench] C3 FP64 (same seeds)")
+        r3d = run_workload(tm, c3d, c3["steps"], args.warmup)
+        side["c3_fp64"] = {k: r3d[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
+        errors["c3"] = dict(rho=rho_error(r3["rows"], r3d["rows"], 3), fields=field_error(r3["final"], r3d["final"]))
+        side["ratio_c4_fp32_over_c2_plain_fp64"] = h["value"] / r2["value"]
+        side["ratio_c3_fp32_over_c2_plain_fp64"] = r3["value"] / r2["value"]
+        if args.with_c5:
+            c5 = c5_batch(tm, tm.F32)
+            log("[bench] C5 FP32 Rietkerk 8192^2")
+            r5 = run_workload(tm, c5, c5["steps"], args.warmup)
+            side["c5_fp32"] = {k: r5[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": METRIC,
+        "value": h["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: i.i.d. Uniform initial fields from Philox(seed, instance)",
+        "config": {"workload": "C4 batched FP32 DP quench: 128x128^2 + 64x256^2 + 48x512^2 + 16x1024^2, "
+                               "a 0.72->0.70 over 10^4 steps, rho(t) FP64 every step",
+                   "instances": len(c4["inst"]), "cells_per_step": cells_of(c4), "steps": K,
+                   "kernel_variant": h["variant"], "graph_steps": h["graph_steps"]},
+        "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": h["frac"], "traffic": None,
+                     "kernel": "dp_step_kernel<float>", "bytes_per_cell": bytes_per_cell(c4),
+                     "kernel_ms": h["kernel_ms"]},
+        "cpu_baseline": cpu,
+        "side_by_side": side,
+        "error_vs_fp64": errors,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

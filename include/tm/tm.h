@@ -1,0 +1,188 @@
+/*
+ * tm.h -- C ABI of libtm, the MI355X (gfx950) Dornic / Rietkerk step library.
+ *
+ * This is the drop-in boundary for the reference's single-species hot loop:
+ * the body of the time loop of
+ *   rietkerk_DorFPE_2D_MultiSp   Rietkerk_FPE/rietkerk_bjork_FKE.cpp:3697 (loop body :4005-4468)
+ *   dP_Dornic_2D_MultiSp         Percolation_FD/multiSPDP.cpp:2421        (loop body :2640-3265)
+ * (paths relative to the reference's simulations/ directory).  The reference
+ * has no FFI for this path -- the step is inline C++ -- so each entry point
+ * below names the reference statements it replaces.  A caller keeps the
+ * reference's drivers (argv contract Rietkerk_FPE/order_1stoc_test_rietkerk.cpp:99-110,
+ * sweep rietkerk_bjork_FKE.cpp:4665) and replaces the loop body with
+ * tm_run() blocks between measurement times.
+ *
+ * Conventions: plain C types only; host arrays are FP64, species-major SoA
+ * [field][L*L] row-major (i = p*L + q), as the reference's
+ * std::vector<std::vector<double>> fields (rietkerk_bjork_FKE.cpp:3902-3904).
+ * Every function returns tm_status; nothing throws across the ABI.  A context
+ * owns its device memory and one HIP stream and is used from one host thread
+ * at a time (the reference's one-OpenMP-thread-per-parameter usage maps to one
+ * context per thread, or better, to one batched context).
+ */
+#ifndef TM_TM_H
+#define TM_TM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tm_status {
+    TM_OK = 0,
+    TM_ERR_ARG = -1,       /* bad argument (sizes, null pointers, unknown enum) */
+    TM_ERR_HIP = -2,       /* a HIP runtime call failed (see tm_last_error)     */
+    TM_ERR_NOMEM = -3,     /* device or pinned allocation failed                */
+    TM_ERR_STATE = -4,     /* call out of order (e.g. tm_run before configure)  */
+    TM_ERR_NODEV = -5,     /* no usable gfx950 device                           */
+    TM_FAULT_NUMERIC = 1   /* a negative or non-finite value was produced; see  *
+                            * tm_fault_info.  Replaces the reference's error-   *
+                            * frame dump + exit(3) (rietkerk_bjork_FKE.cpp:4489-4560) */
+} tm_status;
+
+typedef enum tm_model {
+    TM_MODEL_DP = 1,        /* one field rho                         (multiSPDP.cpp, Sp = 1) */
+    TM_MODEL_RIETKERK = 3   /* three fields P, W, O (rietkerk_bjork_constants_1Sp.h:5, Sp = 3) */
+} tm_model;
+
+typedef enum tm_precision {
+    TM_F64 = 0,   /* FP64 storage and arithmetic: bit-exact with the CPU restatement */
+    TM_F32 = 1    /* FP32 storage and stepping; FP64 Poisson mean, ramp and rho(t) sums */
+} tm_precision;
+
+typedef enum tm_sampler {
+    TM_SAMPLER_EXACT = 0,   /* exact Poisson (inversion / PTRS) and Gamma (Marsaglia-Tsang) */
+    TM_SAMPLER_COMPAT = 1   /* the reference's Gaussian shortcuts above 100
+                             * (rietkerk_bjork_FKE.cpp:4175-4191) */
+} tm_sampler;
+
+typedef enum tm_dp_scheme {
+    TM_DP_EULER = 0,   /* rho <- rho_star - b rho_star^2 dt                           */
+    TM_DP_RK4 = 1,     /* RK4 of -b rho^2 (multiSPDP.cpp:1821-1898, f at :1662).  The
+                        * reference's call site multiSPDP.cpp:3222 passes a in b's slot;
+                        * pass b = a to reproduce that run exactly.                  */
+    TM_DP_EXACT = 2    /* rho_star / (1 + b rho_star dt) (stochasticSPDP.cpp:1064-1068) */
+} tm_dp_scheme;
+
+/* DP-Langevin parameters: dP_Dornic_2D_MultiSp(..., a, b, c, D[Sp], ..., sigma[], ..., dt, dx, ...)
+ * (Percolation_FD/MultiSPDP.h:344-346).  The quench ramp is new: a(n) = a + (a_end - a) *
+ * min(n, ramp_steps) / ramp_steps at instance step n (constant a when ramp_steps == 0). */
+typedef struct tm_dp_params {
+    double a, b, D, sigma, dt, dx;
+    double a_end;
+    int64_t ramp_steps;
+    double flush_eps;   /* 1e-12 (multiSPDP.cpp:2426)  */
+    double flush_neg;   /* -1e-80 (multiSPDP.cpp:3257) */
+    int32_t scheme;     /* tm_dp_scheme */
+    int32_t sampler;    /* tm_sampler   */
+} tm_dp_params;
+
+/* Rietkerk parameters: rietkerk_DorFPE_2D_MultiSp(..., a(=R), c, gmax, alpha, rW, W0, D[Sp], ...,
+ * K[3], sigma[], ..., M[](d = M[0]), ..., dt, dx, ...) (Rietkerk_FPE/rietkerk_bjork_basic.h:450-452).
+ * Units are whatever the caller uses (the model is unit-agnostic). */
+typedef struct tm_rietkerk_params {
+    double R, c, gmax, alpha, rW, W0, d;
+    double D[3];        /* D_P, D_W, D_O                              */
+    double K[3];        /* k0 (unused by the 1-species RHS), k1, k2   */
+    double sigma;       /* sigma_P                                     */
+    double dt, dx;
+    double flush_eps;   /* 1e-10 (rietkerk_bjork_basic.h:119)          */
+    int32_t sampler;    /* tm_sampler */
+    int32_t reserved;
+} tm_rietkerk_params;
+
+/* One lattice instance (a replicate at one parameter point).  Instances of
+ * different L share one launch.  The Philox key is (seed, instance_id); the
+ * counter is (cell, step, block), so a run is reproducible from (seed,
+ * instance_id, step) alone -- the reference's random_device seeding
+ * (rietkerk_bjork_FKE.cpp:3896) becomes a caller choice. */
+typedef struct tm_instance_desc {
+    int32_t L;             /* lattice side g                                  */
+    int32_t params_index;  /* index into the params array of tm_configure     */
+    uint32_t seed;
+    uint32_t instance_id;
+    int64_t step0;         /* instance step number of the next step to take   */
+} tm_instance_desc;
+
+typedef struct tm_ctx tm_ctx;
+
+/* Context lifetime.  device: HIP ordinal.  Replaces the per-call cudaMalloc /
+ * cudaFree of the reference's CUDA seam (rietkerk_bjork_FKE.cpp:3709-3712, :4636-4639). */
+tm_status tm_create(tm_ctx** out, int device);
+void tm_destroy(tm_ctx* ctx);
+const char* tm_last_error(const tm_ctx* ctx);
+
+/* Declare the batch: model, precision, params[n_params] (tm_dp_params or
+ * tm_rietkerk_params), instances[n_inst].  Allocates the double-buffered
+ * fields (zeroed) and the per-instance descriptor and tile tables.  Replaces the
+ * per-replicate field allocation of rietkerk_bjork_FKE.cpp:3902-3913. */
+tm_status tm_configure(tm_ctx* ctx, int model, int precision, const void* params, int n_params,
+                       const tm_instance_desc* instances, int n_inst);
+
+/* Host <-> device fields of one instance, FP64 [nfields][L*L] (converted for TM_F32).
+ * Replace the init_* frame builders' output hand-off (rietkerk_bjork_FKE.cpp:3964-3971)
+ * and give save_frame (rietkerk_bjork_FKE.cpp:2829-2876) its data. */
+tm_status tm_upload_fields(tm_ctx* ctx, int inst, const double* host);
+tm_status tm_download_fields(tm_ctx* ctx, int inst, double* host);
+
+/* i.i.d. Uniform(lo[f], hi[f]) initial fields for every instance, drawn on the
+ * device from Philox(seed, instance_id; cell, 0xFFFFFFFF, 0xFFFFFFFF, 0), word f. */
+tm_status tm_init_uniform(tm_ctx* ctx, const double* lo, const double* hi);
+
+/* Advance every instance n_steps steps: Dornic Poisson-Gamma on P (rho), RK4
+ * (Rietkerk) or the DP nonlinear sub-step, checks, flush and copy -- the loop
+ * body rietkerk_bjork_FKE.cpp:4154-4465 / multiSPDP.cpp:2901-3262.
+ * rho_rows (nullable, host): [n_inst][n_steps][2*nfields] FP64 with, per field,
+ * (#cells > 0, spatial mean) AFTER each step -- occupied_sites_of_vector and
+ * mean_of_vector (rietkerk_bjork_FKE.cpp:1238-1280) as used at :4017-4031, every
+ * step, with Kahan-compensated FP64 sums.  Returns TM_FAULT_NUMERIC if a
+ * negative or non-finite value appeared (the step still completes). */
+tm_status tm_run(tm_ctx* ctx, int64_t n_steps, double* rho_rows);
+
+/* rho(t) row of the current state: [n_inst][2*nfields]. */
+tm_status tm_measure(tm_ctx* ctx, double* rows);
+
+/* Debug aid for parity tests: when enabled, tm_run records the Poisson integer
+ * of every cell of the last step (-1 where the absorbing-state skip applied,
+ * rietkerk_bjork_FKE.cpp:4161-4164); tm_download_draws returns them [L*L]. */
+tm_status tm_set_record_draws(tm_ctx* ctx, int enable);
+tm_status tm_download_draws(tm_ctx* ctx, int inst, int64_t* host);
+
+/* First numeric fault since configure: instance, cell, step; all -1 if none. */
+tm_status tm_fault_info(tm_ctx* ctx, int32_t* inst, int64_t* cell, int64_t* step);
+
+/* Current instance step numbers (step0 + steps taken). */
+tm_status tm_get_step(tm_ctx* ctx, int inst, int64_t* step);
+
+/* Rewind every instance to its step0 and clear the fault record; fields are
+ * left as they are (call tm_init_uniform / tm_upload_fields next).  Launch
+ * shape, graphs and tuning are kept.  Starts a fresh replicate of the batch,
+ * the role of the replicate loop head rietkerk_bjork_FKE.cpp:3887-3899. */
+tm_status tm_reset(tm_ctx* ctx);
+
+/* Launch shape.  variant: index into the kernel table for the configured
+ * model/precision (-1 = default); graph_steps: steps per captured hipGraph
+ * block (0 = eager launches).  tm_variant_count / tm_variant_name describe the table. */
+tm_status tm_set_launch(tm_ctx* ctx, int variant, int graph_steps);
+int tm_variant_count(int model, int precision);
+const char* tm_variant_name(int model, int precision, int variant);
+
+/* Autotuner: time each kernel variant (and graph block sizes) on the configured
+ * batch without advancing it, pick the fastest, and persist the choice in
+ * cache_path (NULL = $TM_TUNE_CACHE or ~/.cache/trophic_maelstorm_amd/tune.txt),
+ * keyed by (device arch, model, precision, batch shape).  A cached choice is
+ * reused without timing.  out_variant / out_graph_steps (nullable) get the choice. */
+tm_status tm_autotune(tm_ctx* ctx, const char* cache_path, int* out_variant, int* out_graph_steps);
+
+/* Timing aid: runs n_steps steps (as tm_run, rows discarded) and reports the
+ * device time between HIP events recorded on the context's stream around the
+ * step kernels only (ms_total), and the number of step-kernel launches. */
+tm_status tm_time_steps(tm_ctx* ctx, int64_t n_steps, double* ms_total, int64_t* launches);
+
+/* The context's HIP stream (hipStream_t) for callers that order their own work. */
+void* tm_stream(tm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TM_TM_H */

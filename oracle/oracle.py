@@ -1,0 +1,211 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) and of the reference's
+own legacy DP integrator (oracle/_ref/libref_ssp.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product package never
+imports this module.  See tm_oracle.h for what the oracle restates and how each
+function is pinned.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_ssp.so")
+
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class DPParams(C.Structure):
+    """tmo_dp_params (mirror of tm_dp_params)."""
+    _fields_ = [("a", C.c_double), ("b", C.c_double), ("D", C.c_double), ("sigma", C.c_double),
+                ("dt", C.c_double), ("dx", C.c_double), ("a_end", C.c_double),
+                ("ramp_steps", C.c_int64), ("flush_eps", C.c_double), ("flush_neg", C.c_double),
+                ("scheme", C.c_int32), ("sampler", C.c_int32)]
+
+
+class RKParams(C.Structure):
+    """tmo_rk_params (mirror of tm_rietkerk_params)."""
+    _fields_ = [("R", C.c_double), ("c", C.c_double), ("gmax", C.c_double), ("alpha", C.c_double),
+                ("rW", C.c_double), ("W0", C.c_double), ("d", C.c_double),
+                ("D", C.c_double * 3), ("K", C.c_double * 3), ("sigma", C.c_double),
+                ("dt", C.c_double), ("dx", C.c_double), ("flush_eps", C.c_double),
+                ("sampler", C.c_int32), ("pad_", C.c_int32)]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so (and the reference library when its sources exist)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+def build_ref() -> str | None:
+    """Compile oracle/_ref/libref_ssp.so from the reference sources, if present."""
+    if os.path.isdir("/root/reference/simulations/Percolation_FD"):
+        subprocess.run(["make", "-C", HERE, "ref"], check=True, stdout=subprocess.DEVNULL)
+    return REF_PATH if os.path.exists(REF_PATH) else None
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.tmo_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        for name in ("tmo_exp", "tmo_log", "tmo_norm_icdf"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double]
+        L.tmo_logfact.restype = C.c_double
+        L.tmo_logfact.argtypes = [C.c_int64]
+        L.tmo_sample.argtypes = [C.c_int, C.c_double, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_int64, C.c_int64, _f64p]
+        L.tmo_dornic_constants.argtypes = [C.c_double] * 5 + [_f64p]
+        L.tmo_dornic_cell.restype = C.c_double
+        L.tmo_dornic_cell.argtypes = [C.c_double] * 5 + [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                         C.c_int64, C.POINTER(C.c_int64)]
+        L.tmo_dornic_batch.argtypes = [C.c_double] * 5 + [C.c_int, C.c_uint32, C.c_uint32, C.c_int64,
+                                                          C.c_int64, _f64p, C.c_void_p]
+        L.tmo_init_uniform.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, _f64p, _f64p, _f64p]
+        L.tmo_dp_run.argtypes = [C.POINTER(DPParams), C.c_int, C.c_uint32, C.c_uint32, C.c_int64,
+                                 C.c_int64, _f64p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
+        L.tmo_rk_run.argtypes = [C.POINTER(RKParams), C.c_int, C.c_uint32, C.c_uint32, C.c_int64,
+                                 C.c_int64, _f64p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
+        L.tmo_laplacian.argtypes = [C.c_int, _f64p, _f64p]
+        L.tmo_rk_rhs.argtypes = [C.POINTER(RKParams), C.c_int, _f64p, _f64p]
+        L.tmo_rk4_deterministic.argtypes = [C.POINTER(RKParams), C.c_int, C.c_int64, _f64p]
+        L.tmo_dp_nonlinear.restype = C.c_double
+        L.tmo_dp_nonlinear.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.tmo_reduce.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's legacy DP integrator, or None when it was not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_PATH):
+            return None
+        R = C.CDLL(REF_PATH)
+        R.ref_ssp_percolation_dornic.restype = C.c_int
+        R.ref_ssp_percolation_dornic.argtypes = [C.c_int] + [C.c_double] * 6 + [
+            C.c_int, _f64p, _f64p, C.c_int, C.c_double, _f64p, C.c_int]
+        R.ref_ssp_parallel.restype = C.c_int
+        R.ref_ssp_parallel.argtypes = [C.c_int, C.c_int] + [C.c_double] * 6 + [
+            C.c_int, _f64p, _f64p, C.c_int, C.c_double]
+        _ref = R
+    return _ref
+
+
+# ---------------------------------------------------------------- helpers
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().tmo_philox4x32_10(c, k, o)
+    return [int(x) for x in o]
+
+
+def sample(kind: str, n: int, param: float = 0.0, mode: int = 0, seed: int = 1, inst: int = 0,
+           cell: int = 0, step: int = 0) -> np.ndarray:
+    kinds = {"uniform": 0, "normal": 1, "poisson": 2, "gamma": 3}
+    out = np.empty(n, np.float64)
+    lib().tmo_sample(kinds[kind], float(param), int(mode), seed, inst, cell, step, n, out)
+    return out
+
+
+def dornic_constants(linear, D, sigma, dt, dx):
+    out = np.empty(3, np.float64)
+    lib().tmo_dornic_constants(linear, D, sigma, dt, dx, out)
+    return out
+
+
+def dornic_batch(alpha_i, rho, lam, lexp, sigma, n, sampler=0, seed=7, inst=0, step=0, draws=False):
+    """n independent Dornic transitions of a cell (alpha_i, rho); returns rho* (and Poisson draws)."""
+    out = np.empty(n, np.float64)
+    k = np.empty(n, np.int64) if draws else None
+    lib().tmo_dornic_batch(alpha_i, rho, lam, lexp, sigma, sampler, seed, inst, step, n, out,
+                           k.ctypes.data if k is not None else None)
+    return (out, k) if draws else out
+
+
+def init_uniform(L, lo, hi, seed, inst):
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray(hi, np.float64)
+    out = np.empty(len(lo) * L * L, np.float64)
+    lib().tmo_init_uniform(L, len(lo), seed, inst, lo, hi, out)
+    return out.reshape(len(lo), L, L)
+
+
+def dp_params(a=0.72, b=2.0, D=1.0, sigma=2.0 ** 0.5, dt=0.1, dx=1.0, a_end=None, ramp_steps=0,
+              flush_eps=1e-12, flush_neg=-1e-80, scheme=0, sampler=0) -> DPParams:
+    return DPParams(a, b, D, sigma, dt, dx, a if a_end is None else a_end, ramp_steps, flush_eps,
+                    flush_neg, scheme, sampler)
+
+
+def rk_params(R=1.0, c=10.0, gmax=0.05, alpha=0.2, rW=0.2, W0=0.2, d=0.25, D=(0.1, 0.1, 100.0),
+              K=(0.0, 5.0, 5.0), sigma=0.5, dt=0.005, dx=2.0, flush_eps=1e-10, sampler=0) -> RKParams:
+    return RKParams(R, c, gmax, alpha, rW, W0, d, (C.c_double * 3)(*D), (C.c_double * 3)(*K), sigma,
+                    dt, dx, flush_eps, sampler, 0)
+
+
+def _run(fn, p, fields, L, seed, inst, step0, nsteps, nf, want_rows, want_draws):
+    f = np.ascontiguousarray(fields, np.float64).copy().reshape(-1)
+    rows = np.zeros((nsteps, 2 * nf), np.float64) if want_rows else None
+    draws = np.zeros(L * L, np.int64) if want_draws else None
+    fc = C.c_int64(-1)
+    st = fn(C.byref(p), L, seed, inst, step0, nsteps, f,
+            rows.ctypes.data if rows is not None else None,
+            draws.ctypes.data if draws is not None else None, C.byref(fc))
+    return {"fields": f.reshape(nf, L, L), "rows": rows,
+            "draws": draws.reshape(L, L) if draws is not None else None,
+            "status": st, "fault_cell": fc.value}
+
+
+def dp_run(p: DPParams, rho, L, seed, inst, step0, nsteps, rows=True, draws=False):
+    return _run(lib().tmo_dp_run, p, rho, L, seed, inst, step0, nsteps, 1, rows, draws)
+
+
+def rk_run(p: RKParams, fields, L, seed, inst, step0, nsteps, rows=True, draws=False):
+    return _run(lib().tmo_rk_run, p, fields, L, seed, inst, step0, nsteps, 3, rows, draws)
+
+
+def laplacian(x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    lib().tmo_laplacian(x.shape[0], x.reshape(-1), out.reshape(-1))
+    return out
+
+
+def rk_rhs(p: RKParams, y):
+    y = np.ascontiguousarray(y, np.float64)
+    f = np.empty_like(y)
+    lib().tmo_rk_rhs(C.byref(p), y.shape[-1], y.reshape(-1), f.reshape(-1))
+    return f
+
+
+def rk4_deterministic(p: RKParams, y, nsteps):
+    y = np.ascontiguousarray(y, np.float64).copy()
+    lib().tmo_rk4_deterministic(C.byref(p), y.shape[-1], nsteps, y.reshape(-1))
+    return y
+
+
+def reduce(fields):
+    f = np.ascontiguousarray(fields, np.float64)
+    nf, L = f.shape[0], f.shape[-1]
+    row = np.empty(2 * nf, np.float64)
+    lib().tmo_reduce(L, nf, f.reshape(-1), row)
+    return row

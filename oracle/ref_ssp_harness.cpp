@@ -1,0 +1,65 @@
+// ref_ssp_harness.cpp -- C entry points around the reference's own legacy DP
+// Dornic integrator, built from the reference source where it lies
+// (Percolation_FD/stochasticSPDP.cpp) into oracle/_ref/libref_ssp.so by
+// oracle/Makefile.  TEST INFRASTRUCTURE ONLY: used for the ensemble parity
+// test of the DP step and as the "reference" CPU baseline of bench.py.
+//
+// percolationDornic_2D (stochasticSPDP.cpp:940-1143) integrates
+//   d rho = (a rho - b rho^2 + D lap rho) dt + sigma sqrt(rho) dW
+// with the Dornic Poisson-Gamma linear/noise step (stochasticSPDP.cpp:1032-1061,
+// libstdc++ poisson/gamma distributions, no Gaussian shortcuts) and the exact
+// nonlinear update rho/(1 + b rho dt) (stochasticSPDP.cpp:1064-1068).  Its
+// stream is std::mt19937_64 seeded from std::random_device (stochasticSPDP.cpp:973),
+// so agreement with it is statistical only.  Output rows (stochasticSPDP.cpp:1130-1141):
+//   a, L, t, <<rho>>_surviving, Var, #surviving replicates, <#active>, surv_rep.
+#include <cstdio>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+using std::vector;
+
+// The definition's signature (stochasticSPDP.cpp:940-941); SPDP.h:96-97 declares a stale one.
+void percolationDornic_2D(vector<vector<double>>& Rho, vector<double>& t_meas, double Rh0[],
+                          double t_max, double a, double b, double D, double sigma, double dt,
+                          double dx, int r, int g);
+
+extern "C" {
+
+// One call of the reference integrator: r replicates of one a-value from the
+// initial frame rho0 (g*g).  Rows are written to out[max_rows][8]; returns the
+// number of rows, or -1 when max_rows is too small.
+int ref_ssp_percolation_dornic(int g, double a, double b, double D, double sigma, double dt,
+                               double dx, int r, const double* rho0, const double* t_meas,
+                               int n_meas, double t_max, double* out, int max_rows)
+{
+    vector<vector<double>> rows;
+    vector<double> tm(t_meas, t_meas + n_meas);
+    vector<double> r0(rho0, rho0 + (size_t)g * g);
+    percolationDornic_2D(rows, tm, r0.data(), t_max, a, b, D, sigma, dt, dx, r, g);
+    if ((int)rows.size() > max_rows) return -1;
+    for (size_t i = 0; i < rows.size(); ++i)
+        for (size_t j = 0; j < 8; ++j) out[i * 8 + j] = j < rows[i].size() ? rows[i][j] : 0.0;
+    return (int)rows.size();
+}
+
+// nthreads concurrent, independent calls (the reference's thread-per-parameter
+// parallelism, rietkerk_bjork_FKE.cpp:4778-4811); rows are discarded.  Used to
+// time the reference on several host cores.
+int ref_ssp_parallel(int nthreads, int g, double a, double b, double D, double sigma, double dt,
+                     double dx, int r, const double* rho0, const double* t_meas, int n_meas,
+                     double t_max)
+{
+    vector<std::thread> th;
+    for (int k = 0; k < nthreads; ++k)
+        th.emplace_back([=]() {
+            vector<vector<double>> rows;
+            vector<double> tm(t_meas, t_meas + n_meas);
+            vector<double> r0(rho0, rho0 + (size_t)g * g);
+            percolationDornic_2D(rows, tm, r0.data(), t_max, a, b, D, sigma, dt, dx, r, g);
+        });
+    for (auto& t : th) t.join();
+    return 0;
+}
+
+}  // extern "C"

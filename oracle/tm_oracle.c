@@ -1,0 +1,565 @@
+/*
+ * tm_oracle.c -- CPU oracle (checker) for the Dornic / Rietkerk step.
+ * TEST INFRASTRUCTURE ONLY; see tm_oracle.h for scope, citations and pins.
+ * Build: gcc -O2 -std=c11 -ffp-contract=off -fno-fast-math (oracle/Makefile).
+ * Reference paths are relative to the reference's simulations/ directory.
+ */
+#include "tm_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random   */
+/* numbers: as easy as 1, 2, 3"): 10 rounds of the 4x32 S-P network.  */
+/* ------------------------------------------------------------------ */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void tmo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+        uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += PHILOX_W0; k1 += PHILOX_W1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* One uniform stream: key (seed, instance), counter (cell, step lo, step hi, block). */
+typedef struct {
+    uint32_t key[2];
+    uint32_t ctr[4];
+    uint32_t buf[4];
+    int pos;
+} stream_t;
+
+static void st_init(stream_t* s, uint32_t seed, uint32_t inst, uint32_t cell, int64_t step)
+{
+    s->key[0] = seed; s->key[1] = inst;
+    s->ctr[0] = cell;
+    s->ctr[1] = (uint32_t)((uint64_t)step);
+    s->ctr[2] = (uint32_t)((uint64_t)step >> 32);
+    s->ctr[3] = 0;
+    s->pos = 4;
+}
+
+static uint32_t st_word(stream_t* s)
+{
+    if (s->pos == 4) {
+        tmo_philox4x32_10(s->ctr, s->key, s->buf);
+        s->ctr[3] += 1u;
+        s->pos = 0;
+    }
+    return s->buf[s->pos++];
+}
+
+/* u in (0,1): (w + 1/2) 2^-32, exact in double. */
+static double st_u(stream_t* s) { return ((double)st_word(s) + 0.5) * 0x1p-32; }
+
+/* ------------------------------------------------------------------ */
+/* exp / log from the mathematics (Cody-Waite reduction + Taylor /     */
+/* atanh series), IEEE basic operations only.                          */
+/* ------------------------------------------------------------------ */
+static double as_double(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+static uint64_t as_u64(double d) { uint64_t b; memcpy(&b, &d, 8); return b; }
+
+#define LN2_HI   0x1.62e42fee00000p-1
+#define LN2_LO   0x1.a39ef35793c76p-33
+#define INV_LN2  0x1.71547652b82fep+0
+#define SQRT2    0x1.6a09e667f3bcdp+0
+
+double tmo_exp(double x)
+{
+    if (x != x) return x;
+    if (x > 0x1.62e42fefa39efp+9) return INFINITY;
+    if (x < -745.13321910194122) return 0.0;
+    double kd = floor(x * INV_LN2 + 0.5);
+    double r = (x - kd * LN2_HI) - kd * LN2_LO;
+    double p = 0x1.6124613a86d09p-33;          /* 1/13! */
+    p = p * r + 0x1.1eed8eff8d898p-29;          /* 1/12! */
+    p = p * r + 0x1.ae64567f544e4p-26;
+    p = p * r + 0x1.27e4fb7789f5cp-22;
+    p = p * r + 0x1.71de3a556c734p-19;
+    p = p * r + 0x1.a01a01a01a01ap-16;
+    p = p * r + 0x1.a01a01a01a01ap-13;
+    p = p * r + 0x1.6c16c16c16c17p-10;
+    p = p * r + 0x1.1111111111111p-7;
+    p = p * r + 0x1.5555555555555p-5;
+    p = p * r + 0x1.5555555555555p-3;
+    p = p * r + 0x1.0000000000000p-1;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    int k = (int)kd;
+    if (k < -1020) { p = p * 0x1p-60; k += 60; }
+    if (k > 1023) { p = p * 2.0; k -= 1; }
+    return p * as_double((uint64_t)(k + 1023) << 52);
+}
+
+double tmo_log(double x)
+{
+    if (x != x || x < 0.0) return NAN;
+    if (x == 0.0) return -INFINITY;
+    if (x == INFINITY) return x;
+    int e = 0;
+    if (x < 0x1p-1022) { x = x * 0x1p54; e = -54; }
+    uint64_t b = as_u64(x);
+    e += (int)((b >> 52) & 0x7ff) - 1023;
+    double m = as_double((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    if (m > SQRT2) { m = m * 0.5; e += 1; }
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double q = 0x1.642c8590b2164p-5;            /* 1/23 */
+    q = q * z + 0x1.8618618618618p-5;            /* 1/21 */
+    q = q * z + 0x1.af286bca1af28p-5;
+    q = q * z + 0x1.e1e1e1e1e1e1ep-5;
+    q = q * z + 0x1.1111111111111p-4;
+    q = q * z + 0x1.3b13b13b13b14p-4;
+    q = q * z + 0x1.745d1745d1746p-4;
+    q = q * z + 0x1.c71c71c71c71cp-4;
+    q = q * z + 0x1.2492492492492p-3;
+    q = q * z + 0x1.999999999999ap-3;
+    q = q * z + 0x1.5555555555555p-2;            /* 1/3 */
+    double s2 = 2.0 * s;
+    double lm = s2 + s2 * (z * q);
+    double ed = (double)e;
+    return ed * LN2_HI + (lm + ed * LN2_LO);
+}
+
+/* Standard normal quantile: P. J. Acklam's rational approximation
+ * (relative error < 1.15e-9), central region |p - 1/2| <= 0.47575. */
+double tmo_norm_icdf(double p)
+{
+    const double plow = 0.02425;
+    if (p < plow) {
+        double q = sqrt(-2.0 * tmo_log(p));
+        double num = ((((-7.784894002430293e-03 * q - 3.223964580411365e-01) * q - 2.400758277161838e+00) * q
+                        - 2.549732539343734e+00) * q + 4.374664141464968e+00) * q + 2.938163982698783e+00;
+        double den = (((7.784695709041462e-03 * q + 3.224671290700398e-01) * q + 2.445134137142996e+00) * q
+                       + 3.754408661907416e+00) * q + 1.0;
+        return num / den;
+    }
+    if (p <= 1.0 - plow) {
+        double q = p - 0.5;
+        double r = q * q;
+        double num = (((((-3.969683028665376e+01 * r + 2.209460984245205e+02) * r - 2.759285104469687e+02) * r
+                         + 1.383577518672690e+02) * r - 3.066479806614716e+01) * r + 2.506628277459239e+00) * q;
+        double den = ((((-5.447609879822406e+01 * r + 1.615858368580409e+02) * r - 1.556989798598866e+02) * r
+                        + 6.680131188771972e+01) * r - 1.328068155288572e+01) * r + 1.0;
+        return num / den;
+    }
+    double q = sqrt(-2.0 * tmo_log(1.0 - p));
+    double num = ((((-7.784894002430293e-03 * q - 3.223964580411365e-01) * q - 2.400758277161838e+00) * q
+                    - 2.549732539343734e+00) * q + 4.374664141464968e+00) * q + 2.938163982698783e+00;
+    double den = (((7.784695709041462e-03 * q + 3.224671290700398e-01) * q + 2.445134137142996e+00) * q
+                   + 3.754408661907416e+00) * q + 1.0;
+    return -(num / den);
+}
+
+/* log(k!): table below 16, Stirling series with 4 correction terms above. */
+static const double LOGFACT[16] = {
+    0x0.0p+0, 0x0.0p+0, 0x1.62e42fefa39efp-1, 0x1.cab0bfa2a2002p+0,
+    0x1.96ca77c922cf9p+1, 0x1.326643c4479c9p+2, 0x1.a51273acf01cap+2, 0x1.10ce1f32dcc30p+3,
+    0x1.5358e82fcb70dp+3, 0x1.99a8921a7f7cfp+3, 0x1.e357590954d15p+3, 0x1.180973f3a8d74p+4,
+    0x1.3fcba16d50143p+4, 0x1.68d5a9c3b32cep+4, 0x1.930f3df162a42p+4, 0x1.be636a63fd346p+4,
+};
+
+double tmo_logfact(int64_t k)
+{
+    if (k < 16) return LOGFACT[k < 0 ? 0 : k];
+    double n = (double)k + 1.0;
+    double in = 1.0 / n;
+    double in2 = in * in;
+    double corr = in * (0x1.5555555555555p-4 - in2 * (0x1.6c16c16c16c17p-9
+                  - in2 * (0x1.a01a01a01a01ap-11 - in2 * 0x1.3813813813814p-11)));
+    return (n - 0.5) * tmo_log(n) - n + 0x1.d67f1c864beb5p-1 + corr;
+}
+
+/* ------------------------------------------------------------------ */
+/* Samplers                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Poisson(z).  z < 12: sequential-search inversion (Devroye 1986, X.3).
+ * z >= 12: PTRS transformed rejection with squeeze (Hoermann 1993,
+ * Insurance: Mathematics and Economics 12, 39-45). */
+static int64_t poisson_exact(stream_t* s, double z)
+{
+    if (!(z > 0.0)) return 0;
+    if (z < 12.0) {
+        double u = st_u(s);
+        double p = tmo_exp(-z);
+        double F = p;
+        int64_t k = 0;
+        while (u > F && k < 200) {
+            k += 1;
+            p = p * z / (double)k;
+            F = F + p;
+        }
+        return k;
+    }
+    double slam = sqrt(z);
+    double loglam = tmo_log(z);
+    double b = 0.931 + 2.53 * slam;
+    double a = -0.059 + 0.02483 * b;
+    double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+    double vr = 0.9277 - 3.6224 / (b - 2.0);
+    double linv = tmo_log(invalpha);
+    for (int it = 0; it < 64; ++it) {
+        double U = st_u(s) - 0.5;
+        double V = st_u(s);
+        double us = 0.5 - fabs(U);
+        double kd = floor((2.0 * a / us + b) * U + z + 0.43);
+        if (us >= 0.07 && V <= vr) return (int64_t)kd;
+        if (kd < 0.0 || (us < 0.013 && V > us)) continue;
+        double lhs = tmo_log(V) + linv - tmo_log(a / (us * us) + b);
+        double rhs = -z + kd * loglam - tmo_logfact((int64_t)kd);
+        if (lhs <= rhs) return (int64_t)kd;
+    }
+    return (int64_t)floor(z);
+}
+
+/* Gamma(shape, 1): Marsaglia & Tsang (2000, ACM TOMS 26, 363-372) with the
+ * U^(1/shape) boost for shape < 1; shape <= 0 gives 0 (Dornic shape 0). */
+static double gamma_mt(stream_t* s, double shape)
+{
+    if (!(shape > 0.0)) return 0.0;
+    double a = shape < 1.0 ? shape + 1.0 : shape;
+    double d = a - 0x1.5555555555555p-2;
+    double c = 1.0 / sqrt(9.0 * d);
+    double g = d;
+    for (int it = 0; it < 64; ++it) {
+        double x = tmo_norm_icdf(st_u(s));
+        double v = 1.0 + c * x;
+        if (v <= 0.0) continue;
+        v = v * v * v;
+        double u = st_u(s);
+        double x2 = x * x;
+        if (u < 1.0 - 0.0331 * x2 * x2) { g = d * v; break; }
+        if (tmo_log(u) < 0.5 * x2 + d * (1.0 - v + tmo_log(v))) { g = d * v; break; }
+    }
+    if (shape < 1.0) g = g * tmo_exp(tmo_log(st_u(s)) / shape);
+    return g;
+}
+
+void tmo_sample(int kind, double param, int mode, uint32_t seed, uint32_t inst,
+                uint32_t cell, int64_t step, int64_t n, double* out)
+{
+    stream_t s;
+    st_init(&s, seed, inst, cell, step);
+    for (int64_t i = 0; i < n; ++i) {
+        switch (kind) {
+        case 0: out[i] = st_u(&s); break;
+        case 1: out[i] = tmo_norm_icdf(st_u(&s)); break;
+        case 2:
+            if (mode == 1 && param > 100.0)
+                out[i] = (double)(int64_t)(tmo_norm_icdf(st_u(&s)) * sqrt(param) + param);
+            else
+                out[i] = (double)poisson_exact(&s, param);
+            break;
+        default: out[i] = gamma_mt(&s, param); break;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Dornic Poisson-Gamma transition of one cell                         */
+/* FKE:4157-4196 (MSP:2904-2943 is the same statement for DP).         */
+/* ------------------------------------------------------------------ */
+static double dornic_cell(stream_t* s, double alpha_i, double cur, double ll, double lambda,
+                          double sigma2_1, int compat, int64_t* n_out)
+{
+    double mu = -1.0 + 2.0 * alpha_i * sigma2_1;          /* FKE:4168 */
+    double ziggy = ll * cur;                              /* FKE:4169, ll = lambda*lambda_exp */
+    double gru;
+    int64_t n;
+    if (ziggy == 0.0) {                                   /* FKE:4171-4174 */
+        n = 0;
+        gru = mu + 1.0;
+    } else if (compat && ziggy > 100.0) {                 /* FKE:4175-4180 */
+        n = (int64_t)(tmo_norm_icdf(st_u(s)) * sqrt(ziggy) + ziggy);
+        gru = mu + 1.0 + (double)n;
+    } else {                                              /* FKE:4181-4185 */
+        n = poisson_exact(s, ziggy);
+        gru = mu + 1.0 + (double)n;
+    }
+    *n_out = n;
+    if (compat && gru > 100.0)                            /* FKE:4186-4191 */
+        return (tmo_norm_icdf(st_u(s)) * sqrt(gru) + gru) / lambda;
+    return gamma_mt(s, gru) / lambda;                     /* FKE:4192-4196 */
+}
+
+void tmo_dornic_constants(double linear, double Dcoef, double sigma, double dt, double dx, double out[3])
+{
+    /* beta = linear - 4 D/dx^2 (FKE:3847 with linear = -d; MSP:2485 with linear = a) */
+    double dx2 = dx * dx;
+    double beta = linear - 4 * Dcoef / (dx2);
+    double lexp = tmo_exp((beta) * dt);
+    double lambda = 2 * (beta) / (sigma * sigma * (lexp - 1.0));
+    out[0] = beta; out[1] = lexp; out[2] = lambda;
+}
+
+double tmo_dornic_cell(double alpha_i, double rho, double lambda, double lambda_exp, double sigma,
+                       int sampler, uint32_t seed, uint32_t inst, uint32_t cell, int64_t step,
+                       int64_t* n_out)
+{
+    stream_t s;
+    st_init(&s, seed, inst, cell, step);
+    int64_t n = -1;
+    double r = rho;
+    if (!(alpha_i == 0 && rho == 0))
+        r = dornic_cell(&s, alpha_i, rho, lambda * lambda_exp, lambda, 1 / (sigma * sigma), sampler == 1, &n);
+    if (n_out) *n_out = n;
+    return r;
+}
+
+void tmo_dornic_batch(double alpha_i, double rho, double lambda, double lambda_exp, double sigma,
+                      int sampler, uint32_t seed, uint32_t inst, int64_t step, int64_t n,
+                      double* out, int64_t* n_out)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t k = -1;
+        out[i] = tmo_dornic_cell(alpha_i, rho, lambda, lambda_exp, sigma, sampler, seed, inst,
+                                 (uint32_t)i, step, &k);
+        if (n_out) n_out[i] = k;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Initial fields, reductions, Laplacian                               */
+/* ------------------------------------------------------------------ */
+void tmo_init_uniform(int L, int nf, uint32_t seed, uint32_t inst,
+                      const double* lo, const double* hi, double* out)
+{
+    const int64_t N = (int64_t)L * L;
+    uint32_t key[2] = {seed, inst};
+    for (int64_t i = 0; i < N; ++i) {
+        uint32_t ctr[4] = {(uint32_t)i, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u};
+        uint32_t w[4];
+        tmo_philox4x32_10(ctr, key, w);
+        for (int f = 0; f < nf; ++f) {
+            double u = ((double)w[f] + 0.5) * 0x1p-32;
+            out[f * N + i] = lo[f] + (hi[f] - lo[f]) * u;
+        }
+    }
+}
+
+/* mean_of_vector (FKE:1238-1246): naive sum / size; occupied_sites_of_vector
+ * (FKE:1271-1280): count of entries > 0.  Row layout FKE:4024 (count, mean). */
+void tmo_reduce(int L, int nf, const double* fields, double* row)
+{
+    const int64_t N = (int64_t)L * L;
+    for (int f = 0; f < nf; ++f) {
+        const double* x = fields + f * N;
+        double sum = 0.0, cnt = 0.0;
+        for (int64_t i = 0; i < N; ++i) {
+            sum += x[i];
+            if (x[i] > 0) cnt += 1;
+        }
+        row[2 * f] = cnt;
+        row[2 * f + 1] = sum / (double)N;
+    }
+}
+
+/* Periodic 5-point neighbours (determine_neighbours_Sq4, FKE:1634-1653):
+ * up = row p-1, down = row p+1, left = column q-1, right = column q+1. */
+static inline int64_t nb_up(int L, int p, int q) { return (int64_t)((p - 1 + L) % L) * L + q; }
+static inline int64_t nb_dn(int L, int p, int q) { return (int64_t)((p + 1) % L) * L + q; }
+static inline int64_t nb_lf(int L, int p, int q) { return (int64_t)p * L + (q - 1 + L) % L; }
+static inline int64_t nb_rt(int L, int p, int q) { return (int64_t)p * L + (q + 1) % L; }
+
+static inline double nsum(const double* x, int L, int p, int q)
+{
+    return x[nb_up(L, p, q)] + x[nb_dn(L, p, q)] + x[nb_lf(L, p, q)] + x[nb_rt(L, p, q)];
+}
+
+void tmo_laplacian(int L, const double* in, double* out)
+{
+    for (int p = 0; p < L; ++p)
+        for (int q = 0; q < L; ++q) {
+            int64_t i = (int64_t)p * L + q;
+            out[i] = nsum(in, L, p, q) - 4 * in[i];
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* DP                                                                   */
+/* ------------------------------------------------------------------ */
+static double dp_f(double b, double y) { return -b * y * y; }   /* f_DP_Dor_1Sp, MSP:1662 */
+
+double tmo_dp_nonlinear(int scheme, double b, double dt, double y)
+{
+    if (scheme == 2)                                  /* stochasticSPDP.cpp:1066 */
+        return (y) / (1 + b * y * dt);
+    if (scheme == 1) {                                /* MSP:1835-1885 */
+        double dt2 = dt / 2.0, dt6 = dt / 6.0;
+        double k1 = dp_f(b, y);
+        double ys = y + (dt2) * k1;
+        double k2 = dp_f(b, ys);
+        ys = y + (dt2) * k2;
+        double k3 = dp_f(b, ys);
+        ys = y + (dt) * k3;
+        double k4 = dp_f(b, ys);
+        return y + (dt6) * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+    }
+    return y + dt * dp_f(b, y);                       /* explicit Euler */
+}
+
+static double ramp_a(const tmo_dp_params* p, int64_t step)
+{
+    if (p->ramp_steps <= 0) return p->a;
+    int64_t m = step < p->ramp_steps ? step : p->ramp_steps;
+    return p->a + (p->a_end - p->a) * ((double)m / (double)p->ramp_steps);
+}
+
+int tmo_dp_run(const tmo_dp_params* p, int L, uint32_t seed, uint32_t inst, int64_t step0,
+               int64_t nsteps, double* rho, double* rows, int64_t* draws, int64_t* fault_cell)
+{
+    const int64_t N = (int64_t)L * L;
+    double* old = (double*)malloc(sizeof(double) * N);
+    const double dx2 = p->dx * p->dx;
+    const double Dc = p->D / (dx2);                   /* MSP:2490 diff_coefficient */
+    const double sigma2_1 = 1 / (p->sigma * p->sigma);
+    const int compat = p->sampler == 1;
+    int fault = 0;
+    if (fault_cell) *fault_cell = -1;
+    for (int64_t n = 0; n < nsteps; ++n) {
+        const int64_t step = step0 + n;
+        double cst[3];
+        tmo_dornic_constants(ramp_a(p, step), p->D, p->sigma, p->dt, p->dx, cst);
+        const double lambda = cst[2], ll = cst[2] * cst[1];
+        memcpy(old, rho, sizeof(double) * N);          /* DRho (MSP:3259) */
+        const int last = (n == nsteps - 1);
+        for (int pp = 0; pp < L; ++pp)
+            for (int q = 0; q < L; ++q) {
+                const int64_t i = (int64_t)pp * L + q;
+                double alpha_i = Dc * (nsum(old, L, pp, q));  /* MSP:2904-2905 */
+                double y = old[i];
+                int64_t draw = -1;
+                if (!(alpha_i == 0 && old[i] == 0)) {          /* MSP:2907-2910 */
+                    stream_t s;
+                    st_init(&s, seed, inst, (uint32_t)i, step);
+                    y = dornic_cell(&s, alpha_i, old[i], ll, lambda, sigma2_1, compat, &draw);
+                }
+                if (last && draws) draws[i] = draw;
+                y = tmo_dp_nonlinear(p->scheme, p->b, p->dt, y);
+                if (y < 0 || !isfinite(y)) {                   /* MSP:3236-3254 */
+                    if (!fault && fault_cell) *fault_cell = i;
+                    fault = 1;
+                }
+                if (y < p->flush_eps && y >= p->flush_neg)     /* MSP:3257-3258 */
+                    y = 0.0;
+                rho[i] = y;
+            }
+        if (rows) tmo_reduce(L, 1, rho, rows + 2 * n);
+        if (fault) break;
+    }
+    free(old);
+    return fault;
+}
+
+/* ------------------------------------------------------------------ */
+/* Rietkerk                                                             */
+/* ------------------------------------------------------------------ */
+/* f_2Dor (FKE:2363-2390): reaction of (P, W, O) plus W/O diffusion.
+ * The P linear and noise terms belong to the Dornic sub-step. */
+static void rk_rhs(const tmo_rk_params* p, int L, const double* y, double* f)
+{
+    const int64_t N = (int64_t)L * L;
+    const double dx2 = p->dx * p->dx;
+    const double Dw = p->D[1] / (dx2), Do = p->D[2] / (dx2);   /* FKE:3866-3881 */
+    const double cgmax = p->c * p->gmax;
+    const double* P = y; const double* W = y + N; const double* O = y + 2 * N;
+    for (int pp = 0; pp < L; ++pp)
+        for (int q = 0; q < L; ++q) {
+            const int64_t i = (int64_t)pp * L + q;
+            f[i] = cgmax * W[i] * P[i] / (W[i] + p->K[1]);
+            double A = p->alpha * (P[i] + p->K[2] * p->W0) / (P[i] + p->K[2]) * O[i];
+            f[N + i] = A - p->gmax * W[i] * P[i] / (W[i] + p->K[1]) - p->rW * W[i]
+                       + (Dw) * (nsum(W, L, pp, q) - 4 * W[i]);
+            f[2 * N + i] = p->R - A + (Do) * (nsum(O, L, pp, q) - 4 * O[i]);
+        }
+}
+
+void tmo_rk_rhs(const tmo_rk_params* p, int L, const double* y, double* f) { rk_rhs(p, L, y, f); }
+
+/* RK4_Integrate_Stochastic_1Sp (FKE:2393-2429) on the full 3-field state. */
+static void rk4_full(const tmo_rk_params* p, int L, double* y, double* ys, double* k1, double* k2,
+                     double* k3, double* k4)
+{
+    const int64_t M = 3 * (int64_t)L * L;
+    const double dt = p->dt, dt2 = dt / 2.0, dt6 = dt / 6.0;   /* FKE:191 */
+    rk_rhs(p, L, y, k1);
+    for (int64_t i = 0; i < M; ++i) ys[i] = y[i] + (dt2) * k1[i];
+    rk_rhs(p, L, ys, k2);
+    for (int64_t i = 0; i < M; ++i) ys[i] = y[i] + (dt2) * k2[i];
+    rk_rhs(p, L, ys, k3);
+    for (int64_t i = 0; i < M; ++i) ys[i] = y[i] + (dt) * k3[i];
+    rk_rhs(p, L, ys, k4);
+    for (int64_t i = 0; i < M; ++i) y[i] += (dt6) * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+}
+
+void tmo_rk4_deterministic(const tmo_rk_params* p, int L, int64_t nsteps, double* y)
+{
+    const int64_t M = 3 * (int64_t)L * L;
+    double* buf = (double*)malloc(sizeof(double) * M * 5);
+    for (int64_t n = 0; n < nsteps; ++n)
+        rk4_full(p, L, y, buf, buf + M, buf + 2 * M, buf + 3 * M, buf + 4 * M);
+    free(buf);
+}
+
+int tmo_rk_run(const tmo_rk_params* p, int L, uint32_t seed, uint32_t inst, int64_t step0,
+               int64_t nsteps, double* fields, double* rows, int64_t* draws, int64_t* fault_cell)
+{
+    const int64_t N = (int64_t)L * L, M = 3 * N;
+    double* buf = (double*)malloc(sizeof(double) * (M * 5 + N));
+    double* old = buf + 5 * M;
+    const double dx2 = p->dx * p->dx;
+    double cst[3];
+    tmo_dornic_constants(-p->d, p->D[0], p->sigma, p->dt, p->dx, cst);   /* FKE:3847-3849 */
+    const double lambda = cst[2], ll = cst[2] * cst[1];
+    const double Dp = p->D[0] / (dx2);
+    const double sigma2_1 = 1 / (p->sigma * p->sigma);
+    const int compat = p->sampler == 1;
+    int fault = 0;
+    if (fault_cell) *fault_cell = -1;
+    for (int64_t n = 0; n < nsteps; ++n) {
+        const int64_t step = step0 + n;
+        const int last = (n == nsteps - 1);
+        memcpy(old, fields, sizeof(double) * N);        /* DRho[0] */
+        for (int pp = 0; pp < L; ++pp)                   /* FKE:4154-4211 */
+            for (int q = 0; q < L; ++q) {
+                const int64_t i = (int64_t)pp * L + q;
+                double alpha_i = Dp * (nsum(old, L, pp, q));
+                int64_t draw = -1;
+                if (!(alpha_i == 0 && old[i] == 0)) {
+                    stream_t s;
+                    st_init(&s, seed, inst, (uint32_t)i, step);
+                    fields[i] = dornic_cell(&s, alpha_i, old[i], ll, lambda, sigma2_1, compat, &draw);
+                }
+                if (last && draws) draws[i] = draw;
+            }
+        rk4_full(p, L, fields, buf, buf + M, buf + 2 * M, buf + 3 * M, buf + 4 * M);   /* FKE:4235 */
+        for (int64_t i = 0; i < M; ++i) {                /* FKE:4352-4462 */
+            double y = fields[i];
+            if (y < 0 || !isfinite(y)) {
+                if (!fault && fault_cell) *fault_cell = i;
+                fault = 1;
+            }
+            if (y < p->flush_eps && y >= 0.0) fields[i] = 0.0;
+        }
+        if (rows) tmo_reduce(L, 3, fields, rows + 6 * n);
+        if (fault) break;
+    }
+    free(buf);
+    return fault;
+}

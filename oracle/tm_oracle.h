@@ -1,0 +1,110 @@
+/*
+ * tm_oracle.h -- CPU oracle for the Dornic / Rietkerk stochastic step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker, never the product: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (trophic_maelstorm_amd/libtm.so) neither links nor calls it.
+ *
+ * A plain-C restatement of the reference's single-species hot path
+ * (paths relative to the reference's simulations/ directory):
+ *   Rietkerk  Rietkerk_FPE/rietkerk_bjork_FKE.cpp:3842-3865 (constants),
+ *             :4154-4211 (Dornic P step), :2363-2444 (f_2Dor + RK4),
+ *             :4352-4465 (checks, flush, copy), :4017-4031 + :1238-1280 (rho(t)).
+ *   DP        Percolation_FD/multiSPDP.cpp:2480-2504 (constants),
+ *             :2901-2958 (Dornic rho step), :1647-1664 + :1821-1898 (RK4 of -b rho^2),
+ *             :3231-3262 (flush, copy); exact update stochasticSPDP.cpp:1064-1068.
+ * fed by counter-based Philox4x32-10 uniforms instead of the reference's
+ * std::mt19937_64 stream (FKE:3896-3899), with samplers written from the
+ * published methods named at each function.  Every integer decision uses only
+ * IEEE-exact operations (+ - * / sqrt floor) and this file's own exp/log, so a
+ * GPU build that performs the same operations in the same order reproduces the
+ * draws bit-for-bit.  Build with -ffp-contract=off.
+ *
+ * Parity pins (tests/test_oracle_*.py): Philox against the Random123
+ * known-answer vectors and rocRAND's host Philox; exp/log against numpy; the
+ * normal quantile against scipy.special.ndtri; Poisson/Gamma samplers by
+ * exact pmf chi-square and moment tests; the Dornic step by the closed-form
+ * CIR transition moments; the Laplacian by its Fourier eigenvalues; RK4 by the
+ * Rietkerk mean-field fixed points and the P=0 closed form; the DP nonlinear
+ * sub-steps by the closed form rho/(1+b rho t); the whole DP step by ensemble
+ * comparison with the reference's own percolationDornic_2D built from source.
+ */
+#ifndef TM_ORACLE_H
+#define TM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- parameter packs (mirror the product's C-ABI structs field for field) ---- */
+typedef struct tmo_dp_params {
+    double a, b, D, sigma, dt, dx;
+    double a_end;          /* quench target; used when ramp_steps > 0 */
+    int64_t ramp_steps;    /* a(n) = a + (a_end - a) * min(n, ramp_steps) / ramp_steps */
+    double flush_eps;      /* multiSPDP.cpp:2426  1e-12 */
+    double flush_neg;      /* multiSPDP.cpp:3257  -1e-80 */
+    int32_t scheme;        /* 0 Euler, 1 RK4 (multiSPDP.cpp:1821), 2 exact (stochasticSPDP.cpp:1066) */
+    int32_t sampler;       /* 0 exact Poisson/Gamma, 1 reference Gaussian shortcuts above 100 */
+} tmo_dp_params;
+
+typedef struct tmo_rk_params {
+    double R, c, gmax, alpha, rW, W0, d;
+    double D[3];           /* Dp, Dw, Do */
+    double K[3];           /* k0 (unused by the 1-species RHS), k1, k2 */
+    double sigma;          /* sigma_P */
+    double dt, dx;
+    double flush_eps;      /* rietkerk_bjork_basic.h:119  1e-10 */
+    int32_t sampler;
+    int32_t pad_;
+} tmo_rk_params;
+
+/* ---- primitives exposed for the oracle's own pins ---- */
+void tmo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double tmo_exp(double x);
+double tmo_log(double x);
+double tmo_norm_icdf(double p);
+double tmo_logfact(int64_t k);
+
+/* Draw n samples from one stream (key = (seed, inst), counter = (cell, step, draw)).
+ * kind: 0 uniform, 1 normal, 2 poisson(param, sampler mode = mode), 3 gamma(param). */
+void tmo_sample(int kind, double param, int mode, uint32_t seed, uint32_t inst,
+                uint32_t cell, int64_t step, int64_t n, double* out);
+
+/* Dornic constants (FKE:3847-3849, MSP:2485-2486): out = {beta, exp(beta dt), lambda} */
+void tmo_dornic_constants(double linear, double Dcoef, double sigma, double dt, double dx, double out[3]);
+
+/* One Dornic transition for a single cell with given alpha_i and rho (moment pins). */
+double tmo_dornic_cell(double alpha_i, double rho, double lambda, double lambda_exp, double sigma,
+                       int sampler, uint32_t seed, uint32_t inst, uint32_t cell, int64_t step,
+                       int64_t* n_out);
+
+/* n independent transitions (cells 0..n-1 of step `step`) of one cell state. */
+void tmo_dornic_batch(double alpha_i, double rho, double lambda, double lambda_exp, double sigma,
+                      int sampler, uint32_t seed, uint32_t inst, int64_t step, int64_t n,
+                      double* out, int64_t* n_out);
+
+/* i.i.d. Uniform(lo[f], hi[f]) initial fields, species-major [nf][L*L]. */
+void tmo_init_uniform(int L, int nf, uint32_t seed, uint32_t inst,
+                      const double* lo, const double* hi, double* out);
+
+/* Run nsteps steps in place on host fields (species-major SoA, FKE:3902).
+ * rows (nullable): [nsteps][2*nf] = per field (#cells > 0, spatial mean) after each step.
+ * draws (nullable): [L*L] Poisson integer of the LAST step, -1 where the cell was skipped.
+ * Returns 0, or 1 if a negative / non-finite value was seen (first one in *fault_cell). */
+int tmo_dp_run(const tmo_dp_params* p, int L, uint32_t seed, uint32_t inst, int64_t step0,
+               int64_t nsteps, double* rho, double* rows, int64_t* draws, int64_t* fault_cell);
+int tmo_rk_run(const tmo_rk_params* p, int L, uint32_t seed, uint32_t inst, int64_t step0,
+               int64_t nsteps, double* fields, double* rows, int64_t* draws, int64_t* fault_cell);
+
+/* Deterministic pieces exposed for closed-form pins. */
+void tmo_laplacian(int L, const double* in, double* out);            /* 5-point periodic, FKE:1634-1653 */
+void tmo_rk_rhs(const tmo_rk_params* p, int L, const double* y, double* f); /* f_2Dor, FKE:2363-2390 */
+void tmo_rk4_deterministic(const tmo_rk_params* p, int L, int64_t nsteps, double* y); /* FKE:2393-2429 */
+double tmo_dp_nonlinear(int scheme, double b, double dt, double rho);  /* one nonlinear sub-step */
+void tmo_reduce(int L, int nf, const double* fields, double* row);    /* (#>0, mean) per field */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,186 @@
+"""GPU parity: the HIP step (through the C ABI) against the CPU oracle.
+
+FP64: bit-exact per-cell Poisson integers and fields / rho(t) within 1e-12
+relative (the fields in fact match to the last bit; the rho(t) means differ only
+by summation order).  Run on an MI355X with `pytest -m gpu`.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12   # FP64 tolerance on fields and rho(t) means
+
+
+def assert_rows_close(got, want, rel=REL):
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.shape == want.shape
+    cnt = got[..., 0::2], want[..., 0::2]
+    assert np.array_equal(cnt[0], cnt[1]), "occupied-site counts differ"
+    m_g, m_w = got[..., 1::2], want[..., 1::2]
+    assert np.all(np.abs(m_g - m_w) <= rel * np.maximum(np.abs(m_w), 1e-300)), \
+        f"max rel {np.max(np.abs(m_g - m_w) / np.maximum(np.abs(m_w), 1e-300))}"
+
+
+def dp_case(tm, oracle, L, steps, seed=11, inst=3, step0=0, sampler=0, scheme=0, ramp=0,
+            variant=-1, graph=100, a=0.72, init_hi=1.0):
+    p = tm.DPParams(a=a, a_end=0.70 if ramp else None, ramp_steps=ramp, sampler=sampler, scheme=scheme)
+    po = oracle.dp_params(a=a, a_end=0.70 if ramp else None, ramp_steps=ramp, sampler=sampler,
+                          scheme=scheme)
+    rho0 = oracle.init_uniform(L, [0.0], [init_hi], seed, inst)
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_DP, tm.F64, p, [tm.Instance(L, 0, seed, inst, step0)])
+        ctx.set_launch(variant, graph)
+        ctx.upload(0, rho0)
+        ctx.record_draws(True)
+        rows = ctx.run(steps)
+        got = ctx.download(0)
+        draws = ctx.draws(0)
+    ref = oracle.dp_run(po, rho0, L, seed, inst, step0, steps, rows=True, draws=True)
+    return got, rows[0], draws, ref
+
+
+@pytest.mark.parametrize("L,steps", [(16, 300), (40, 120), (64, 100), (128, 60)])
+def test_dp_f64_bit_exact(tm, oracle, L, steps):
+    got, rows, draws, ref = dp_case(tm, oracle, L, steps)
+    assert ref["status"] == 0
+    assert np.array_equal(draws, ref["draws"]), "Poisson integers differ"
+    assert np.array_equal(got, ref["fields"]), \
+        f"max rel {np.max(np.abs(got - ref['fields']) / np.maximum(np.abs(ref['fields']), 1e-300))}"
+    assert_rows_close(rows, ref["rows"])
+
+
+@pytest.mark.parametrize("sampler,scheme,ramp,a,hi", [
+    (1, 0, 0, 0.72, 1.0),       # reference Gaussian shortcuts (z > 100 needs rho > 11.8)
+    (1, 2, 0, 0.72, 40.0),      # ... with rho up to 40 (exact update: Euler would go negative)
+    (0, 2, 0, 0.72, 40.0),      # PTRS Poisson branch (z >= 12)
+    (0, 1, 0, 0.72, 1.0),       # RK4 nonlinear sub-step
+    (0, 2, 0, 0.72, 1.0),       # exact nonlinear update
+    (0, 0, 40, 0.72, 1.0),      # quench ramp a(t)
+    (0, 0, 0, 1.6, 1.0),        # active phase
+])
+def test_dp_f64_modes(tm, oracle, sampler, scheme, ramp, a, hi):
+    got, rows, draws, ref = dp_case(tm, oracle, 32, 60, sampler=sampler, scheme=scheme, ramp=ramp,
+                                    a=a, init_hi=hi)
+    assert np.array_equal(draws, ref["draws"])
+    assert np.array_equal(got, ref["fields"])
+    assert_rows_close(rows, ref["rows"])
+
+
+def test_dp_f64_tiles_graphs_and_step0_do_not_change_results(tm, oracle):
+    base = None
+    for variant in range(len(tm.variant_names(tm.MODEL_DP, tm.F64))):
+        for graph in (0, 7):
+            got, rows, draws, ref = dp_case(tm, oracle, 48, 23, step0=1000003, variant=variant,
+                                            graph=graph)
+            assert np.array_equal(got, ref["fields"])
+            if base is None:
+                base = got
+            assert np.array_equal(got, base)
+
+
+def rk_case(tm, oracle, L, steps, seed=5, inst=1, sampler=0, variant=-1, graph=100, R=1.0,
+            lo=(0.0, 0.0, 0.0), hi=(90.0, 1.0, 1.0)):
+    p = tm.RietkerkParams(R=R, sampler=sampler)
+    po = oracle.rk_params(R=R, sampler=sampler)
+    f0 = oracle.init_uniform(L, list(lo), list(hi), seed, inst)
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_RIETKERK, tm.F64, p, [tm.Instance(L, 0, seed, inst, 0)])
+        ctx.set_launch(variant, graph)
+        ctx.upload(0, f0)
+        ctx.record_draws(True)
+        rows = ctx.run(steps)
+        got = ctx.download(0)
+        draws = ctx.draws(0)
+    ref = oracle.rk_run(po, f0, L, seed, inst, 0, steps, rows=True, draws=True)
+    return got, rows[0], draws, ref
+
+
+@pytest.mark.parametrize("L,steps", [(16, 40), (24, 30), (64, 20)])
+def test_rietkerk_f64_bit_exact(tm, oracle, L, steps):
+    got, rows, draws, ref = rk_case(tm, oracle, L, steps)
+    assert ref["status"] == 0
+    assert np.array_equal(draws, ref["draws"]), "Poisson integers differ"
+    assert np.array_equal(got, ref["fields"]), \
+        f"max rel {np.max(np.abs(got - ref['fields']) / np.maximum(np.abs(ref['fields']), 1e-300))}"
+    assert_rows_close(rows, ref["rows"])
+
+
+def test_rietkerk_f64_compat_and_tiles(tm, oracle):
+    for variant in range(len(tm.variant_names(tm.MODEL_RIETKERK, tm.F64))):
+        got, rows, draws, ref = rk_case(tm, oracle, 40, 12, sampler=1, variant=variant, graph=5)
+        assert np.array_equal(draws, ref["draws"])
+        assert np.array_equal(got, ref["fields"])
+        assert_rows_close(rows, ref["rows"])
+
+
+def test_mixed_size_batch_matches_single_runs(tm, oracle):
+    Ls = [16, 32, 24, 64, 16]
+    ps = [tm.DPParams(a=0.72), tm.DPParams(a=1.1, a_end=1.0, ramp_steps=10)]
+    pos = [oracle.dp_params(a=0.72), oracle.dp_params(a=1.1, a_end=1.0, ramp_steps=10)]
+    insts = [tm.Instance(L, k % 2, 100 + k, 7 * k, 3 * k) for k, L in enumerate(Ls)]
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_DP, tm.F64, ps, insts)
+        ctx.init_uniform([0.0], [1.0])
+        init = [ctx.download(k) for k in range(len(Ls))]
+        rows = ctx.run(37)
+        got = [ctx.download(k) for k in range(len(Ls))]
+    for k, d in enumerate(insts):
+        want0 = oracle.init_uniform(d.L, [0.0], [1.0], d.seed, d.instance_id)
+        assert np.array_equal(init[k], want0)
+        ref = oracle.dp_run(pos[d.params_index], want0, d.L, d.seed, d.instance_id, d.step0, 37)
+        assert np.array_equal(got[k], ref["fields"])
+        assert_rows_close(rows[k], ref["rows"])
+
+
+def test_init_uniform_and_measure(tm, oracle):
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_RIETKERK, tm.F64, tm.RietkerkParams(), [tm.Instance(33, 0, 9, 4)])
+        ctx.init_uniform([0.0, 0.0, 0.0], [90.0, 1.0, 1.0])
+        f = ctx.download(0)
+        m = ctx.measure()
+    want = oracle.init_uniform(33, [0, 0, 0], [90, 1, 1], 9, 4)
+    assert np.array_equal(f, want)
+    assert_rows_close(m[0], oracle.reduce(want))
+
+
+def test_absorbing_state_is_exact(tm, oracle):
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_DP, tm.F64, tm.DPParams(), [tm.Instance(32)])
+        ctx.record_draws(True)
+        rows = ctx.run(10)
+        assert np.all(ctx.download(0) == 0.0)
+        assert np.all(ctx.draws(0) == -1)
+        assert np.all(rows == 0.0)
+
+
+def test_numeric_fault_is_reported(tm, oracle):
+    rho = np.zeros((1, 16, 16))
+    rho[0, 5, 7] = 100.0     # explicit Euler drives it negative: 100 - 0.1*2*100^2 < 0
+    po = oracle.dp_params()
+    ref = oracle.dp_run(po, rho, 16, 1, 0, 0, 1)
+    assert ref["status"] == 1
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_DP, tm.F64, tm.DPParams(), [tm.Instance(16, 0, 1, 0)])
+        ctx.upload(0, rho)
+        with pytest.raises(tm.NumericFault):
+            ctx.run(1)
+        inst, cell, step = ctx.fault_info()
+        assert (inst, step) == (0, 0)
+        assert cell == ref["fault_cell"]
+
+
+def test_autotune_does_not_change_state(tm, oracle, tmp_path):
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_DP, tm.F64, tm.DPParams(), [tm.Instance(64, 0, 1, 0), tm.Instance(32, 0, 1, 1)])
+        ctx.init_uniform([0.0], [1.0])
+        ctx.run(3)
+        before = [ctx.download(k) for k in range(2)]
+        v, g = ctx.autotune(str(tmp_path / "tune.txt"))
+        assert 0 <= v < len(tm.variant_names(tm.MODEL_DP, tm.F64))
+        assert [np.array_equal(ctx.download(k), before[k]) for k in range(2)] == [True, True]
+        assert ctx.step(0) == 3
+        # cached choice is reused
+        assert ctx.autotune(str(tmp_path / "tune.txt")) == (v, g)
+        ms = ctx.time_steps(4)
+        assert ms > 0 and ctx.step(1) == 7

@@ -1,0 +1,293 @@
+"""trophic_maelstorm_amd -- MI355X (gfx950) integrator for the Dornic / Rietkerk
+stochastic step of Trophic_Maelstorm.
+
+The compute path is libtm.so (hand-written HIP for gfx950) behind the C ABI in
+include/tm/tm.h.  This module is a thin ctypes binding of that ABI with the
+reference's parameter packs as Python dataclasses; it performs no computation
+of its own and raises if libtm.so is missing or no gfx950 device is present.
+
+Reference call sites (relative to the reference's simulations/ directory):
+  rietkerk_DorFPE_2D_MultiSp   Rietkerk_FPE/rietkerk_bjork_FKE.cpp:3697
+  dP_Dornic_2D_MultiSp         Percolation_FD/multiSPDP.cpp:2421
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtm.so")
+
+MODEL_DP, MODEL_RIETKERK = 1, 3
+F64, F32 = 0, 1
+SAMPLER_EXACT, SAMPLER_COMPAT = 0, 1
+DP_EULER, DP_RK4, DP_EXACT = 0, 1, 2
+
+TM_OK, TM_FAULT_NUMERIC = 0, 1
+_STATUS = {0: "TM_OK", -1: "TM_ERR_ARG", -2: "TM_ERR_HIP", -3: "TM_ERR_NOMEM", -4: "TM_ERR_STATE",
+           -5: "TM_ERR_NODEV", 1: "TM_FAULT_NUMERIC"}
+
+
+class TMError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class NumericFault(TMError):
+    """A negative or non-finite value appeared (the reference's exit(3) path,
+    rietkerk_bjork_FKE.cpp:4489-4560)."""
+
+
+class _DP(C.Structure):
+    _fields_ = [("a", C.c_double), ("b", C.c_double), ("D", C.c_double), ("sigma", C.c_double),
+                ("dt", C.c_double), ("dx", C.c_double), ("a_end", C.c_double),
+                ("ramp_steps", C.c_int64), ("flush_eps", C.c_double), ("flush_neg", C.c_double),
+                ("scheme", C.c_int32), ("sampler", C.c_int32)]
+
+
+class _RK(C.Structure):
+    _fields_ = [("R", C.c_double), ("c", C.c_double), ("gmax", C.c_double), ("alpha", C.c_double),
+                ("rW", C.c_double), ("W0", C.c_double), ("d", C.c_double),
+                ("D", C.c_double * 3), ("K", C.c_double * 3), ("sigma", C.c_double),
+                ("dt", C.c_double), ("dx", C.c_double), ("flush_eps", C.c_double),
+                ("sampler", C.c_int32), ("reserved", C.c_int32)]
+
+
+class _Inst(C.Structure):
+    _fields_ = [("L", C.c_int32), ("params_index", C.c_int32), ("seed", C.c_uint32),
+                ("instance_id", C.c_uint32), ("step0", C.c_int64)]
+
+
+@dataclass
+class DPParams:
+    """dP_Dornic_2D_MultiSp(..., a, b, c, D[Sp], ..., sigma[], ..., dt, dx, ...)
+    (Percolation_FD/MultiSPDP.h:344-346) plus the quench ramp a -> a_end."""
+    a: float = 0.72
+    b: float = 2.0
+    D: float = 1.0
+    sigma: float = 2.0 ** 0.5
+    dt: float = 0.1
+    dx: float = 1.0
+    a_end: float | None = None
+    ramp_steps: int = 0
+    flush_eps: float = 1e-12
+    flush_neg: float = -1e-80
+    scheme: int = DP_EULER
+    sampler: int = SAMPLER_EXACT
+
+    def _c(self) -> _DP:
+        return _DP(self.a, self.b, self.D, self.sigma, self.dt, self.dx,
+                   self.a if self.a_end is None else self.a_end, self.ramp_steps, self.flush_eps,
+                   self.flush_neg, self.scheme, self.sampler)
+
+
+@dataclass
+class RietkerkParams:
+    """rietkerk_DorFPE_2D_MultiSp(..., a(=R), c, gmax, alpha, rW, W0, D[Sp], ..., K[3],
+    sigma[], ..., M[0](=d), ..., dt, dx, ...) (Rietkerk_FPE/rietkerk_bjork_basic.h:450-452)."""
+    R: float = 1.0
+    c: float = 10.0
+    gmax: float = 0.05
+    alpha: float = 0.2
+    rW: float = 0.2
+    W0: float = 0.2
+    d: float = 0.25
+    D: tuple = (0.1, 0.1, 100.0)
+    K: tuple = (0.0, 5.0, 5.0)
+    sigma: float = 0.5
+    dt: float = 0.005
+    dx: float = 2.0
+    flush_eps: float = 1e-10
+    sampler: int = SAMPLER_EXACT
+
+    def _c(self) -> _RK:
+        return _RK(self.R, self.c, self.gmax, self.alpha, self.rW, self.W0, self.d,
+                   (C.c_double * 3)(*self.D), (C.c_double * 3)(*self.K), self.sigma, self.dt,
+                   self.dx, self.flush_eps, self.sampler, 0)
+
+
+@dataclass
+class Instance:
+    L: int
+    params_index: int = 0
+    seed: int = 1
+    instance_id: int = 0
+    step0: int = 0
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libtm.so; raises when it has not been built (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run trophic_maelstorm_amd.build.build() "
+                           "(hipcc, gfx950) before using the HIP integrator")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, i32, dp = C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)
+    sig = {
+        "tm_create": ([C.POINTER(vp), C.c_int], C.c_int),
+        "tm_destroy": ([vp], None),
+        "tm_last_error": ([vp], C.c_char_p),
+        "tm_configure": ([vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int], C.c_int),
+        "tm_upload_fields": ([vp, C.c_int, dp], C.c_int),
+        "tm_download_fields": ([vp, C.c_int, dp], C.c_int),
+        "tm_init_uniform": ([vp, dp, dp], C.c_int),
+        "tm_run": ([vp, i64, vp], C.c_int),
+        "tm_measure": ([vp, dp], C.c_int),
+        "tm_set_record_draws": ([vp, C.c_int], C.c_int),
+        "tm_download_draws": ([vp, C.c_int, C.POINTER(i64)], C.c_int),
+        "tm_fault_info": ([vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)], C.c_int),
+        "tm_get_step": ([vp, C.c_int, C.POINTER(i64)], C.c_int),
+        "tm_reset": ([vp], C.c_int),
+        "tm_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
+        "tm_variant_count": ([C.c_int, C.c_int], C.c_int),
+        "tm_variant_name": ([C.c_int, C.c_int, C.c_int], C.c_char_p),
+        "tm_autotune": ([vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "tm_time_steps": ([vp, i64, dp, C.POINTER(i64)], C.c_int),
+        "tm_stream": ([vp], vp),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def variant_names(model: int, precision: int) -> list[str]:
+    L = lib()
+    return [L.tm_variant_name(model, precision, v).decode()
+            for v in range(L.tm_variant_count(model, precision))]
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One tm_ctx: a batch of instances of one model at one precision on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self._lib = lib()
+        h = C.c_void_p()
+        st = self._lib.tm_create(C.byref(h), device)
+        if st != TM_OK:
+            raise TMError(st, f"tm_create(device={device}) failed: a gfx950 GPU is required")
+        self._h = h
+        self.model = None
+        self.instances: list[Instance] = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int):
+        if st == TM_OK:
+            return
+        msg = self._lib.tm_last_error(self._h).decode()
+        if st == TM_FAULT_NUMERIC:
+            raise NumericFault(st, msg)
+        raise TMError(st, msg)
+
+    @property
+    def nfields(self) -> int:
+        return int(self.model)
+
+    def configure(self, model: int, precision: int, params, instances):
+        plist = params if isinstance(params, (list, tuple)) else [params]
+        ctype = _DP if model == MODEL_DP else _RK
+        parr = (ctype * len(plist))(*[p._c() for p in plist])
+        iarr = (_Inst * len(instances))(*[_Inst(i.L, i.params_index, i.seed, i.instance_id, i.step0)
+                                          for i in instances])
+        self._check(self._lib.tm_configure(self._h, model, precision, C.cast(parr, C.c_void_p),
+                                           len(plist), C.cast(iarr, C.c_void_p), len(instances)))
+        self.model, self.precision, self.instances = model, precision, list(instances)
+        return self
+
+    def upload(self, i: int, fields: np.ndarray):
+        f = np.ascontiguousarray(fields, np.float64)
+        L = self.instances[i].L
+        if f.size != self.nfields * L * L:
+            raise ValueError(f"instance {i}: expected {self.nfields}x{L}x{L} values, got {f.shape}")
+        self._check(self._lib.tm_upload_fields(self._h, i, _dptr(f)))
+
+    def download(self, i: int) -> np.ndarray:
+        L = self.instances[i].L
+        out = np.empty((self.nfields, L, L), np.float64)
+        self._check(self._lib.tm_download_fields(self._h, i, _dptr(out)))
+        return out
+
+    def init_uniform(self, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        self._check(self._lib.tm_init_uniform(self._h, _dptr(lo), _dptr(hi)))
+
+    def run(self, n_steps: int, rows: bool = True):
+        """Advance n_steps; returns rows [n_inst, n_steps, 2*nf] of (#>0, mean) per field."""
+        out = np.empty((len(self.instances), n_steps, 2 * self.nfields), np.float64) if rows else None
+        self._check(self._lib.tm_run(self._h, n_steps, out.ctypes.data if out is not None else None))
+        return out
+
+    def measure(self) -> np.ndarray:
+        out = np.empty((len(self.instances), 2 * self.nfields), np.float64)
+        self._check(self._lib.tm_measure(self._h, _dptr(out)))
+        return out
+
+    def record_draws(self, enable: bool = True):
+        self._check(self._lib.tm_set_record_draws(self._h, int(enable)))
+
+    def draws(self, i: int) -> np.ndarray:
+        L = self.instances[i].L
+        out = np.empty((L, L), np.int64)
+        self._check(self._lib.tm_download_draws(self._h, i, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+    def fault_info(self):
+        inst, cell, step = C.c_int32(), C.c_int64(), C.c_int64()
+        self._check(self._lib.tm_fault_info(self._h, C.byref(inst), C.byref(cell), C.byref(step)))
+        return inst.value, cell.value, step.value
+
+    def step(self, i: int = 0) -> int:
+        s = C.c_int64()
+        self._check(self._lib.tm_get_step(self._h, i, C.byref(s)))
+        return s.value
+
+    def reset(self):
+        """Rewind every instance to its step0 and clear the fault record (fields untouched)."""
+        self._check(self._lib.tm_reset(self._h))
+
+    def set_launch(self, variant: int = -1, graph_steps: int = 100):
+        self._check(self._lib.tm_set_launch(self._h, variant, graph_steps))
+
+    def autotune(self, cache_path: str | None = None):
+        v, g = C.c_int(), C.c_int()
+        self._check(self._lib.tm_autotune(self._h, cache_path.encode() if cache_path else None,
+                                          C.byref(v), C.byref(g)))
+        return v.value, g.value
+
+    def time_steps(self, n_steps: int) -> float:
+        """Device ms of n_steps step-kernel launches (HIP events on the context stream)."""
+        ms, n = C.c_double(), C.c_int64()
+        self._check(self._lib.tm_time_steps(self._h, n_steps, C.byref(ms), C.byref(n)))
+        return ms.value

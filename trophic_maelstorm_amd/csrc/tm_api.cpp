@@ -1,0 +1,771 @@
+// tm_api.cpp -- host side of the C ABI declared in include/tm/tm.h.
+//
+// A context owns: the instance descriptor and tile tables, the double-buffered
+// fields of every instance (one allocation, instance blocks [nf][L*L] back to
+// back), the tile partials of the per-step rho(t) sums, the device step state,
+// the rho(t) rows, one HIP stream, and up to two instantiated hipGraphs (one
+// per field-buffer parity) each replaying `graph_steps` step launches plus the
+// rows fold.  The host tracks the same step counter the kernels advance.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "tm/tm.h"
+#include "tm_launch.h"
+
+using namespace tmk;
+
+struct tm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::string arch;
+    int n_cu = 0;
+
+    // configuration
+    int model = 0, prec = 0, nf = 0;
+    size_t param_size = 0;
+    std::vector<unsigned char> params;
+    int n_params = 0;
+    std::vector<tm_instance_desc> inst;
+    std::vector<int64_t> cell_off;
+    int64_t total_cells = 0;
+    size_t esize = 0;
+
+    // launch shape
+    int variant = -1;
+    int graph_steps = 100;
+    int xcd = 1;
+    std::vector<DevTile> tiles;
+    std::vector<DevInst> dinst;
+
+    // device memory
+    void* d_params = nullptr;
+    DevInst* d_inst = nullptr;
+    DevTile* d_tiles = nullptr;
+    void* d_fields[2] = {nullptr, nullptr};
+    double2* d_partials = nullptr;
+    void* d_consts = nullptr;      // [nslots][n_inst] per-step constants
+    int nslots = 0;
+    DevState* d_state = nullptr;
+    double* d_rows = nullptr;
+    int64_t rows_cap = 0;
+    int64_t* d_draws = nullptr;
+    bool record_draws = false;
+    double* h_pinned = nullptr;   // small pinned staging for state writes
+
+    int parity = 0;
+    int64_t counter = 0;
+
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+const char* kDefaultTuneCache = "/.cache/trophic_maelstorm_amd/tune.txt";
+
+#define HIPCHK(ctx, call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                \
+            return TM_ERR_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+tm_status fail(tm_ctx* c, tm_status s, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return s;
+}
+
+const Variant* variant_table(int model, int prec, int* n)
+{
+    if (model == TM_MODEL_DP) return prec == TM_F64 ? dp_variants_f64(n) : dp_variants_f32(n);
+    if (model == TM_MODEL_RIETKERK) return prec == TM_F64 ? rk_variants_f64(n) : rk_variants_f32(n);
+    *n = 0;
+    return nullptr;
+}
+
+void drop_graphs(tm_ctx* c)
+{
+    for (auto& g : c->graph) {
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+template <typename P>
+void free_dev(P*& p)
+{
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+void release_config(tm_ctx* c)
+{
+    drop_graphs(c);
+    free_dev(c->d_params);
+    free_dev(c->d_inst);
+    free_dev(c->d_tiles);
+    free_dev(c->d_fields[0]);
+    free_dev(c->d_fields[1]);
+    free_dev(c->d_partials);
+    free_dev(c->d_consts);
+    free_dev(c->d_rows);
+    free_dev(c->d_draws);
+    c->rows_cap = 0;
+    c->nslots = 0;
+    c->model = 0;
+}
+
+const Variant& cur_variant(tm_ctx* c)
+{
+    int n;
+    const Variant* v = variant_table(c->model, c->prec, &n);
+    return v[c->variant];
+}
+
+// (Re)build the tile table for the current variant and upload it with the
+// instance descriptors; (re)size the partials buffer.
+tm_status build_tiles(tm_ctx* c)
+{
+    drop_graphs(c);
+    const Variant& v = cur_variant(c);
+    c->tiles.clear();
+    c->dinst.clear();
+    for (size_t i = 0; i < c->inst.size(); ++i) {
+        const tm_instance_desc& d = c->inst[i];
+        DevInst di{};
+        di.L = d.L;
+        di.pidx = d.params_index;
+        di.seed = d.seed;
+        di.inst_id = d.instance_id;
+        di.step0 = d.step0;
+        di.cell_off = c->cell_off[i];
+        di.tile0 = (int32_t)c->tiles.size();
+        for (int y0 = 0; y0 < d.L; y0 += v.ty)
+            for (int x0 = 0; x0 < d.L; x0 += v.tx) c->tiles.push_back(DevTile{(int32_t)i, x0, y0, 0});
+        di.ntiles = (int32_t)c->tiles.size() - di.tile0;
+        c->dinst.push_back(di);
+    }
+    free_dev(c->d_tiles);
+    free_dev(c->d_partials);
+    HIPCHK(c, hipMalloc((void**)&c->d_tiles, sizeof(DevTile) * c->tiles.size()));
+    HIPCHK(c, hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(DevTile) * c->tiles.size(), hipMemcpyHostToDevice));
+    if (!c->d_inst) HIPCHK(c, hipMalloc((void**)&c->d_inst, sizeof(DevInst) * c->dinst.size()));
+    HIPCHK(c, hipMemcpy(c->d_inst, c->dinst.data(), sizeof(DevInst) * c->dinst.size(), hipMemcpyHostToDevice));
+    c->nslots = std::max(c->graph_steps, 64);
+    HIPCHK(c, hipMalloc((void**)&c->d_partials, sizeof(double2) * (size_t)c->nslots * c->tiles.size() * c->nf));
+    free_dev(c->d_consts);
+    HIPCHK(c, hipMalloc(&c->d_consts, const_size(c->model) * (size_t)c->nslots * c->inst.size()));
+    return TM_OK;
+}
+
+StepArgs step_args(tm_ctx* c, int parity, int k)
+{
+    StepArgs a{};
+    a.inst = c->d_inst;
+    a.tiles = c->d_tiles;
+    a.consts = c->d_consts;
+    a.in = c->d_fields[parity];
+    a.out = c->d_fields[parity ^ 1];
+    a.partials = c->d_partials;
+    a.state = c->d_state;
+    a.draws = c->record_draws ? c->d_draws : nullptr;
+    a.ntiles = (int32_t)c->tiles.size();
+    a.nslots = c->nslots;
+    a.k = k;
+    a.n_inst = (int32_t)c->inst.size();
+    a.xcd = c->xcd;
+    return a;
+}
+
+int grid_of(tm_ctx* c) { return (int)c->tiles.size(); }
+
+void launch_step(tm_ctx* c, int parity, int k)
+{
+    const StepArgs a = step_args(c, parity, k);
+    cur_variant(c).fn(a, grid_of(c), c->stream);
+}
+
+void launch_fold(tm_ctx* c, int nsteps)
+{
+    CombineArgs a{};
+    a.inst = c->d_inst;
+    a.partials = c->d_partials;
+    a.state = c->d_state;
+    a.rows = c->d_rows;
+    a.ntiles = (int32_t)c->tiles.size();
+    a.nslots = c->nslots;
+    a.nf = c->nf;
+    a.nsteps = nsteps;
+    a.rows_cap = c->rows_cap;
+    a.measure = 0;
+    launch_combine(a, (int)c->inst.size(), nsteps, c->stream);
+}
+
+// One block of n <= nslots steps starting from buffer `parity`: constants,
+// n step launches, the rows fold (optional) and the counter bump.
+void enqueue_block(tm_ctx* c, int parity, int n, bool fold)
+{
+    launch_prep(c->model, c->d_inst, c->d_params, c->d_state, (int)c->inst.size(), n, c->d_consts, c->stream);
+    for (int k = 0; k < n; ++k) {
+        launch_step(c, parity, k);
+        parity ^= 1;
+    }
+    if (fold) launch_fold(c, n);
+    launch_bump(c->d_state, n, c->stream);
+}
+
+// Capture one block of graph_steps steps (with the rows fold) for a parity.
+tm_status capture_block(tm_ctx* c, int parity, int n, bool fold, hipGraphExec_t* out)
+{
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_block(c, parity, n, fold);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) return fail(c, TM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(c, TM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return TM_OK;
+}
+
+tm_status capture(tm_ctx* c, int parity)
+{
+    if (c->graph[parity]) return TM_OK;
+    return capture_block(c, parity, c->graph_steps, true, &c->graph[parity]);
+}
+
+tm_status ensure_rows(tm_ctx* c, int64_t n_steps)
+{
+    if (c->rows_cap >= n_steps) return TM_OK;
+    drop_graphs(c);
+    free_dev(c->d_rows);
+    const int64_t cap = std::max<int64_t>(n_steps, 256);
+    HIPCHK(c, hipMalloc((void**)&c->d_rows, sizeof(double) * (size_t)cap * c->inst.size() * 2 * c->nf));
+    c->rows_cap = cap;
+    return TM_OK;
+}
+
+tm_status check_ready(tm_ctx* c)
+{
+    if (!c) return TM_ERR_ARG;
+    if (!c->model) return fail(c, TM_ERR_STATE, "tm_configure has not been called");
+    return TM_OK;
+}
+
+tm_status set_run_base(tm_ctx* c)
+{
+    // run_base <- counter, ordered on the stream before the step launches.
+    *(int64_t*)c->h_pinned = c->counter;
+    HIPCHK(c, hipMemcpyAsync((char*)c->d_state + offsetof(DevState, run_base), c->h_pinned, sizeof(int64_t),
+                             hipMemcpyHostToDevice, c->stream));
+    return TM_OK;
+}
+
+// Advance n_steps with the rows fold; rows stay on the device.
+tm_status advance(tm_ctx* c, int64_t n_steps)
+{
+    tm_status s = ensure_rows(c, n_steps);
+    if (s != TM_OK) return s;
+    if ((s = set_run_base(c)) != TM_OK) return s;
+    int64_t left = n_steps;
+    const int B = c->graph_steps;
+    while (B > 0 && left >= B) {
+        if ((s = capture(c, c->parity)) != TM_OK) return s;
+        HIPCHK(c, hipGraphLaunch(c->graph[c->parity], c->stream));
+        c->counter += B;
+        c->parity ^= (B & 1);
+        left -= B;
+    }
+    while (left > 0) {
+        const int chunk = (int)std::min<int64_t>(left, c->nslots);
+        enqueue_block(c, c->parity, chunk, true);
+        HIPCHK(c, hipGetLastError());
+        c->parity ^= (chunk & 1);
+        c->counter += chunk;
+        left -= chunk;
+    }
+    return TM_OK;
+}
+
+// First fault recorded by the kernels: (instance, cell, instance step), or all -1.
+tm_status read_fault(tm_ctx* c, int32_t* inst, int64_t* cell, int64_t* step)
+{
+    DevState st;
+    HIPCHK(c, hipMemcpy(&st, c->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    *inst = -1;
+    *cell = -1;
+    *step = -1;
+    if (st.fault_counter != ~0ull) {
+        *inst = (int32_t)(st.fault_key >> 40);
+        *cell = (int64_t)(st.fault_key & ((1ull << 40) - 1));
+        *step = (*inst >= 0 && *inst < (int)c->inst.size() ? c->inst[*inst].step0 : 0) + (int64_t)st.fault_counter;
+    }
+    return TM_OK;
+}
+
+tm_status fault_status(tm_ctx* c)
+{
+    int32_t inst;
+    int64_t cell, step;
+    tm_status s = read_fault(c, &inst, &cell, &step);
+    if (s != TM_OK) return s;
+    if (inst >= 0) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "numeric fault: instance %d cell %lld step %lld", inst, (long long)cell,
+                 (long long)step);
+        c->err = buf;
+        return TM_FAULT_NUMERIC;
+    }
+    return TM_OK;
+}
+
+// Zero state with no fault recorded.
+tm_status reset_state(tm_ctx* c)
+{
+    DevState st{};
+    st.fault_counter = ~0ull;
+    st.fault_key = ~0ull;
+    HIPCHK(c, hipMemcpy(c->d_state, &st, sizeof(st), hipMemcpyHostToDevice));
+    return TM_OK;
+}
+
+std::string tune_key(tm_ctx* c)
+{
+    std::map<int, int> sizes;
+    for (auto& d : c->inst) sizes[d.L] += 1;
+    std::ostringstream k;
+    k << c->arch << "|m" << c->model << "|p" << c->prec << "|";
+    for (auto& kv : sizes) k << kv.first << "x" << kv.second << ",";
+    return k.str();
+}
+
+std::string tune_path(const char* p)
+{
+    if (p && *p) return p;
+    if (const char* e = getenv("TM_TUNE_CACHE")) return e;
+    const char* h = getenv("HOME");
+    return std::string(h ? h : "/tmp") + kDefaultTuneCache;
+}
+
+void mkdirs(const std::string& path)
+{
+    for (size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '/') mkdir(path.substr(0, i).c_str(), 0755);
+}
+
+}  // namespace
+
+extern "C" {
+
+tm_status tm_create(tm_ctx** out, int device)
+{
+    if (!out) return TM_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return TM_ERR_NODEV;
+    tm_ctx* c = new tm_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return TM_ERR_NODEV;
+    }
+    c->arch = prop.gcnArchName;
+    c->n_cu = prop.multiProcessorCount;
+    if (c->arch.rfind("gfx950", 0) != 0) {
+        delete c;
+        return TM_ERR_NODEV;   // code objects are built for gfx950 only
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&c->d_state, sizeof(DevState)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_pinned, 64, hipHostMallocDefault) != hipSuccess) {
+        tm_destroy(c);
+        return TM_ERR_HIP;
+    }
+    *out = c;
+    return TM_OK;
+}
+
+void tm_destroy(tm_ctx* c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    release_config(c);
+    free_dev(c->d_state);
+    if (c->h_pinned) hipHostFree(c->h_pinned);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* tm_last_error(const tm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, int n_params,
+                       const tm_instance_desc* instances, int n_inst)
+{
+    if (!c) return TM_ERR_ARG;
+    if (model != TM_MODEL_DP && model != TM_MODEL_RIETKERK) return fail(c, TM_ERR_ARG, "unknown model");
+    if (precision != TM_F64 && precision != TM_F32) return fail(c, TM_ERR_ARG, "unknown precision");
+    if (!params || n_params <= 0 || !instances || n_inst <= 0) return fail(c, TM_ERR_ARG, "empty batch");
+    for (int i = 0; i < n_inst; ++i) {
+        const tm_instance_desc& d = instances[i];
+        if (d.L < 1 || d.L > 65536) return fail(c, TM_ERR_ARG, "instance L out of range [1, 65536]");
+        if ((int64_t)d.L * d.L > 0xFFFFFFFFll) return fail(c, TM_ERR_ARG, "instance too large for 32-bit cell ids");
+        if (d.params_index < 0 || d.params_index >= n_params) return fail(c, TM_ERR_ARG, "params_index out of range");
+        if (d.step0 < 0) return fail(c, TM_ERR_ARG, "negative step0");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    release_config(c);
+    c->model = model;
+    c->prec = precision;
+    c->nf = model;   // enum value = number of fields
+    c->esize = precision == TM_F64 ? sizeof(double) : sizeof(float);
+    c->param_size = model == TM_MODEL_DP ? sizeof(tm_dp_params) : sizeof(tm_rietkerk_params);
+    c->params.assign((const unsigned char*)params, (const unsigned char*)params + c->param_size * n_params);
+    c->n_params = n_params;
+    c->inst.assign(instances, instances + n_inst);
+    c->cell_off.resize(n_inst);
+    c->total_cells = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        c->cell_off[i] = c->total_cells;
+        c->total_cells += (int64_t)instances[i].L * instances[i].L;
+    }
+    c->variant = default_variant(model, precision);
+    HIPCHK(c, hipMalloc(&c->d_params, c->param_size * n_params));
+    HIPCHK(c, hipMemcpy(c->d_params, c->params.data(), c->param_size * n_params, hipMemcpyHostToDevice));
+    const size_t fbytes = c->esize * (size_t)c->nf * c->total_cells;
+    for (int b = 0; b < 2; ++b) {
+        if (hipMalloc(&c->d_fields[b], fbytes) != hipSuccess) {
+            release_config(c);
+            return fail(c, TM_ERR_NOMEM, "field allocation failed");
+        }
+        HIPCHK(c, hipMemsetAsync(c->d_fields[b], 0, fbytes, c->stream));
+    }
+    HIPCHK(c, hipMalloc((void**)&c->d_draws, sizeof(int64_t) * (size_t)c->total_cells));
+    HIPCHK(c, hipMemsetAsync(c->d_draws, 0xFF, sizeof(int64_t) * (size_t)c->total_cells, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (reset_state(c) != TM_OK) return TM_ERR_HIP;
+    c->parity = 0;
+    c->counter = 0;
+    tm_status s = build_tiles(c);
+    if (s != TM_OK) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TM_OK;
+}
+
+tm_status tm_upload_fields(tm_ctx* c, int i, const double* host)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (i < 0 || i >= (int)c->inst.size() || !host) return fail(c, TM_ERR_ARG, "bad instance or pointer");
+    const int64_t n = (int64_t)c->nf * c->inst[i].L * c->inst[i].L;
+    char* dst = (char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[i];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->prec == TM_F64) {
+        HIPCHK(c, hipMemcpy(dst, host, sizeof(double) * n, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(host, host + n);
+        HIPCHK(c, hipMemcpy(dst, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
+    return TM_OK;
+}
+
+tm_status tm_download_fields(tm_ctx* c, int i, double* host)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (i < 0 || i >= (int)c->inst.size() || !host) return fail(c, TM_ERR_ARG, "bad instance or pointer");
+    const int64_t n = (int64_t)c->nf * c->inst[i].L * c->inst[i].L;
+    const char* src = (const char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[i];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->prec == TM_F64) {
+        HIPCHK(c, hipMemcpy(host, src, sizeof(double) * n, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        HIPCHK(c, hipMemcpy(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < n; ++k) host[k] = tmp[k];
+    }
+    return TM_OK;
+}
+
+tm_status tm_init_uniform(tm_ctx* c, const double* lo, const double* hi)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (!lo || !hi) return fail(c, TM_ERR_ARG, "null bounds");
+    for (size_t i = 0; i < c->inst.size(); ++i) {
+        void* dst = (char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[i];
+        const tm_instance_desc& d = c->inst[i];
+        if (c->prec == TM_F64) launch_init_f64(c->nf, d.L, d.seed, d.instance_id, lo, hi, dst, c->stream);
+        else launch_init_f32(c->nf, d.L, d.seed, d.instance_id, lo, hi, dst, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TM_OK;
+}
+
+tm_status tm_run(tm_ctx* c, int64_t n_steps, double* rows)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (n_steps < 0) return fail(c, TM_ERR_ARG, "negative n_steps");
+    if (n_steps == 0) return TM_OK;
+    if ((s = advance(c, n_steps)) != TM_OK) return s;
+    if (rows) {
+        const size_t w = sizeof(double) * 2 * c->nf;
+        HIPCHK(c, hipMemcpy2DAsync(rows, w * n_steps, c->d_rows, w * c->rows_cap, w * n_steps, c->inst.size(),
+                                   hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return fault_status(c);
+}
+
+tm_status tm_measure(tm_ctx* c, double* rows)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (!rows) return fail(c, TM_ERR_ARG, "null rows");
+    const Variant& v = cur_variant(c);
+    double* d_out = nullptr;
+    const size_t bytes = sizeof(double) * 2 * c->nf * c->inst.size();
+    HIPCHK(c, hipMalloc((void**)&d_out, bytes));
+    if (c->prec == TM_F64)
+        launch_measure_f64(c->nf, c->d_inst, c->d_tiles, (int)c->tiles.size(), v.tx, v.ty, c->d_fields[c->parity],
+                           c->d_partials, c->stream);
+    else
+        launch_measure_f32(c->nf, c->d_inst, c->d_tiles, (int)c->tiles.size(), v.tx, v.ty, c->d_fields[c->parity],
+                           c->d_partials, c->stream);
+    CombineArgs a{};
+    a.inst = c->d_inst;
+    a.partials = c->d_partials;
+    a.state = c->d_state;
+    a.rows = d_out;
+    a.ntiles = (int32_t)c->tiles.size();
+    a.nslots = c->nslots;
+    a.nf = c->nf;
+    a.nsteps = 1;
+    a.rows_cap = 1;
+    a.measure = 1;
+    launch_combine(a, (int)c->inst.size(), 1, c->stream);
+    hipError_t e = hipMemcpyAsync(rows, d_out, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d_out);
+    if (e != hipSuccess) return fail(c, TM_ERR_HIP, std::string("tm_measure: ") + hipGetErrorString(e));
+    return TM_OK;
+}
+
+tm_status tm_set_record_draws(tm_ctx* c, int enable)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (c->record_draws != (enable != 0)) drop_graphs(c);
+    c->record_draws = enable != 0;
+    return TM_OK;
+}
+
+tm_status tm_download_draws(tm_ctx* c, int i, int64_t* host)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (i < 0 || i >= (int)c->inst.size() || !host) return fail(c, TM_ERR_ARG, "bad instance or pointer");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(host, c->d_draws + c->cell_off[i], sizeof(int64_t) * c->inst[i].L * c->inst[i].L,
+                        hipMemcpyDeviceToHost));
+    return TM_OK;
+}
+
+tm_status tm_fault_info(tm_ctx* c, int32_t* inst, int64_t* cell, int64_t* step)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int32_t i;
+    int64_t ce, st;
+    if ((s = read_fault(c, &i, &ce, &st)) != TM_OK) return s;
+    if (inst) *inst = i;
+    if (cell) *cell = ce;
+    if (step) *step = st;
+    return TM_OK;
+}
+
+tm_status tm_get_step(tm_ctx* c, int i, int64_t* step)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (i < 0 || i >= (int)c->inst.size() || !step) return fail(c, TM_ERR_ARG, "bad instance or pointer");
+    *step = c->inst[i].step0 + c->counter;
+    return TM_OK;
+}
+
+int tm_variant_count(int model, int precision)
+{
+    int n = 0;
+    variant_table(model, precision, &n);
+    return n;
+}
+
+const char* tm_variant_name(int model, int precision, int variant)
+{
+    int n = 0;
+    const Variant* v = variant_table(model, precision, &n);
+    if (!v || variant < 0 || variant >= n) return nullptr;
+    return v[variant].name;
+}
+
+tm_status tm_set_launch(tm_ctx* c, int variant, int graph_steps)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (graph_steps < 0 || graph_steps > 4096) return fail(c, TM_ERR_ARG, "graph_steps out of range [0, 4096]");
+    const int n = tm_variant_count(c->model, c->prec);
+    if (variant < -1 || variant >= n) return fail(c, TM_ERR_ARG, "variant out of range");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->variant = variant < 0 ? default_variant(c->model, c->prec) : variant;
+    c->graph_steps = graph_steps;
+    return build_tiles(c);
+}
+
+void* tm_stream(tm_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+tm_status tm_time_steps(tm_ctx* c, int64_t n_steps, double* ms_total, int64_t* launches)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (n_steps <= 0 || !ms_total) return fail(c, TM_ERR_ARG, "n_steps must be positive");
+    // Step kernels only (no rows fold): graphs of up to 100 step launches, one
+    // per starting parity, replayed between two events on the context stream.
+    const int B = (int)std::min<int64_t>(std::min<int64_t>(n_steps, 100), c->nslots);
+    hipGraphExec_t ge[2] = {nullptr, nullptr};
+    for (int par = 0; par < 2; ++par) {
+        if (par == 1 && (B & 1) == 0) break;   // even blocks keep the parity
+        if ((s = capture_block(c, c->parity ^ par, B, false, &ge[par])) != TM_OK) return s;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    const int p0 = c->parity;
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    int64_t left = n_steps;
+    while (left >= B) {
+        HIPCHK(c, hipGraphLaunch(ge[(c->parity ^ p0) & 1], c->stream));
+        c->parity ^= (B & 1);
+        left -= B;
+    }
+    if (left > 0) {
+        enqueue_block(c, c->parity, (int)left, false);
+        c->parity ^= (int)(left & 1);
+    }
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    for (auto g : ge)
+        if (g) hipGraphExecDestroy(g);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->counter += n_steps;
+    *ms_total = ms;
+    if (launches) *launches = n_steps;
+    return fault_status(c);
+}
+
+tm_status tm_reset(tm_ctx* c)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((s = reset_state(c)) != TM_OK) return s;
+    c->counter = 0;
+    c->parity = 0;
+    return TM_OK;
+}
+
+tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* out_graph_steps)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    const std::string key = tune_key(c);
+    const std::string path = tune_path(cache_path);
+    {   // cached choice
+        std::ifstream f(path);
+        std::string k;
+        int v, g;
+        while (f >> k >> v >> g) {
+            if (k == key && v >= 0 && v < tm_variant_count(c->model, c->prec)) {
+                if ((s = tm_set_launch(c, v, g)) != TM_OK) return s;
+                if (out_variant) *out_variant = v;
+                if (out_graph_steps) *out_graph_steps = g;
+                return TM_OK;
+            }
+        }
+    }
+    // Dry timing: launches read the current buffer and write the other one
+    // without advancing the step counter, so the batch state is unchanged.
+    const int nv = tm_variant_count(c->model, c->prec);
+    const int graph_steps = c->graph_steps;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    int best_v = c->variant;
+    double best = 1e30;
+    const int reps = 6;
+    for (int v = 0; v < nv; ++v) {
+        c->variant = v;
+        if ((s = build_tiles(c)) != TM_OK) return s;
+        launch_prep(c->model, c->d_inst, c->d_params, c->d_state, (int)c->inst.size(), 1, c->d_consts,
+                    c->stream);
+        launch_step(c, c->parity, 0);   // warm-up
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r) launch_step(c, c->parity, 0);
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_v = v;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    // Graph block: short steps are launch-bound and want long graphs.
+    const double per_step_ms = best / reps;
+    const int g = per_step_ms < 0.05 ? 200 : (graph_steps > 0 ? graph_steps : 100);
+    c->variant = best_v;
+    if ((s = tm_set_launch(c, best_v, g)) != TM_OK) return s;
+    mkdirs(path);
+    std::ofstream f(path, std::ios::app);
+    if (f) f << key << " " << best_v << " " << g << "\n";
+    if (out_variant) *out_variant = best_v;
+    if (out_graph_steps) *out_graph_steps = g;
+    return TM_OK;
+}
+
+}  // extern "C"
+
+namespace tmk {
+int default_variant(int model, int precision)
+{
+    if (model == TM_MODEL_DP) return 1;                       // 64 x 16
+    return precision == TM_F64 ? 1 : 2;                       // 32 x 16 / 32 x 32
+}
+}  // namespace tmk

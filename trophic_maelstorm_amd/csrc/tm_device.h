@@ -1,0 +1,363 @@
+// tm_device.h -- counter-based randomness, FP64 transcendentals and the
+// Dornic Poisson-Gamma samplers, shared by the FP64 and FP32 step kernels.
+//
+// Everything that can move an integer draw (Philox words, the FP64 uniforms,
+// exp/log, the Poisson sampler, the FP64 Gamma sampler) uses only IEEE basic
+// operations in a fixed order with contraction off, so the FP64 kernels draw
+// exactly what a CPU restatement of the same steps draws.
+//
+// Reference semantics (paths relative to the reference's simulations/):
+//   Dornic transition   Rietkerk_FPE/rietkerk_bjork_FKE.cpp:4157-4196
+//                       (Percolation_FD/multiSPDP.cpp:2904-2943 for DP)
+//   constants           rietkerk_bjork_FKE.cpp:3847-3849, multiSPDP.cpp:2485-2486
+// The reference samples with libstdc++ <random> on one std::mt19937_64 stream
+// per replicate (rietkerk_bjork_FKE.cpp:3896-3899); here every (seed, instance,
+// cell, step) owns an independent Philox4x32-10 stream, so results do not
+// depend on how the lattice is tiled or ordered.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TM_HD __host__ __device__ __forceinline__
+
+namespace tmk {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Key = (seed, instance id); counter =
+// (cell, step lo, step hi, block).  Initial fields use step words 0xFFFFFFFF.
+// ---------------------------------------------------------------------------
+struct Philox4 {
+    uint32_t v0, v1, v2, v3;
+};
+
+TM_HD Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                            uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return Philox4{c0, c1, c2, c3};
+}
+
+// A per-(cell, step) stream of 32-bit words, four per Philox block.  Words are
+// handed out by rotation (no dynamic register indexing).
+struct Stream {
+    uint32_t k0, k1, c0, c1, c2, blk;
+    uint32_t b0, b1, b2, b3;
+    int left;
+
+    TM_HD Stream(uint32_t seed, uint32_t inst, uint32_t cell, int64_t step)
+        : k0(seed), k1(inst), c0(cell), c1((uint32_t)(uint64_t)step),
+          c2((uint32_t)((uint64_t)step >> 32)), blk(0), b0(0), b1(0), b2(0), b3(0), left(0)
+    {
+    }
+
+    TM_HD uint32_t word()
+    {
+        if (left == 0) {
+            const Philox4 r = philox4x32_10(c0, c1, c2, blk, k0, k1);
+            blk += 1u;
+            b0 = r.v0; b1 = r.v1; b2 = r.v2; b3 = r.v3;
+            left = 4;
+        }
+        const uint32_t w = b0;
+        b0 = b1; b1 = b2; b2 = b3;
+        left -= 1;
+        return w;
+    }
+    // (w + 1/2) 2^-32: exact in double, never 0 or 1.
+    TM_HD double u64() { return ((double)word() + 0.5) * 0x1p-32; }
+    // The FP64 uniform rounded toward zero to FP32: never 0 or 1, and the same
+    // value as the FP64 path's to FP32 precision (fine resolution near 0).
+    __device__ float u32() { return __double2float_rz(u64()); }
+};
+
+// ---------------------------------------------------------------------------
+// FP64 exp / log written from the mathematics: Cody-Waite reduction by ln 2 and
+// a degree-13 Taylor polynomial; log via m = 2^-e x in [sqrt(1/2), sqrt 2) and
+// the atanh series of s = (m-1)/(m+1) to s^23.  (<= 1 and <= 2 ulp.)
+// ---------------------------------------------------------------------------
+TM_HD double bits_to_double(uint64_t b)
+{
+    union { uint64_t u; double d; } x;
+    x.u = b;
+    return x.d;
+}
+TM_HD uint64_t double_to_bits(double d)
+{
+    union { uint64_t u; double d; } x;
+    x.d = d;
+    return x.u;
+}
+
+TM_HD double dexp(double x)
+{
+#pragma clang fp contract(off)
+    if (x != x) return x;
+    if (x > 0x1.62e42fefa39efp+9) return __builtin_inf();
+    if (x < -745.13321910194122) return 0.0;
+    const double kd = __builtin_floor(x * 0x1.71547652b82fep+0 + 0.5);
+    const double r = (x - kd * 0x1.62e42fee00000p-1) - kd * 0x1.a39ef35793c76p-33;
+    double p = 0x1.6124613a86d09p-33;
+    p = p * r + 0x1.1eed8eff8d898p-29;
+    p = p * r + 0x1.ae64567f544e4p-26;
+    p = p * r + 0x1.27e4fb7789f5cp-22;
+    p = p * r + 0x1.71de3a556c734p-19;
+    p = p * r + 0x1.a01a01a01a01ap-16;
+    p = p * r + 0x1.a01a01a01a01ap-13;
+    p = p * r + 0x1.6c16c16c16c17p-10;
+    p = p * r + 0x1.1111111111111p-7;
+    p = p * r + 0x1.5555555555555p-5;
+    p = p * r + 0x1.5555555555555p-3;
+    p = p * r + 0x1.0000000000000p-1;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    int k = (int)kd;
+    if (k < -1020) { p = p * 0x1p-60; k += 60; }
+    if (k > 1023) { p = p * 2.0; k -= 1; }
+    return p * bits_to_double((uint64_t)(k + 1023) << 52);
+}
+
+TM_HD double dlog(double x)
+{
+#pragma clang fp contract(off)
+    if (x != x || x < 0.0) return __builtin_nan("");
+    if (x == 0.0) return -__builtin_inf();
+    if (x == __builtin_inf()) return x;
+    int e = 0;
+    if (x < 0x1p-1022) { x = x * 0x1p54; e = -54; }
+    const uint64_t b = double_to_bits(x);
+    e += (int)((b >> 52) & 0x7ff) - 1023;
+    double m = bits_to_double((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    if (m > 0x1.6a09e667f3bcdp+0) { m = m * 0.5; e += 1; }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double q = 0x1.642c8590b2164p-5;
+    q = q * z + 0x1.8618618618618p-5;
+    q = q * z + 0x1.af286bca1af28p-5;
+    q = q * z + 0x1.e1e1e1e1e1e1ep-5;
+    q = q * z + 0x1.1111111111111p-4;
+    q = q * z + 0x1.3b13b13b13b14p-4;
+    q = q * z + 0x1.745d1745d1746p-4;
+    q = q * z + 0x1.c71c71c71c71cp-4;
+    q = q * z + 0x1.2492492492492p-3;
+    q = q * z + 0x1.999999999999ap-3;
+    q = q * z + 0x1.5555555555555p-2;
+    const double s2 = 2.0 * s;
+    const double lm = s2 + s2 * (z * q);
+    const double ed = (double)e;
+    return ed * 0x1.62e42fee00000p-1 + (lm + ed * 0x1.a39ef35793c76p-33);
+}
+
+TM_HD double dsqrt(double x) { return __builtin_sqrt(x); }
+
+// Acklam's rational approximation of the standard normal quantile (|rel err|
+// < 1.15e-9); one uniform in, one deviate out, no rejection.
+TM_HD double norm_icdf_d(double p)
+{
+#pragma clang fp contract(off)
+    const double plow = 0.02425;
+    double q, num, den;
+    if (p < plow || p > 1.0 - plow) {
+        q = dsqrt(-2.0 * dlog(p < plow ? p : 1.0 - p));
+        num = ((((-7.784894002430293e-03 * q - 3.223964580411365e-01) * q - 2.400758277161838e+00) * q
+                - 2.549732539343734e+00) * q + 4.374664141464968e+00) * q + 2.938163982698783e+00;
+        den = (((7.784695709041462e-03 * q + 3.224671290700398e-01) * q + 2.445134137142996e+00) * q
+               + 3.754408661907416e+00) * q + 1.0;
+        const double x = num / den;
+        return p < plow ? x : -x;
+    }
+    q = p - 0.5;
+    const double r = q * q;
+    num = (((((-3.969683028665376e+01 * r + 2.209460984245205e+02) * r - 2.759285104469687e+02) * r
+             + 1.383577518672690e+02) * r - 3.066479806614716e+01) * r + 2.506628277459239e+00) * q;
+    den = ((((-5.447609879822406e+01 * r + 1.615858368580409e+02) * r - 1.556989798598866e+02) * r
+            + 6.680131188771972e+01) * r - 1.328068155288572e+01) * r + 1.0;
+    return num / den;
+}
+
+// Same approximation in FP32 for the FP32 Gamma sampler.
+__device__ __forceinline__ float norm_icdf_f(float p)
+{
+    const float plow = 0.02425f;
+    if (p < plow || p > 1.0f - plow) {
+        const float q = __builtin_sqrtf(-2.0f * __logf(p < plow ? p : 1.0f - p));
+        const float num = ((((-7.784894002430293e-03f * q - 3.223964580411365e-01f) * q - 2.400758277161838e+00f) * q
+                            - 2.549732539343734e+00f) * q + 4.374664141464968e+00f) * q + 2.938163982698783e+00f;
+        const float den = (((7.784695709041462e-03f * q + 3.224671290700398e-01f) * q + 2.445134137142996e+00f) * q
+                           + 3.754408661907416e+00f) * q + 1.0f;
+        const float x = __fdividef(num, den);
+        return p < plow ? x : -x;
+    }
+    const float q = p - 0.5f;
+    const float r = q * q;
+    const float num = (((((-3.969683028665376e+01f * r + 2.209460984245205e+02f) * r - 2.759285104469687e+02f) * r
+                         + 1.383577518672690e+02f) * r - 3.066479806614716e+01f) * r + 2.506628277459239e+00f) * q;
+    const float den = ((((-5.447609879822406e+01f * r + 1.615858368580409e+02f) * r - 1.556989798598866e+02f) * r
+                        + 6.680131188771972e+01f) * r - 1.328068155288572e+01f) * r + 1.0f;
+    return __fdividef(num, den);
+}
+
+// log(k!): exact table below 16, Stirling series (4 terms) above.
+TM_HD double logfact(int64_t k)
+{
+#pragma clang fp contract(off)
+    if (k < 16) {
+        switch (k < 0 ? 0 : (int)k) {
+        case 2: return 0x1.62e42fefa39efp-1;
+        case 3: return 0x1.cab0bfa2a2002p+0;
+        case 4: return 0x1.96ca77c922cf9p+1;
+        case 5: return 0x1.326643c4479c9p+2;
+        case 6: return 0x1.a51273acf01cap+2;
+        case 7: return 0x1.10ce1f32dcc30p+3;
+        case 8: return 0x1.5358e82fcb70dp+3;
+        case 9: return 0x1.99a8921a7f7cfp+3;
+        case 10: return 0x1.e357590954d15p+3;
+        case 11: return 0x1.180973f3a8d74p+4;
+        case 12: return 0x1.3fcba16d50143p+4;
+        case 13: return 0x1.68d5a9c3b32cep+4;
+        case 14: return 0x1.930f3df162a42p+4;
+        case 15: return 0x1.be636a63fd346p+4;
+        default: return 0.0;
+        }
+    }
+    const double n = (double)k + 1.0;
+    const double in = 1.0 / n;
+    const double in2 = in * in;
+    const double corr = in * (0x1.5555555555555p-4 - in2 * (0x1.6c16c16c16c17p-9
+                        - in2 * (0x1.a01a01a01a01ap-11 - in2 * 0x1.3813813813814p-11)));
+    return (n - 0.5) * dlog(n) - n + 0x1.d67f1c864beb5p-1 + corr;
+}
+
+// ---------------------------------------------------------------------------
+// Poisson(z) in FP64 for both storage precisions (z = lambda e^{beta dt} rho is
+// always formed in FP64).  z < 12: sequential-search inversion; z >= 12: PTRS
+// transformed rejection (Hoermann 1993).  compat: the reference's Gaussian
+// shortcut trunc(N sqrt z + z) above 100 (rietkerk_bjork_FKE.cpp:4175-4180).
+// ---------------------------------------------------------------------------
+TM_HD int64_t poisson_d(Stream& s, double z, bool compat)
+{
+#pragma clang fp contract(off)
+    if (!(z > 0.0)) return 0;
+    if (compat && z > 100.0) return (int64_t)(norm_icdf_d(s.u64()) * dsqrt(z) + z);
+    if (z < 12.0) {
+        const double u = s.u64();
+        double p = dexp(-z);
+        double F = p;
+        int64_t k = 0;
+        while (u > F && k < 200) {
+            k += 1;
+            p = p * z / (double)k;
+            F = F + p;
+        }
+        return k;
+    }
+    const double slam = dsqrt(z);
+    const double loglam = dlog(z);
+    const double b = 0.931 + 2.53 * slam;
+    const double a = -0.059 + 0.02483 * b;
+    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    const double linv = dlog(invalpha);
+    for (int it = 0; it < 64; ++it) {
+        const double U = s.u64() - 0.5;
+        const double V = s.u64();
+        const double us = 0.5 - __builtin_fabs(U);
+        const double kd = __builtin_floor((2.0 * a / us + b) * U + z + 0.43);
+        if (us >= 0.07 && V <= vr) return (int64_t)kd;
+        if (kd < 0.0 || (us < 0.013 && V > us)) continue;
+        const double lhs = dlog(V) + linv - dlog(a / (us * us) + b);
+        const double rhs = -z + kd * loglam - logfact((int64_t)kd);
+        if (lhs <= rhs) return (int64_t)kd;
+    }
+    return (int64_t)__builtin_floor(z);
+}
+
+// Gamma(shape, 1), Marsaglia & Tsang (2000) with the U^(1/shape) boost below
+// shape 1; shape <= 0 gives 0 (the Dornic shape-0 case).  FP64 version.
+TM_HD double gamma_d(Stream& s, double shape)
+{
+#pragma clang fp contract(off)
+    if (!(shape > 0.0)) return 0.0;
+    const double a = shape < 1.0 ? shape + 1.0 : shape;
+    const double d = a - 0x1.5555555555555p-2;
+    const double c = 1.0 / dsqrt(9.0 * d);
+    double g = d;
+    for (int it = 0; it < 64; ++it) {
+        const double x = norm_icdf_d(s.u64());
+        double v = 1.0 + c * x;
+        if (v <= 0.0) continue;
+        v = v * v * v;
+        const double u = s.u64();
+        const double x2 = x * x;
+        if (u < 1.0 - 0.0331 * x2 * x2) { g = d * v; break; }
+        if (dlog(u) < 0.5 * x2 + d * (1.0 - v + dlog(v))) { g = d * v; break; }
+    }
+    if (shape < 1.0) g = g * dexp(dlog(s.u64()) / shape);
+    return g;
+}
+
+// FP32 version: proposal and squeeze in FP32; the rare log-space acceptance
+// test in FP64 because d(1 - v + log v) cancels badly in FP32 at large shape.
+__device__ __forceinline__ float gamma_f(Stream& s, float shape)
+{
+    if (!(shape > 0.0f)) return 0.0f;
+    const float a = shape < 1.0f ? shape + 1.0f : shape;
+    const float d = a - 0.333333343f;
+    const float c = __frsqrt_rn(9.0f * d);
+    float g = d;
+    for (int it = 0; it < 64; ++it) {
+        const float x = norm_icdf_f(s.u32());
+        float v = 1.0f + c * x;
+        if (v <= 0.0f) continue;
+        v = v * v * v;
+        const float u = s.u32();
+        const float x2 = x * x;
+        if (u < 1.0f - 0.0331f * x2 * x2) { g = d * v; break; }
+        const double vd = (double)v;
+        if (dlog((double)u) < 0.5 * (double)x2 + (double)d * (1.0 - vd + dlog(vd))) { g = d * v; break; }
+    }
+    // U^(1/shape) amplifies FP32 rounding by 1/shape: evaluate it in FP64.
+    if (shape < 1.0f) g = (float)((double)g * dexp(dlog(s.u64()) / (double)shape));
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Dornic constants for one parameter point (rietkerk_bjork_FKE.cpp:3847-3849,
+// multiSPDP.cpp:2485-2486): beta = linear - 4 D/dx^2, e^{beta dt},
+// lambda = 2 beta / (sigma^2 (e^{beta dt} - 1)).
+// ---------------------------------------------------------------------------
+struct DornicConst {
+    double lambda;     // lambda
+    double ll;         // lambda * e^{beta dt}   (Poisson mean per unit rho)
+    double dc;         // D / dx^2
+    double sigma2_1;   // 1 / sigma^2
+};
+
+TM_HD DornicConst dornic_constants(double linear, double D, double sigma, double dt, double dx)
+{
+#pragma clang fp contract(off)
+    const double dx2 = dx * dx;
+    const double beta = linear - 4 * D / (dx2);
+    const double lexp = dexp((beta) * dt);
+    const double lambda = 2 * (beta) / (sigma * sigma * (lexp - 1.0));
+    DornicConst c;
+    c.lambda = lambda;
+    c.ll = lambda * lexp;
+    c.dc = D / (dx2);
+    c.sigma2_1 = 1 / (sigma * sigma);
+    return c;
+}
+
+}  // namespace tmk

@@ -1,0 +1,836 @@
+// tm_kernels.h -- fused step kernels for gfx950 (templated on storage type and
+// tile shape; instantiated in tm_kernels_f64.hip / tm_kernels_f32.hip).
+//
+// One workgroup owns a TX x TY tile of one instance.  It stages the tile and a
+// periodic halo of every field in LDS (halo 1 for DP, 4 for Rietkerk, whose
+// RK4 re-applies the W/O Laplacian in each of its four stages), computes the
+// whole step there, and writes each field once: the HBM traffic is one read and
+// one write of every field per step plus halo re-reads that hit in L2.  The
+// tile is a unit of work only -- every cell draws from its own Philox stream
+// keyed by its global index, so results do not depend on tile shape.
+//
+// Per-step rho(t): each workgroup writes FP64 (sum, count) partials of its
+// tile; combine_rows later folds them in tile order with compensated sums.
+//
+// Steps run in blocks (one captured hipGraph, or one eager chunk):
+//   prep_consts  -- per-(step, instance) constants of the block into a table
+//   step k       -- k = 0 .. B-1: step number = device counter + k
+//   combine_rows -- rho(t) rows of the block
+//   bump_counter -- device counter += B
+// so a graph replays with correct step numbers, and the step kernels carry no
+// atomics, fences or serial per-tile set-up.
+#pragma once
+#include "tm_device.h"
+#include "tm/tm.h"
+
+namespace tmk {
+
+struct DevInst {
+    int32_t L;
+    int32_t pidx;
+    uint32_t seed;
+    uint32_t inst_id;
+    int64_t step0;      // instance step at device counter 0
+    int64_t cell_off;   // first cell of this instance in cell-indexed arrays
+    int32_t tile0;      // first tile in the tile table
+    int32_t ntiles;
+    int32_t pad0, pad1;
+};
+
+struct DevTile {
+    int32_t inst, x0, y0, pad;
+};
+
+struct DevState {
+    int64_t counter;     // steps taken since configure
+    int64_t run_base;    // counter at the start of the current tm_run (row index base)
+    uint32_t done;       // workgroups finished in the current launch
+    uint32_t pad;
+    // First numeric fault, as the minimum over (launch counter, instance, cell);
+    // all ones while none.  Launches run in stream order, so a later launch only
+    // records when no earlier one did, and within a launch atomicMin picks the
+    // same cell the reference's sequential sweep reports first.
+    uint64_t fault_counter;
+    uint64_t fault_key;  // (instance << 40) | cell
+};
+
+struct StepArgs {
+    const DevInst* inst;
+    const DevTile* tiles;
+    const void* consts;  // [B][n_inst] DPConst / RKConstD of this block (prep_consts)
+    const void* in;      // T*: fields, instance block [nf][L*L] at cell_off * nf
+    void* out;
+    double2* partials;   // [nslots][ntiles][nf]: (FP64 sum, count)
+    DevState* state;
+    int64_t* draws;      // nullable: Poisson integer per cell of the last step
+    int32_t ntiles;
+    int32_t nslots;
+    int32_t k;           // position of this launch in its block
+    int32_t n_inst;
+    int32_t xcd;         // XCD-aware tile order
+    int32_t pad;
+};
+
+// Periodic wrap for |v| < a few L.
+__device__ __forceinline__ int wrap(int v, int L)
+{
+    while (v < 0) v += L;
+    while (v >= L) v -= L;
+    return v;
+}
+
+// blockIdx -> tile.  Workgroups are dealt round-robin to the 8 XCDs; give each
+// XCD a contiguous run of tiles so neighbouring tiles (which re-read each
+// other's halo rows) share that XCD's L2.
+__device__ __forceinline__ int tile_of(int t, int ntiles, int xcd)
+{
+    if (!xcd) return t;
+    const int x = t & 7, j = t >> 3;
+    const int q = ntiles >> 3, r = ntiles & 7;
+    return x * q + (x < r ? x : r) + j;
+}
+
+__device__ __forceinline__ void record_fault(DevState* st, int inst, int64_t cell, int64_t counter)
+{
+    if (st->fault_counter < (uint64_t)counter) return;
+    atomicMin((unsigned long long*)&st->fault_counter, (unsigned long long)counter);
+    atomicMin((unsigned long long*)&st->fault_key, ((unsigned long long)inst << 40) | (unsigned long long)cell);
+}
+
+// Fixed-order block reduction of NF (sum, count) pairs; thread 0 writes them.
+template <int NT, int NF>
+__device__ __forceinline__ void block_partials(double (&sum)[NF], double (&cnt)[NF], double2* dst,
+                                               double* red)
+{
+    constexpr int NW = NT / 64;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sum[f] += __shfl_xor(sum[f], o);
+            cnt[f] += __shfl_xor(cnt[f], o);
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            red[(2 * f) * NW + w] = sum[f];
+            red[(2 * f + 1) * NW + w] = cnt[f];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            double s = 0.0, c = 0.0;
+            for (int i = 0; i < NW; ++i) {
+                s += red[(2 * f) * NW + i];
+                c += red[(2 * f + 1) * NW + i];
+            }
+            dst[f] = make_double2(s, c);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Arithmetic policy: FP64 keeps the reference's divisions and operation order
+// (bit-exact with the CPU restatement); FP32 uses the hardware reciprocal.
+// ---------------------------------------------------------------------------
+template <typename T> struct Arith;
+template <> struct Arith<double> {
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
+};
+template <> struct Arith<float> {
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ bool finite(float x) { return __builtin_isfinite(x); }
+};
+
+// Dornic Poisson-Gamma transition of one live cell (rietkerk_bjork_FKE.cpp:4168-4196).
+__device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double cur, const DornicConst& k,
+                                              bool compat, int64_t& n)
+{
+#pragma clang fp contract(off)
+    const double mu = -1.0 + 2.0 * alpha_i * k.sigma2_1;
+    const double ziggy = k.ll * cur;
+    double gru;
+    if (ziggy == 0.0) {
+        n = 0;
+        gru = mu + 1.0;
+    } else {
+        n = poisson_d(s, ziggy, compat);
+        gru = mu + 1.0 + (double)n;
+    }
+    if (compat && gru > 100.0) return (norm_icdf_d(s.u64()) * dsqrt(gru) + gru) / k.lambda;
+    return gamma_d(s, gru) / k.lambda;
+}
+
+// FP32 storage: alpha_i, the Poisson mean, the Poisson draw and the Gamma shape
+// in FP64 (the same values the FP64 path forms from the same fields), the
+// Gamma deviate in FP32.
+__device__ __forceinline__ float dornic_cell(Stream& s, double alpha_i, float cur, const DornicConst& k,
+                                             bool compat, int64_t& n)
+{
+#pragma clang fp contract(off)
+    const double mu = -1.0 + 2.0 * alpha_i * k.sigma2_1;
+    const double ziggy = k.ll * (double)cur;
+    n = poisson_d(s, ziggy, compat);
+    const double gru = mu + 1.0 + (double)n;
+    float g;
+    if (compat && gru > 100.0) {
+        const float gf = (float)gru;
+        g = norm_icdf_f(s.u32()) * __builtin_sqrtf(gf) + gf;
+    } else {
+        g = gamma_f(s, (float)gru);
+    }
+    return (float)((double)g / k.lambda);
+}
+
+// alpha_i = (D/dx^2)(up + down + left + right) in FP64, in the reference's order
+// (rietkerk_bjork_FKE.cpp:4158-4159).
+template <typename T>
+__device__ __forceinline__ double nbr_alpha(double dc, T up, T dn, T lf, T rt)
+{
+#pragma clang fp contract(off)
+    return dc * ((double)up + (double)dn + (double)lf + (double)rt);
+}
+
+// ---------------------------------------------------------------------------
+// DP nonlinear sub-step (f = -b rho^2, multiSPDP.cpp:1662).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T dp_f(T b, T y) { return -b * y * y; }
+
+template <typename T>
+__device__ __forceinline__ T dp_nonlinear(int scheme, T b, T dt, T y)
+{
+    if (scheme == TM_DP_EXACT) return Arith<T>::div(y, (T)1 + b * y * dt);
+    if (scheme == TM_DP_RK4) {
+        const T dt2 = dt / (T)2.0, dt6 = dt / (T)6.0;
+        const T k1 = dp_f(b, y);
+        T ys = y + dt2 * k1;
+        const T k2 = dp_f(b, ys);
+        ys = y + dt2 * k2;
+        const T k3 = dp_f(b, ys);
+        ys = y + dt * k3;
+        const T k4 = dp_f(b, ys);
+        return y + dt6 * (k1 + (T)2.0 * k2 + (T)2.0 * k3 + k4);
+    }
+    return y + dt * dp_f(b, y);
+}
+
+struct DPConst {
+    DornicConst k;
+    double b, dt, eps, neg;
+    int scheme, compat;
+    int64_t step;
+};
+
+__device__ __forceinline__ DPConst dp_constants(const tm_dp_params& p, int64_t step)
+{
+#pragma clang fp contract(off)
+    double a = p.a;
+    if (p.ramp_steps > 0) {
+        const int64_t m = step < p.ramp_steps ? step : p.ramp_steps;
+        a = p.a + (p.a_end - p.a) * ((double)m / (double)p.ramp_steps);
+    }
+    DPConst c;
+    c.k = dornic_constants(a, p.D, p.sigma, p.dt, p.dx);
+    c.b = p.b;
+    c.dt = p.dt;
+    c.eps = p.flush_eps;
+    c.neg = p.flush_neg;
+    c.scheme = p.scheme;
+    c.compat = p.sampler == TM_SAMPLER_COMPAT;
+    c.step = step;
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// DP step: Dornic + nonlinear sub-step + checks + flush + rho(t) partials.
+// multiSPDP.cpp:2901-2958 and :3222-3262.
+// ---------------------------------------------------------------------------
+template <typename T, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
+{
+    constexpr int H = 1, RW = TX + 2 * H, RH = TY + 2 * H;
+    constexpr int ROWS = NT / TX;          // tile rows covered per pass
+    static_assert(NT % TX == 0 && TY % ROWS == 0, "tile shape");
+    __shared__ T s_rho[RH * RW];
+    __shared__ double s_red[2 * (NT / 64)];
+
+    const int64_t counter = a.state->counter + a.k;
+    const int64_t slot = counter % a.nslots;
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly0 = tid / TX;
+    const T* in = (const T*)a.in;
+    T* out = (T*)a.out;
+    const DPConst* consts = (const DPConst*)a.consts + (int64_t)a.k * a.n_inst;
+
+    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+        const int tile = tile_of(t, a.ntiles, a.xcd);
+        const DevTile tl = a.tiles[tile];
+        const DevInst di = a.inst[tl.inst];
+        const int L = di.L;
+        const T* fin = in + di.cell_off;
+        T* fout = out + di.cell_off;
+        for (int idx = tid; idx < RH * RW; idx += NT) {
+            const int ry = idx / RW, rx = idx - (idx / RW) * RW;
+            const int gy = wrap(tl.y0 + ry - H, L), gx = wrap(tl.x0 + rx - H, L);
+            s_rho[idx] = fin[(int64_t)gy * L + gx];
+        }
+        const DPConst c = consts[tl.inst];
+        __syncthreads();
+        const T b = (T)c.b, dt = (T)c.dt, eps = (T)c.eps, neg = (T)c.neg;
+        double sum[1] = {0.0}, cnt[1] = {0.0};
+        for (int j = 0; j < TY / ROWS; ++j) {
+            const int ly = ly0 + j * ROWS;
+            const int gy = tl.y0 + ly, gx = tl.x0 + lx;
+            if (gy >= L || gx >= L) continue;
+            const int r = (ly + 1) * RW + lx + 1;
+            const T cur = s_rho[r];
+            const double alpha_i = nbr_alpha(c.k.dc, s_rho[r - RW], s_rho[r + RW], s_rho[r - 1], s_rho[r + 1]);
+            const int64_t cell = (int64_t)gy * L + gx;
+            T y = cur;
+            int64_t n = -1;
+            if (!(alpha_i == 0.0 && cur == (T)0)) {
+                Stream st(di.seed, di.inst_id, (uint32_t)cell, c.step);
+                y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
+            }
+            y = dp_nonlinear<T>(c.scheme, b, dt, y);
+            if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
+            if (y < eps && y >= neg) y = (T)0;
+            fout[cell] = y;
+            if (a.draws) a.draws[di.cell_off + cell] = n;
+            sum[0] += (double)y;
+            cnt[0] += y > (T)0 ? 1.0 : 0.0;
+        }
+        double2* dst = a.partials + (slot * a.ntiles + tile);
+        block_partials<NT, 1>(sum, cnt, dst, s_red);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rietkerk (SPB = 1): Dornic on P, then RK4 of f_2Dor on (P*, W, O), checks,
+// flush, rho(t) partials -- rietkerk_bjork_FKE.cpp:4154-4465, :2363-2444.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RKConst {
+    DornicConst k;          // P Dornic constants (beta = -d - 4 D_P/dx^2)
+    T R, cgmax, gmax, alpha, rW, K1, K2, K2W0, dw, dO, dt, dt2, dt6, eps;
+    int compat;
+    int64_t step;
+};
+
+// FP64 constants as tabled by prep_consts; rk_cast gives the stepping type's copy.
+struct RKConstD {
+    DornicConst k;
+    double R, cgmax, gmax, alpha, rW, K1, K2, K2W0, dw, dO, dt, dt2, dt6, eps;
+    int compat, pad;
+    int64_t step;
+};
+
+__device__ __forceinline__ RKConstD rk_constants(const tm_rietkerk_params& p, int64_t step)
+{
+#pragma clang fp contract(off)
+    RKConstD c;
+    const double dx2 = p.dx * p.dx;
+    c.k = dornic_constants(-p.d, p.D[0], p.sigma, p.dt, p.dx);
+    c.R = p.R;
+    c.cgmax = p.c * p.gmax;
+    c.gmax = p.gmax;
+    c.alpha = p.alpha;
+    c.rW = p.rW;
+    c.K1 = p.K[1];
+    c.K2 = p.K[2];
+    c.K2W0 = p.K[2] * p.W0;
+    c.dw = p.D[1] / (dx2);
+    c.dO = p.D[2] / (dx2);
+    c.dt = p.dt;
+    c.dt2 = p.dt / 2.0;
+    c.dt6 = p.dt / 6.0;
+    c.eps = p.flush_eps;
+    c.compat = p.sampler == TM_SAMPLER_COMPAT;
+    c.pad = 0;
+    c.step = step;
+    return c;
+}
+
+template <typename T>
+__device__ __forceinline__ RKConst<T> rk_cast(const RKConstD& d)
+{
+    RKConst<T> c;
+    c.k = d.k;
+    c.R = (T)d.R; c.cgmax = (T)d.cgmax; c.gmax = (T)d.gmax; c.alpha = (T)d.alpha; c.rW = (T)d.rW;
+    c.K1 = (T)d.K1; c.K2 = (T)d.K2; c.K2W0 = (T)d.K2W0; c.dw = (T)d.dw; c.dO = (T)d.dO;
+    c.dt = (T)d.dt; c.dt2 = (T)d.dt2; c.dt6 = (T)d.dt6; c.eps = (T)d.eps;
+    c.compat = d.compat;
+    c.step = d.step;
+    return c;
+}
+
+// f_2Dor at one cell (rietkerk_bjork_FKE.cpp:2380-2386), with nW / nO the sums
+// up + down + left + right of the W / O neighbours.
+template <typename T>
+__device__ __forceinline__ void rk_rhs(const RKConst<T>& c, T P, T W, T O, T nW, T nO, T& fP, T& fW,
+                                       T& fO)
+{
+#pragma clang fp contract(off)
+    fP = Arith<T>::div(c.cgmax * W * P, W + c.K1);
+    const T A = Arith<T>::div(c.alpha * (P + c.K2W0), P + c.K2) * O;
+    fW = A - Arith<T>::div(c.gmax * W * P, W + c.K1) - c.rW * W + c.dw * (nW - (T)4 * W);
+    fO = c.R - A + c.dO * (nO - (T)4 * O);
+}
+
+// Ring of radius r around a TX x TY tile: index k -> tile-relative (ly, lx).
+template <int TX, int TY, int R>
+__device__ __forceinline__ void ring_cell(int k, int& ly, int& lx)
+{
+    constexpr int W = TX + 2 * R;
+    if (k < R * W) {
+        ly = -R + k / W;
+        lx = -R + k % W;
+    } else if (k < 2 * R * W) {
+        k -= R * W;
+        ly = TY + k / W;
+        lx = -R + k % W;
+    } else {
+        k -= 2 * R * W;
+        ly = k / (2 * R);
+        const int cc = k % (2 * R);
+        lx = cc < R ? cc - R : TX + (cc - R);
+    }
+}
+
+template <int TX, int TY, int R>
+struct Ring {
+    static constexpr int N = (TX + 2 * R) * (TY + 2 * R) - TX * TY;
+};
+
+template <typename T, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
+{
+    constexpr int W4 = TX + 8, H4 = TY + 8;        // radius-4 frame (loaded fields)
+    constexpr int W3 = TX + 6, H3 = TY + 6;        // radius-3 frame (P*, stage states)
+    constexpr int NOWN = TX * TY / NT;             // owned cells per thread
+    static_assert((TX * TY) % NT == 0, "tile shape");
+    constexpr int NR3 = (Ring<TX, TY, 3>::N + NT - 1) / NT;
+    constexpr int NR2 = (Ring<TX, TY, 2>::N + NT - 1) / NT;
+    constexpr int NR1 = (Ring<TX, TY, 1>::N + NT - 1) / NT;
+    __shared__ T sP0[H4 * W4], sW0[H4 * W4], sO0[H4 * W4];
+    __shared__ T sPs[H3 * W3];
+    __shared__ T sS[3][H3 * W3];
+    __shared__ double s_red[6 * (NT / 64)];
+
+    const int64_t counter = a.state->counter + a.k;
+    const int64_t slot = counter % a.nslots;
+    const int tid = threadIdx.x;
+    const T* in = (const T*)a.in;
+    T* out = (T*)a.out;
+    const RKConstD* consts = (const RKConstD*)a.consts + (int64_t)a.k * a.n_inst;
+
+    auto i4 = [](int ly, int lx) { return (ly + 4) * W4 + (lx + 4); };
+    auto i3 = [](int ly, int lx) { return (ly + 3) * W3 + (lx + 3); };
+    auto own = [&](int j, int& ly, int& lx) {
+        const int k = tid + j * NT;
+        ly = k / TX;
+        lx = k % TX;
+    };
+
+    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+        const int tile = tile_of(t, a.ntiles, a.xcd);
+        const DevTile tl = a.tiles[tile];
+        const DevInst di = a.inst[tl.inst];
+        const int L = di.L;
+        const int64_t N = (int64_t)L * L;
+        const T* fin = in + 3 * di.cell_off;
+        T* fout = out + 3 * di.cell_off;
+        for (int idx = tid; idx < H4 * W4; idx += NT) {
+            const int ry = idx / W4, rx = idx - (idx / W4) * W4;
+            const int gy = wrap(tl.y0 + ry - 4, L), gx = wrap(tl.x0 + rx - 4, L);
+            const int64_t g = (int64_t)gy * L + gx;
+            sP0[idx] = fin[g];
+            sW0[idx] = fin[N + g];
+            sO0[idx] = fin[2 * N + g];
+        }
+        const RKConst<T> c = rk_cast<T>(consts[tl.inst]);
+        __syncthreads();
+
+        // ---- stage 0: Dornic P* on the radius-3 frame (FKE:4154-4211) ----
+        auto dornic_at = [&](int ly, int lx, int64_t* n_out) {
+            const int r = i4(ly, lx);
+            const T cur = sP0[r];
+            const double alpha_i = nbr_alpha(c.k.dc, sP0[r - W4], sP0[r + W4], sP0[r - 1], sP0[r + 1]);
+            T y = cur;
+            int64_t n = -1;
+            if (!(alpha_i == 0.0 && cur == (T)0)) {
+                const int gy = wrap(tl.y0 + ly, L), gx = wrap(tl.x0 + lx, L);
+                Stream st(di.seed, di.inst_id, (uint32_t)((int64_t)gy * L + gx), c.step);
+                y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
+            }
+            *n_out = n;
+            sPs[i3(ly, lx)] = y;
+        };
+        for (int j = 0; j < NOWN; ++j) {
+            int ly, lx;
+            own(j, ly, lx);
+            int64_t n;
+            dornic_at(ly, lx, &n);
+            const int gy = tl.y0 + ly, gx = tl.x0 + lx;
+            if (a.draws && gy < L && gx < L) a.draws[di.cell_off + (int64_t)gy * L + gx] = n;
+        }
+        for (int j = 0; j < NR3; ++j) {
+            const int k = tid + j * NT;
+            if (k < Ring<TX, TY, 3>::N) {
+                int ly, lx;
+                ring_cell<TX, TY, 3>(k, ly, lx);
+                int64_t n;
+                dornic_at(ly, lx, &n);
+            }
+        }
+        __syncthreads();
+
+        // ---- stage 1: K1 from (P*, W, O) on the radius-3 frame ----
+        T accP[NOWN], accW[NOWN], accO[NOWN];
+        auto stage1 = [&](int ly, int lx, T& kP, T& kW, T& kO) {
+            const int r4 = i4(ly, lx);
+            const T P = sPs[i3(ly, lx)], W = sW0[r4], O = sO0[r4];
+            const T nW = sW0[r4 - W4] + sW0[r4 + W4] + sW0[r4 - 1] + sW0[r4 + 1];
+            const T nO = sO0[r4 - W4] + sO0[r4 + W4] + sO0[r4 - 1] + sO0[r4 + 1];
+            rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
+            const int r3 = i3(ly, lx);
+            sS[0][r3] = P + c.dt2 * kP;
+            sS[1][r3] = W + c.dt2 * kW;
+            sS[2][r3] = O + c.dt2 * kO;
+        };
+#pragma unroll
+        for (int j = 0; j < NOWN; ++j) {
+            int ly, lx;
+            own(j, ly, lx);
+            stage1(ly, lx, accP[j], accW[j], accO[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NR3; ++j) {
+            const int k = tid + j * NT;
+            if (k < Ring<TX, TY, 3>::N) {
+                int ly, lx;
+                ring_cell<TX, TY, 3>(k, ly, lx);
+                T kP, kW, kO;
+                stage1(ly, lx, kP, kW, kO);
+            }
+        }
+        __syncthreads();
+
+        // ---- stages 2 and 3: K from the stage state in sS, new state held in
+        //      registers until every thread has read the old one ----
+        auto kstage = [&](int ly, int lx, T& kP, T& kW, T& kO) {
+            const int r = i3(ly, lx);
+            const T P = sS[0][r], W = sS[1][r], O = sS[2][r];
+            const T nW = sS[1][r - W3] + sS[1][r + W3] + sS[1][r - 1] + sS[1][r + 1];
+            const T nO = sS[2][r - W3] + sS[2][r + W3] + sS[2][r - 1] + sS[2][r + 1];
+            rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
+        };
+        auto ybase = [&](int ly, int lx, T& P, T& W, T& O) {
+            const int r4 = i4(ly, lx);
+            P = sPs[i3(ly, lx)];
+            W = sW0[r4];
+            O = sO0[r4];
+        };
+        {   // stage 2: K2 on radius 2, ys3 = y + dt/2 K2
+            T nP[NOWN], nW[NOWN], nO[NOWN];
+            T rP[NR2], rW[NR2], rO[NR2];
+#pragma unroll
+            for (int j = 0; j < NOWN; ++j) {
+                int ly, lx;
+                own(j, ly, lx);
+                T kP, kW, kO, P, W, O;
+                kstage(ly, lx, kP, kW, kO);
+                ybase(ly, lx, P, W, O);
+                accP[j] = accP[j] + (T)2.0 * kP;
+                accW[j] = accW[j] + (T)2.0 * kW;
+                accO[j] = accO[j] + (T)2.0 * kO;
+                nP[j] = P + c.dt2 * kP;
+                nW[j] = W + c.dt2 * kW;
+                nO[j] = O + c.dt2 * kO;
+            }
+#pragma unroll
+            for (int j = 0; j < NR2; ++j) {
+                const int k = tid + j * NT;
+                if (k < Ring<TX, TY, 2>::N) {
+                    int ly, lx;
+                    ring_cell<TX, TY, 2>(k, ly, lx);
+                    T kP, kW, kO, P, W, O;
+                    kstage(ly, lx, kP, kW, kO);
+                    ybase(ly, lx, P, W, O);
+                    rP[j] = P + c.dt2 * kP;
+                    rW[j] = W + c.dt2 * kW;
+                    rO[j] = O + c.dt2 * kO;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < NOWN; ++j) {
+                int ly, lx;
+                own(j, ly, lx);
+                const int r = i3(ly, lx);
+                sS[0][r] = nP[j]; sS[1][r] = nW[j]; sS[2][r] = nO[j];
+            }
+#pragma unroll
+            for (int j = 0; j < NR2; ++j) {
+                const int k = tid + j * NT;
+                if (k < Ring<TX, TY, 2>::N) {
+                    int ly, lx;
+                    ring_cell<TX, TY, 2>(k, ly, lx);
+                    const int r = i3(ly, lx);
+                    sS[0][r] = rP[j]; sS[1][r] = rW[j]; sS[2][r] = rO[j];
+                }
+            }
+            __syncthreads();
+        }
+        {   // stage 3: K3 on radius 1, ys4 = y + dt K3
+            T nP[NOWN], nW[NOWN], nO[NOWN];
+            T rP[NR1], rW[NR1], rO[NR1];
+#pragma unroll
+            for (int j = 0; j < NOWN; ++j) {
+                int ly, lx;
+                own(j, ly, lx);
+                T kP, kW, kO, P, W, O;
+                kstage(ly, lx, kP, kW, kO);
+                ybase(ly, lx, P, W, O);
+                accP[j] = accP[j] + (T)2.0 * kP;
+                accW[j] = accW[j] + (T)2.0 * kW;
+                accO[j] = accO[j] + (T)2.0 * kO;
+                nP[j] = P + c.dt * kP;
+                nW[j] = W + c.dt * kW;
+                nO[j] = O + c.dt * kO;
+            }
+#pragma unroll
+            for (int j = 0; j < NR1; ++j) {
+                const int k = tid + j * NT;
+                if (k < Ring<TX, TY, 1>::N) {
+                    int ly, lx;
+                    ring_cell<TX, TY, 1>(k, ly, lx);
+                    T kP, kW, kO, P, W, O;
+                    kstage(ly, lx, kP, kW, kO);
+                    ybase(ly, lx, P, W, O);
+                    rP[j] = P + c.dt * kP;
+                    rW[j] = W + c.dt * kW;
+                    rO[j] = O + c.dt * kO;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < NOWN; ++j) {
+                int ly, lx;
+                own(j, ly, lx);
+                const int r = i3(ly, lx);
+                sS[0][r] = nP[j]; sS[1][r] = nW[j]; sS[2][r] = nO[j];
+            }
+#pragma unroll
+            for (int j = 0; j < NR1; ++j) {
+                const int k = tid + j * NT;
+                if (k < Ring<TX, TY, 1>::N) {
+                    int ly, lx;
+                    ring_cell<TX, TY, 1>(k, ly, lx);
+                    const int r = i3(ly, lx);
+                    sS[0][r] = rP[j]; sS[1][r] = rW[j]; sS[2][r] = rO[j];
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- stage 4: K4 on the tile, combine, checks, flush, write ----
+        double sum[3] = {0.0, 0.0, 0.0}, cnt[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < NOWN; ++j) {
+            int ly, lx;
+            own(j, ly, lx);
+            T kP, kW, kO, P, W, O;
+            kstage(ly, lx, kP, kW, kO);
+            ybase(ly, lx, P, W, O);
+            T y[3];
+            y[0] = P + c.dt6 * (accP[j] + kP);
+            y[1] = W + c.dt6 * (accW[j] + kW);
+            y[2] = O + c.dt6 * (accO[j] + kO);
+            const int gy = tl.y0 + ly, gx = tl.x0 + lx;
+            if (gy < L && gx < L) {
+                const int64_t cell = (int64_t)gy * L + gx;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    T v = y[f];
+                    if (v < (T)0 || !Arith<T>::finite(v)) record_fault(a.state, tl.inst, f * N + cell, counter);
+                    if (v < c.eps && v >= (T)0) v = (T)0;
+                    fout[f * N + cell] = v;
+                    sum[f] += (double)v;
+                    cnt[f] += v > (T)0 ? 1.0 : 0.0;
+                }
+            }
+        }
+        double2* dst = a.partials + (slot * a.ntiles + tile) * 3;
+        block_partials<NT, 3>(sum, cnt, dst, s_red);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block prologue and epilogue.
+// prep_consts: table[k][i] = constants of instance i at step counter + k.
+// bump_counter: counter += n (one thread; runs after the block's rows fold).
+// ---------------------------------------------------------------------------
+template <int MODEL>
+__global__ __launch_bounds__(256) void prep_consts_kernel(const DevInst* inst, const void* params,
+                                                          const DevState* state, int n_inst, int nsteps,
+                                                          void* table)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)n_inst * nsteps) return;
+    const int k = (int)(idx / n_inst), i = (int)(idx % n_inst);
+    const DevInst di = inst[i];
+    const int64_t step = di.step0 + state->counter + k;
+    if (MODEL == TM_MODEL_DP)
+        ((DPConst*)table)[idx] = dp_constants(((const tm_dp_params*)params)[di.pidx], step);
+    else
+        ((RKConstD*)table)[idx] = rk_constants(((const tm_rietkerk_params*)params)[di.pidx], step);
+}
+
+#ifdef TM_DEFINE_COMBINE   // defined by exactly one translation unit
+__global__ void bump_counter_kernel(DevState* state, int n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) state->counter += n;
+}
+#endif
+
+// ---------------------------------------------------------------------------
+// Partials of the current fields only (tm_measure).
+// ---------------------------------------------------------------------------
+template <typename T, int NF>
+__global__ __launch_bounds__(256) void measure_kernel(const DevInst* inst, const DevTile* tiles, int ntiles,
+                                                      int tx, int ty, const T* fields, double2* partials)
+{
+    __shared__ double s_red[2 * NF * 4];
+    const int tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const DevTile tl = tiles[tile];
+    const DevInst di = inst[tl.inst];
+    const int L = di.L;
+    const int64_t N = (int64_t)L * L;
+    const T* f0 = fields + NF * di.cell_off;
+    double sum[NF], cnt[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) { sum[f] = 0.0; cnt[f] = 0.0; }
+    for (int k = threadIdx.x; k < tx * ty; k += 256) {
+        const int gy = tl.y0 + k / tx, gx = tl.x0 + k % tx;
+        if (gy >= L || gx >= L) continue;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const T v = f0[f * N + (int64_t)gy * L + gx];
+            sum[f] += (double)v;
+            cnt[f] += v > (T)0 ? 1.0 : 0.0;
+        }
+    }
+    block_partials<256, NF>(sum, cnt, partials + (int64_t)tile * NF, s_red);
+}
+
+// ---------------------------------------------------------------------------
+// Fold tile partials into rho(t) rows: one workgroup per (instance, step).
+// Per-thread Kahan sums over a fixed tile stride, then a fixed pairwise tree
+// of compensated (TwoSum) additions -- deterministic run to run.
+// rows: [n_inst][rows_cap][2 nf]; row = step counter - run_base.
+// ---------------------------------------------------------------------------
+struct CombineArgs {
+    const DevInst* inst;
+    const double2* partials;
+    const DevState* state;
+    double* rows;
+    int32_t ntiles, nslots, nf, nsteps;  // nsteps: steps folded by this launch
+    int64_t rows_cap;
+    int32_t measure;                     // 1: slot 0, row 0 (tm_measure)
+    int32_t pad;
+};
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+#pragma clang fp contract(off)
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+#ifdef TM_DEFINE_COMBINE   // defined by exactly one translation unit
+__global__ __launch_bounds__(256) void combine_rows_kernel(CombineArgs a)
+{
+#pragma clang fp contract(off)
+    __shared__ double s_s[3][256], s_e[3][256], s_n[3][256];
+    const int i = blockIdx.x, j = blockIdx.y;
+    const DevInst di = a.inst[i];
+    int64_t slot = 0, row = 0;
+    if (!a.measure) {
+        const int64_t c = a.state->counter + j;   // step counter of this row (before the bump)
+        slot = c % a.nslots;
+        row = c - a.state->run_base;
+    }
+    const double2* p = a.partials + (slot * a.ntiles + di.tile0) * a.nf;
+    const int tid = threadIdx.x;
+    for (int f = 0; f < a.nf; ++f) {
+        double s = 0.0, e = 0.0, n = 0.0;
+        for (int t = tid; t < di.ntiles; t += 256) {
+            const double2 v = p[(int64_t)t * a.nf + f];
+            const double y = v.x - e;     // Kahan
+            const double tt = s + y;
+            e = (tt - s) - y;
+            s = tt;
+            n += v.y;
+        }
+        s_s[f][tid] = s;
+        s_e[f][tid] = -e;
+        s_n[f][tid] = n;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            for (int f = 0; f < a.nf; ++f) {
+                double s, e;
+                two_sum(s_s[f][tid], s_s[f][tid + w], s, e);
+                s_s[f][tid] = s;
+                s_e[f][tid] = s_e[f][tid] + s_e[f][tid + w] + e;
+                s_n[f][tid] += s_n[f][tid + w];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double cells = (double)di.L * (double)di.L;
+        double* out = a.rows + ((int64_t)i * a.rows_cap + row) * (2 * a.nf);
+        for (int f = 0; f < a.nf; ++f) {
+            out[2 * f] = s_n[f][0];
+            out[2 * f + 1] = (s_s[f][0] + s_e[f][0]) / cells;
+        }
+    }
+}
+#endif
+
+// ---------------------------------------------------------------------------
+// i.i.d. Uniform(lo, hi) initial fields, one thread per cell of one instance.
+// ---------------------------------------------------------------------------
+template <typename T, int NF>
+__global__ __launch_bounds__(256) void init_uniform_kernel(int L, uint32_t seed, uint32_t inst_id, double lo0,
+                                                           double hi0, double lo1, double hi1, double lo2,
+                                                           double hi2, T* fields)
+{
+#pragma clang fp contract(off)
+    const int64_t N = (int64_t)L * L;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const Philox4 w = philox4x32_10((uint32_t)i, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, seed, inst_id);
+    const uint32_t ws[3] = {w.v0, w.v1, w.v2};
+    const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const double u = ((double)ws[f] + 0.5) * 0x1p-32;
+        fields[f * N + i] = (T)(lo[f] + (hi[f] - lo[f]) * u);
+    }
+}
+
+}  // namespace tmk

@@ -1,0 +1,63 @@
+// tm_kernels_f32.hip -- FP32-storage instantiations of the step kernels.
+// Fields, stencil and RK4 stages are FP32 (contraction allowed, hardware
+// reciprocal); the Poisson mean and draw, the ramp and the rho(t) sums stay FP64
+// (their helpers pin contraction off locally).
+#include "tm_launch.h"
+
+namespace tmk {
+
+template <typename T, int TX, int TY, int NT>
+static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+template <typename T, int TX, int TY, int NT>
+static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((rk_step_kernel<T, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+static const Variant kDP[] = {
+    {64, 8, 256, "dp_f32_64x8_t256", &launch_dp<float, 64, 8, 256>},
+    {64, 16, 256, "dp_f32_64x16_t256", &launch_dp<float, 64, 16, 256>},
+    {64, 32, 256, "dp_f32_64x32_t256", &launch_dp<float, 64, 32, 256>},
+    {32, 8, 256, "dp_f32_32x8_t256", &launch_dp<float, 32, 8, 256>},
+    {128, 16, 256, "dp_f32_128x16_t256", &launch_dp<float, 128, 16, 256>},
+};
+
+static const Variant kRK[] = {
+    {16, 16, 256, "rk_f32_16x16_t256", &launch_rk<float, 16, 16, 256>},
+    {32, 16, 256, "rk_f32_32x16_t256", &launch_rk<float, 32, 16, 256>},
+    {32, 32, 256, "rk_f32_32x32_t256", &launch_rk<float, 32, 32, 256>},
+    {64, 32, 256, "rk_f32_64x32_t256", &launch_rk<float, 64, 32, 256>},
+};
+
+const Variant* dp_variants_f32(int* n) { *n = (int)(sizeof(kDP) / sizeof(kDP[0])); return kDP; }
+const Variant* rk_variants_f32(int* n) { *n = (int)(sizeof(kRK) / sizeof(kRK[0])); return kRK; }
+
+void launch_measure_f32(int nf, const DevInst* inst, const DevTile* tiles, int ntiles, int tx, int ty,
+                        const void* fields, double2* partials, hipStream_t s)
+{
+    if (nf == 1)
+        hipLaunchKernelGGL((measure_kernel<float, 1>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
+                           ty, (const float*)fields, partials);
+    else
+        hipLaunchKernelGGL((measure_kernel<float, 3>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
+                           ty, (const float*)fields, partials);
+}
+
+void launch_init_f32(int nf, int L, uint32_t seed, uint32_t inst_id, const double* lo, const double* hi,
+                     void* fields, hipStream_t s)
+{
+    const int64_t N = (int64_t)L * L;
+    const int grid = (int)((N + 255) / 256);
+    if (nf == 1)
+        hipLaunchKernelGGL((init_uniform_kernel<float, 1>), dim3(grid), dim3(256), 0, s, L, seed, inst_id, lo[0],
+                           hi[0], 0.0, 0.0, 0.0, 0.0, (float*)fields);
+    else
+        hipLaunchKernelGGL((init_uniform_kernel<float, 3>), dim3(grid), dim3(256), 0, s, L, seed, inst_id, lo[0],
+                           hi[0], lo[1], hi[1], lo[2], hi[2], (float*)fields);
+}
+
+}  // namespace tmk

@@ -1,0 +1,87 @@
+// tm_kernels_f64.hip -- FP64 instantiations of the step kernels, plus the
+// precision-independent rho(t) combine.  Compiled with -ffp-contract=off so
+// every FP64 operation rounds exactly as the CPU restatement's does.
+#define TM_DEFINE_COMBINE
+#include "tm_launch.h"
+
+namespace tmk {
+
+template <typename T, int TX, int TY, int NT>
+static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+template <typename T, int TX, int TY, int NT>
+static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((rk_step_kernel<T, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+static const Variant kDP[] = {
+    {64, 8, 256, "dp_f64_64x8_t256", &launch_dp<double, 64, 8, 256>},
+    {64, 16, 256, "dp_f64_64x16_t256", &launch_dp<double, 64, 16, 256>},
+    {64, 32, 256, "dp_f64_64x32_t256", &launch_dp<double, 64, 32, 256>},
+    {32, 8, 256, "dp_f64_32x8_t256", &launch_dp<double, 32, 8, 256>},
+    {128, 16, 256, "dp_f64_128x16_t256", &launch_dp<double, 128, 16, 256>},
+};
+
+static const Variant kRK[] = {
+    {16, 16, 256, "rk_f64_16x16_t256", &launch_rk<double, 16, 16, 256>},
+    {32, 16, 256, "rk_f64_32x16_t256", &launch_rk<double, 32, 16, 256>},
+    {32, 32, 256, "rk_f64_32x32_t256", &launch_rk<double, 32, 32, 256>},
+};
+
+const Variant* dp_variants_f64(int* n) { *n = (int)(sizeof(kDP) / sizeof(kDP[0])); return kDP; }
+const Variant* rk_variants_f64(int* n) { *n = (int)(sizeof(kRK) / sizeof(kRK[0])); return kRK; }
+
+void launch_measure_f64(int nf, const DevInst* inst, const DevTile* tiles, int ntiles, int tx, int ty,
+                        const void* fields, double2* partials, hipStream_t s)
+{
+    if (nf == 1)
+        hipLaunchKernelGGL((measure_kernel<double, 1>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
+                           ty, (const double*)fields, partials);
+    else
+        hipLaunchKernelGGL((measure_kernel<double, 3>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
+                           ty, (const double*)fields, partials);
+}
+
+void launch_init_f64(int nf, int L, uint32_t seed, uint32_t inst_id, const double* lo, const double* hi,
+                     void* fields, hipStream_t s)
+{
+    const int64_t N = (int64_t)L * L;
+    const int grid = (int)((N + 255) / 256);
+    if (nf == 1)
+        hipLaunchKernelGGL((init_uniform_kernel<double, 1>), dim3(grid), dim3(256), 0, s, L, seed, inst_id, lo[0],
+                           hi[0], 0.0, 0.0, 0.0, 0.0, (double*)fields);
+    else
+        hipLaunchKernelGGL((init_uniform_kernel<double, 3>), dim3(grid), dim3(256), 0, s, L, seed, inst_id, lo[0],
+                           hi[0], lo[1], hi[1], lo[2], hi[2], (double*)fields);
+}
+
+void launch_combine(const CombineArgs& a, int n_inst, int nsteps, hipStream_t s)
+{
+    hipLaunchKernelGGL(combine_rows_kernel, dim3(n_inst, nsteps), dim3(256), 0, s, a);
+}
+
+void launch_prep(int model, const DevInst* inst, const void* params, const DevState* state, int n_inst,
+                 int nsteps, void* table, hipStream_t s)
+{
+    const int64_t n = (int64_t)n_inst * nsteps;
+    const int grid = (int)((n + 255) / 256);
+    if (model == TM_MODEL_DP)
+        hipLaunchKernelGGL(prep_consts_kernel<TM_MODEL_DP>, dim3(grid), dim3(256), 0, s, inst, params, state,
+                           n_inst, nsteps, table);
+    else
+        hipLaunchKernelGGL(prep_consts_kernel<TM_MODEL_RIETKERK>, dim3(grid), dim3(256), 0, s, inst, params,
+                           state, n_inst, nsteps, table);
+}
+
+void launch_bump(DevState* state, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(64), 0, s, state, n);
+}
+
+size_t const_size(int model) { return model == TM_MODEL_DP ? sizeof(DPConst) : sizeof(RKConstD); }
+
+}  // namespace tmk

@@ -1,0 +1,41 @@
+// tm_launch.h -- host-side launch table for the step kernels.  Each kernel
+// variant is one (tile width, tile height, threads) instantiation; the host
+// context picks one by default or through the autotuner.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "tm_kernels.h"
+
+namespace tmk {
+
+typedef void (*StepLaunch)(const StepArgs&, int grid, hipStream_t);
+
+struct Variant {
+    int tx, ty, nt;
+    const char* name;
+    StepLaunch fn;
+};
+
+// Variant tables per (model, precision); defined in tm_kernels_f64.hip / _f32.hip.
+const Variant* dp_variants_f64(int* n);
+const Variant* rk_variants_f64(int* n);
+const Variant* dp_variants_f32(int* n);
+const Variant* rk_variants_f32(int* n);
+
+// Default variant index per (model, precision).
+int default_variant(int model, int precision);
+
+void launch_measure_f64(int nf, const DevInst* inst, const DevTile* tiles, int ntiles, int tx, int ty,
+                        const void* fields, double2* partials, hipStream_t s);
+void launch_measure_f32(int nf, const DevInst* inst, const DevTile* tiles, int ntiles, int tx, int ty,
+                        const void* fields, double2* partials, hipStream_t s);
+void launch_init_f64(int nf, int L, uint32_t seed, uint32_t inst_id, const double* lo, const double* hi,
+                     void* fields, hipStream_t s);
+void launch_init_f32(int nf, int L, uint32_t seed, uint32_t inst_id, const double* lo, const double* hi,
+                     void* fields, hipStream_t s);
+void launch_combine(const CombineArgs& a, int n_inst, int nsteps, hipStream_t s);
+void launch_prep(int model, const DevInst* inst, const void* params, const DevState* state, int n_inst,
+                 int nsteps, void* table, hipStream_t s);
+void launch_bump(DevState* state, int n, hipStream_t s);
+size_t const_size(int model);
+
+}  // namespace tmk

@@ -81,8 +81,24 @@ def test_fp32_one_step_tracks_fp64(tm, model):
     # that falls within rounding of its threshold gives that cell a different
     # (equally valid) deviate, which must stay rare.
     assert np.median(rel) < 2e-7, np.median(rel)
-    assert np.quantile(rel, 0.99) < 2e-5, np.quantile(rel, 0.99)
+    # the U^(1/shape) boost of small shapes multiplies the FP32 error of log2(U) by 1/shape
+    assert np.quantile(rel, 0.99) < 1e-4, np.quantile(rel, 0.99)
     assert np.mean(rel > 1e-2) < 1e-3, np.mean(rel > 1e-2)
+
+
+def ensemble_checks(r32, r64, field=0, floor=0.1, rel_tol=1e-3):
+    """FP32 vs FP64 ensemble rho(t) on the same seeds: within rel_tol while the
+    ensemble mean is at least `floor` of its initial value, and no bias over the
+    whole run (paired per-instance differences within 4 standard errors)."""
+    c = 2 * field + 1
+    m32, m64 = r32[:, :, c].mean(0), r64[:, :, c].mean(0)
+    live = m64 >= floor * m64[0]
+    rel = np.abs(m32 - m64)[live] / m64[live]
+    assert rel.max() < rel_tol, rel.max()
+    d = r32[:, :, c] - r64[:, :, c]
+    se = d.std(0, ddof=1) / np.sqrt(d.shape[0])
+    z = np.abs(d.mean(0)) / np.maximum(se, 1e-300)
+    assert np.all((z < 4.0) | (np.abs(d.mean(0)) <= 1e-6 * np.abs(m64[0]))), z.max()
 
 
 def test_fp32_dp_ensemble_matches_fp64(tm):
@@ -90,9 +106,7 @@ def test_fp32_dp_ensemble_matches_fp64(tm):
     p = tm.DPParams(a=0.72, a_end=0.70, ramp_steps=400)
     r32 = run_batch(tm, tm.MODEL_DP, tm.F32, p, insts, [0.0], [1.0], 400)
     r64 = run_batch(tm, tm.MODEL_DP, tm.F64, p, insts, [0.0], [1.0], 400)
-    m32, m64 = r32[:, :, 1].mean(0), r64[:, :, 1].mean(0)
-    rel = np.abs(m32 - m64) / m64
-    assert rel.max() < 1e-3, rel.max()
+    ensemble_checks(r32, r64)
 
 
 def test_fp32_rietkerk_ensemble_matches_fp64(tm):
@@ -101,6 +115,4 @@ def test_fp32_rietkerk_ensemble_matches_fp64(tm):
     r32 = run_batch(tm, tm.MODEL_RIETKERK, tm.F32, ps, insts, [0, 0, 0], [90, 1, 1], 200)
     r64 = run_batch(tm, tm.MODEL_RIETKERK, tm.F64, ps, insts, [0, 0, 0], [90, 1, 1], 200)
     for f in range(3):
-        m32, m64 = r32[:, :, 2 * f + 1].mean(0), r64[:, :, 2 * f + 1].mean(0)
-        rel = np.abs(m32 - m64) / np.abs(m64)
-        assert rel.max() < 1e-3, (f, rel.max())
+        ensemble_checks(r32, r64, field=f)

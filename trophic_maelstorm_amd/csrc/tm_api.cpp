@@ -155,8 +155,9 @@ tm_status build_tiles(tm_ctx* c)
         di.step0 = d.step0;
         di.cell_off = c->cell_off[i];
         di.tile0 = (int32_t)c->tiles.size();
-        for (int y0 = 0; y0 < d.L; y0 += v.ty)
-            for (int x0 = 0; x0 < d.L; x0 += v.tx) c->tiles.push_back(DevTile{(int32_t)i, x0, y0, 0});
+        // vertical strips: consecutive tiles share a halo row
+        for (int x0 = 0; x0 < d.L; x0 += v.tx)
+            for (int y0 = 0; y0 < d.L; y0 += v.ty) c->tiles.push_back(DevTile{(int32_t)i, x0, y0, 0});
         di.ntiles = (int32_t)c->tiles.size() - di.tile0;
         c->dinst.push_back(di);
     }
@@ -167,7 +168,8 @@ tm_status build_tiles(tm_ctx* c)
     if (!c->d_inst) HIPCHK(c, hipMalloc((void**)&c->d_inst, sizeof(DevInst) * c->dinst.size()));
     HIPCHK(c, hipMemcpy(c->d_inst, c->dinst.data(), sizeof(DevInst) * c->dinst.size(), hipMemcpyHostToDevice));
     c->nslots = std::max(c->graph_steps, 64);
-    HIPCHK(c, hipMalloc((void**)&c->d_partials, sizeof(double2) * (size_t)c->nslots * c->tiles.size() * c->nf));
+    HIPCHK(c, hipMalloc((void**)&c->d_partials,
+                        sizeof(double2) * (size_t)c->nslots * c->tiles.size() * v.pw * c->nf));
     free_dev(c->d_consts);
     HIPCHK(c, hipMalloc(&c->d_consts, const_size(c->model) * (size_t)c->nslots * c->inst.size()));
     return TM_OK;
@@ -192,7 +194,15 @@ StepArgs step_args(tm_ctx* c, int parity, int k)
     return a;
 }
 
-int grid_of(tm_ctx* c) { return (int)c->tiles.size(); }
+// Persistent variants launch wg_per_cu workgroups per CU, each walking a
+// contiguous tile range; the others launch one workgroup per tile.
+int grid_of(tm_ctx* c)
+{
+    const Variant& v = cur_variant(c);
+    const int n = (int)c->tiles.size();
+    if (v.wg_per_cu <= 0) return n;
+    return std::max(1, std::min(n, c->n_cu * v.wg_per_cu));
+}
 
 void launch_step(tm_ctx* c, int parity, int k)
 {
@@ -213,6 +223,7 @@ void launch_fold(tm_ctx* c, int nsteps)
     a.nsteps = nsteps;
     a.rows_cap = c->rows_cap;
     a.measure = 0;
+    a.pw = cur_variant(c).pw;
     launch_combine(a, (int)c->inst.size(), nsteps, c->stream);
 }
 
@@ -565,6 +576,7 @@ tm_status tm_measure(tm_ctx* c, double* rows)
     a.nsteps = 1;
     a.rows_cap = 1;
     a.measure = 1;
+    a.pw = 1;
     launch_combine(a, (int)c->inst.size(), 1, c->stream);
     hipError_t e = hipMemcpyAsync(rows, d_out, bytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -765,7 +777,7 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
 namespace tmk {
 int default_variant(int model, int precision)
 {
-    if (model == TM_MODEL_DP) return 1;                       // 64 x 16
+    if (model == TM_MODEL_DP) return 1;                       // 128 x 32, one tile per workgroup
     return precision == TM_F64 ? 1 : 2;                       // 32 x 16 / 32 x 32
 }
 }  // namespace tmk

@@ -77,9 +77,11 @@ struct Stream {
     }
     // (w + 1/2) 2^-32: exact in double, never 0 or 1.
     TM_HD double u64() { return ((double)word() + 0.5) * 0x1p-32; }
-    // The FP64 uniform rounded toward zero to FP32: never 0 or 1, and the same
-    // value as the FP64 path's to FP32 precision (fine resolution near 0).
-    __device__ float u32() { return __double2float_rz(u64()); }
+    // FP32 uniform ((w >> 9) + 1/2) 2^-23: exact, never 0 or 1.
+    __device__ float u32() { return ((float)(word() >> 9) + 0.5f) * 0x1p-23f; }
+    // 1 - u from the same kind of word with full FP32 resolution near 0:
+    // ((2^32 - 1 - w) + 1/2) 2^-32, in (0, 1] (1 only through rounding at the top).
+    __device__ float v32() { return ((float)(~word()) + 0.5f) * 0x1p-32f; }
 };
 
 // ---------------------------------------------------------------------------
@@ -308,31 +310,6 @@ TM_HD double gamma_d(Stream& s, double shape)
     return g;
 }
 
-// FP32 version: proposal and squeeze in FP32; the rare log-space acceptance
-// test in FP64 because d(1 - v + log v) cancels badly in FP32 at large shape.
-__device__ __forceinline__ float gamma_f(Stream& s, float shape)
-{
-    if (!(shape > 0.0f)) return 0.0f;
-    const float a = shape < 1.0f ? shape + 1.0f : shape;
-    const float d = a - 0.333333343f;
-    const float c = __frsqrt_rn(9.0f * d);
-    float g = d;
-    for (int it = 0; it < 64; ++it) {
-        const float x = norm_icdf_f(s.u32());
-        float v = 1.0f + c * x;
-        if (v <= 0.0f) continue;
-        v = v * v * v;
-        const float u = s.u32();
-        const float x2 = x * x;
-        if (u < 1.0f - 0.0331f * x2 * x2) { g = d * v; break; }
-        const double vd = (double)v;
-        if (dlog((double)u) < 0.5 * (double)x2 + (double)d * (1.0 - vd + dlog(vd))) { g = d * v; break; }
-    }
-    // U^(1/shape) amplifies FP32 rounding by 1/shape: evaluate it in FP64.
-    if (shape < 1.0f) g = (float)((double)g * dexp(dlog(s.u64()) / (double)shape));
-    return g;
-}
-
 // ---------------------------------------------------------------------------
 // Dornic constants for one parameter point (rietkerk_bjork_FKE.cpp:3847-3849,
 // multiSPDP.cpp:2485-2486): beta = linear - 4 D/dx^2, e^{beta dt},
@@ -343,6 +320,8 @@ struct DornicConst {
     double ll;         // lambda * e^{beta dt}   (Poisson mean per unit rho)
     double dc;         // D / dx^2
     double sigma2_1;   // 1 / sigma^2
+    double inv_lambda; // 1 / lambda
+    float inv_lambda_hi, inv_lambda_lo;   // 1 / lambda as an FP32 pair (the FP32 path)
 };
 
 TM_HD DornicConst dornic_constants(double linear, double D, double sigma, double dt, double dx)
@@ -355,6 +334,9 @@ TM_HD DornicConst dornic_constants(double linear, double D, double sigma, double
     DornicConst c;
     c.lambda = lambda;
     c.ll = lambda * lexp;
+    c.inv_lambda = 1.0 / lambda;
+    c.inv_lambda_hi = (float)c.inv_lambda;
+    c.inv_lambda_lo = (float)(c.inv_lambda - (double)c.inv_lambda_hi);
     c.dc = D / (dx2);
     c.sigma2_1 = 1 / (sigma * sigma);
     return c;

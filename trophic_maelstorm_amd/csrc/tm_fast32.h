@@ -1,0 +1,170 @@
+// tm_fast32.h -- FP32 Poisson and Gamma samplers for the FP32-storage step.
+//
+// Same methods as the FP64 samplers in tm_device.h (inversion below 12, PTRS
+// above; Marsaglia-Tsang Gamma), same Philox words in the same order, but
+// evaluated in FP32 with the hardware transcendentals.  Where FP32 would lose
+// the answer the formulation changes, not the precision:
+//   * inversion runs on the upper tail S_k = P(N > k) against v = 1 - u, with
+//     S_0 = -expm1(-z) from its series below 1/2, so P(N >= 1) = z stays
+//     resolved as z -> 0 (the near-absorbing regime);
+//   * PTRS forms k with the large mean added in FP64, and its log-pmf as
+//     -stirlerr(k) - bd0(k, z) - log(2 pi k)/2 (Loader 2000), whose bd0 series
+//     has no cancellation at k ~ z ~ 10^5;
+//   * Marsaglia-Tsang's log test uses the series of d(1 - v + log v) + x^2/2 in
+//     y = c x, which also has no cancellation at large shape.
+#pragma once
+#include "tm_device.h"
+
+namespace tmk {
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float log2_hw(float x) { return __builtin_amdgcn_logf(x); }   // v_log_f32
+__device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }  // v_exp_f32
+__device__ __forceinline__ float logf_hw(float x) { return log2_hw(x) * 0.693147181f; }
+
+// -expm1(-z) for z >= 0 in FP32.
+__device__ __forceinline__ float neg_expm1_neg(float z)
+{
+    if (z < 0.5f) {
+        // z - z^2/2 + z^3/6 - ... to z^8 (truncation < 3e-8 relative at z = 1/2)
+        float p = -2.48015873e-5f;
+        p = p * z + 1.98412698e-4f;
+        p = p * z - 1.38888889e-3f;
+        p = p * z + 8.33333333e-3f;
+        p = p * z - 4.16666667e-2f;
+        p = p * z + 1.66666667e-1f;
+        p = p * z - 0.5f;
+        p = p * z + 1.0f;
+        return p * z;
+    }
+    return 1.0f - __expf(-z);
+}
+
+// log(k!) - [(k + 1/2) log k - k + log(2 pi)/2]  (Stirling's error), k >= 1.
+__device__ __forceinline__ float stirlerr_f(float k)
+{
+    if (k < 16.0f) {
+        // exact values: lgamma(k+1) - ((k+0.5) log k - k + 0.5 log 2pi)
+        const int i = (int)k;
+        switch (i) {
+        case 1: return 0.0810614667953272f;
+        case 2: return 0.0413406959554092f;
+        case 3: return 0.0276779256849983f;
+        case 4: return 0.0207906721037650f;
+        case 5: return 0.0166446911898211f;
+        case 6: return 0.0138761288230707f;
+        case 7: return 0.0118967099458917f;
+        case 8: return 0.0104112652619720f;
+        case 9: return 0.00925546218271273f;
+        case 10: return 0.00833056343336287f;
+        case 11: return 0.00757367548795184f;
+        case 12: return 0.00694284010720953f;
+        case 13: return 0.00640899418800420f;
+        case 14: return 0.00595137011275885f;
+        case 15: return 0.00555473355196280f;
+        default: return 0.0f;
+        }
+    }
+    const float r = rcpf(k), r2 = r * r;
+    return r * (0.0833333333f - r2 * (0.00277777778f - r2 * 0.000793650794f));
+}
+
+// Poisson(z), z formed in FP64 by the caller.
+__device__ __forceinline__ int64_t poisson_f(Stream& s, double zd, bool compat)
+{
+    if (!(zd > 0.0)) return 0;
+    if (compat && zd > 100.0) {
+        const float zf = (float)zd;
+        return (int64_t)((double)(norm_icdf_f(s.u32()) * __builtin_sqrtf(zf)) + zd);
+    }
+    if (zd < 12.0) {
+        // Both searches stop once the next term can no longer move the running
+        // sum at FP32 resolution: the remaining tail mass is below 2^-24, and a
+        // uniform that falls in the rounding gap left between the sum's limit and
+        // 1 (or 0) maps to that extreme quantile instead of running on.
+        const float z = (float)zd;
+        float p = __expf(-z);                   // P(N = 0)
+        int k = 0;
+        if (z < 0.5f) {
+            const float v = s.v32();            // 1 - u, fine near 0
+            float S = neg_expm1_neg(z);         // P(N > 0), resolved as z -> 0
+            while (v < S && k < 200) {
+                k += 1;
+                p = p * z * rcpf((float)k);
+                if (p <= S * 0x1p-24f) break;
+                S = S - p;
+            }
+        } else {
+            const float u = s.u32();
+            float F = p;                        // P(N <= 0)
+            while (u > F && k < 200) {
+                k += 1;
+                p = p * z * rcpf((float)k);
+                if (p <= 0x1p-25f) break;
+                F = F + p;
+            }
+        }
+        return k;
+    }
+    const float z = (float)zd;
+    const float slam = __builtin_sqrtf(z);
+    const float b = 0.931f + 2.53f * slam;
+    const float a = -0.059f + 0.02483f * b;
+    const float invalpha = 1.1239f + 1.1328f * rcpf(b - 3.4f);
+    const float vr = 0.9277f - 3.6224f * rcpf(b - 2.0f);
+    for (int it = 0; it < 64; ++it) {
+        const float U = s.u32() - 0.5f;
+        const float V = s.u32();
+        const float us = 0.5f - __builtin_fabsf(U);
+        const double kd = __builtin_floor((double)((2.0f * a * rcpf(us) + b) * U) + zd + 0.43);
+        if (us >= 0.07f && V <= vr) return (int64_t)kd;
+        if (kd < 0.0 || (us < 0.013f && V > us)) continue;
+        const float k = (float)kd;
+        // log p(k; z) = -stirlerr(k) - bd0(k, z) - log(2 pi k)/2, bd0 = k log(k/z) + z - k
+        const float dk = (float)(kd - zd);
+        float bd0;
+        if (__builtin_fabsf(dk) < 0.1f * (k + z)) {
+            const float w = dk * rcpf(k + z), w2 = w * w;
+            bd0 = dk * w + 2.0f * k * w * w2 * (0.333333333f + w2 * (0.2f + w2 * (0.142857143f + w2 * 0.111111111f)));
+        } else {
+            bd0 = k * logf_hw(k * rcpf(z)) + z - k;
+        }
+        const float logp = k > 0.0f ? -stirlerr_f(k) - bd0 - 0.5f * logf_hw(6.28318531f * k) : -z;
+        const float lhs = logf_hw(V * invalpha * rcpf(a * rcpf(us * us) + b));
+        if (lhs <= logp) return (int64_t)kd;
+    }
+    return (int64_t)__builtin_floor(zd);
+}
+
+// Gamma(shape, 1), Marsaglia-Tsang in FP32.
+__device__ __forceinline__ float gamma_f(Stream& s, float shape)
+{
+    if (!(shape > 0.0f)) return 0.0f;
+    const float a = shape < 1.0f ? shape + 1.0f : shape;
+    const float d = a - 0.333333343f;
+    const float c = __builtin_amdgcn_rsqf(9.0f * d);
+    float g = d;
+    for (int it = 0; it < 64; ++it) {
+        const float x = norm_icdf_f(s.u32());
+        const float y = c * x;
+        float v = 1.0f + y;
+        if (v <= 0.0f) continue;
+        v = v * v * v;
+        const float u = s.u32();
+        const float x2 = x * x;
+        if (u < 1.0f - 0.0331f * x2 * x2) { g = d * v; break; }
+        float acc;
+        if (__builtin_fabsf(y) < 0.1f) {
+            // x^2/2 + d(1 - v + log v) = d y^4 (-3/4 + 3/5 y - 1/2 y^2 + 3/7 y^3 - 3/8 y^4)
+            const float y2 = y * y;
+            acc = d * y2 * y2 * (-0.75f + y * (0.6f + y * (-0.5f + y * (0.428571429f - 0.375f * y))));
+        } else {
+            acc = 0.5f * x2 + d * (1.0f - v + logf_hw(v));
+        }
+        if (logf_hw(u) < acc) { g = d * v; break; }
+    }
+    if (shape < 1.0f) g = g * exp2_hw(log2_hw(s.u32()) * rcpf(shape));
+    return g;
+}
+
+}  // namespace tmk

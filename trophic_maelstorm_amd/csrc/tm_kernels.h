@@ -21,6 +21,7 @@
 // atomics, fences or serial per-tile set-up.
 #pragma once
 #include "tm_device.h"
+#include "tm_fast32.h"
 #include "tm/tm.h"
 
 namespace tmk {
@@ -166,16 +167,16 @@ __device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double 
     return gamma_d(s, gru) / k.lambda;
 }
 
-// FP32 storage: alpha_i, the Poisson mean, the Poisson draw and the Gamma shape
-// in FP64 (the same values the FP64 path forms from the same fields), the
-// Gamma deviate in FP32.
+// FP32 storage: alpha_i, the Poisson mean and the Gamma shape in FP64 (the same
+// values the FP64 path forms from the same fields); the Poisson and Gamma
+// deviates from the FP32 samplers of tm_fast32.h.
 __device__ __forceinline__ float dornic_cell(Stream& s, double alpha_i, float cur, const DornicConst& k,
                                              bool compat, int64_t& n)
 {
 #pragma clang fp contract(off)
     const double mu = -1.0 + 2.0 * alpha_i * k.sigma2_1;
     const double ziggy = k.ll * (double)cur;
-    n = poisson_d(s, ziggy, compat);
+    n = poisson_f(s, ziggy, compat);
     const double gru = mu + 1.0 + (double)n;
     float g;
     if (compat && gru > 100.0) {
@@ -184,7 +185,9 @@ __device__ __forceinline__ float dornic_cell(Stream& s, double alpha_i, float cu
     } else {
         g = gamma_f(s, (float)gru);
     }
-    return (float)((double)g / k.lambda);
+    // g / lambda with 1/lambda carried as hi + lo: a single FP32 rounding of 1/lambda
+    // would bias every cell by the same relative amount every step.
+    return __builtin_fmaf(g, k.inv_lambda_hi, g * k.inv_lambda_lo);
 }
 
 // alpha_i = (D/dx^2)(up + down + left + right) in FP64, in the reference's order
@@ -251,64 +254,211 @@ __device__ __forceinline__ DPConst dp_constants(const tm_dp_params& p, int64_t s
 // DP step: Dornic + nonlinear sub-step + checks + flush + rho(t) partials.
 // multiSPDP.cpp:2901-2958 and :3222-3262.
 // ---------------------------------------------------------------------------
-template <typename T, int TX, int TY, int NT>
+__device__ __forceinline__ uint32_t as_bits(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ uint32_t as_bits(double x)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    return (uint32_t)b | (uint32_t)(b >> 32);
+}
+
+// Four +0 cells starting at p (16-byte aligned when possible, else four stores).
+__device__ __forceinline__ void store4_zero(float* p)
+{
+    if (((uintptr_t)p & 15) == 0) *(float4*)p = make_float4(0.f, 0.f, 0.f, 0.f);
+    else { p[0] = 0.f; p[1] = 0.f; p[2] = 0.f; p[3] = 0.f; }
+}
+__device__ __forceinline__ void store4_zero(double* p)
+{
+    if (((uintptr_t)p & 15) == 0) {
+        ((double2*)p)[0] = make_double2(0.0, 0.0);
+        ((double2*)p)[1] = make_double2(0.0, 0.0);
+    } else { p[0] = 0.0; p[1] = 0.0; p[2] = 0.0; p[3] = 0.0; }
+}
+
+// Wave-level (sum, count) reduction; lane 0 writes the wave's partial.
+template <int NF>
+__device__ __forceinline__ void wave_partials(double (&sum)[NF], double (&cnt)[NF], double2* dst)
+{
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sum[f] += __shfl_xor(sum[f], o);
+            cnt[f] += __shfl_xor(cnt[f], o);
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) dst[f] = make_double2(sum[f], cnt[f]);
+    }
+}
+
+// Asynchronous copy of a tile and its halo into LDS with global_load_lds: each
+// wave instruction lands 64 lanes' data at consecutive LDS addresses, so the
+// staging costs no registers and overlaps whatever the workgroup does until
+// its next s_waitcnt.  The LDS image is row-major, RH rows of RWV = TX + 2V
+// elements covering columns [x0 - V, x0 + TX + V), V = 16 / sizeof(T), rows
+// [y0 - 1, y0 + TY + 1), periodic.  When L is a multiple of V every 16-byte
+// chunk of a row is contiguous in memory and one global_load_lds_dwordx4 moves
+// it; otherwise the image is filled dword by dword.
+template <typename T, int TX, int RH, int NT>
+__device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0, uint32_t* lds)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr int V = 16 / (int)sizeof(T);
+    constexpr int RWV = TX + 2 * V;
+    const int tid = threadIdx.x;
+    const int wave_base = tid & ~63;
+    if (L % V == 0) {
+        constexpr int CPR = RWV / V;                  // 16-byte chunks per row
+        constexpr int NC = RH * CPR;
+        constexpr int NLD = (NC + NT - 1) / NT;
+        // not unrolled: the copies are fire-and-forget, nothing stays live
+#pragma unroll 1
+        for (int j = 0; j < NLD; ++j) {
+            const int q = j * NT + tid;
+            uint32_t off = 0;
+            if (q < NC) {
+                const int ry = q / CPR, cx = q - (q / CPR) * CPR;
+                int gy = y0 + ry - 1, gx = x0 - V + cx * V;
+                gy = gy < 0 ? gy + L : (gy >= L ? gy - L : gy);   // one wrap: L >= RH, L >= RWV
+                gx = gx < 0 ? gx + L : (gx >= L ? gx - L : gx);
+                if (L < RWV || L < RH) {
+                    gy = wrap(y0 + ry - 1, L);
+                    gx = wrap(x0 - V + cx * V, L);
+                }
+                off = (uint32_t)gy * (uint32_t)L + (uint32_t)gx;
+            }
+            __builtin_amdgcn_global_load_lds(fin + off, lds + 4 * (j * NT + wave_base), 16, 0, 0);
+        }
+    } else {
+        constexpr int DW = sizeof(T) / 4;             // dwords per element
+        constexpr int NQ = RH * RWV * DW;
+        constexpr int NLD = (NQ + NT - 1) / NT;
+        const uint32_t* src0 = (const uint32_t*)fin;
+#pragma unroll 1
+        for (int j = 0; j < NLD; ++j) {
+            const int q = j * NT + tid;
+            uint32_t off = 0;
+            if (q < NQ) {
+                const int e = q / DW, w = q - (q / DW) * DW;
+                const int ry = e / RWV, rx = e - (e / RWV) * RWV;
+                const int gy = wrap(y0 + ry - 1, L), gx = wrap(x0 - V + rx, L);
+                off = ((uint32_t)gy * (uint32_t)L + (uint32_t)gx) * DW + w;
+            }
+            __builtin_amdgcn_global_load_lds(src0 + off, lds + j * NT + wave_base, 4, 0, 0);
+        }
+    }
+#endif
+}
+
+template <typename T, int TX, int TY, int NT, int NIMG>
 __global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
 {
-    constexpr int H = 1, RW = TX + 2 * H, RH = TY + 2 * H;
-    constexpr int ROWS = NT / TX;          // tile rows covered per pass
-    static_assert(NT % TX == 0 && TY % ROWS == 0, "tile shape");
-    __shared__ T s_rho[RH * RW];
-    __shared__ double s_red[2 * (NT / 64)];
+    constexpr int V = 16 / (int)sizeof(T);
+    constexpr int RWV = TX + 2 * V, RH = TY + 2;      // LDS image (see tile_to_lds)
+    constexpr int CG = TX / 4;                        // 4-cell groups per tile row
+    constexpr int R4 = NT / CG;                       // tile rows covered per pass
+    constexpr int NW = NT / 64;                       // waves = partials per tile
+    constexpr int NQ = RH * RWV * (int)(sizeof(T) / 4);
+    constexpr int NBUF = ((NQ + 4 * NT - 1) / (4 * NT)) * 4 * NT;   // dwords per image
+    static_assert(NT % CG == 0 && TY % R4 == 0 && TX % 64 == 0, "tile shape");
+    static_assert(NIMG == 1 || NIMG == 2, "one image (a tile per workgroup) or two (prefetch)");
+    __shared__ uint32_t s_img[NIMG][NBUF];
 
     const int64_t counter = a.state->counter + a.k;
     const int64_t slot = counter % a.nslots;
     const int tid = threadIdx.x;
-    const int lx = tid % TX, ly0 = tid / TX;
+    const int cg = tid % CG, ly0 = tid / CG;
     const T* in = (const T*)a.in;
     T* out = (T*)a.out;
     const DPConst* consts = (const DPConst*)a.consts + (int64_t)a.k * a.n_inst;
 
-    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-        const int tile = tile_of(t, a.ntiles, a.xcd);
+    // Each workgroup walks a contiguous range of tiles (ordered down vertical
+    // strips, so consecutive tiles share a halo row in L2); tile t+1 streams
+    // into the second LDS image while tile t is computed from the first.
+    const int t0 = (int)((int64_t)blockIdx.x * a.ntiles / gridDim.x);
+    const int t1 = (int)((int64_t)(blockIdx.x + 1) * a.ntiles / gridDim.x);
+    if (t0 < t1) {
+        const DevTile tl = a.tiles[t0];
+        const DevInst di = a.inst[tl.inst];
+        tile_to_lds<T, TX, RH, NT>(in + di.cell_off, di.L, tl.x0, tl.y0, s_img[0]);
+    }
+    for (int tile = t0; tile < t1; ++tile) {
+        const int buf = NIMG == 2 ? (tile - t0) & 1 : 0;
         const DevTile tl = a.tiles[tile];
         const DevInst di = a.inst[tl.inst];
         const int L = di.L;
-        const T* fin = in + di.cell_off;
         T* fout = out + di.cell_off;
-        for (int idx = tid; idx < RH * RW; idx += NT) {
-            const int ry = idx / RW, rx = idx - (idx / RW) * RW;
-            const int gy = wrap(tl.y0 + ry - H, L), gx = wrap(tl.x0 + rx - H, L);
-            s_rho[idx] = fin[(int64_t)gy * L + gx];
-        }
         const DPConst c = consts[tl.inst];
-        __syncthreads();
-        const T b = (T)c.b, dt = (T)c.dt, eps = (T)c.eps, neg = (T)c.neg;
-        double sum[1] = {0.0}, cnt[1] = {0.0};
-        for (int j = 0; j < TY / ROWS; ++j) {
-            const int ly = ly0 + j * ROWS;
-            const int gy = tl.y0 + ly, gx = tl.x0 + lx;
-            if (gy >= L || gx >= L) continue;
-            const int r = (ly + 1) * RW + lx + 1;
-            const T cur = s_rho[r];
-            const double alpha_i = nbr_alpha(c.k.dc, s_rho[r - RW], s_rho[r + RW], s_rho[r - 1], s_rho[r + 1]);
-            const int64_t cell = (int64_t)gy * L + gx;
-            T y = cur;
-            int64_t n = -1;
-            if (!(alpha_i == 0.0 && cur == (T)0)) {
-                Stream st(di.seed, di.inst_id, (uint32_t)cell, c.step);
-                y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
-            }
-            y = dp_nonlinear<T>(c.scheme, b, dt, y);
-            if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
-            if (y < eps && y >= neg) y = (T)0;
-            fout[cell] = y;
-            if (a.draws) a.draws[di.cell_off + cell] = n;
-            sum[0] += (double)y;
-            cnt[0] += y > (T)0 ? 1.0 : 0.0;
+        __builtin_amdgcn_s_waitcnt(0);   // this thread's copies of tile `tile` have landed
+        __syncthreads();                 // ... everyone's; and tile-1's image is free
+        if (NIMG == 2 && tile + 1 < t1) {
+            const DevTile tn = a.tiles[tile + 1];
+            const DevInst dn = a.inst[tn.inst];
+            tile_to_lds<T, TX, RH, NT>(in + dn.cell_off, dn.L, tn.x0, tn.y0, s_img[(buf ^ 1) & (NIMG - 1)]);
         }
-        double2* dst = a.partials + (slot * a.ntiles + tile);
-        block_partials<NT, 1>(sum, cnt, dst, s_red);
-        __syncthreads();
+        const T* s_rho = (const T*)s_img[buf];
+        const T b = (T)c.b, dt = (T)c.dt, eps = (T)c.eps, neg = (T)c.neg;
+        double sum[1] = {0.0};
+        uint32_t nlive = 0;
+        // Thread -> groups of 4 adjacent cells (cg) on rows ly0 + j * R4.  A group
+        // whose 14-value stencil is all +0 is absorbing (the reference skips each
+        // of its cells, FKE:4161-4164) and only stores zeros; other groups run
+        // the per-cell step.
+        for (int j = 0; j < TY / R4; ++j) {
+            const int ly = ly0 + j * R4;
+            const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
+            if (gy >= L || gx0 >= L) continue;
+            const int r = (ly + 1) * RWV + V + 4 * cg;
+            const uint32_t cell0 = (uint32_t)gy * (uint32_t)L + (uint32_t)gx0;
+            const bool full = gx0 + 4 <= L;
+            uint32_t bits = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                bits |= as_bits(s_rho[r + q]) | as_bits(s_rho[r - RWV + q]) | as_bits(s_rho[r + RWV + q]);
+            bits |= as_bits(s_rho[r - 1]) | as_bits(s_rho[r + 4]);
+            if (bits == 0 && full) {
+                store4_zero(fout + cell0);
+                if (a.draws) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) a.draws[di.cell_off + cell0 + q] = -1;
+                }
+                continue;
+            }
+            for (int q = 0; q < 4; ++q) {
+                if (gx0 + q >= L) break;
+                const int rq = r + q;
+                const T cur = s_rho[rq];
+                const T up = s_rho[rq - RWV], dn = s_rho[rq + RWV], lf = s_rho[rq - 1], rt = s_rho[rq + 1];
+                const uint32_t cell = cell0 + q;
+                T y = (T)0;
+                int64_t n = -1;
+                if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
+                    y = cur;
+                    const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
+                    if (!(alpha_i == 0.0 && cur == (T)0)) {
+                        Stream st(di.seed, di.inst_id, cell, c.step);
+                        y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
+                    }
+                    y = dp_nonlinear<T>(c.scheme, b, dt, y);
+                    if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
+                    if (y < eps && y >= neg) y = (T)0;
+                }
+                fout[cell] = y;
+                if (a.draws) a.draws[di.cell_off + cell] = n;
+                sum[0] += (double)y;
+                nlive += y > (T)0 ? 1u : 0u;
+            }
+        }
+        double cnt[1] = {(double)nlive};
+        wave_partials<1>(sum, cnt, a.partials + (slot * a.ntiles + tile) * NW + (tid >> 6));
+        if (NIMG == 1 && tile + 1 < t1) {     // one image: reload it after everyone is done
+            __syncthreads();
+            const DevTile tn = a.tiles[tile + 1];
+            const DevInst dn = a.inst[tn.inst];
+            tile_to_lds<T, TX, RH, NT>(in + dn.cell_off, dn.L, tn.x0, tn.y0, s_img[0]);
+        }
     }
 }
 
@@ -747,7 +897,7 @@ struct CombineArgs {
     int32_t ntiles, nslots, nf, nsteps;  // nsteps: steps folded by this launch
     int64_t rows_cap;
     int32_t measure;                     // 1: slot 0, row 0 (tm_measure)
-    int32_t pad;
+    int32_t pw;                          // partials per tile (waves of the step kernel)
 };
 
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
@@ -771,11 +921,12 @@ __global__ __launch_bounds__(256) void combine_rows_kernel(CombineArgs a)
         slot = c % a.nslots;
         row = c - a.state->run_base;
     }
-    const double2* p = a.partials + (slot * a.ntiles + di.tile0) * a.nf;
+    const double2* p = a.partials + ((slot * a.ntiles + di.tile0) * a.pw) * a.nf;
+    const int nsub = di.ntiles * a.pw;
     const int tid = threadIdx.x;
     for (int f = 0; f < a.nf; ++f) {
         double s = 0.0, e = 0.0, n = 0.0;
-        for (int t = tid; t < di.ntiles; t += 256) {
+        for (int t = tid; t < nsub; t += 256) {
             const double2 v = p[(int64_t)t * a.nf + f];
             const double y = v.x - e;     // Kahan
             const double tt = s + y;

@@ -6,10 +6,10 @@
 
 namespace tmk {
 
-template <typename T, int TX, int TY, int NT>
+template <typename T, int TX, int TY, int NT, int NIMG>
 static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, NIMG>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 template <typename T, int TX, int TY, int NT>
@@ -19,17 +19,19 @@ static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
 }
 
 static const Variant kDP[] = {
-    {64, 8, 256, "dp_f64_64x8_t256", &launch_dp<double, 64, 8, 256>},
-    {64, 16, 256, "dp_f64_64x16_t256", &launch_dp<double, 64, 16, 256>},
-    {64, 32, 256, "dp_f64_64x32_t256", &launch_dp<double, 64, 32, 256>},
-    {32, 8, 256, "dp_f64_32x8_t256", &launch_dp<double, 32, 8, 256>},
-    {128, 16, 256, "dp_f64_128x16_t256", &launch_dp<double, 128, 16, 256>},
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256", &launch_dp<double, 64, 32, 256, 1>},
+    {128, 32, 256, 0, 4, "dp_f64_128x32_t256", &launch_dp<double, 128, 32, 256, 1>},
+    {128, 64, 256, 0, 4, "dp_f64_128x64_t256", &launch_dp<double, 128, 64, 256, 1>},
+    {64, 64, 256, 0, 4, "dp_f64_64x64_t256", &launch_dp<double, 64, 64, 256, 1>},
+    {256, 32, 256, 0, 4, "dp_f64_256x32_t256", &launch_dp<double, 256, 32, 256, 1>},
+    {64, 32, 256, 3, 4, "dp_f64_64x32_t256_p3", &launch_dp<double, 64, 32, 256, 2>},
+    {128, 32, 256, 2, 4, "dp_f64_128x32_t256_p2", &launch_dp<double, 128, 32, 256, 2>},
 };
 
 static const Variant kRK[] = {
-    {16, 16, 256, "rk_f64_16x16_t256", &launch_rk<double, 16, 16, 256>},
-    {32, 16, 256, "rk_f64_32x16_t256", &launch_rk<double, 32, 16, 256>},
-    {32, 32, 256, "rk_f64_32x32_t256", &launch_rk<double, 32, 32, 256>},
+    {16, 16, 256, 0, 1, "rk_f64_16x16_t256", &launch_rk<double, 16, 16, 256>},
+    {32, 16, 256, 0, 1, "rk_f64_32x16_t256", &launch_rk<double, 32, 16, 256>},
+    {32, 32, 256, 0, 1, "rk_f64_32x32_t256", &launch_rk<double, 32, 32, 256>},
 };
 
 const Variant* dp_variants_f64(int* n) { *n = (int)(sizeof(kDP) / sizeof(kDP[0])); return kDP; }

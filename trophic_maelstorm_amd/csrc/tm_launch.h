@@ -11,6 +11,8 @@ typedef void (*StepLaunch)(const StepArgs&, int grid, hipStream_t);
 
 struct Variant {
     int tx, ty, nt;
+    int wg_per_cu;   // > 0: persistent grid of wg_per_cu workgroups per CU
+    int pw;          // rho(t) partials written per tile
     const char* name;
     StepLaunch fn;
 };

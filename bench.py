@@ -16,7 +16,7 @@ Roofline: achieved HBM GB/s of the step kernel = algorithmic bytes per launch
 write of every field) / the kernel's average duration, measured with HIP
 events on the context stream over a replay of the same trajectory.
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--with-c5] [--no-cpu]
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--c5-steps N] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -35,6 +35,12 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s + % HBM peak: batched FP32 vs plain FP64, ρ(t) err ≤1e-3"
 HBM_PEAK_GBS = 8000.0
 SEED = 20251202
+# HBM bytes per launch of the FP32 DP step kernel on the C4 batch, from counter
+# runs (profiles/dp_step_f32_c4_quench_hbm_traffic.md): FETCH_SIZE x 2.000 and
+# WRITE_SIZE x 1.000, the factors a plain 256 MiB copy needs to read its known
+# size.  Variant dp_f32_256x32_t256.
+C4_F32_TRAFFIC = {"variant": "dp_f32_256x32_t256", "fetch_kib": 75081.6, "fetch_scale": 2.0,
+                  "write_kib": 142393.3, "write_scale": 1.0}
 
 
 def c4_batch(tm, precision):
@@ -43,7 +49,7 @@ def c4_batch(tm, precision):
                     dt=0.1, dx=1.0, scheme=tm.DP_EULER, sampler=tm.SAMPLER_EXACT)
     inst = [tm.Instance(L, 0, SEED, k) for k, L in enumerate(sizes)]
     return dict(model=tm.MODEL_DP, precision=precision, params=[p], inst=inst, lo=[0.0], hi=[1.0],
-                steps=10_000, nf=1)
+                steps=10_000, nf=1, chunk=5000)
 
 
 def c3_batch(tm, precision):
@@ -51,7 +57,7 @@ def c3_batch(tm, precision):
     ps = [tm.RietkerkParams(R=float(R)) for R in Rs]
     inst = [tm.Instance(512, k // 16, SEED, k) for k in range(256)]
     return dict(model=tm.MODEL_RIETKERK, precision=precision, params=ps, inst=inst,
-                lo=[0.0, 0.0, 0.0], hi=[90.0, 1.0, 1.0], steps=5_000, nf=3)
+                lo=[0.0, 0.0, 0.0], hi=[90.0, 1.0, 1.0], steps=5_000, nf=3, chunk=1000)
 
 
 def c2_batch(tm, precision):
@@ -60,10 +66,10 @@ def c2_batch(tm, precision):
                 steps=20_000, nf=3)
 
 
-def c5_batch(tm, precision):
+def c5_batch(tm, precision, steps=100_000):
     return dict(model=tm.MODEL_RIETKERK, precision=precision, params=[tm.RietkerkParams(R=0.75)],
                 inst=[tm.Instance(8192, 0, SEED, 0)], lo=[0.0, 0.0, 0.0], hi=[90.0, 1.0, 1.0],
-                steps=100_000, nf=3)
+                steps=steps, nf=3, chunk=500)
 
 
 def cells_of(b):
@@ -74,9 +80,17 @@ def bytes_per_cell(b):
     return b["nf"] * 2 * (8 if b["precision"] == 0 else 4)
 
 
+def traffic_bytes(variant):
+    t = C4_F32_TRAFFIC
+    if variant != t["variant"]:
+        return None
+    return int(round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024))
+
+
 def run_workload(tm, b, steps, warmup, want_rows=True, tune=True):
-    """Warm up, then time `steps` steps of the real trajectory from t = 0 and replay
-    it with HIP events around the step kernels only."""
+    """Warm up, then time `steps` steps of the real trajectory from t = 0: wall
+    clock around the whole job, and the device time of every step launch (HIP
+    events on the context stream inside each run() call, summed over chunks)."""
     with tm.Context(0) as ctx:
         ctx.configure(b["model"], b["precision"], b["params"], b["inst"])
         variant = None
@@ -87,39 +101,64 @@ def run_workload(tm, b, steps, warmup, want_rows=True, tune=True):
             ctx.run(warmup, rows=want_rows)
         ctx.reset()
         ctx.init_uniform(b["lo"], b["hi"])
+        chunk = max(1, min(steps, b.get("chunk", steps)))
+        parts = []
+        dev_ms = 0.0
         t0 = time.perf_counter()
-        rows = ctx.run(steps, rows=want_rows)
+        done = 0
+        while done < steps:
+            n = min(chunk, steps - done)
+            parts.append(ctx.run(n, rows=want_rows))
+            dev_ms += ctx.last_run_ms()
+            done += n
+            if done < steps:
+                print(f"[bench]   ... {done}/{steps} steps", file=sys.stderr, flush=True)
         t1 = time.perf_counter()
+        rows = np.concatenate(parts, axis=1) if want_rows else None
         final = [ctx.download(k) for k in range(min(len(b["inst"]), 4))]
-        # replay of the same trajectory: step-kernel time only
-        ctx.reset()
-        ctx.init_uniform(b["lo"], b["hi"])
-        kms = ctx.time_steps(steps)
         names = tm.variant_names(b["model"], b["precision"])
         vname = names[variant[0]] if variant else None
     cells = cells_of(b)
     wall = t1 - t0
-    avg_kernel_s = kms / 1e3 / steps
-    achieved = cells * bytes_per_cell(b) / avg_kernel_s / 1e9
+    avg_step_s = dev_ms / 1e3 / steps
+    achieved = cells * bytes_per_cell(b) / avg_step_s / 1e9
     return dict(value=cells * steps / wall, wall_s=wall, ms_per_step=wall / steps * 1e3,
-                kernel_ms=avg_kernel_s * 1e3, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
+                kernel_ms=avg_step_s * 1e3, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
                 cells=cells, steps=steps, variant=vname, graph_steps=variant[1] if variant else None,
                 rows=rows, final=final)
 
 
 def rho_error(r32, r64, nf):
-    """Errors of FP32 rho(t) against the FP64 run of the same seeds.  Ensemble: the
-    mean over instances of the first field's spatial mean, per step."""
-    m32 = r32[:, :, 1].mean(axis=0)
-    m64 = r64[:, :, 1].mean(axis=0)
-    live = np.abs(m64) > 0
-    ens = np.abs(m32[live] - m64[live]) / np.abs(m64[live])
-    per = np.abs(r32[:, :, 1::2] - r64[:, :, 1::2]) / np.maximum(np.abs(r64[:, :, 1::2]), 1e-300)
-    per = per[np.abs(r64[:, :, 1::2]) > 0]
-    return dict(ensemble_rel_max=float(ens.max()) if ens.size else 0.0,
-                ensemble_rel_mean=float(ens.mean()) if ens.size else 0.0,
-                instance_rel_max=float(per.max()) if per.size else 0.0,
-                instance_rel_mean=float(per.mean()) if per.size else 0.0)
+    """Errors of FP32 rho(t) against the FP64 run of the same seeds, per field.
+    ensemble: mean over instances of the spatial mean, per step -- relative error
+    max / mean over all steps with a nonzero FP64 ensemble, and over the steps
+    where the ensemble still holds >= 10% of its initial density (later, same-seed
+    trajectories decorrelate cluster by cluster and the difference is sampling
+    noise); bias_z_max: largest |paired mean difference| / standard error over
+    steps (a systematic FP32 bias shows up here); survival: max difference of the
+    fraction of instances with a live cell."""
+    out = {}
+    names = "rho" if nf == 1 else "PWO"
+    for f in range(nf):
+        c = 2 * f + 1
+        m32, m64 = r32[:, :, c].mean(axis=0), r64[:, :, c].mean(axis=0)
+        nz = np.abs(m64) > 0
+        rel = np.abs(m32[nz] - m64[nz]) / np.abs(m64[nz])
+        pop = m64 >= 0.1 * m64[0]
+        relp = np.abs(m32[pop] - m64[pop]) / np.abs(m64[pop])
+        d = r32[:, :, c] - r64[:, :, c]
+        se = d.std(axis=0, ddof=1) / np.sqrt(max(d.shape[0], 1))
+        zb = np.abs(d.mean(axis=0)) / np.where(se > 0, se, np.inf)
+        s32 = (r32[:, :, 2 * f] > 0).mean(axis=0)
+        s64 = (r64[:, :, 2 * f] > 0).mean(axis=0)
+        out[names[f]] = dict(
+            ensemble_rel_max=float(rel.max()) if rel.size else 0.0,
+            ensemble_rel_mean=float(rel.mean()) if rel.size else 0.0,
+            ensemble_rel_max_populated=float(relp.max()) if relp.size else 0.0,
+            populated_steps=int(pop.sum()),
+            bias_z_max=float(zb.max()) if zb.size else 0.0,
+            survival_diff_max=float(np.abs(s32 - s64).max()))
+    return out
 
 
 def field_error(f32, f64):
@@ -173,10 +212,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps of the headline sweep (default: full 10^4)")
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--with-c5", action="store_true", help="also run the 8192^2 Rietkerk run (10^5 steps)")
+    ap.add_argument("--c5-steps", type=int, default=0,
+                    help="also time the 8192^2 Rietkerk lattice for this many steps (its full run is 10^5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-side", action="store_true", help="headline only")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+        x = torch.empty(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); instances are independent, "
               "so a batch shards across GPUs with no communication.", file=sys.stderr)
@@ -193,9 +246,15 @@ def main():
     h = run_workload(tm, c4, K, args.warmup)
     log(f"[bench]   {h['value']:.4g} cell-updates/s, kernel {h['kernel_ms']:.4f} ms, "
         f"{h['achieved_gbs']:.0f} GB/s ({100 * h['frac']:.1f}% of peak)")
+    if args.headline_only:
+        print(json.dumps({"metric": METRIC, "value": h["value"], "unit": "cell-updates/s", "steps": K,
+                          "ms_per_step": h["ms_per_step"], "step_device_ms": h["kernel_ms"],
+                          "achieved_gbs": h["achieved_gbs"], "variant": h["variant"]}), flush=True)
+        return
     c4d = c4_batch(tm, tm.F64)
     log("[bench] C4 FP64 (same seeds)")
     h64 = run_workload(tm, c4d, K, args.warmup)
+    log(f"[bench]   {h64['value']:.4g} cell-updates/s, {h64['achieved_gbs']:.0f} GB/s")
     errors = {"c4": dict(rho=rho_error(h["rows"], h64["rows"], 1), fields=field_error(h["final"], h64["final"]))}
     side = {"c4_fp64": {k: h64[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}}
 
@@ -203,6 +262,7 @@ def main():
         c2 = c2_batch(tm, tm.F64)
         log("[bench] C2 plain FP64 Rietkerk 256^2")
         r2 = run_workload(tm, c2, c2["steps"], args.warmup)
+        log(f"[bench]   {r2['value']:.4g} cell-updates/s, kernel {r2['kernel_ms']:.4f} ms")
         side["c2_plain_fp64"] = {k: r2[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
         c3 = c3_batch(tm, tm.F32)
         log("[bench] C3 FP32 batched Rietkerk 256 x 512^2")
@@ -212,15 +272,17 @@ def main():
         c3d = c3_batch(tm, tm.F64)
         log("[bench] C3 FP64 (same seeds)")
         r3d = run_workload(tm, c3d, c3["steps"], args.warmup)
+        log(f"[bench]   {r3d['value']:.4g} cell-updates/s, {r3d['achieved_gbs']:.0f} GB/s")
         side["c3_fp64"] = {k: r3d[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
         errors["c3"] = dict(rho=rho_error(r3["rows"], r3d["rows"], 3), fields=field_error(r3["final"], r3d["final"]))
         side["ratio_c4_fp32_over_c2_plain_fp64"] = h["value"] / r2["value"]
         side["ratio_c3_fp32_over_c2_plain_fp64"] = r3["value"] / r2["value"]
-        if args.with_c5:
-            c5 = c5_batch(tm, tm.F32)
-            log("[bench] C5 FP32 Rietkerk 8192^2")
+        if args.c5_steps > 0:
+            c5 = c5_batch(tm, tm.F32, args.c5_steps)
+            log(f"[bench] C5 FP32 Rietkerk 8192^2, {args.c5_steps} of its 10^5 steps")
             r5 = run_workload(tm, c5, c5["steps"], args.warmup)
-            side["c5_fp32"] = {k: r5[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
+            side["c5_fp32"] = {k: r5[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac",
+                                                 "variant", "steps")}
 
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
@@ -241,9 +303,12 @@ def main():
                    "instances": len(c4["inst"]), "cells_per_step": cells_of(c4), "steps": K,
                    "kernel_variant": h["variant"], "graph_steps": h["graph_steps"]},
         "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": h["frac"], "traffic": None,
+                     "frac": h["frac"], "traffic": traffic_bytes(h["variant"]),
                      "kernel": "dp_step_kernel<float>", "bytes_per_cell": bytes_per_cell(c4),
-                     "kernel_ms": h["kernel_ms"]},
+                     "cells_per_launch": cells_of(c4), "step_device_ms": h["kernel_ms"],
+                     "timing": "HIP events on the context stream around every run() of the timed region; "
+                               "the per-step share also holds the per-block constants, rho(t) fold and "
+                               "counter bump launches"},
         "cpu_baseline": cpu,
         "side_by_side": side,
         "error_vs_fp64": errors,

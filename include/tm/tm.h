@@ -179,6 +179,12 @@ tm_status tm_autotune(tm_ctx* ctx, const char* cache_path, int* out_variant, int
  * step kernels only (ms_total), and the number of step-kernel launches. */
 tm_status tm_time_steps(tm_ctx* ctx, int64_t n_steps, double* ms_total, int64_t* launches);
 
+/* Device time of the last tm_run: elapsed time between HIP events recorded on
+ * the context's stream before its first launch and after its last (step
+ * kernels, per-block constants, rho(t) fold and counter bump; the rows copy to
+ * the host is outside).  n_steps (nullable) gets that call's step count. */
+tm_status tm_last_run_time(const tm_ctx* ctx, double* ms, int64_t* n_steps);
+
 /* The context's HIP stream (hipStream_t) for callers that order their own work. */
 void* tm_stream(tm_ctx* ctx);
 

@@ -152,6 +152,7 @@ def lib() -> C.CDLL:
         "tm_variant_name": ([C.c_int, C.c_int, C.c_int], C.c_char_p),
         "tm_autotune": ([vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "tm_time_steps": ([vp, i64, dp, C.POINTER(i64)], C.c_int),
+        "tm_last_run_time": ([vp, dp, C.POINTER(i64)], C.c_int),
         "tm_stream": ([vp], vp),
     }
     for name, (args, res) in sig.items():
@@ -285,6 +286,13 @@ class Context:
         self._check(self._lib.tm_autotune(self._h, cache_path.encode() if cache_path else None,
                                           C.byref(v), C.byref(g)))
         return v.value, g.value
+
+    def last_run_ms(self) -> float:
+        """Device ms of the last run()'s launches (HIP events on the context stream
+        around them; the rows copy to the host is outside)."""
+        ms = C.c_double()
+        self._check(self._lib.tm_last_run_time(self._h, C.byref(ms), None))
+        return ms.value
 
     def time_steps(self, n_steps: int) -> float:
         """Device ms of n_steps step-kernel launches (HIP events on the context stream)."""

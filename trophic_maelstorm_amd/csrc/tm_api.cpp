@@ -69,6 +69,11 @@ struct tm_ctx {
     int64_t counter = 0;
 
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+
+    // device time of the last tm_run's launches (events on the context stream)
+    hipEvent_t run_ev[2] = {nullptr, nullptr};
+    double last_run_ms = 0.0;
+    int64_t last_run_steps = 0;
 };
 
 namespace {
@@ -407,7 +412,8 @@ tm_status tm_create(tm_ctx** out, int device)
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&c->d_state, sizeof(DevState)) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_pinned, 64, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&c->h_pinned, 64, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&c->run_ev[0]) != hipSuccess || hipEventCreate(&c->run_ev[1]) != hipSuccess) {
         tm_destroy(c);
         return TM_ERR_HIP;
     }
@@ -423,6 +429,8 @@ void tm_destroy(tm_ctx* c)
     release_config(c);
     free_dev(c->d_state);
     if (c->h_pinned) hipHostFree(c->h_pinned);
+    for (auto e : c->run_ev)
+        if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -540,14 +548,28 @@ tm_status tm_run(tm_ctx* c, int64_t n_steps, double* rows)
     if (s != TM_OK) return s;
     if (n_steps < 0) return fail(c, TM_ERR_ARG, "negative n_steps");
     if (n_steps == 0) return TM_OK;
+    HIPCHK(c, hipEventRecord(c->run_ev[0], c->stream));
     if ((s = advance(c, n_steps)) != TM_OK) return s;
+    HIPCHK(c, hipEventRecord(c->run_ev[1], c->stream));
     if (rows) {
         const size_t w = sizeof(double) * 2 * c->nf;
         HIPCHK(c, hipMemcpy2DAsync(rows, w * n_steps, c->d_rows, w * c->rows_cap, w * n_steps, c->inst.size(),
                                    hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->run_ev[0], c->run_ev[1]));
+    c->last_run_ms = ms;
+    c->last_run_steps = n_steps;
     return fault_status(c);
+}
+
+tm_status tm_last_run_time(const tm_ctx* c, double* ms, int64_t* n_steps)
+{
+    if (!c || !ms) return TM_ERR_ARG;
+    *ms = c->last_run_ms;
+    if (n_steps) *n_steps = c->last_run_steps;
+    return TM_OK;
 }
 
 tm_status tm_measure(tm_ctx* c, double* rows)

@@ -89,6 +89,8 @@ def lib():
         L.tmo_dp_nonlinear.restype = C.c_double
         L.tmo_dp_nonlinear.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.tmo_reduce.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
+        L.tmo_flush.restype = C.c_double
+        L.tmo_flush.argtypes = [C.c_double] * 3
         _lib = L
     return _lib
 
@@ -201,6 +203,14 @@ def rk4_deterministic(p: RKParams, y, nsteps):
     y = np.ascontiguousarray(y, np.float64).copy()
     lib().tmo_rk4_deterministic(C.byref(p), y.shape[-1], nsteps, y.reshape(-1))
     return y
+
+
+def dp_nonlinear(scheme, b, dt, rho):
+    return lib().tmo_dp_nonlinear(int(scheme), float(b), float(dt), float(rho))
+
+
+def flush(x, eps, neg):
+    return lib().tmo_flush(float(x), float(eps), float(neg))
 
 
 def reduce(fields):

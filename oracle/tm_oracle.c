@@ -353,6 +353,11 @@ void tmo_init_uniform(int L, int nf, uint32_t seed, uint32_t inst,
     }
 }
 
+/* Post-step flush: x < eps && x >= neg -> +0.0, anything else unchanged (NaN
+ * included).  Rietkerk eps 1e-10, neg 0 (FKE:4459-4461); DP eps 1e-12,
+ * neg -1e-80 (MSP:3257-3258). */
+double tmo_flush(double x, double eps, double neg) { return (x < eps && x >= neg) ? 0.0 : x; }
+
 /* mean_of_vector (FKE:1238-1246): naive sum / size; occupied_sites_of_vector
  * (FKE:1271-1280): count of entries > 0.  Row layout FKE:4024 (count, mean). */
 void tmo_reduce(int L, int nf, const double* fields, double* row)
@@ -456,9 +461,7 @@ int tmo_dp_run(const tmo_dp_params* p, int L, uint32_t seed, uint32_t inst, int6
                     if (!fault && fault_cell) *fault_cell = i;
                     fault = 1;
                 }
-                if (y < p->flush_eps && y >= p->flush_neg)     /* MSP:3257-3258 */
-                    y = 0.0;
-                rho[i] = y;
+                rho[i] = tmo_flush(y, p->flush_eps, p->flush_neg);   /* MSP:3257-3258 */
             }
         if (rows) tmo_reduce(L, 1, rho, rows + 2 * n);
         if (fault) break;
@@ -555,7 +558,7 @@ int tmo_rk_run(const tmo_rk_params* p, int L, uint32_t seed, uint32_t inst, int6
                 if (!fault && fault_cell) *fault_cell = i;
                 fault = 1;
             }
-            if (y < p->flush_eps && y >= 0.0) fields[i] = 0.0;
+            fields[i] = tmo_flush(y, p->flush_eps, 0.0);   /* FKE:4459-4461 */
         }
         if (rows) tmo_reduce(L, 3, fields, rows + 6 * n);
         if (fault) break;

@@ -103,6 +103,7 @@ void tmo_rk_rhs(const tmo_rk_params* p, int L, const double* y, double* f); /* f
 void tmo_rk4_deterministic(const tmo_rk_params* p, int L, int64_t nsteps, double* y); /* FKE:2393-2429 */
 double tmo_dp_nonlinear(int scheme, double b, double dt, double rho);  /* one nonlinear sub-step */
 void tmo_reduce(int L, int nf, const double* fields, double* row);    /* (#>0, mean) per field */
+double tmo_flush(double x, double eps, double neg);                   /* post-step flush rule */
 
 #ifdef __cplusplus
 }

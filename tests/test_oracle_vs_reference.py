@@ -1,0 +1,81 @@
+"""The oracle's DP step against the reference's own integrator, in distribution.
+
+percolationDornic_2D (Percolation_FD/stochasticSPDP.cpp:940-1143) is built from
+the reference source into oracle/_ref by oracle/Makefile.  It seeds
+std::mt19937_64 from std::random_device (stochasticSPDP.cpp:973-976) and uses
+libstdc++'s Poisson and Gamma distributions, so only ensembles can agree.  Both
+sides integrate the same initial frame with the Dornic step and the exact
+nonlinear update rho / (1 + b rho dt) (stochasticSPDP.cpp:1064-1068; oracle
+scheme 2, exact sampler: the legacy integrator has no Gaussian shortcuts, and
+no flush of tiny values, so the oracle's flush is off here).
+
+Compared per logged time (t = 0.1 ... 1.0 every step, then 2, 5, 10, 20): the
+mean over surviving replicates of the spatial mean rho, and of the number of
+occupied cells (the reference's columns 3 and 6).  Each difference must sit
+within 4.5 standard errors, estimated from the oracle ensemble for both sides
+(plus the Poisson variance of the empty-cell totals for the occupied count).
+Over 28 comparisons the chance of a false failure is below 2e-4.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+A_VALUES = [0.72, 1.0]
+L, R, DT, B, D, SIGMA = 32, 64, 0.1, 2.0, 1.0, math.sqrt(2.0)
+T_MEAS = [2.0, 5.0, 10.0, 20.0]
+
+
+def reference_rows(ref, a, rho0):
+    out = np.zeros((64, 8), np.float64)
+    tm = np.array(T_MEAS)
+    fd = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)     # the reference prints a line per replicate
+    try:
+        n = ref.ref_ssp_percolation_dornic(L, a, B, D, SIGMA, DT, 1.0, R, rho0, tm, len(tm), T_MEAS[-1],
+                                           out, out.shape[0])
+    finally:
+        os.dup2(fd, 1)
+        os.close(fd)
+        os.close(devnull)
+    assert n > 0
+    return out[:n]
+
+
+@pytest.mark.parametrize("a", A_VALUES)
+def test_dp_ensemble_matches_reference_integrator(oracle, a):
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("reference library not built: /root/reference sources absent")
+    rho0 = oracle.init_uniform(L, [0.0], [1.0], 77, 0).reshape(-1).copy()
+    rows_ref = reference_rows(ref, a, rho0)
+    nsteps = int(round(T_MEAS[-1] / DT))
+    # the legacy integrator has no post-step flush (multiSPDP.cpp:3257-3258 added it later)
+    p = oracle.dp_params(a=a, b=B, D=D, sigma=SIGMA, dt=DT, scheme=2, sampler=0, flush_eps=0.0, flush_neg=0.0)
+    orc = np.stack([oracle.dp_run(p, rho0, L, 1234, k, 0, nsteps)["rows"] for k in range(R)])  # [R, steps, 2]
+    checked = 0
+    for row in rows_ref:
+        t = row[2]
+        n = int(round(t / DT))
+        if n == 0:
+            continue
+        cnt, mean = orc[:, n - 1, 0], orc[:, n - 1, 1]
+        alive = cnt > 0
+        assert alive.sum() >= R // 2, "too few surviving oracle replicates to compare"
+        m = alive.sum()
+        x = mean[alive]
+        se = x.std(ddof=1) / math.sqrt(m)
+        zscore = abs(row[3] - x.mean()) / (math.sqrt(2.0) * se)
+        assert zscore < 4.5, (a, t, "mean", row[3], x.mean(), zscore)
+        # occupied cells: compare the empty-cell totals; while empty cells are
+        # rare their count is Poisson-like and the oracle's sample variance can
+        # be zero, so the Poisson variance of both totals is added
+        empty_orc = float(np.sum(L * L - cnt[alive]))
+        empty_ref = (L * L - row[6]) * m
+        var = 2.0 * m * cnt[alive].var(ddof=1) + empty_orc + empty_ref + 1.0
+        zscore = abs(empty_ref - empty_orc) / math.sqrt(var)
+        assert zscore < 4.5, (a, t, "occupied", row[6], cnt[alive].mean(), zscore)
+        checked += 2
+    assert checked == 2 * (10 + len(T_MEAS))

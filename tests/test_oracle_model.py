@@ -289,3 +289,59 @@ def test_reduce_counts_and_means(oracle):
     for k in range(3):
         assert row[2 * k] == np.count_nonzero(f[k] > 0)
         assert row[2 * k + 1] == math.fsum(f[k].ravel()) / (L * L)
+
+
+# ------------------------------------------------------------------ step composition
+def linear_flow(P, beta, dt, Dc):
+    """The sigma -> 0 limit of the Dornic transition: the mean of the CIR step with
+    alpha_i = Dc * (sum of the four old neighbours)."""
+    nsum = np.roll(P, 1, 0) + np.roll(P, -1, 0) + np.roll(P, 1, 1) + np.roll(P, -1, 1)
+    e = math.exp(beta * dt)
+    return P * e + Dc * nsum / beta * (e - 1)
+
+
+def test_rietkerk_step_is_dornic_then_rk4(oracle):
+    """One full step with sigma -> 0 equals RK4 started from the linearly evolved P
+    and the OLD W, O (rietkerk_bjork_FKE.cpp:4154-4235 order).  A long dt (with
+    slow O diffusion so RK4 stays stable) makes the other order visibly different."""
+    L, sig, dt = 8, 1e-6, 0.5
+    p = oracle.rk_params(R=1.1, sigma=sig, dt=dt, D=(0.1, 0.1, 0.1))
+    y0 = oracle.init_uniform(L, [5.0, 0.5, 0.5], [90.0, 1.0, 1.0], 4, 0)
+    got = oracle.rk_run(p, y0, L, 9, 0, 0, 1)["fields"]
+    beta = -0.25 - 4 * 0.1 / 4.0
+    mid = y0.copy()
+    mid[0] = linear_flow(y0[0], beta, dt, 0.1 / 4.0)
+    want = oracle.rk4_deterministic(p, mid, 1)
+    np.testing.assert_allclose(got, want, rtol=1e-5)
+    # and not RK4 first, then Dornic
+    other = oracle.rk4_deterministic(p, y0, 1)
+    other[0] = linear_flow(other[0], beta, dt, 0.1 / 4.0)
+    assert np.max(np.abs(got[0] - other[0]) / other[0]) > 1e-3
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_dp_step_is_dornic_then_nonlinear(oracle, scheme):
+    L, sig, a, b, dt = 8, 1e-6, 0.72, 2.0, 0.1
+    p = oracle.dp_params(a=a, b=b, sigma=sig, dt=dt, scheme=scheme)
+    rho0 = oracle.init_uniform(L, [0.2], [1.0], 5, 0)[0]
+    got = oracle.dp_run(p, rho0, L, 9, 0, 0, 1)["fields"][0]
+    lin = linear_flow(rho0, a - 4.0, dt, 1.0)
+    want = np.vectorize(lambda r: oracle.dp_nonlinear(scheme, b, dt, r))(lin)
+    np.testing.assert_allclose(got, want, rtol=1e-5)
+
+
+def test_dornic_compat_moments(oracle):
+    """Compat sampler at the Rietkerk C2 scale: n = trunc(N(0,1) sqrt z + z) and
+    rho* = (N(0,1) sqrt s + s) / lambda with s = mu + 1 + n.  For z >> 1,
+    E[trunc(X)] = z - 1/2 and Var = z + 1/12 (Sheppard), so
+    E[lambda rho*] = mu + 1/2 + z and Var[lambda rho*] = E[s] + z + 1/12."""
+    beta, e, lam = oracle.dornic_constants(-0.25, 0.1, 0.5, 0.005, 2.0)
+    alpha_i, rho = 0.1 / 4.0 * 160.0, 40.0
+    x = oracle.dornic_batch(alpha_i, rho, lam, e, 0.5, 1_000_000, sampler=1, seed=31) * lam
+    z = lam * e * rho
+    mu = -1 + 2 * alpha_i / 0.25
+    m = mu + 0.5 + z
+    v = (mu + 0.5 + z) + z + 1.0 / 12
+    N = len(x)
+    assert abs(x.mean() - m) < 5 * math.sqrt(v / N), (x.mean() - m) / math.sqrt(v / N)
+    assert abs(x.var() - v) < 5 * v * math.sqrt(2.0 / N)

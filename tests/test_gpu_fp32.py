@@ -116,3 +116,23 @@ def test_fp32_rietkerk_ensemble_matches_fp64(tm):
     r64 = run_batch(tm, tm.MODEL_RIETKERK, tm.F64, ps, insts, [0, 0, 0], [90, 1, 1], 200)
     for f in range(3):
         ensemble_checks(r32, r64, field=f)
+
+
+def test_fp32_rietkerk_split_matches_fused(tm):
+    """The split FP32 step runs the same per-cell arithmetic as the fused kernel."""
+    names = tm.variant_names(tm.MODEL_RIETKERK, tm.F32)
+    out = {}
+    for want in ("rk_f32_32x32_t256", "rk_f32_split_32x32_t256"):
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_RIETKERK, tm.F32, tm.RietkerkParams(R=1.1),
+                          [tm.Instance(96, 0, 8, k) for k in range(3)])
+            ctx.set_launch(names.index(want), 10)
+            ctx.init_uniform([0, 0, 0], [90, 1, 1])
+            ctx.record_draws(True)
+            rows = ctx.run(20)
+            out[want] = (np.stack([ctx.download(k) for k in range(3)]), ctx.draws(1), rows)
+    (f_a, d_a, r_a), (f_b, d_b, r_b) = out.values()
+    assert np.mean(d_a == d_b) >= 0.999
+    rel = np.abs(f_a - f_b) / np.maximum(np.abs(f_b), 1e-30)
+    assert np.quantile(rel, 0.99) < 1e-5, np.quantile(rel, 0.99)
+    np.testing.assert_allclose(r_a[:, :, 1::2], r_b[:, :, 1::2], rtol=1e-5)

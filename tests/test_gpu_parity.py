@@ -114,6 +114,34 @@ def test_rietkerk_f64_compat_and_tiles(tm, oracle):
         assert_rows_close(rows, ref["rows"])
 
 
+def test_rietkerk_split_step_mixed_batch_bit_exact(tm, oracle):
+    """Split step (Dornic kernel on its own tiles, then RK4) on a batch of uneven
+    L, none a multiple of the Dornic tile, exact sampler, against the oracle."""
+    names = tm.variant_names(tm.MODEL_RIETKERK, tm.F64)
+    split = [k for k, n in enumerate(names) if "split" in n]
+    assert split
+    Ls = [24, 40, 100, 70]
+    ps = [tm.RietkerkParams(R=0.9), tm.RietkerkParams(R=1.3)]
+    pos = [oracle.rk_params(R=0.9), oracle.rk_params(R=1.3)]
+    insts = [tm.Instance(L, k % 2, 40 + k, 3 * k + 1, 5 * k) for k, L in enumerate(Ls)]
+    for variant in split:
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_RIETKERK, tm.F64, ps, insts)
+            ctx.set_launch(variant, 4)
+            ctx.init_uniform([0.0, 0.0, 0.0], [90.0, 1.0, 1.0])
+            init = [ctx.download(k) for k in range(len(Ls))]
+            ctx.record_draws(True)
+            rows = ctx.run(9)
+            got = [ctx.download(k) for k in range(len(Ls))]
+            draws = [ctx.draws(k) for k in range(len(Ls))]
+        for k, d in enumerate(insts):
+            ref = oracle.rk_run(pos[d.params_index], init[k], d.L, d.seed, d.instance_id, d.step0, 9,
+                                rows=True, draws=True)
+            assert np.array_equal(draws[k], ref["draws"]), (names[variant], d.L)
+            assert np.array_equal(got[k], ref["fields"]), (names[variant], d.L)
+            assert_rows_close(rows[k], ref["rows"])
+
+
 def test_mixed_size_batch_matches_single_runs(tm, oracle):
     Ls = [16, 32, 24, 64, 16]
     ps = [tm.DPParams(a=0.72), tm.DPParams(a=1.1, a_end=1.0, ramp_steps=10)]

@@ -48,12 +48,15 @@ struct tm_ctx {
     int graph_steps = 100;
     int xcd = 1;
     std::vector<DevTile> tiles;
+    std::vector<DevTile> tiles_a;   // split Rietkerk step: Dornic kernel tiles
     std::vector<DevInst> dinst;
 
     // device memory
     void* d_params = nullptr;
     DevInst* d_inst = nullptr;
     DevTile* d_tiles = nullptr;
+    DevTile* d_tiles_a = nullptr;
+    void* d_aux = nullptr;         // split Rietkerk step: P* field [total_cells]
     void* d_fields[2] = {nullptr, nullptr};
     double2* d_partials = nullptr;
     void* d_consts = nullptr;      // [nslots][n_inst] per-step constants
@@ -124,6 +127,9 @@ void release_config(tm_ctx* c)
     free_dev(c->d_params);
     free_dev(c->d_inst);
     free_dev(c->d_tiles);
+    free_dev(c->d_tiles_a);
+    free_dev(c->d_aux);
+    c->tiles_a.clear();
     free_dev(c->d_fields[0]);
     free_dev(c->d_fields[1]);
     free_dev(c->d_partials);
@@ -170,6 +176,18 @@ tm_status build_tiles(tm_ctx* c)
     free_dev(c->d_partials);
     HIPCHK(c, hipMalloc((void**)&c->d_tiles, sizeof(DevTile) * c->tiles.size()));
     HIPCHK(c, hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(DevTile) * c->tiles.size(), hipMemcpyHostToDevice));
+    free_dev(c->d_tiles_a);
+    c->tiles_a.clear();
+    if (v.fn_a) {   // split step: the Dornic kernel's own tiles, and the P* field
+        for (size_t i = 0; i < c->inst.size(); ++i)
+            for (int x0 = 0; x0 < c->inst[i].L; x0 += v.ax)
+                for (int y0 = 0; y0 < c->inst[i].L; y0 += v.ay) c->tiles_a.push_back(DevTile{(int32_t)i, x0, y0, 0});
+        HIPCHK(c, hipMalloc((void**)&c->d_tiles_a, sizeof(DevTile) * c->tiles_a.size()));
+        HIPCHK(c, hipMemcpy(c->d_tiles_a, c->tiles_a.data(), sizeof(DevTile) * c->tiles_a.size(),
+                            hipMemcpyHostToDevice));
+        if (!c->d_aux && hipMalloc(&c->d_aux, c->esize * (size_t)c->total_cells) != hipSuccess)
+            return fail(c, TM_ERR_NOMEM, "P* field allocation failed");
+    }
     if (!c->d_inst) HIPCHK(c, hipMalloc((void**)&c->d_inst, sizeof(DevInst) * c->dinst.size()));
     HIPCHK(c, hipMemcpy(c->d_inst, c->dinst.data(), sizeof(DevInst) * c->dinst.size(), hipMemcpyHostToDevice));
     c->nslots = std::max(c->graph_steps, 64);
@@ -191,6 +209,9 @@ StepArgs step_args(tm_ctx* c, int parity, int k)
     a.partials = c->d_partials;
     a.state = c->d_state;
     a.draws = c->record_draws ? c->d_draws : nullptr;
+    a.tiles_a = c->d_tiles_a;
+    a.aux = c->d_aux;
+    a.ntiles_a = (int32_t)c->tiles_a.size();
     a.ntiles = (int32_t)c->tiles.size();
     a.nslots = c->nslots;
     a.k = k;
@@ -212,7 +233,9 @@ int grid_of(tm_ctx* c)
 void launch_step(tm_ctx* c, int parity, int k)
 {
     const StepArgs a = step_args(c, parity, k);
-    cur_variant(c).fn(a, grid_of(c), c->stream);
+    const Variant& v = cur_variant(c);
+    if (v.fn_a) v.fn_a(a, (int)c->tiles_a.size(), c->stream);   // split: P* first
+    v.fn(a, grid_of(c), c->stream);
 }
 
 void launch_fold(tm_ctx* c, int nsteps)
@@ -468,7 +491,7 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
         c->cell_off[i] = c->total_cells;
         c->total_cells += (int64_t)instances[i].L * instances[i].L;
     }
-    c->variant = default_variant(model, precision);
+    c->variant = default_variant(model, precision, c->total_cells);
     HIPCHK(c, hipMalloc(&c->d_params, c->param_size * n_params));
     HIPCHK(c, hipMemcpy(c->d_params, c->params.data(), c->param_size * n_params, hipMemcpyHostToDevice));
     const size_t fbytes = c->esize * (size_t)c->nf * c->total_cells;
@@ -673,7 +696,7 @@ tm_status tm_set_launch(tm_ctx* c, int variant, int graph_steps)
     const int n = tm_variant_count(c->model, c->prec);
     if (variant < -1 || variant >= n) return fail(c, TM_ERR_ARG, "variant out of range");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->variant = variant < 0 ? default_variant(c->model, c->prec) : variant;
+    c->variant = variant < 0 ? default_variant(c->model, c->prec, c->total_cells) : variant;
     c->graph_steps = graph_steps;
     return build_tiles(c);
 }
@@ -797,9 +820,16 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
 }  // extern "C"
 
 namespace tmk {
-int default_variant(int model, int precision)
+int default_variant(int model, int precision, int64_t total_cells)
 {
     if (model == TM_MODEL_DP) return 1;                       // 128 x 32, one tile per workgroup
-    return precision == TM_F64 ? 1 : 2;                       // 32 x 16 / 32 x 32
+    // Rietkerk: small lattices are latency-bound and want the single fused
+    // 16 x 16 launch; batches want the split step (each P* sampled once).
+    if (total_cells < (1 << 20)) return 0;
+    const int n = tm_variant_count(model, precision);
+    const char* want = precision == TM_F64 ? "rk_f64_split_32x16_t256" : "rk_f32_split_32x16_t256";
+    for (int v = 0; v < n; ++v)
+        if (std::string(tm_variant_name(model, precision, v)) == want) return v;
+    return 0;
 }
 }  // namespace tmk

@@ -64,12 +64,14 @@ struct StepArgs {
     double2* partials;   // [nslots][ntiles][nf]: (FP64 sum, count)
     DevState* state;
     int64_t* draws;      // nullable: Poisson integer per cell of the last step
+    const DevTile* tiles_a;  // split Rietkerk step: tiles of the Dornic kernel
+    void* aux;           // split Rietkerk step: T* P* field, instance block [L*L] at cell_off
     int32_t ntiles;
     int32_t nslots;
     int32_t k;           // position of this launch in its block
     int32_t n_inst;
     int32_t xcd;         // XCD-aware tile order
-    int32_t pad;
+    int32_t ntiles_a;
 };
 
 // Periodic wrap for |v| < a few L.
@@ -352,7 +354,15 @@ __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0,
 #endif
 }
 
-template <typename T, int TX, int TY, int NT, int NIMG>
+// MODE 0: the DP step.  MODE 1: the Dornic sub-step of the Rietkerk P field
+// alone (rietkerk_bjork_FKE.cpp:4154-4211), writing P* to a.aux for the RK4
+// kernel -- same staging, absorbing-group skip and sampler, on its own tiles.
+struct RKConstD;
+template <int MODE> struct StepConst;
+template <> struct StepConst<0> { typedef DPConst type; };
+template <> struct StepConst<1> { typedef RKConstD type; };
+
+template <typename T, int TX, int TY, int NT, int NIMG, int MODE>
 __global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
 {
     constexpr int V = 16 / (int)sizeof(T);
@@ -362,8 +372,10 @@ __global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
     constexpr int NW = NT / 64;                       // waves = partials per tile
     constexpr int NQ = RH * RWV * (int)(sizeof(T) / 4);
     constexpr int NBUF = ((NQ + 4 * NT - 1) / (4 * NT)) * 4 * NT;   // dwords per image
+    constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
     static_assert(NT % CG == 0 && TY % R4 == 0 && TX % 64 == 0, "tile shape");
     static_assert(NIMG == 1 || NIMG == 2, "one image (a tile per workgroup) or two (prefetch)");
+    typedef typename StepConst<MODE>::type Const;
     __shared__ uint32_t s_img[NIMG][NBUF];
 
     const int64_t counter = a.state->counter + a.k;
@@ -371,35 +383,37 @@ __global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
     const int tid = threadIdx.x;
     const int cg = tid % CG, ly0 = tid / CG;
     const T* in = (const T*)a.in;
-    T* out = (T*)a.out;
-    const DPConst* consts = (const DPConst*)a.consts + (int64_t)a.k * a.n_inst;
+    T* out = MODE == 1 ? (T*)a.aux : (T*)a.out;
+    const Const* consts = (const Const*)a.consts + (int64_t)a.k * a.n_inst;
+    const DevTile* tiles = MODE == 1 ? a.tiles_a : a.tiles;
+    const int ntiles = MODE == 1 ? a.ntiles_a : a.ntiles;
 
     // Each workgroup walks a contiguous range of tiles (ordered down vertical
     // strips, so consecutive tiles share a halo row in L2); tile t+1 streams
     // into the second LDS image while tile t is computed from the first.
-    const int t0 = (int)((int64_t)blockIdx.x * a.ntiles / gridDim.x);
-    const int t1 = (int)((int64_t)(blockIdx.x + 1) * a.ntiles / gridDim.x);
+    const int t0 = (int)((int64_t)blockIdx.x * ntiles / gridDim.x);
+    const int t1 = (int)((int64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
     if (t0 < t1) {
-        const DevTile tl = a.tiles[t0];
+        const DevTile tl = tiles[t0];
         const DevInst di = a.inst[tl.inst];
-        tile_to_lds<T, TX, RH, NT>(in + di.cell_off, di.L, tl.x0, tl.y0, s_img[0]);
+        tile_to_lds<T, TX, RH, NT>(in + NFIN * di.cell_off, di.L, tl.x0, tl.y0, s_img[0]);
     }
     for (int tile = t0; tile < t1; ++tile) {
         const int buf = NIMG == 2 ? (tile - t0) & 1 : 0;
-        const DevTile tl = a.tiles[tile];
+        const DevTile tl = tiles[tile];
         const DevInst di = a.inst[tl.inst];
         const int L = di.L;
         T* fout = out + di.cell_off;
-        const DPConst c = consts[tl.inst];
+        const Const c = consts[tl.inst];
         __builtin_amdgcn_s_waitcnt(0);   // this thread's copies of tile `tile` have landed
         __syncthreads();                 // ... everyone's; and tile-1's image is free
         if (NIMG == 2 && tile + 1 < t1) {
-            const DevTile tn = a.tiles[tile + 1];
+            const DevTile tn = tiles[tile + 1];
             const DevInst dn = a.inst[tn.inst];
-            tile_to_lds<T, TX, RH, NT>(in + dn.cell_off, dn.L, tn.x0, tn.y0, s_img[(buf ^ 1) & (NIMG - 1)]);
+            tile_to_lds<T, TX, RH, NT>(in + NFIN * dn.cell_off, dn.L, tn.x0, tn.y0,
+                                       s_img[(buf ^ 1) & (NIMG - 1)]);
         }
         const T* s_rho = (const T*)s_img[buf];
-        const T b = (T)c.b, dt = (T)c.dt, eps = (T)c.eps, neg = (T)c.neg;
         double sum[1] = {0.0};
         uint32_t nlive = 0;
         // Thread -> groups of 4 adjacent cells (cg) on rows ly0 + j * R4.  A group
@@ -441,23 +455,29 @@ __global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
                         Stream st(di.seed, di.inst_id, cell, c.step);
                         y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
                     }
-                    y = dp_nonlinear<T>(c.scheme, b, dt, y);
-                    if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
-                    if (y < eps && y >= neg) y = (T)0;
+                    if constexpr (MODE == 0) {
+                        y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
+                        if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
+                        if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
+                    }
                 }
                 fout[cell] = y;
                 if (a.draws) a.draws[di.cell_off + cell] = n;
-                sum[0] += (double)y;
-                nlive += y > (T)0 ? 1u : 0u;
+                if constexpr (MODE == 0) {
+                    sum[0] += (double)y;
+                    nlive += y > (T)0 ? 1u : 0u;
+                }
             }
         }
-        double cnt[1] = {(double)nlive};
-        wave_partials<1>(sum, cnt, a.partials + (slot * a.ntiles + tile) * NW + (tid >> 6));
+        if constexpr (MODE == 0) {
+            double cnt[1] = {(double)nlive};
+            wave_partials<1>(sum, cnt, a.partials + (slot * a.ntiles + tile) * NW + (tid >> 6));
+        }
         if (NIMG == 1 && tile + 1 < t1) {     // one image: reload it after everyone is done
             __syncthreads();
-            const DevTile tn = a.tiles[tile + 1];
+            const DevTile tn = tiles[tile + 1];
             const DevInst dn = a.inst[tn.inst];
-            tile_to_lds<T, TX, RH, NT>(in + dn.cell_off, dn.L, tn.x0, tn.y0, s_img[0]);
+            tile_to_lds<T, TX, RH, NT>(in + NFIN * dn.cell_off, dn.L, tn.x0, tn.y0, s_img[0]);
         }
     }
 }
@@ -559,7 +579,11 @@ struct Ring {
     static constexpr int N = (TX + 2 * R) * (TY + 2 * R) - TX * TY;
 };
 
-template <typename T, int TX, int TY, int NT>
+// SPLIT = false: the whole step in one kernel, the Dornic P* computed on the
+// radius-3 frame of every tile (ring cells are sampled by both neighbouring
+// tiles).  SPLIT = true: P* comes from the Dornic kernel (dp_step_kernel
+// MODE 1) through a.aux, and this kernel runs the RK4 part only.
+template <typename T, int TX, int TY, int NT, bool SPLIT>
 __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
 {
     constexpr int W4 = TX + 8, H4 = TY + 8;        // radius-4 frame (loaded fields)
@@ -569,7 +593,7 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
     constexpr int NR3 = (Ring<TX, TY, 3>::N + NT - 1) / NT;
     constexpr int NR2 = (Ring<TX, TY, 2>::N + NT - 1) / NT;
     constexpr int NR1 = (Ring<TX, TY, 1>::N + NT - 1) / NT;
-    __shared__ T sP0[H4 * W4], sW0[H4 * W4], sO0[H4 * W4];
+    __shared__ T sP0[SPLIT ? 1 : H4 * W4], sW0[H4 * W4], sO0[H4 * W4];
     __shared__ T sPs[H3 * W3];
     __shared__ T sS[3][H3 * W3];
     __shared__ double s_red[6 * (NT / 64)];
@@ -601,9 +625,17 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
             const int ry = idx / W4, rx = idx - (idx / W4) * W4;
             const int gy = wrap(tl.y0 + ry - 4, L), gx = wrap(tl.x0 + rx - 4, L);
             const int64_t g = (int64_t)gy * L + gx;
-            sP0[idx] = fin[g];
+            if constexpr (!SPLIT) sP0[idx] = fin[g];
             sW0[idx] = fin[N + g];
             sO0[idx] = fin[2 * N + g];
+        }
+        if constexpr (SPLIT) {
+            const T* pst = (const T*)a.aux + di.cell_off;
+            for (int idx = tid; idx < H3 * W3; idx += NT) {
+                const int ry = idx / W3, rx = idx - (idx / W3) * W3;
+                const int gy = wrap(tl.y0 + ry - 3, L), gx = wrap(tl.x0 + rx - 3, L);
+                sPs[idx] = pst[(int64_t)gy * L + gx];
+            }
         }
         const RKConst<T> c = rk_cast<T>(consts[tl.inst]);
         __syncthreads();
@@ -623,24 +655,26 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
             *n_out = n;
             sPs[i3(ly, lx)] = y;
         };
-        for (int j = 0; j < NOWN; ++j) {
-            int ly, lx;
-            own(j, ly, lx);
-            int64_t n;
-            dornic_at(ly, lx, &n);
-            const int gy = tl.y0 + ly, gx = tl.x0 + lx;
-            if (a.draws && gy < L && gx < L) a.draws[di.cell_off + (int64_t)gy * L + gx] = n;
-        }
-        for (int j = 0; j < NR3; ++j) {
-            const int k = tid + j * NT;
-            if (k < Ring<TX, TY, 3>::N) {
+        if constexpr (!SPLIT) {
+            for (int j = 0; j < NOWN; ++j) {
                 int ly, lx;
-                ring_cell<TX, TY, 3>(k, ly, lx);
+                own(j, ly, lx);
                 int64_t n;
                 dornic_at(ly, lx, &n);
+                const int gy = tl.y0 + ly, gx = tl.x0 + lx;
+                if (a.draws && gy < L && gx < L) a.draws[di.cell_off + (int64_t)gy * L + gx] = n;
             }
+            for (int j = 0; j < NR3; ++j) {
+                const int k = tid + j * NT;
+                if (k < Ring<TX, TY, 3>::N) {
+                    int ly, lx;
+                    ring_cell<TX, TY, 3>(k, ly, lx);
+                    int64_t n;
+                    dornic_at(ly, lx, &n);
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
 
         // ---- stage 1: K1 from (P*, W, O) on the radius-3 frame ----
         T accP[NOWN], accW[NOWN], accO[NOWN];

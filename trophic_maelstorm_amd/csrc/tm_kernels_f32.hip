@@ -6,16 +6,16 @@
 
 namespace tmk {
 
-template <typename T, int TX, int TY, int NT, int NIMG>
+template <typename T, int TX, int TY, int NT, int NIMG, int MODE = 0>
 static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, NIMG>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, NIMG, MODE>), dim3(grid), dim3(NT), 0, s, a);
 }
 
-template <typename T, int TX, int TY, int NT>
+template <typename T, int TX, int TY, int NT, bool SPLIT = false>
 static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((rk_step_kernel<T, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((rk_step_kernel<T, TX, TY, NT, SPLIT>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 static const Variant kDP[] = {
@@ -33,6 +33,13 @@ static const Variant kRK[] = {
     {32, 16, 256, 0, 1, "rk_f32_32x16_t256", &launch_rk<float, 32, 16, 256>},
     {32, 32, 256, 0, 1, "rk_f32_32x32_t256", &launch_rk<float, 32, 32, 256>},
     {64, 32, 256, 0, 1, "rk_f32_64x32_t256", &launch_rk<float, 64, 32, 256>},
+    // split step: Dornic kernel on 128 x 32 tiles, then the RK4 kernel
+    {32, 16, 256, 0, 1, "rk_f32_split_32x16_t256", &launch_rk<float, 32, 16, 256, true>, 128, 32,
+     &launch_dp<float, 128, 32, 256, 1, 1>},
+    {32, 32, 256, 0, 1, "rk_f32_split_32x32_t256", &launch_rk<float, 32, 32, 256, true>, 128, 32,
+     &launch_dp<float, 128, 32, 256, 1, 1>},
+    {64, 32, 256, 0, 1, "rk_f32_split_64x32_t256", &launch_rk<float, 64, 32, 256, true>, 128, 32,
+     &launch_dp<float, 128, 32, 256, 1, 1>},
 };
 
 const Variant* dp_variants_f32(int* n) { *n = (int)(sizeof(kDP) / sizeof(kDP[0])); return kDP; }

@@ -6,16 +6,16 @@
 
 namespace tmk {
 
-template <typename T, int TX, int TY, int NT, int NIMG>
+template <typename T, int TX, int TY, int NT, int NIMG, int MODE = 0>
 static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, NIMG>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, NIMG, MODE>), dim3(grid), dim3(NT), 0, s, a);
 }
 
-template <typename T, int TX, int TY, int NT>
+template <typename T, int TX, int TY, int NT, bool SPLIT = false>
 static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((rk_step_kernel<T, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((rk_step_kernel<T, TX, TY, NT, SPLIT>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 static const Variant kDP[] = {
@@ -32,6 +32,13 @@ static const Variant kRK[] = {
     {16, 16, 256, 0, 1, "rk_f64_16x16_t256", &launch_rk<double, 16, 16, 256>},
     {32, 16, 256, 0, 1, "rk_f64_32x16_t256", &launch_rk<double, 32, 16, 256>},
     {32, 32, 256, 0, 1, "rk_f64_32x32_t256", &launch_rk<double, 32, 32, 256>},
+    // split step: Dornic kernel on 64 x 32 tiles, then the RK4 kernel
+    {16, 16, 256, 0, 1, "rk_f64_split_16x16_t256", &launch_rk<double, 16, 16, 256, true>, 64, 32,
+     &launch_dp<double, 64, 32, 256, 1, 1>},
+    {32, 16, 256, 0, 1, "rk_f64_split_32x16_t256", &launch_rk<double, 32, 16, 256, true>, 64, 32,
+     &launch_dp<double, 64, 32, 256, 1, 1>},
+    {32, 32, 256, 0, 1, "rk_f64_split_32x32_t256", &launch_rk<double, 32, 32, 256, true>, 64, 32,
+     &launch_dp<double, 64, 32, 256, 1, 1>},
 };
 
 const Variant* dp_variants_f64(int* n) { *n = (int)(sizeof(kDP) / sizeof(kDP[0])); return kDP; }

@@ -15,6 +15,8 @@ struct Variant {
     int pw;          // rho(t) partials written per tile
     const char* name;
     StepLaunch fn;
+    int ax, ay;      // split Rietkerk step: tile shape of the Dornic kernel fn_a,
+    StepLaunch fn_a; // launched (one tile per workgroup) before fn; 0 = fused step
 };
 
 // Variant tables per (model, precision); defined in tm_kernels_f64.hip / _f32.hip.
@@ -23,8 +25,8 @@ const Variant* rk_variants_f64(int* n);
 const Variant* dp_variants_f32(int* n);
 const Variant* rk_variants_f32(int* n);
 
-// Default variant index per (model, precision).
-int default_variant(int model, int precision);
+// Default variant index per (model, precision, batch size in cells).
+int default_variant(int model, int precision, int64_t total_cells);
 
 void launch_measure_f64(int nf, const DevInst* inst, const DevTile* tiles, int ntiles, int tx, int ty,
                         const void* fields, double2* partials, hipStream_t s);

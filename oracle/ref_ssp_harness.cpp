@@ -14,6 +14,7 @@
 //   a, L, t, <<rho>>_surviving, Var, #surviving replicates, <#active>, surv_rep.
 #include <cstdio>
 #include <cstring>
+#include <iostream>
 #include <thread>
 #include <vector>
 
@@ -37,6 +38,8 @@ int ref_ssp_percolation_dornic(int g, double a, double b, double D, double sigma
     vector<double> tm(t_meas, t_meas + n_meas);
     vector<double> r0(rho0, rho0 + (size_t)g * g);
     percolationDornic_2D(rows, tm, r0.data(), t_max, a, b, D, sigma, dt, dx, r, g);
+    std::cout.flush();   // the integrator's progress lines go out while the caller's redirect holds
+    fflush(stdout);
     if ((int)rows.size() > max_rows) return -1;
     for (size_t i = 0; i < rows.size(); ++i)
         for (size_t j = 0; j < 8; ++j) out[i * 8 + j] = j < rows[i].size() ? rows[i][j] : 0.0;
@@ -59,6 +62,8 @@ int ref_ssp_parallel(int nthreads, int g, double a, double b, double D, double s
             percolationDornic_2D(rows, tm, r0.data(), t_max, a, b, D, sigma, dt, dx, r, g);
         });
     for (auto& t : th) t.join();
+    std::cout.flush();
+    fflush(stdout);
     return 0;
 }
 

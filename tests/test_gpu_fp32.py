@@ -136,3 +136,22 @@ def test_fp32_rietkerk_split_matches_fused(tm):
     rel = np.abs(f_a - f_b) / np.maximum(np.abs(f_b), 1e-30)
     assert np.quantile(rel, 0.99) < 1e-5, np.quantile(rel, 0.99)
     np.testing.assert_allclose(r_a[:, :, 1::2], r_b[:, :, 1::2], rtol=1e-5)
+
+
+def test_fp32_dp_variants_agree(tm):
+    """Every FP32 DP variant runs the same per-cell arithmetic from the same Philox
+    words, so tile shape, register budget and tile order leave results unchanged."""
+    names = tm.variant_names(tm.MODEL_DP, tm.F32)
+    insts = [tm.Instance(L, 0, 9, k) for k, L in enumerate([96, 130, 64, 50])]
+    out = []
+    for v in range(len(names)):
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_DP, tm.F32, tm.DPParams(a=0.9), insts)
+            ctx.set_launch(v, 10)
+            ctx.init_uniform([0.0], [1.0])
+            rows = ctx.run(30)
+            out.append((np.concatenate([ctx.download(k).ravel() for k in range(len(insts))]), rows))
+    f0, r0 = out[0]
+    for v, (f, r) in enumerate(out[1:], 1):
+        assert np.array_equal(f, f0), names[v]
+        np.testing.assert_allclose(r, r0, rtol=1e-12, err_msg=names[v])

@@ -79,6 +79,33 @@ def test_dp_f64_tiles_graphs_and_step0_do_not_change_results(tm, oracle):
             assert np.array_equal(got, base)
 
 
+def test_dp_odd_sizes_every_variant_bit_exact(tm, oracle):
+    """Every DP kernel variant on a batch of lattice sides that are not multiples
+    of 4 or of the tile width (per-cell stores, partial groups and tiles, wrapped
+    halos, unaligned instance offsets), Poisson integers recorded, against the
+    oracle."""
+    names = tm.variant_names(tm.MODEL_DP, tm.F64)
+    Ls = [17, 130, 30, 64, 50]
+    p = tm.DPParams(a=1.1)
+    po = oracle.dp_params(a=1.1)
+    insts = [tm.Instance(L, 0, 70 + k, k, 11 * k) for k, L in enumerate(Ls)]
+    for variant in range(len(names)):
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_DP, tm.F64, p, insts)
+            ctx.set_launch(variant, 6)
+            ctx.init_uniform([0.0], [1.0])
+            init = [ctx.download(k) for k in range(len(Ls))]
+            ctx.record_draws(True)
+            rows = ctx.run(13)
+            got = [ctx.download(k) for k in range(len(Ls))]
+            draws = [ctx.draws(k) for k in range(len(Ls))]
+        for k, d in enumerate(insts):
+            ref = oracle.dp_run(po, init[k][0], d.L, d.seed, d.instance_id, d.step0, 13, draws=True)
+            assert np.array_equal(draws[k], ref["draws"]), (names[variant], d.L)
+            assert np.array_equal(got[k], ref["fields"]), (names[variant], d.L)
+            assert_rows_close(rows[k], ref["rows"])
+
+
 def rk_case(tm, oracle, L, steps, seed=5, inst=1, sampler=0, variant=-1, graph=100, R=1.0,
             lo=(0.0, 0.0, 0.0), hi=(90.0, 1.0, 1.0)):
     p = tm.RietkerkParams(R=R, sampler=sampler)

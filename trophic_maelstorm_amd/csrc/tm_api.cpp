@@ -166,9 +166,18 @@ tm_status build_tiles(tm_ctx* c)
         di.step0 = d.step0;
         di.cell_off = c->cell_off[i];
         di.tile0 = (int32_t)c->tiles.size();
-        // vertical strips: consecutive tiles share a halo row
-        for (int x0 = 0; x0 < d.L; x0 += v.tx)
-            for (int y0 = 0; y0 < d.L; y0 += v.ty) c->tiles.push_back(DevTile{(int32_t)i, x0, y0, 0});
+        // Vertical strips: consecutive tiles share a halo row, but the
+        // workgroups in flight then read the same columns a row stride apart.
+        // Along rows: the workgroups in flight read contiguous addresses.
+        if (v.row_major) {
+            for (int y0 = 0; y0 < d.L; y0 += v.ty)
+                for (int x0 = 0; x0 < d.L; x0 += v.tx)
+                    c->tiles.push_back(DevTile{(int32_t)i, x0, y0, d.L, c->cell_off[i]});
+        } else {
+            for (int x0 = 0; x0 < d.L; x0 += v.tx)
+                for (int y0 = 0; y0 < d.L; y0 += v.ty)
+                    c->tiles.push_back(DevTile{(int32_t)i, x0, y0, d.L, c->cell_off[i]});
+        }
         di.ntiles = (int32_t)c->tiles.size() - di.tile0;
         c->dinst.push_back(di);
     }
@@ -181,7 +190,8 @@ tm_status build_tiles(tm_ctx* c)
     if (v.fn_a) {   // split step: the Dornic kernel's own tiles, and the P* field
         for (size_t i = 0; i < c->inst.size(); ++i)
             for (int x0 = 0; x0 < c->inst[i].L; x0 += v.ax)
-                for (int y0 = 0; y0 < c->inst[i].L; y0 += v.ay) c->tiles_a.push_back(DevTile{(int32_t)i, x0, y0, 0});
+                for (int y0 = 0; y0 < c->inst[i].L; y0 += v.ay)
+                    c->tiles_a.push_back(DevTile{(int32_t)i, x0, y0, c->inst[i].L, c->cell_off[i]});
         HIPCHK(c, hipMalloc((void**)&c->d_tiles_a, sizeof(DevTile) * c->tiles_a.size()));
         HIPCHK(c, hipMemcpy(c->d_tiles_a, c->tiles_a.data(), sizeof(DevTile) * c->tiles_a.size(),
                             hipMemcpyHostToDevice));
@@ -190,7 +200,8 @@ tm_status build_tiles(tm_ctx* c)
     }
     if (!c->d_inst) HIPCHK(c, hipMalloc((void**)&c->d_inst, sizeof(DevInst) * c->dinst.size()));
     HIPCHK(c, hipMemcpy(c->d_inst, c->dinst.data(), sizeof(DevInst) * c->dinst.size(), hipMemcpyHostToDevice));
-    c->nslots = std::max(c->graph_steps, 64);
+    c->nslots = 64;   // a power of two >= the graph block (kernels mask the slot index)
+    while (c->nslots < c->graph_steps) c->nslots *= 2;
     HIPCHK(c, hipMalloc((void**)&c->d_partials,
                         sizeof(double2) * (size_t)c->nslots * c->tiles.size() * v.pw * c->nf));
     free_dev(c->d_consts);
@@ -822,7 +833,7 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
 namespace tmk {
 int default_variant(int model, int precision, int64_t total_cells)
 {
-    if (model == TM_MODEL_DP) return 1;                       // 128 x 32, one tile per workgroup
+    if (model == TM_MODEL_DP) return precision == TM_F64 ? 2 : 4;   // 64 x 64 / 128 x 32 (<= 80 VGPRs)
     // Rietkerk: small lattices are latency-bound and want the single fused
     // 16 x 16 launch; batches want the split step (each P* sampled once).
     if (total_cells < (1 << 20)) return 0;

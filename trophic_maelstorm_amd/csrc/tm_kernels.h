@@ -38,8 +38,11 @@ struct DevInst {
     int32_t pad0, pad1;
 };
 
+// A tile carries its instance's side and cell offset, so a workgroup can
+// issue its field loads after a single descriptor load.
 struct DevTile {
-    int32_t inst, x0, y0, pad;
+    int32_t inst, x0, y0, L;
+    int64_t cell_off;
 };
 
 struct DevState {
@@ -263,20 +266,6 @@ __device__ __forceinline__ uint32_t as_bits(double x)
     return (uint32_t)b | (uint32_t)(b >> 32);
 }
 
-// Four +0 cells starting at p (16-byte aligned when possible, else four stores).
-__device__ __forceinline__ void store4_zero(float* p)
-{
-    if (((uintptr_t)p & 15) == 0) *(float4*)p = make_float4(0.f, 0.f, 0.f, 0.f);
-    else { p[0] = 0.f; p[1] = 0.f; p[2] = 0.f; p[3] = 0.f; }
-}
-__device__ __forceinline__ void store4_zero(double* p)
-{
-    if (((uintptr_t)p & 15) == 0) {
-        ((double2*)p)[0] = make_double2(0.0, 0.0);
-        ((double2*)p)[1] = make_double2(0.0, 0.0);
-    } else { p[0] = 0.0; p[1] = 0.0; p[2] = 0.0; p[3] = 0.0; }
-}
-
 // Wave-level (sum, count) reduction; lane 0 writes the wave's partial.
 template <int NF>
 __device__ __forceinline__ void wave_partials(double (&sum)[NF], double (&cnt)[NF], double2* dst)
@@ -354,6 +343,36 @@ __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0,
 #endif
 }
 
+template <typename T> struct Quad { T v[4]; };
+
+template <typename T>
+__device__ __forceinline__ T sel4(const Quad<T>& q, int i)
+{
+    return i == 0 ? q.v[0] : (i == 1 ? q.v[1] : (i == 2 ? q.v[2] : q.v[3]));
+}
+
+
+// Store of the group of 4 cells at row[gx0 ..]: one 16-byte store (two for
+// FP64) when the instance's rows are 16-byte aligned and the group is whole,
+// else per cell.  The per-cell form is a rolled loop on purpose: written as
+// straight-line single stores next to the 16-byte form, the two were merged by
+// the optimiser into four single-word stores on every path.
+template <typename T>
+__device__ __forceinline__ void store_quad(T* row, int gx0, int nvalid, bool vec, const Quad<T>& q)
+{
+    if (vec && nvalid == 4) {
+        if constexpr (sizeof(T) == 4) {
+            *(float4*)(row + gx0) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+        } else {
+            *(double2*)(row + gx0) = make_double2(q.v[0], q.v[1]);
+            *(double2*)(row + gx0 + 2) = make_double2(q.v[2], q.v[3]);
+        }
+    } else {
+#pragma unroll 1
+        for (int i = 0; i < nvalid; ++i) row[gx0 + i] = sel4(q, i);
+    }
+}
+
 // MODE 0: the DP step.  MODE 1: the Dornic sub-step of the Rietkerk P field
 // alone (rietkerk_bjork_FKE.cpp:4154-4211), writing P* to a.aux for the RK4
 // kernel -- same staging, absorbing-group skip and sampler, on its own tiles.
@@ -362,8 +381,8 @@ template <int MODE> struct StepConst;
 template <> struct StepConst<0> { typedef DPConst type; };
 template <> struct StepConst<1> { typedef RKConstD type; };
 
-template <typename T, int TX, int TY, int NT, int NIMG, int MODE>
-__global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
+template <typename T, int TX, int TY, int NT, int MODE, int MINW = 1>
+__global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
 {
     constexpr int V = 16 / (int)sizeof(T);
     constexpr int RWV = TX + 2 * V, RH = TY + 2;      // LDS image (see tile_to_lds)
@@ -374,12 +393,11 @@ __global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
     constexpr int NBUF = ((NQ + 4 * NT - 1) / (4 * NT)) * 4 * NT;   // dwords per image
     constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
     static_assert(NT % CG == 0 && TY % R4 == 0 && TX % 64 == 0, "tile shape");
-    static_assert(NIMG == 1 || NIMG == 2, "one image (a tile per workgroup) or two (prefetch)");
     typedef typename StepConst<MODE>::type Const;
-    __shared__ uint32_t s_img[NIMG][NBUF];
+    __shared__ uint32_t s_img[NBUF];
 
     const int64_t counter = a.state->counter + a.k;
-    const int64_t slot = counter % a.nslots;
+    const int64_t slot = counter & (int64_t)(a.nslots - 1);   // nslots: a power of two
     const int tid = threadIdx.x;
     const int cg = tid % CG, ly0 = tid / CG;
     const T* in = (const T*)a.in;
@@ -388,38 +406,17 @@ __global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
     const DevTile* tiles = MODE == 1 ? a.tiles_a : a.tiles;
     const int ntiles = MODE == 1 ? a.ntiles_a : a.ntiles;
 
-    // Each workgroup walks a contiguous range of tiles (ordered down vertical
-    // strips, so consecutive tiles share a halo row in L2); tile t+1 streams
-    // into the second LDS image while tile t is computed from the first.
-    const int t0 = (int)((int64_t)blockIdx.x * ntiles / gridDim.x);
-    const int t1 = (int)((int64_t)(blockIdx.x + 1) * ntiles / gridDim.x);
-    if (t0 < t1) {
-        const DevTile tl = tiles[t0];
-        const DevInst di = a.inst[tl.inst];
-        tile_to_lds<T, TX, RH, NT>(in + NFIN * di.cell_off, di.L, tl.x0, tl.y0, s_img[0]);
-    }
-    for (int tile = t0; tile < t1; ++tile) {
-        const int buf = NIMG == 2 ? (tile - t0) & 1 : 0;
-        const DevTile tl = tiles[tile];
-        const DevInst di = a.inst[tl.inst];
-        const int L = di.L;
-        T* fout = out + di.cell_off;
-        const Const c = consts[tl.inst];
-        __builtin_amdgcn_s_waitcnt(0);   // this thread's copies of tile `tile` have landed
-        __syncthreads();                 // ... everyone's; and tile-1's image is free
-        if (NIMG == 2 && tile + 1 < t1) {
-            const DevTile tn = tiles[tile + 1];
-            const DevInst dn = a.inst[tn.inst];
-            tile_to_lds<T, TX, RH, NT>(in + NFIN * dn.cell_off, dn.L, tn.x0, tn.y0,
-                                       s_img[(buf ^ 1) & (NIMG - 1)]);
-        }
-        const T* s_rho = (const T*)s_img[buf];
+    // One tile from its LDS image: the per-cell step on every 4-cell group of
+    // rows ly0 + j R4, then (MODE 0) the wave's rho(t) partial.
+    auto compute = [&](int tile, const DevTile& tl, const DevInst& di, const Const& c, const T* s_rho) {
+        const int L = tl.L;
+        T* fout = out + tl.cell_off;
+        const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
         double sum[1] = {0.0};
         uint32_t nlive = 0;
-        // Thread -> groups of 4 adjacent cells (cg) on rows ly0 + j * R4.  A group
-        // whose 14-value stencil is all +0 is absorbing (the reference skips each
-        // of its cells, FKE:4161-4164) and only stores zeros; other groups run
-        // the per-cell step.
+        // A group whose 14-value stencil is all +0 is absorbing (the reference
+        // skips each of its cells, FKE:4161-4164) and only stores zeros; other
+        // groups run the per-cell step.
         for (int j = 0; j < TY / R4; ++j) {
             const int ly = ly0 + j * R4;
             const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
@@ -432,54 +429,72 @@ __global__ __launch_bounds__(NT) void dp_step_kernel(StepArgs a)
             for (int q = 0; q < 4; ++q)
                 bits |= as_bits(s_rho[r + q]) | as_bits(s_rho[r - RWV + q]) | as_bits(s_rho[r + RWV + q]);
             bits |= as_bits(s_rho[r - 1]) | as_bits(s_rho[r + 4]);
+            const int nvalid = min(4, L - gx0);
+            Quad<T> yq;
+            yq.v[0] = yq.v[1] = yq.v[2] = yq.v[3] = (T)0;
             if (bits == 0 && full) {
-                store4_zero(fout + cell0);
                 if (a.draws) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) a.draws[di.cell_off + cell0 + q] = -1;
+                    for (int q = 0; q < 4; ++q) a.draws[tl.cell_off + cell0 + q] = -1;
                 }
-                continue;
-            }
-            for (int q = 0; q < 4; ++q) {
-                if (gx0 + q >= L) break;
-                const int rq = r + q;
-                const T cur = s_rho[rq];
-                const T up = s_rho[rq - RWV], dn = s_rho[rq + RWV], lf = s_rho[rq - 1], rt = s_rho[rq + 1];
-                const uint32_t cell = cell0 + q;
-                T y = (T)0;
-                int64_t n = -1;
-                if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
-                    y = cur;
-                    const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
-                    if (!(alpha_i == 0.0 && cur == (T)0)) {
-                        Stream st(di.seed, di.inst_id, cell, c.step);
-                        y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
+            } else {
+#pragma unroll 1
+                for (int q = 0; q < nvalid; ++q) {
+                    const int rq = r + q;
+                    const T cur = s_rho[rq];
+                    const T up = s_rho[rq - RWV], dn = s_rho[rq + RWV], lf = s_rho[rq - 1], rt = s_rho[rq + 1];
+                    const uint32_t cell = cell0 + q;
+                    T y = (T)0;
+                    int64_t n = -1;
+                    if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
+                        y = cur;
+                        const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
+                        if (!(alpha_i == 0.0 && cur == (T)0)) {
+                            Stream st(di.seed, di.inst_id, cell, c.step);
+                            y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
+                        }
+                        if constexpr (MODE == 0) {
+                            y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
+                            if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
+                            if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
+                        }
                     }
+                    if (a.draws) a.draws[tl.cell_off + cell] = n;
                     if constexpr (MODE == 0) {
-                        y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
-                        if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
-                        if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
+                        sum[0] += (double)y;
+                        nlive += y > (T)0 ? 1u : 0u;
                     }
-                }
-                fout[cell] = y;
-                if (a.draws) a.draws[di.cell_off + cell] = n;
-                if constexpr (MODE == 0) {
-                    sum[0] += (double)y;
-                    nlive += y > (T)0 ? 1u : 0u;
+                    yq.v[0] = q == 0 ? y : yq.v[0];
+                    yq.v[1] = q == 1 ? y : yq.v[1];
+                    yq.v[2] = q == 2 ? y : yq.v[2];
+                    yq.v[3] = q == 3 ? y : yq.v[3];
                 }
             }
+            store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
         }
         if constexpr (MODE == 0) {
-            double cnt[1] = {(double)nlive};
-            wave_partials<1>(sum, cnt, a.partials + (slot * a.ntiles + tile) * NW + (tid >> 6));
+            double2* dst = a.partials + (slot * a.ntiles + tile) * NW + (tid >> 6);
+            if (__ballot(nlive != 0 || sum[0] != 0.0) == 0) {   // an all-zero wave: no reduction
+                if ((tid & 63) == 0) *dst = make_double2(0.0, 0.0);
+            } else {
+                double cnt[1] = {(double)nlive};
+                wave_partials<1>(sum, cnt, dst);
+            }
         }
-        if (NIMG == 1 && tile + 1 < t1) {     // one image: reload it after everyone is done
-            __syncthreads();
-            const DevTile tn = tiles[tile + 1];
-            const DevInst dn = a.inst[tn.inst];
-            tile_to_lds<T, TX, RH, NT>(in + NFIN * dn.cell_off, dn.L, tn.x0, tn.y0, s_img[0]);
-        }
-    }
+    };
+
+    // One tile per workgroup.  Its descriptor, instance and constants are read
+    // before the kernel writes anything, so they are scalar loads issued
+    // together; the image copy follows at once.
+    const int tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const DevTile tl = tiles[tile];
+    const DevInst di = a.inst[tl.inst];
+    const Const c = consts[tl.inst];
+    tile_to_lds<T, TX, RH, NT>(in + NFIN * tl.cell_off, tl.L, tl.x0, tl.y0, s_img);
+    __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
+    __syncthreads();                 // ... everyone's
+    compute(tile, tl, di, c, (const T*)s_img);
 }
 
 // ---------------------------------------------------------------------------
@@ -599,7 +614,7 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
     __shared__ double s_red[6 * (NT / 64)];
 
     const int64_t counter = a.state->counter + a.k;
-    const int64_t slot = counter % a.nslots;
+    const int64_t slot = counter & (int64_t)(a.nslots - 1);   // nslots: a power of two
     const int tid = threadIdx.x;
     const T* in = (const T*)a.in;
     T* out = (T*)a.out;
@@ -952,7 +967,7 @@ __global__ __launch_bounds__(256) void combine_rows_kernel(CombineArgs a)
     int64_t slot = 0, row = 0;
     if (!a.measure) {
         const int64_t c = a.state->counter + j;   // step counter of this row (before the bump)
-        slot = c % a.nslots;
+        slot = c & (int64_t)(a.nslots - 1);
         row = c - a.state->run_base;
     }
     const double2* p = a.partials + ((slot * a.ntiles + di.tile0) * a.pw) * a.nf;

@@ -6,10 +6,10 @@
 
 namespace tmk {
 
-template <typename T, int TX, int TY, int NT, int NIMG, int MODE = 0>
+template <typename T, int TX, int TY, int NT, int MODE = 0, int MINW = 1>
 static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, NIMG, MODE>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, MODE, MINW>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 template <typename T, int TX, int TY, int NT, bool SPLIT = false>
@@ -18,28 +18,29 @@ static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
     hipLaunchKernelGGL((rk_step_kernel<T, TX, TY, NT, SPLIT>), dim3(grid), dim3(NT), 0, s, a);
 }
 
+// _w6: at most 80 VGPRs (six waves per SIMD); _rm: tiles ordered along rows.
 static const Variant kDP[] = {
-    {64, 32, 256, 0, 4, "dp_f32_64x32_t256", &launch_dp<float, 64, 32, 256, 1>},
-    {128, 32, 256, 0, 4, "dp_f32_128x32_t256", &launch_dp<float, 128, 32, 256, 1>},
-    {128, 64, 256, 0, 4, "dp_f32_128x64_t256", &launch_dp<float, 128, 64, 256, 1>},
-    {64, 64, 256, 0, 4, "dp_f32_64x64_t256", &launch_dp<float, 64, 64, 256, 1>},
-    {256, 32, 256, 0, 4, "dp_f32_256x32_t256", &launch_dp<float, 256, 32, 256, 1>},
-    {64, 32, 256, 5, 4, "dp_f32_64x32_t256_p5", &launch_dp<float, 64, 32, 256, 2>},
-    {128, 32, 256, 4, 4, "dp_f32_128x32_t256_p4", &launch_dp<float, 128, 32, 256, 2>},
+    {64, 32, 256, 0, 4, "dp_f32_64x32_t256", &launch_dp<float, 64, 32, 256>},
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256", &launch_dp<float, 128, 32, 256>},
+    {64, 64, 256, 0, 4, "dp_f32_64x64_t256", &launch_dp<float, 64, 64, 256>},
+    {256, 32, 256, 0, 4, "dp_f32_256x32_t256", &launch_dp<float, 256, 32, 256>},
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_w6", &launch_dp<float, 128, 32, 256, 0, 6>},
+    {64, 64, 256, 0, 4, "dp_f32_64x64_t256_w6", &launch_dp<float, 64, 64, 256, 0, 6>},
+    {128, 16, 256, 0, 4, "dp_f32_128x16_t256_rm", &launch_dp<float, 128, 16, 256>, 0, 0, nullptr, 1},
+    {256, 32, 256, 0, 4, "dp_f32_256x32_t256_rm", &launch_dp<float, 256, 32, 256>, 0, 0, nullptr, 1},
 };
 
 static const Variant kRK[] = {
     {16, 16, 256, 0, 1, "rk_f32_16x16_t256", &launch_rk<float, 16, 16, 256>},
     {32, 16, 256, 0, 1, "rk_f32_32x16_t256", &launch_rk<float, 32, 16, 256>},
     {32, 32, 256, 0, 1, "rk_f32_32x32_t256", &launch_rk<float, 32, 32, 256>},
-    {64, 32, 256, 0, 1, "rk_f32_64x32_t256", &launch_rk<float, 64, 32, 256>},
-    // split step: Dornic kernel on 128 x 32 tiles, then the RK4 kernel
+    // split step: Dornic kernel (dp_step_kernel MODE 1) on 128 x 32 tiles, then the RK4 kernel
     {32, 16, 256, 0, 1, "rk_f32_split_32x16_t256", &launch_rk<float, 32, 16, 256, true>, 128, 32,
-     &launch_dp<float, 128, 32, 256, 1, 1>},
+     &launch_dp<float, 128, 32, 256, 1>},
     {32, 32, 256, 0, 1, "rk_f32_split_32x32_t256", &launch_rk<float, 32, 32, 256, true>, 128, 32,
-     &launch_dp<float, 128, 32, 256, 1, 1>},
-    {64, 32, 256, 0, 1, "rk_f32_split_64x32_t256", &launch_rk<float, 64, 32, 256, true>, 128, 32,
-     &launch_dp<float, 128, 32, 256, 1, 1>},
+     &launch_dp<float, 128, 32, 256, 1>},
+    {32, 16, 256, 0, 1, "rk_f32_split_32x16_t256_w6", &launch_rk<float, 32, 16, 256, true>, 128, 32,
+     &launch_dp<float, 128, 32, 256, 1, 6>},
 };
 
 const Variant* dp_variants_f32(int* n) { *n = (int)(sizeof(kDP) / sizeof(kDP[0])); return kDP; }

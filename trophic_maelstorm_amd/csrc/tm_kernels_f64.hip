@@ -6,10 +6,10 @@
 
 namespace tmk {
 
-template <typename T, int TX, int TY, int NT, int NIMG, int MODE = 0>
+template <typename T, int TX, int TY, int NT, int MODE = 0, int MINW = 1>
 static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, NIMG, MODE>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, MODE, MINW>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 template <typename T, int TX, int TY, int NT, bool SPLIT = false>
@@ -19,26 +19,23 @@ static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
 }
 
 static const Variant kDP[] = {
-    {64, 32, 256, 0, 4, "dp_f64_64x32_t256", &launch_dp<double, 64, 32, 256, 1>},
-    {128, 32, 256, 0, 4, "dp_f64_128x32_t256", &launch_dp<double, 128, 32, 256, 1>},
-    {128, 64, 256, 0, 4, "dp_f64_128x64_t256", &launch_dp<double, 128, 64, 256, 1>},
-    {64, 64, 256, 0, 4, "dp_f64_64x64_t256", &launch_dp<double, 64, 64, 256, 1>},
-    {256, 32, 256, 0, 4, "dp_f64_256x32_t256", &launch_dp<double, 256, 32, 256, 1>},
-    {64, 32, 256, 3, 4, "dp_f64_64x32_t256_p3", &launch_dp<double, 64, 32, 256, 2>},
-    {128, 32, 256, 2, 4, "dp_f64_128x32_t256_p2", &launch_dp<double, 128, 32, 256, 2>},
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256", &launch_dp<double, 64, 32, 256>},
+    {128, 32, 256, 0, 4, "dp_f64_128x32_t256", &launch_dp<double, 128, 32, 256>},
+    {64, 64, 256, 0, 4, "dp_f64_64x64_t256", &launch_dp<double, 64, 64, 256>},
+    {256, 32, 256, 0, 4, "dp_f64_256x32_t256", &launch_dp<double, 256, 32, 256>},
 };
 
 static const Variant kRK[] = {
     {16, 16, 256, 0, 1, "rk_f64_16x16_t256", &launch_rk<double, 16, 16, 256>},
     {32, 16, 256, 0, 1, "rk_f64_32x16_t256", &launch_rk<double, 32, 16, 256>},
     {32, 32, 256, 0, 1, "rk_f64_32x32_t256", &launch_rk<double, 32, 32, 256>},
-    // split step: Dornic kernel on 64 x 32 tiles, then the RK4 kernel
+    // split step: Dornic kernel (dp_step_kernel MODE 1) on 64 x 32 tiles, then the RK4 kernel
     {16, 16, 256, 0, 1, "rk_f64_split_16x16_t256", &launch_rk<double, 16, 16, 256, true>, 64, 32,
-     &launch_dp<double, 64, 32, 256, 1, 1>},
+     &launch_dp<double, 64, 32, 256, 1>},
     {32, 16, 256, 0, 1, "rk_f64_split_32x16_t256", &launch_rk<double, 32, 16, 256, true>, 64, 32,
-     &launch_dp<double, 64, 32, 256, 1, 1>},
+     &launch_dp<double, 64, 32, 256, 1>},
     {32, 32, 256, 0, 1, "rk_f64_split_32x32_t256", &launch_rk<double, 32, 32, 256, true>, 64, 32,
-     &launch_dp<double, 64, 32, 256, 1, 1>},
+     &launch_dp<double, 64, 32, 256, 1>},
 };
 
 const Variant* dp_variants_f64(int* n) { *n = (int)(sizeof(kDP) / sizeof(kDP[0])); return kDP; }

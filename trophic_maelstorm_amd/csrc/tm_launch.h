@@ -17,6 +17,7 @@ struct Variant {
     StepLaunch fn;
     int ax, ay;      // split Rietkerk step: tile shape of the Dornic kernel fn_a,
     StepLaunch fn_a; // launched (one tile per workgroup) before fn; 0 = fused step
+    int row_major;   // tile order within an instance: 0 = down vertical strips, 1 = along rows
 };
 
 // Variant tables per (model, precision); defined in tm_kernels_f64.hip / _f32.hip.

@@ -35,12 +35,22 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s + % HBM peak: batched FP32 vs plain FP64, ρ(t) err ≤1e-3"
 HBM_PEAK_GBS = 8000.0
 SEED = 20251202
-# HBM bytes per launch of the FP32 DP step kernel on the C4 batch, from counter
-# runs (profiles/dp_step_f32_c4_quench_hbm_traffic.md): FETCH_SIZE x 2.000 and
-# WRITE_SIZE x 1.000, the factors a plain 256 MiB copy needs to read its known
-# size.  Variant dp_f32_256x32_t256.
-C4_F32_TRAFFIC = {"variant": "dp_f32_256x32_t256", "fetch_kib": 75081.6, "fetch_scale": 2.0,
-                  "write_kib": 142393.3, "write_scale": 1.0}
+# HBM bytes per launch of the FP32 DP step kernel on the C4 batch, per kernel
+# variant, from counter runs over the first 10^3 steps
+# (profiles/dp_step_f32_c4_quench_hbm_traffic.md): FETCH_SIZE x 2.000 and
+# WRITE_SIZE x 1.000 KiB, the factors a plain 256 MiB copy needs to read its
+# known size.
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+C4_F32_TRAFFIC_KIB = {   # variant: (FETCH_SIZE, WRITE_SIZE) per dispatch
+    "dp_f32_128x32_t256_w6": (75198.8, 139824.1),
+    "dp_f32_64x32_t256": (80202.1, 140385.4),
+    "dp_f32_128x32_t256": (75199.2, 139824.1),
+    "dp_f32_64x64_t256": (85723.7, 139837.5),
+    "dp_f32_256x32_t256": (74841.1, 139556.2),
+    "dp_f32_64x64_t256_w6": (85720.6, 139837.4),
+    "dp_f32_128x16_t256_rm": (110913.2, 140372.5),
+    "dp_f32_256x32_t256_rm": (89995.4, 139556.0),
+}
 
 
 def c4_batch(tm, precision):
@@ -81,20 +91,25 @@ def bytes_per_cell(b):
 
 
 def traffic_bytes(variant):
-    t = C4_F32_TRAFFIC
-    if variant != t["variant"]:
+    t = C4_F32_TRAFFIC_KIB.get(variant)
+    if t is None:
         return None
-    return int(round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024))
+    return int(round((t[0] * FETCH_SCALE + t[1] * WRITE_SCALE) * 1024))
 
 
-def run_workload(tm, b, steps, warmup, want_rows=True, tune=True):
+def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=None):
     """Warm up, then time `steps` steps of the real trajectory from t = 0: wall
     clock around the whole job, and the device time of every step launch (HIP
-    events on the context stream inside each run() call, summed over chunks)."""
+    events on the context stream inside each run() call, summed over chunks).
+    The launch shape comes from the autotuner, or is the named variant."""
     with tm.Context(0) as ctx:
         ctx.configure(b["model"], b["precision"], b["params"], b["inst"])
         variant = None
-        if tune:
+        if variant_name:
+            v = tm.variant_names(b["model"], b["precision"]).index(variant_name)
+            ctx.set_launch(v, 100)
+            variant = (v, 100)
+        elif tune:
             variant = ctx.autotune(os.environ.get("TM_TUNE_CACHE"))
         ctx.init_uniform(b["lo"], b["hi"])
         if warmup:
@@ -216,6 +231,8 @@ def main():
                     help="also time the 8192^2 Rietkerk lattice for this many steps (its full run is 10^5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-side", action="store_true", help="headline only")
+    ap.add_argument("--variant", default=None,
+                    help="kernel variant for the headline run instead of the autotuner's choice")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -243,7 +260,7 @@ def main():
     c4 = c4_batch(tm, tm.F32)
     K = args.steps or c4["steps"]
     log(f"[bench] C4 FP32 batched DP quench: {cells_of(c4)} cells/step x {K} steps")
-    h = run_workload(tm, c4, K, args.warmup)
+    h = run_workload(tm, c4, K, args.warmup, variant_name=args.variant)
     log(f"[bench]   {h['value']:.4g} cell-updates/s, kernel {h['kernel_ms']:.4f} ms, "
         f"{h['achieved_gbs']:.0f} GB/s ({100 * h['frac']:.1f}% of peak)")
     if args.headline_only:

@@ -77,12 +77,12 @@ struct StepArgs {
     int32_t ntiles_a;
 };
 
-// Periodic wrap for |v| < a few L.
+// Periodic index: v mod L in [0, L).
 __device__ __forceinline__ int wrap(int v, int L)
 {
-    while (v < 0) v += L;
-    while (v >= L) v -= L;
-    return v;
+    if (v >= -L && v < 2 * L) return v < 0 ? v + L : (v >= L ? v - L : v);   // the usual case
+    const int m = v % L;   // lattices smaller than a tile plus its halo
+    return m < 0 ? m + L : m;
 }
 
 // blockIdx -> tile.  Workgroups are dealt round-robin to the 8 XCDs; give each

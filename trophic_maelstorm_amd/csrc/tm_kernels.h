@@ -287,17 +287,25 @@ __device__ __forceinline__ void wave_partials(double (&sum)[NF], double (&cnt)[N
 // Asynchronous copy of a tile and its halo into LDS with global_load_lds: each
 // wave instruction lands 64 lanes' data at consecutive LDS addresses, so the
 // staging costs no registers and overlaps whatever the workgroup does until
-// its next s_waitcnt.  The LDS image is row-major, RH rows of RWV = TX + 2V
-// elements covering columns [x0 - V, x0 + TX + V), V = 16 / sizeof(T), rows
-// [y0 - 1, y0 + TY + 1), periodic.  When L is a multiple of V every 16-byte
-// chunk of a row is contiguous in memory and one global_load_lds_dwordx4 moves
-// it; otherwise the image is filled dword by dword.
-template <typename T, int TX, int RH, int NT>
+// its next s_waitcnt.  The LDS image is row-major: RH rows [y0 - H, ...) of
+// RWV = TX + 2 HC elements covering columns [x0 - HC, x0 + TX + HC), periodic,
+// where HC is the halo H rounded up to whole 16-byte chunks of V = 16 / sizeof(T)
+// elements.  When L is a multiple of V every 16-byte chunk of a row is
+// contiguous in memory and one global_load_lds_dwordx4 moves it; otherwise the
+// image is filled dword by dword.
+template <typename T, int H>
+struct LdsFrame {
+    static constexpr int V = 16 / (int)sizeof(T);
+    static constexpr int HC = ((H + V - 1) / V) * V;
+};
+
+template <typename T, int TX, int RH, int NT, int H = 1>
 __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0, uint32_t* lds)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    constexpr int V = 16 / (int)sizeof(T);
-    constexpr int RWV = TX + 2 * V;
+    constexpr int V = LdsFrame<T, H>::V;
+    constexpr int HC = LdsFrame<T, H>::HC;
+    constexpr int RWV = TX + 2 * HC;
     const int tid = threadIdx.x;
     const int wave_base = tid & ~63;
     if (L % V == 0) {
@@ -311,12 +319,12 @@ __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0,
             uint32_t off = 0;
             if (q < NC) {
                 const int ry = q / CPR, cx = q - (q / CPR) * CPR;
-                int gy = y0 + ry - 1, gx = x0 - V + cx * V;
+                int gy = y0 + ry - H, gx = x0 - HC + cx * V;
                 gy = gy < 0 ? gy + L : (gy >= L ? gy - L : gy);   // one wrap: L >= RH, L >= RWV
                 gx = gx < 0 ? gx + L : (gx >= L ? gx - L : gx);
                 if (L < RWV || L < RH) {
-                    gy = wrap(y0 + ry - 1, L);
-                    gx = wrap(x0 - V + cx * V, L);
+                    gy = wrap(y0 + ry - H, L);
+                    gx = wrap(x0 - HC + cx * V, L);
                 }
                 off = (uint32_t)gy * (uint32_t)L + (uint32_t)gx;
             }
@@ -334,7 +342,7 @@ __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0,
             if (q < NQ) {
                 const int e = q / DW, w = q - (q / DW) * DW;
                 const int ry = e / RWV, rx = e - (e / RWV) * RWV;
-                const int gy = wrap(y0 + ry - 1, L), gx = wrap(x0 - V + rx, L);
+                const int gy = wrap(y0 + ry - H, L), gx = wrap(x0 - HC + rx, L);
                 off = ((uint32_t)gy * (uint32_t)L + (uint32_t)gx) * DW + w;
             }
             __builtin_amdgcn_global_load_lds(src0 + off, lds + j * NT + wave_base, 4, 0, 0);
@@ -608,10 +616,16 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
     constexpr int NR3 = (Ring<TX, TY, 3>::N + NT - 1) / NT;
     constexpr int NR2 = (Ring<TX, TY, 2>::N + NT - 1) / NT;
     constexpr int NR1 = (Ring<TX, TY, 1>::N + NT - 1) / NT;
-    __shared__ T sP0[SPLIT ? 1 : H4 * W4], sW0[H4 * W4], sO0[H4 * W4];
-    __shared__ T sPs[H3 * W3];
+    static_assert(LdsFrame<T, 4>::HC == 4, "radius-4 frames are W4 = TX + 8 wide");
+    constexpr int NBUF4 = ((H4 * W4 * (int)sizeof(T) / 4 + 4 * NT - 1) / (4 * NT)) * 4 * NT;
+    // radius-4 frames of P (P* for the split step), W and O, staged by DMA
+    __shared__ uint32_t s_fr[3][NBUF4];
+    __shared__ T sPs[SPLIT ? 1 : H3 * W3];   // fused step: P* on the radius-3 frame
     __shared__ T sS[3][H3 * W3];
     __shared__ double s_red[6 * (NT / 64)];
+    T* const sP0 = (T*)s_fr[0];
+    T* const sW0 = (T*)s_fr[1];
+    T* const sO0 = (T*)s_fr[2];
 
     const int64_t counter = a.state->counter + a.k;
     const int64_t slot = counter & (int64_t)(a.nslots - 1);   // nslots: a power of two
@@ -622,38 +636,32 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
 
     auto i4 = [](int ly, int lx) { return (ly + 4) * W4 + (lx + 4); };
     auto i3 = [](int ly, int lx) { return (ly + 3) * W3 + (lx + 3); };
+    auto ps = [&](int ly, int lx) { return SPLIT ? sP0[i4(ly, lx)] : sPs[i3(ly, lx)]; };   // P*
     auto own = [&](int j, int& ly, int& lx) {
         const int k = tid + j * NT;
         ly = k / TX;
         lx = k % TX;
     };
 
-    for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-        const int tile = tile_of(t, a.ntiles, a.xcd);
+    {   // one tile per workgroup; its descriptor, instance and constants are
+        // read before the kernel writes anything (scalar loads), then the
+        // three frames are staged by DMA
+        if ((int)blockIdx.x >= a.ntiles) return;
+        const int tile = tile_of(blockIdx.x, a.ntiles, a.xcd);
         const DevTile tl = a.tiles[tile];
         const DevInst di = a.inst[tl.inst];
-        const int L = di.L;
+        const RKConstD cd = consts[tl.inst];
+        const int L = tl.L;
         const int64_t N = (int64_t)L * L;
-        const T* fin = in + 3 * di.cell_off;
-        T* fout = out + 3 * di.cell_off;
-        for (int idx = tid; idx < H4 * W4; idx += NT) {
-            const int ry = idx / W4, rx = idx - (idx / W4) * W4;
-            const int gy = wrap(tl.y0 + ry - 4, L), gx = wrap(tl.x0 + rx - 4, L);
-            const int64_t g = (int64_t)gy * L + gx;
-            if constexpr (!SPLIT) sP0[idx] = fin[g];
-            sW0[idx] = fin[N + g];
-            sO0[idx] = fin[2 * N + g];
-        }
-        if constexpr (SPLIT) {
-            const T* pst = (const T*)a.aux + di.cell_off;
-            for (int idx = tid; idx < H3 * W3; idx += NT) {
-                const int ry = idx / W3, rx = idx - (idx / W3) * W3;
-                const int gy = wrap(tl.y0 + ry - 3, L), gx = wrap(tl.x0 + rx - 3, L);
-                sPs[idx] = pst[(int64_t)gy * L + gx];
-            }
-        }
-        const RKConst<T> c = rk_cast<T>(consts[tl.inst]);
-        __syncthreads();
+        const T* fin = in + 3 * tl.cell_off;
+        T* fout = out + 3 * tl.cell_off;
+        const T* p_src = SPLIT ? (const T*)a.aux + tl.cell_off : fin;
+        tile_to_lds<T, TX, H4, NT, 4>(p_src, L, tl.x0, tl.y0, s_fr[0]);
+        tile_to_lds<T, TX, H4, NT, 4>(fin + N, L, tl.x0, tl.y0, s_fr[1]);
+        tile_to_lds<T, TX, H4, NT, 4>(fin + 2 * N, L, tl.x0, tl.y0, s_fr[2]);
+        const RKConst<T> c = rk_cast<T>(cd);
+        __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
+        __syncthreads();                 // ... everyone's
 
         // ---- stage 0: Dornic P* on the radius-3 frame (FKE:4154-4211) ----
         auto dornic_at = [&](int ly, int lx, int64_t* n_out) {
@@ -695,7 +703,7 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         T accP[NOWN], accW[NOWN], accO[NOWN];
         auto stage1 = [&](int ly, int lx, T& kP, T& kW, T& kO) {
             const int r4 = i4(ly, lx);
-            const T P = sPs[i3(ly, lx)], W = sW0[r4], O = sO0[r4];
+            const T P = ps(ly, lx), W = sW0[r4], O = sO0[r4];
             const T nW = sW0[r4 - W4] + sW0[r4 + W4] + sW0[r4 - 1] + sW0[r4 + 1];
             const T nO = sO0[r4 - W4] + sO0[r4 + W4] + sO0[r4 - 1] + sO0[r4 + 1];
             rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
@@ -733,7 +741,7 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         };
         auto ybase = [&](int ly, int lx, T& P, T& W, T& O) {
             const int r4 = i4(ly, lx);
-            P = sPs[i3(ly, lx)];
+            P = ps(ly, lx);
             W = sW0[r4];
             O = sO0[r4];
         };

@@ -389,120 +389,131 @@ template <int MODE> struct StepConst;
 template <> struct StepConst<0> { typedef DPConst type; };
 template <> struct StepConst<1> { typedef RKConstD type; };
 
+template <typename T, int TX, int TY, int NT>
+struct DPShape {
+    static constexpr int V = 16 / (int)sizeof(T);
+    static constexpr int RWV = TX + 2 * V, RH = TY + 2;      // LDS image (see tile_to_lds)
+    static constexpr int CG = TX / 4;                        // 4-cell groups per tile row
+    static constexpr int R4 = NT / CG;                       // tile rows covered per pass
+    static constexpr int G = TY / R4;                        // groups per thread per tile
+    static constexpr int NW = NT / 64;                       // waves = partials per tile
+    static constexpr int NQ = RH * RWV * (int)(sizeof(T) / 4);
+    static constexpr int NBUF = ((NQ + 4 * NT - 1) / (4 * NT)) * 4 * NT;   // dwords per image
+    static_assert(NT % CG == 0 && TY % R4 == 0 && TX % 64 == 0, "tile shape");
+};
+
+// One tile from its LDS image: the per-cell step on each of this thread's G
+// 4-cell groups (rows ly0 + j R4), then (MODE 0) the wave's rho(t) partial.
+template <typename T, int TX, int TY, int NT, int MODE>
+__device__ __forceinline__ void dp_tile(const StepArgs& a, int tile, const DevTile& tl, const DevInst& di,
+                                        const typename StepConst<MODE>::type& c, const T* s_rho,
+                                        int64_t counter, int64_t slot)
+{
+    typedef DPShape<T, TX, TY, NT> S;
+    constexpr int V = S::V, RWV = S::RWV, R4 = S::R4;
+    const int tid = threadIdx.x;
+    const int cg = tid % S::CG, ly0 = tid / S::CG;
+    const int L = tl.L;
+    T* fout = (MODE == 1 ? (T*)a.aux : (T*)a.out) + tl.cell_off;
+    const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
+    double sum[1] = {0.0};
+    uint32_t nlive = 0;
+    // A group whose 14-value stencil is all +0 is absorbing (the reference
+    // skips each of its cells, FKE:4161-4164) and only stores zeros; other
+    // groups run the per-cell step.
+    for (int j = 0; j < S::G; ++j) {
+        const int ly = ly0 + j * R4;
+        const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
+        if (gy >= L || gx0 >= L) continue;
+        const int r = (ly + 1) * RWV + V + 4 * cg;
+        const uint32_t cell0 = (uint32_t)gy * (uint32_t)L + (uint32_t)gx0;
+        const bool full = gx0 + 4 <= L;
+        uint32_t bits = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            bits |= as_bits(s_rho[r + q]) | as_bits(s_rho[r - RWV + q]) | as_bits(s_rho[r + RWV + q]);
+        bits |= as_bits(s_rho[r - 1]) | as_bits(s_rho[r + 4]);
+        const int nvalid = min(4, L - gx0);
+        Quad<T> yq;
+        yq.v[0] = yq.v[1] = yq.v[2] = yq.v[3] = (T)0;
+        if (bits == 0 && full) {
+            if (a.draws) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a.draws[tl.cell_off + cell0 + q] = -1;
+            }
+        } else {
+#pragma unroll 1
+            for (int q = 0; q < nvalid; ++q) {
+                const int rq = r + q;
+                const T cur = s_rho[rq];
+                const T up = s_rho[rq - RWV], dn = s_rho[rq + RWV], lf = s_rho[rq - 1], rt = s_rho[rq + 1];
+                const uint32_t cell = cell0 + q;
+                T y = (T)0;
+                int64_t n = -1;
+                if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
+                    y = cur;
+                    const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
+                    if (!(alpha_i == 0.0 && cur == (T)0)) {
+                        Stream st(di.seed, di.inst_id, cell, c.step);
+                        y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
+                    }
+                    if constexpr (MODE == 0) {
+                        y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
+                        if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
+                        if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
+                    }
+                }
+                if (a.draws) a.draws[tl.cell_off + cell] = n;
+                if constexpr (MODE == 0) {
+                    sum[0] += (double)y;
+                    nlive += y > (T)0 ? 1u : 0u;
+                }
+                yq.v[0] = q == 0 ? y : yq.v[0];
+                yq.v[1] = q == 1 ? y : yq.v[1];
+                yq.v[2] = q == 2 ? y : yq.v[2];
+                yq.v[3] = q == 3 ? y : yq.v[3];
+            }
+        }
+        store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
+    }
+    if constexpr (MODE == 0) {
+        double2* dst = a.partials + (slot * a.ntiles + tile) * S::NW + (tid >> 6);
+        if (__ballot(nlive != 0 || sum[0] != 0.0) == 0) {   // an all-zero wave: no reduction
+            if ((tid & 63) == 0) *dst = make_double2(0.0, 0.0);
+        } else {
+            double cnt[1] = {(double)nlive};
+            wave_partials<1>(sum, cnt, dst);
+        }
+    }
+}
+
+// MODE 0: the DP step.  MODE 1: the Dornic sub-step of the Rietkerk P field
+// alone (rietkerk_bjork_FKE.cpp:4154-4211), writing P* to a.aux for the RK4
+// kernel -- same staging, absorbing-group skip and sampler, on its own tiles.
+// One tile per workgroup: its descriptor, instance and constants are read
+// before the kernel writes anything, so they are scalar loads issued
+// together, and the image copy follows at once.
 template <typename T, int TX, int TY, int NT, int MODE, int MINW = 1>
 __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
 {
-    constexpr int V = 16 / (int)sizeof(T);
-    constexpr int RWV = TX + 2 * V, RH = TY + 2;      // LDS image (see tile_to_lds)
-    constexpr int CG = TX / 4;                        // 4-cell groups per tile row
-    constexpr int R4 = NT / CG;                       // tile rows covered per pass
-    constexpr int NW = NT / 64;                       // waves = partials per tile
-    constexpr int NQ = RH * RWV * (int)(sizeof(T) / 4);
-    constexpr int NBUF = ((NQ + 4 * NT - 1) / (4 * NT)) * 4 * NT;   // dwords per image
-    constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
-    static_assert(NT % CG == 0 && TY % R4 == 0 && TX % 64 == 0, "tile shape");
+    typedef DPShape<T, TX, TY, NT> S;
     typedef typename StepConst<MODE>::type Const;
-    __shared__ uint32_t s_img[NBUF];
+    constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
+    __shared__ uint32_t s_img[S::NBUF];
 
     const int64_t counter = a.state->counter + a.k;
     const int64_t slot = counter & (int64_t)(a.nslots - 1);   // nslots: a power of two
-    const int tid = threadIdx.x;
-    const int cg = tid % CG, ly0 = tid / CG;
-    const T* in = (const T*)a.in;
-    T* out = MODE == 1 ? (T*)a.aux : (T*)a.out;
-    const Const* consts = (const Const*)a.consts + (int64_t)a.k * a.n_inst;
     const DevTile* tiles = MODE == 1 ? a.tiles_a : a.tiles;
     const int ntiles = MODE == 1 ? a.ntiles_a : a.ntiles;
-
-    // One tile from its LDS image: the per-cell step on every 4-cell group of
-    // rows ly0 + j R4, then (MODE 0) the wave's rho(t) partial.
-    auto compute = [&](int tile, const DevTile& tl, const DevInst& di, const Const& c, const T* s_rho) {
-        const int L = tl.L;
-        T* fout = out + tl.cell_off;
-        const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
-        double sum[1] = {0.0};
-        uint32_t nlive = 0;
-        // A group whose 14-value stencil is all +0 is absorbing (the reference
-        // skips each of its cells, FKE:4161-4164) and only stores zeros; other
-        // groups run the per-cell step.
-        for (int j = 0; j < TY / R4; ++j) {
-            const int ly = ly0 + j * R4;
-            const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
-            if (gy >= L || gx0 >= L) continue;
-            const int r = (ly + 1) * RWV + V + 4 * cg;
-            const uint32_t cell0 = (uint32_t)gy * (uint32_t)L + (uint32_t)gx0;
-            const bool full = gx0 + 4 <= L;
-            uint32_t bits = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                bits |= as_bits(s_rho[r + q]) | as_bits(s_rho[r - RWV + q]) | as_bits(s_rho[r + RWV + q]);
-            bits |= as_bits(s_rho[r - 1]) | as_bits(s_rho[r + 4]);
-            const int nvalid = min(4, L - gx0);
-            Quad<T> yq;
-            yq.v[0] = yq.v[1] = yq.v[2] = yq.v[3] = (T)0;
-            if (bits == 0 && full) {
-                if (a.draws) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) a.draws[tl.cell_off + cell0 + q] = -1;
-                }
-            } else {
-#pragma unroll 1
-                for (int q = 0; q < nvalid; ++q) {
-                    const int rq = r + q;
-                    const T cur = s_rho[rq];
-                    const T up = s_rho[rq - RWV], dn = s_rho[rq + RWV], lf = s_rho[rq - 1], rt = s_rho[rq + 1];
-                    const uint32_t cell = cell0 + q;
-                    T y = (T)0;
-                    int64_t n = -1;
-                    if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
-                        y = cur;
-                        const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
-                        if (!(alpha_i == 0.0 && cur == (T)0)) {
-                            Stream st(di.seed, di.inst_id, cell, c.step);
-                            y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
-                        }
-                        if constexpr (MODE == 0) {
-                            y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
-                            if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
-                            if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
-                        }
-                    }
-                    if (a.draws) a.draws[tl.cell_off + cell] = n;
-                    if constexpr (MODE == 0) {
-                        sum[0] += (double)y;
-                        nlive += y > (T)0 ? 1u : 0u;
-                    }
-                    yq.v[0] = q == 0 ? y : yq.v[0];
-                    yq.v[1] = q == 1 ? y : yq.v[1];
-                    yq.v[2] = q == 2 ? y : yq.v[2];
-                    yq.v[3] = q == 3 ? y : yq.v[3];
-                }
-            }
-            store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
-        }
-        if constexpr (MODE == 0) {
-            double2* dst = a.partials + (slot * a.ntiles + tile) * NW + (tid >> 6);
-            if (__ballot(nlive != 0 || sum[0] != 0.0) == 0) {   // an all-zero wave: no reduction
-                if ((tid & 63) == 0) *dst = make_double2(0.0, 0.0);
-            } else {
-                double cnt[1] = {(double)nlive};
-                wave_partials<1>(sum, cnt, dst);
-            }
-        }
-    };
-
-    // One tile per workgroup.  Its descriptor, instance and constants are read
-    // before the kernel writes anything, so they are scalar loads issued
-    // together; the image copy follows at once.
     const int tile = blockIdx.x;
     if (tile >= ntiles) return;
     const DevTile tl = tiles[tile];
     const DevInst di = a.inst[tl.inst];
-    const Const c = consts[tl.inst];
-    tile_to_lds<T, TX, RH, NT>(in + NFIN * tl.cell_off, tl.L, tl.x0, tl.y0, s_img);
+    const Const c = ((const Const*)a.consts)[(int64_t)a.k * a.n_inst + tl.inst];
+    tile_to_lds<T, TX, S::RH, NT>((const T*)a.in + NFIN * tl.cell_off, tl.L, tl.x0, tl.y0, s_img);
     __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
     __syncthreads();                 // ... everyone's
-    compute(tile, tl, di, c, (const T*)s_img);
+    dp_tile<T, TX, TY, NT, MODE>(a, tile, tl, di, c, (const T*)s_img, counter, slot);
 }
 
 // ---------------------------------------------------------------------------

@@ -232,7 +232,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-side", action="store_true", help="headline only")
     ap.add_argument("--variant", default=None,
-                    help="kernel variant for the headline run instead of the autotuner's choice")
+                    help="kernel variant for the headline (or --only) run instead of the autotuner's choice")
+    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c5"], default=None,
+                    help="only this side workload, --steps steps (default: its full length), one JSON line")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -256,6 +258,18 @@ def main():
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
+
+    if args.only:
+        b = {"c2": lambda: c2_batch(tm, tm.F64), "c3": lambda: c3_batch(tm, tm.F32),
+             "c3-f64": lambda: c3_batch(tm, tm.F64), "c5": lambda: c5_batch(tm, tm.F32)}[args.only]()
+        n = args.steps or b["steps"]
+        log(f"[bench] {args.only}: {cells_of(b)} cells/step x {n} steps")
+        r = run_workload(tm, b, n, args.warmup, variant_name=args.variant)
+        print(json.dumps({"workload": args.only, "value": r["value"], "unit": "cell-updates/s", "steps": n,
+                          "ms_per_step": r["ms_per_step"], "step_device_ms": r["kernel_ms"],
+                          "achieved_gbs": r["achieved_gbs"], "frac": r["frac"], "variant": r["variant"]}),
+              flush=True)
+        return
 
     c4 = c4_batch(tm, tm.F32)
     K = args.steps or c4["steps"]

@@ -109,9 +109,9 @@ def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=N
             v = tm.variant_names(b["model"], b["precision"]).index(variant_name)
             ctx.set_launch(v, 100)
             variant = (v, 100)
-        elif tune:
-            variant = ctx.autotune(os.environ.get("TM_TUNE_CACHE"))
         ctx.init_uniform(b["lo"], b["hi"])
+        if not variant_name and tune:   # tuned on the initial state of the run
+            variant = ctx.autotune(os.environ.get("TM_TUNE_CACHE"))
         if warmup:
             ctx.run(warmup, rows=want_rows)
         ctx.reset()

@@ -168,10 +168,13 @@ int tm_variant_count(int model, int precision);
 const char* tm_variant_name(int model, int precision, int variant);
 
 /* Autotuner: time each kernel variant (and graph block sizes) on the configured
- * batch without advancing it, pick the fastest, and persist the choice in
- * cache_path (NULL = $TM_TUNE_CACHE or ~/.cache/trophic_maelstorm_amd/tune.txt),
- * keyed by (device arch, model, precision, batch shape).  A cached choice is
- * reused without timing.  out_variant / out_graph_steps (nullable) get the choice. */
+ * batch and its current fields without advancing it, and pick the fastest; the
+ * variant in place when tuning starts is kept unless another is at least 5%
+ * faster.  The choice persists in cache_path (NULL = $TM_TUNE_CACHE or
+ * ~/.cache/trophic_maelstorm_amd/tune.txt), keyed by (device arch, model,
+ * precision, batch shape); a cached choice is reused without timing.  Tune on a
+ * representative state (e.g. the initial fields of the run).
+ * out_variant / out_graph_steps (nullable) get the choice. */
 tm_status tm_autotune(tm_ctx* ctx, const char* cache_path, int* out_variant, int* out_graph_steps);
 
 /* Timing aid: runs n_steps steps (as tm_run, rows discarded) and reports the

@@ -793,8 +793,11 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    int best_v = c->variant;
-    double best = 1e30;
+    // The variant in place when tuning starts (the default unless the caller
+    // chose one) is kept unless another is at least 5% faster on this state.
+    const int start_v = c->variant;
+    int best_v = start_v;
+    double best = 1e30, start_time = 1e30;
     int reps = 6;
     for (int v = 0; v < nv; ++v) {
         c->variant = v;
@@ -815,10 +818,15 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
             vbest = std::min(vbest, (double)ms / reps);
             if (v == 0 && trial == 0 && ms < 1.0f) reps = std::min(200, (int)(reps * 1.0 / std::max(ms, 1e-3f)) + 1);
         }
+        if (v == start_v) start_time = vbest;
         if (vbest < best) {
             best = vbest;
             best_v = v;
         }
+    }
+    if (best > 0.95 * start_time) {
+        best_v = start_v;
+        best = start_time;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);

@@ -191,6 +191,63 @@ tm_status tm_last_run_time(const tm_ctx* ctx, double* ms, int64_t* n_steps);
 /* The context's HIP stream (hipStream_t) for callers that order their own work. */
 void* tm_stream(tm_ctx* ctx);
 
+/* ---- Sweep measurement and replicate statistics ----------------------------
+ * The reference's sweep (first_order_critical_exp_delta_stochastic_MultiSp,
+ * rietkerk_bjork_FKE.cpp:4665-4887 / multiSPDP.cpp:3456-3702) runs one OpenMP
+ * thread per parameter value and, inside it, r replicates one after another,
+ * logging (#occupied, mean) per field at the times t_meas and folding each
+ * replicate into running statistics over the replicates still alive.  Here the
+ * whole sweep is one batch: instance k is replicate j of parameter point
+ * params_index, where j counts the earlier instances with the same
+ * params_index (instance order is replicate order). */
+
+/* Host only (no device needed).  The measurement times of the reference:
+ * logarithm10_time_bins(t_max, dt) (rietkerk_bjork_FKE.cpp:1497-1536,
+ * multiSPDP.cpp:1251-1287): 10^(0.04 k) rounded down to the dt grid, kept above
+ * 1, duplicates and points closer than dt/2 removed.  Writes at most cap values
+ * to out (nullable) and returns how many there are (0 when t_max <= 1). */
+int tm_log10_time_bins(double t_max, double dt, double* out, int cap);
+
+/* Host only.  The step numbers at which the reference's time loop logs each
+ * t_meas[i]: t accumulates t += dt from 0 and the state after n steps is logged
+ * for the current index when t == 0 or t_meas[index] - dt/2 <= t < t_meas[index]
+ * + dt/2 (rietkerk_bjork_FKE.cpp:4005-4017,4467 / multiSPDP.cpp:2640-2650,3264),
+ * while t < t_meas[n_meas-1] + dt.  Returns how many points were reached (a
+ * point off the dt grid stalls the rest, as in the reference). */
+int tm_measure_steps(const double* t_meas, int n_meas, double dt, int64_t* steps);
+
+/* Advance every instance to step steps[n_meas-1] (counted from now) as tm_run
+ * would, and record each instance's rho(t) row (tm_run's row format) at each
+ * steps[i] into a device series [n_inst][n_meas][2*nfields] kept by the context
+ * for tm_replicate_stats; steps[i] == 0 records the current state.  steps must
+ * be non-decreasing and >= 0.  series (nullable, host) gets a copy.  Replaces
+ * the per-replicate Rho_M logging of rietkerk_bjork_FKE.cpp:4017-4031. */
+tm_status tm_run_series(tm_ctx* ctx, const int64_t* steps, int n_meas, double* series);
+
+/* Flags of tm_replicate_stats. */
+#define TM_STATS_INTDIV_VARIANCE 1     /* the reference's (1/(r-1)) in integer arithmetic
+                                        * (rietkerk_bjork_FKE.cpp:1351, multiSPDP.cpp:1112):
+                                        * the old-variance term drops for r >= 3          */
+#define TM_STATS_STOP_AT_EXTINCTION 2  /* a replicate's rows after its first measurement
+                                        * with no occupied cell of field 0 read as zero: the
+                                        * reference ends the replicate there
+                                        * (rietkerk_bjork_FKE.cpp:3086,3135,4068-4072)    */
+#define TM_STATS_REFERENCE 3           /* both: the reference's numbers                   */
+
+/* Across-replicate statistics of the last tm_run_series, on the device, one
+ * row per (parameter point, measurement): stats [n_params][n_meas][nfields+3] =
+ *   means over all replicates of fields 1..nfields-1 (W, O),
+ *   then for field 0: mean over surviving replicates of the spatial mean, its
+ *   variance over them, the number of surviving replicates, and their mean
+ *   number of occupied cells
+ * -- rho_rep_avg_var columns 1..(nfields+2) of rietkerk_bjork_FKE.cpp:4578-4602
+ * (first replicate) and var_mean_incremental_surv_runs (rietkerk_bjork_FKE.cpp:
+ * 1319-1369, multiSPDP.cpp:1080-1125; later replicates, in order).  A replicate
+ * survives at a measurement when field 0 is occupied there and at every earlier
+ * one.  Without TM_STATS_INTDIV_VARIANCE the variance is the unbiased sample
+ * variance.  Parameter points with no instance get zero rows. */
+tm_status tm_replicate_stats(tm_ctx* ctx, int flags, double* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,9 @@ def lib():
         L.tmo_reduce.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
         L.tmo_flush.restype = C.c_double
         L.tmo_flush.argtypes = [C.c_double] * 3
+        L.tmo_replicate_stats.argtypes = [_f64p, C.c_int, C.c_int, C.c_int,
+                                          np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
+                                          C.c_int, C.c_int, _f64p]
         _lib = L
     return _lib
 
@@ -108,6 +111,10 @@ def ref_lib():
         R.ref_ssp_parallel.restype = C.c_int
         R.ref_ssp_parallel.argtypes = [C.c_int, C.c_int] + [C.c_double] * 6 + [
             C.c_int, _f64p, _f64p, C.c_int, C.c_double]
+        R.ref_ssp_var_mean_surv.restype = None
+        R.ref_ssp_var_mean_surv.argtypes = [_f64p, _f64p, C.c_int]
+        R.ref_ssp_log10_time_bins.restype = C.c_int
+        R.ref_ssp_log10_time_bins.argtypes = [C.c_double, C.c_double, C.c_void_p, C.c_int]
         _ref = R
     return _ref
 
@@ -219,3 +226,31 @@ def reduce(fields):
     row = np.empty(2 * nf, np.float64)
     lib().tmo_reduce(L, nf, f.reshape(-1), row)
     return row
+
+
+def replicate_stats(series, params_index, n_params, flags):
+    """Across-replicate statistics (tmo_replicate_stats): [n_params, n_meas, nf+3]."""
+    x = np.ascontiguousarray(series, np.float64)
+    n_inst, n_meas, w = x.shape
+    pi = np.ascontiguousarray(params_index, np.int32)
+    out = np.empty((n_params, n_meas, w // 2 + 3), np.float64)
+    lib().tmo_replicate_stats(x.reshape(-1), n_inst, n_meas, w // 2, pi, n_params, int(flags), out.reshape(-1))
+    return out
+
+
+def ref_var_mean_surv(tab, x):
+    """The reference's var_mean_incremental_surv_runs (stochasticSPDP.cpp:173-207) on
+    tab[size][5] (updated in place) with one replicate's x[size][2]."""
+    x = np.ascontiguousarray(x, np.float64)
+    ref_lib().ref_ssp_var_mean_surv(tab, x, tab.shape[0])
+    return tab
+
+
+def ref_log10_time_bins(t_max, dt):
+    """The reference's logarithm10_time_bins (stochasticSPDP.cpp:336-377)."""
+    R = ref_lib()
+    n = R.ref_ssp_log10_time_bins(t_max, dt, None, 0)
+    out = np.empty(n, np.float64)
+    if n:
+        R.ref_ssp_log10_time_bins(t_max, dt, out.ctypes.data, n)
+    return out

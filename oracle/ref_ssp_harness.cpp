@@ -1,5 +1,6 @@
 // ref_ssp_harness.cpp -- C entry points around the reference's own legacy DP
-// Dornic integrator, built from the reference source where it lies
+// Dornic integrator, its running replicate statistics and its measurement
+// times, built from the reference source where it lies
 // (Percolation_FD/stochasticSPDP.cpp) into oracle/_ref/libref_ssp.so by
 // oracle/Makefile.  TEST INFRASTRUCTURE ONLY: used for the ensemble parity
 // test of the DP step and as the "reference" CPU baseline of bench.py.
@@ -25,7 +26,29 @@ void percolationDornic_2D(vector<vector<double>>& Rho, vector<double>& t_meas, d
                           double t_max, double a, double b, double D, double sigma, double dt,
                           double dx, int r, int g);
 
+// Running statistics over surviving replicates (stochasticSPDP.cpp:173-207) and
+// the log-spaced measurement times (stochasticSPDP.cpp:336-377), as defined there.
+void var_mean_incremental_surv_runs(double t_avg_var_rep_N[][5], double X_curr[][2], int size);
+std::vector<double> logarithm10_time_bins(double t_max, double dt);
+
 extern "C" {
+
+// One replicate folded into the running table: tab[size][5] = (t, mean, var,
+// #surviving, mean #active), x[size][2] = (#active, mean) of the replicate.
+void ref_ssp_var_mean_surv(double* tab, double* x, int size)
+{
+    var_mean_incremental_surv_runs(reinterpret_cast<double(*)[5]>(tab), reinterpret_cast<double(*)[2]>(x), size);
+}
+
+// The reference's measurement times; returns their number, writes at most cap.
+int ref_ssp_log10_time_bins(double t_max, double dt, double* out, int cap)
+{
+    const vector<double> t = logarithm10_time_bins(t_max, dt);
+    std::cout.flush();
+    fflush(stdout);
+    for (size_t i = 0; i < t.size() && (int)i < cap; ++i) out[i] = t[i];
+    return (int)t.size();
+}
 
 // One call of the reference integrator: r replicates of one a-value from the
 // initial frame rho0 (g*g).  Rows are written to out[max_rows][8]; returns the

@@ -566,3 +566,76 @@ int tmo_rk_run(const tmo_rk_params* p, int L, uint32_t seed, uint32_t inst, int6
     free(buf);
     return fault;
 }
+
+/* ---- across-replicate statistics ------------------------------------------
+ * Restated replicate by replicate, as the reference folds them: the first
+ * replicate of a parameter point initialises the table (FKE:4578-4597), every
+ * later one updates it through the running-statistics recurrence
+ * (FKE:1319-1369), whose time loop stops at the replicate's first measurement
+ * with no occupied cell.  With flag 2 a replicate's rows after that measurement
+ * are zero, as the reference's Rho_M rows are when the time loop has ended
+ * there (FKE:3086-3135, 4068-4072).  Table row: [1..nf-1] other-field means,
+ * then mean, var, surviving count, mean occupied count of field 0. */
+void tmo_replicate_stats(const double* series, int n_inst, int n_meas, int nf,
+                         const int32_t* params_index, int n_params, int flags, double* out)
+{
+    const int w = 2 * nf, nc = nf + 3;
+    const int intdiv = (flags & 1) != 0, stop = (flags & 2) != 0;
+    double* x = (double*)malloc(sizeof(double) * (size_t)(n_meas > 0 ? n_meas : 1) * w);
+    memset(out, 0, sizeof(double) * (size_t)n_params * n_meas * nc);
+    for (int g = 0; g < n_params; ++g) {
+        double* tab = out + (int64_t)g * n_meas * nc;
+        int j = 0;
+        for (int k = 0; k < n_inst; ++k) {
+            if (params_index[k] != g) continue;
+            /* this replicate's logged rows (X_curr), zero after its end if asked */
+            int ended = 0;
+            for (int i = 0; i < n_meas; ++i) {
+                const double* src = series + ((int64_t)k * n_meas + i) * w;
+                for (int c = 0; c < w; ++c) x[(int64_t)i * w + c] = ended ? 0.0 : src[c];
+                if (stop && src[0] == 0.0) ended = 1;
+            }
+            if (j == 0) {                                         /* FKE:4581-4596 */
+                for (int i = 0; i < n_meas; ++i) {
+                    double* row = tab + (int64_t)i * nc;
+                    const double* xi = x + (int64_t)i * w;
+                    row[nf - 1] = xi[1];
+                    row[nf] = 0.0;
+                    row[nf + 2] = xi[0];
+                    if (xi[0] > 0) row[nf + 1] = 1;
+                    else if (xi[0] == 0) row[nf + 1] = 0;
+                    for (int f = 1; f < nf; ++f) row[f - 1] = xi[2 * f + 1];
+                }
+            } else {
+                for (int i = 0; i < n_meas; ++i) {                /* FKE:1330-1358 */
+                    double* row = tab + (int64_t)i * nc;
+                    const double* xi = x + (int64_t)i * w;
+                    const int r = row[nf + 1] + 1;
+                    if (xi[0] == 0.0) break;
+                    const double mean_prev = row[nf - 1];
+                    if (row[nf + 1] == 0.0) {
+                        row[nf - 1] = xi[1];
+                        row[nf] = 0.0;
+                        row[nf + 2] = xi[0];
+                    } else {
+                        row[nf - 1] = (xi[1] + (r - 1) * row[nf - 1]) / r;
+                        row[nf + 2] = (xi[0] + (r - 1) * row[nf + 2]) / r;
+                        const double inner = (r - 2) * row[nf] + (xi[1] - row[nf - 1]) * (xi[1] - row[nf - 1]);
+                        if (intdiv)
+                            row[nf] = (row[nf - 1] - mean_prev) * (row[nf - 1] - mean_prev) + (1 / (r - 1)) * inner;
+                        else
+                            row[nf] = (row[nf - 1] - mean_prev) * (row[nf - 1] - mean_prev) + inner / (r - 1);
+                    }
+                    row[nf + 1] = r;
+                }
+                for (int i = 0; i < n_meas; ++i)                  /* FKE:1361-1366 */
+                    for (int f = 1; f < nf; ++f) {
+                        double* row = tab + (int64_t)i * nc;
+                        row[f - 1] = (x[(int64_t)i * w + 2 * f + 1] + j * row[f - 1]) / (j + 1);
+                    }
+            }
+            ++j;
+        }
+    }
+    free(x);
+}

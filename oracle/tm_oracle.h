@@ -27,7 +27,10 @@
  * CIR transition moments; the Laplacian by its Fourier eigenvalues; RK4 by the
  * Rietkerk mean-field fixed points and the P=0 closed form; the DP nonlinear
  * sub-steps by the closed form rho/(1+b rho t); the whole DP step by ensemble
- * comparison with the reference's own percolationDornic_2D built from source.
+ * comparison with the reference's own percolationDornic_2D built from source;
+ * the replicate statistics against the reference's own
+ * var_mean_incremental_surv_runs (stochasticSPDP.cpp:173-207) built from
+ * source, and against numpy means and unbiased variances.
  */
 #ifndef TM_ORACLE_H
 #define TM_ORACLE_H
@@ -104,6 +107,17 @@ void tmo_rk4_deterministic(const tmo_rk_params* p, int L, int64_t nsteps, double
 double tmo_dp_nonlinear(int scheme, double b, double dt, double rho);  /* one nonlinear sub-step */
 void tmo_reduce(int L, int nf, const double* fields, double* row);    /* (#>0, mean) per field */
 double tmo_flush(double x, double eps, double neg);                   /* post-step flush rule */
+
+/* Across-replicate statistics of a sweep (FKE:4578-4602 first replicate,
+ * FKE:1319-1369 / MSP:1080-1125 later ones).  series: [n_inst][n_meas][2*nf]
+ * rows (#>0, mean per field); params_index[n_inst] groups the instances, in
+ * instance order = replicate order.  flags: 1 = integer-division variance
+ * term (FKE:1351), 2 = rows after a replicate's first extinct measurement read
+ * as zero (FKE:3086,3135).  out: [n_params][n_meas][nf+3] = means of fields
+ * 1..nf-1 over all replicates, then mean, variance, surviving count and mean
+ * occupied count of field 0 over the surviving replicates. */
+void tmo_replicate_stats(const double* series, int n_inst, int n_meas, int nf,
+                         const int32_t* params_index, int n_params, int flags, double* out);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,8 @@ F64, F32 = 0, 1
 SAMPLER_EXACT, SAMPLER_COMPAT = 0, 1
 DP_EULER, DP_RK4, DP_EXACT = 0, 1, 2
 
+STATS_INTDIV_VARIANCE, STATS_STOP_AT_EXTINCTION, STATS_REFERENCE = 1, 2, 3
+
 TM_OK, TM_FAULT_NUMERIC = 0, 1
 _STATUS = {0: "TM_OK", -1: "TM_ERR_ARG", -2: "TM_ERR_HIP", -3: "TM_ERR_NOMEM", -4: "TM_ERR_STATE",
            -5: "TM_ERR_NODEV", 1: "TM_FAULT_NUMERIC"}
@@ -154,6 +156,10 @@ def lib() -> C.CDLL:
         "tm_time_steps": ([vp, i64, dp, C.POINTER(i64)], C.c_int),
         "tm_last_run_time": ([vp, dp, C.POINTER(i64)], C.c_int),
         "tm_stream": ([vp], vp),
+        "tm_log10_time_bins": ([C.c_double, C.c_double, dp, C.c_int], C.c_int),
+        "tm_measure_steps": ([dp, C.c_int, C.c_double, C.POINTER(i64)], C.c_int),
+        "tm_run_series": ([vp, C.POINTER(i64), C.c_int, vp], C.c_int),
+        "tm_replicate_stats": ([vp, C.c_int, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -173,6 +179,45 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def log10_time_bins(t_max: float, dt: float) -> np.ndarray:
+    """The reference's logarithm10_time_bins(t_max, dt) (rietkerk_bjork_FKE.cpp:1497-1536)."""
+    n = lib().tm_log10_time_bins(t_max, dt, None, 0)
+    out = np.empty(n, np.float64)
+    if n:
+        lib().tm_log10_time_bins(t_max, dt, _dptr(out), n)
+    return out
+
+
+def sweep_times(t_max: float, dt: float) -> np.ndarray:
+    """The sweep's t_measure: the log-spaced bins with 0 in front, sorted, repeats
+    removed (rietkerk_bjork_FKE.cpp:4676-4703, multiSPDP.cpp:3469-3490)."""
+    return np.unique(np.concatenate([[0.0], log10_time_bins(t_max, dt)]))
+
+
+def measure_steps(t_meas, dt: float) -> np.ndarray:
+    """Steps after which the reference's time loop logs each t_meas[i]
+    (rietkerk_bjork_FKE.cpp:4005-4017); raises if a point is never reached."""
+    t = np.ascontiguousarray(t_meas, np.float64)
+    out = np.empty(t.size, np.int64)
+    n = lib().tm_measure_steps(_dptr(t), t.size, dt, out.ctypes.data_as(C.POINTER(C.c_int64)))
+    if n != t.size:
+        raise ValueError(f"measurement time {t[n]} is not reached on the dt = {dt} grid")
+    return out
+
+
+def sweep_values(start: float, end: float, num: int) -> np.ndarray:
+    """a_space = linspace(a_start, a_end, div) (rietkerk_bjork_FKE.cpp:1374-1398,
+    start + delta i with the last point = end, as numpy.linspace)."""
+    return np.linspace(start, end, num)
+
+
+# Column names of Context.replicate_stats rows, per model (field 0 is P / rho).
+STATS_COLUMNS = {
+    MODEL_DP: ["mean", "var", "surviving", "active"],
+    MODEL_RIETKERK: ["mean_W", "mean_O", "mean", "var", "surviving", "active"],
+}
+
+
 class Context:
     """One tm_ctx: a batch of instances of one model at one precision on one GPU."""
 
@@ -185,6 +230,7 @@ class Context:
         self._h = h
         self.model = None
         self.instances: list[Instance] = []
+        self._n_params = self._n_series = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -224,6 +270,7 @@ class Context:
         self._check(self._lib.tm_configure(self._h, model, precision, C.cast(parr, C.c_void_p),
                                            len(plist), C.cast(iarr, C.c_void_p), len(instances)))
         self.model, self.precision, self.instances = model, precision, list(instances)
+        self._n_params, self._n_series = len(plist), 0
         return self
 
     def upload(self, i: int, fields: np.ndarray):
@@ -248,6 +295,23 @@ class Context:
         """Advance n_steps; returns rows [n_inst, n_steps, 2*nf] of (#>0, mean) per field."""
         out = np.empty((len(self.instances), n_steps, 2 * self.nfields), np.float64) if rows else None
         self._check(self._lib.tm_run(self._h, n_steps, out.ctypes.data if out is not None else None))
+        return out
+
+    def run_series(self, steps, copy: bool = True):
+        """Advance to steps[-1] (from now), keeping every instance's rho(t) row at each
+        steps[i] on the device; returns the series [n_inst, n_meas, 2*nf] (or None)."""
+        st = np.ascontiguousarray(steps, np.int64)
+        out = np.empty((len(self.instances), st.size, 2 * self.nfields), np.float64) if copy else None
+        self._check(self._lib.tm_run_series(self._h, st.ctypes.data_as(C.POINTER(C.c_int64)), st.size,
+                                            out.ctypes.data if out is not None else None))
+        self._n_series = st.size
+        return out
+
+    def replicate_stats(self, flags: int = STATS_REFERENCE) -> np.ndarray:
+        """Across-replicate statistics of the last run_series, computed on the device:
+        [n_params, n_meas, nf+3], columns STATS_COLUMNS[model]."""
+        out = np.empty((self._n_params, self._n_series, self.nfields + 3), np.float64)
+        self._check(self._lib.tm_replicate_stats(self._h, flags, _dptr(out)))
         return out
 
     def measure(self) -> np.ndarray:

@@ -26,6 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INCLUDE,
 UNITS = {
     "tm_kernels_f64.hip": ["-ffp-contract=off"],
     "tm_kernels_f32.hip": ["-ffp-contract=fast"],
+    "tm_ensemble.hip": ["-ffp-contract=off"],
     "tm_api.cpp": ["-ffp-contract=off"],
 }
 

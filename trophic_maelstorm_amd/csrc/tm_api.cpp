@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cerrno>
 #include <chrono>
 #include <cstdio>
@@ -77,6 +78,17 @@ struct tm_ctx {
     hipEvent_t run_ev[2] = {nullptr, nullptr};
     double last_run_ms = 0.0;
     int64_t last_run_steps = 0;
+
+    // sweep series (tm_run_series) and replicate statistics (tm_replicate_stats)
+    double* d_series = nullptr;        // [n_inst][n_series][2 nf]
+    int64_t series_cap = 0;            // doubles
+    int n_series = 0;                  // measurements of the current series (0: none)
+    int64_t* d_series_steps = nullptr;
+    int series_steps_cap = 0;
+    int32_t* d_groups = nullptr;       // [n_params + 1] offsets, [n_inst] members, [n_inst] first extinct
+    int64_t groups_cap = 0;
+    double* d_stats = nullptr;         // [n_params][n_series][nf + 3]
+    int64_t stats_cap = 0;
 };
 
 namespace {
@@ -136,6 +148,12 @@ void release_config(tm_ctx* c)
     free_dev(c->d_consts);
     free_dev(c->d_rows);
     free_dev(c->d_draws);
+    free_dev(c->d_series);
+    free_dev(c->d_series_steps);
+    free_dev(c->d_groups);
+    free_dev(c->d_stats);
+    c->series_cap = c->groups_cap = c->stats_cap = 0;
+    c->series_steps_cap = c->n_series = 0;
     c->rows_cap = 0;
     c->nslots = 0;
     c->model = 0;
@@ -606,15 +624,11 @@ tm_status tm_last_run_time(const tm_ctx* c, double* ms, int64_t* n_steps)
     return TM_OK;
 }
 
-tm_status tm_measure(tm_ctx* c, double* rows)
+// rho(t) row of the current state of every instance into out[i * stride * 2 nf]
+// (device): tile partials of the current fields, then the rows fold.
+static void enqueue_measure(tm_ctx* c, double* out, int64_t stride)
 {
-    tm_status s = check_ready(c);
-    if (s != TM_OK) return s;
-    if (!rows) return fail(c, TM_ERR_ARG, "null rows");
     const Variant& v = cur_variant(c);
-    double* d_out = nullptr;
-    const size_t bytes = sizeof(double) * 2 * c->nf * c->inst.size();
-    HIPCHK(c, hipMalloc((void**)&d_out, bytes));
     if (c->prec == TM_F64)
         launch_measure_f64(c->nf, c->d_inst, c->d_tiles, (int)c->tiles.size(), v.tx, v.ty, c->d_fields[c->parity],
                            c->d_partials, c->stream);
@@ -625,15 +639,26 @@ tm_status tm_measure(tm_ctx* c, double* rows)
     a.inst = c->d_inst;
     a.partials = c->d_partials;
     a.state = c->d_state;
-    a.rows = d_out;
+    a.rows = out;
     a.ntiles = (int32_t)c->tiles.size();
     a.nslots = c->nslots;
     a.nf = c->nf;
     a.nsteps = 1;
-    a.rows_cap = 1;
+    a.rows_cap = stride;
     a.measure = 1;
     a.pw = 1;
     launch_combine(a, (int)c->inst.size(), 1, c->stream);
+}
+
+tm_status tm_measure(tm_ctx* c, double* rows)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (!rows) return fail(c, TM_ERR_ARG, "null rows");
+    double* d_out = nullptr;
+    const size_t bytes = sizeof(double) * 2 * c->nf * c->inst.size();
+    HIPCHK(c, hipMalloc((void**)&d_out, bytes));
+    enqueue_measure(c, d_out, 1);
     hipError_t e = hipMemcpyAsync(rows, d_out, bytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(d_out);
@@ -840,6 +865,144 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
     if (f) f << key << " " << best_v << " " << g << "\n";
     if (out_variant) *out_variant = best_v;
     if (out_graph_steps) *out_graph_steps = g;
+    return TM_OK;
+}
+
+// ---- Sweep measurement and replicate statistics ----------------------------
+
+int tm_log10_time_bins(double t_max, double dt, double* out, int cap)
+{
+    // 10^p for p = 0, 0.04, 0.08, ... (p accumulated) up to floor(10 log10 t_max)/10,
+    // each rounded down to the dt grid (an int number of dt) and kept above 1,
+    // skipping a repeat of the previous point; then, from the end, every point
+    // less than dt/2 above its predecessor is dropped
+    // (rietkerk_bjork_FKE.cpp:1497-1536).
+    std::vector<double> t;
+    if (!(t_max > 1.0) || !(dt > 0.0)) return 0;
+    const double top = std::floor(std::log10(t_max) * 10.0) / 10.0;
+    for (double p = 0.0; p <= top; p += 0.04) {
+        double v = (double)(int)(std::pow(10.0, p) / dt) * dt;
+        if (v <= 1.0) v = v + dt;
+        if (t.empty() || v != t.back()) t.push_back(v);
+    }
+    for (int i = (int)t.size() - 1; i > 0; --i)
+        if (t[i] < t[i - 1] + dt / 2.0) t.erase(t.begin() + i);
+    if (out)
+        for (int i = 0; i < (int)t.size() && i < cap; ++i) out[i] = t[i];
+    return (int)t.size();
+}
+
+int tm_measure_steps(const double* t_meas, int n_meas, double dt, int64_t* steps)
+{
+    if (!t_meas || !steps || n_meas <= 0 || !(dt > 0.0)) return 0;
+    const double t_end = t_meas[n_meas - 1] + dt;
+    double t = 0.0;
+    int idx = 0;
+    for (int64_t n = 0; t < t_end && idx < n_meas; ++n) {
+        if (t == 0 || (t >= t_meas[idx] - dt / 2.0 && t < t_meas[idx] + dt / 2.0)) steps[idx++] = n;
+        t += dt;
+    }
+    return idx;
+}
+
+tm_status tm_run_series(tm_ctx* c, const int64_t* steps, int n_meas, double* series)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (!steps || n_meas <= 0) return fail(c, TM_ERR_ARG, "tm_run_series: need steps[n_meas] with n_meas > 0");
+    for (int m = 0; m < n_meas; ++m)
+        if (steps[m] < 0 || (m > 0 && steps[m] < steps[m - 1]))
+            return fail(c, TM_ERR_ARG, "tm_run_series: steps must be non-decreasing and >= 0");
+    const int n_inst = (int)c->inst.size();
+    const int w = 2 * c->nf;
+    const int64_t need = (int64_t)n_inst * n_meas * w;
+    c->n_series = 0;
+    if (need > c->series_cap) {
+        free_dev(c->d_series);
+        c->series_cap = 0;
+        if (hipMalloc((void**)&c->d_series, sizeof(double) * (size_t)need) != hipSuccess)
+            return fail(c, TM_ERR_NOMEM, "series allocation failed");
+        c->series_cap = need;
+    }
+    if (n_meas > c->series_steps_cap) {
+        free_dev(c->d_series_steps);
+        c->series_steps_cap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->d_series_steps, sizeof(int64_t) * (size_t)n_meas));
+        c->series_steps_cap = n_meas;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_series_steps, steps, sizeof(int64_t) * (size_t)n_meas, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipEventRecord(c->run_ev[0], c->stream));
+    int m = 0;
+    for (; m < n_meas && steps[m] == 0; ++m) enqueue_measure(c, c->d_series + (int64_t)m * w, n_meas);
+    // Advance in chunks of whole graph blocks whose rows fit a ~32 MiB rows
+    // buffer (or the one already there); after each chunk, keep the rows of the
+    // measurements it reached.
+    int64_t chunk = std::max<int64_t>(c->rows_cap, (int64_t{32} << 20) / ((int64_t)sizeof(double) * n_inst * w));
+    if (c->graph_steps > 0 && chunk >= c->graph_steps) chunk -= chunk % c->graph_steps;
+    chunk = std::max<int64_t>(chunk, 1);
+    const int64_t total = steps[n_meas - 1];
+    for (int64_t s0 = 0; s0 < total;) {
+        const int64_t len = std::min(chunk, total - s0);
+        if ((s = advance(c, len)) != TM_OK) return s;
+        int m1 = m;
+        while (m1 < n_meas && steps[m1] <= s0 + len) ++m1;
+        launch_gather_series(c->d_rows, c->rows_cap, w, c->d_series_steps, m, m1, s0, n_meas, n_inst, c->d_series,
+                             c->stream);
+        m = m1;
+        s0 += len;
+        // the next advance() rewrites the pinned staging its first copy reads
+        if (s0 < total) HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->run_ev[1], c->stream));
+    if (series)
+        HIPCHK(c, hipMemcpyAsync(series, c->d_series, sizeof(double) * (size_t)need, hipMemcpyDeviceToHost,
+                                 c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->run_ev[0], c->run_ev[1]));
+    c->last_run_ms = ms;
+    c->last_run_steps = total;
+    c->n_series = n_meas;
+    return fault_status(c);
+}
+
+tm_status tm_replicate_stats(tm_ctx* c, int flags, double* stats)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (!stats) return fail(c, TM_ERR_ARG, "null stats");
+    if (flags & ~TM_STATS_REFERENCE) return fail(c, TM_ERR_ARG, "tm_replicate_stats: unknown flags");
+    if (c->n_series <= 0) return fail(c, TM_ERR_STATE, "tm_run_series has not been called");
+    const int n_inst = (int)c->inst.size(), ng = c->n_params, n_meas = c->n_series;
+    // Instances grouped by parameter point, in instance (= replicate) order.
+    std::vector<int32_t> grp((size_t)ng + 1 + 2 * (size_t)n_inst, 0);
+    for (const tm_instance_desc& d : c->inst) ++grp[(size_t)d.params_index + 1];
+    for (int g = 0; g < ng; ++g) grp[(size_t)g + 1] += grp[(size_t)g];
+    std::vector<int32_t> cur(grp.begin(), grp.begin() + ng);
+    for (int i = 0; i < n_inst; ++i) grp[(size_t)ng + 1 + cur[(size_t)c->inst[(size_t)i].params_index]++] = i;
+    const int64_t gsize = (int64_t)grp.size();
+    if (gsize > c->groups_cap) {
+        free_dev(c->d_groups);
+        c->groups_cap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->d_groups, sizeof(int32_t) * (size_t)gsize));
+        c->groups_cap = gsize;
+    }
+    const int64_t nout = (int64_t)ng * n_meas * (c->nf + 3);
+    if (nout > c->stats_cap) {
+        free_dev(c->d_stats);
+        c->stats_cap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->d_stats, sizeof(double) * (size_t)nout));
+        c->stats_cap = nout;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_groups, grp.data(), sizeof(int32_t) * (size_t)(ng + 1 + n_inst),
+                             hipMemcpyHostToDevice, c->stream));
+    launch_replicate_stats(c->d_series, n_inst, n_meas, c->nf, c->d_groups, c->d_groups + ng + 1, ng, flags,
+                           c->d_groups + ng + 1 + n_inst, c->d_stats, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(stats, c->d_stats, sizeof(double) * (size_t)nout, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return TM_OK;
 }
 

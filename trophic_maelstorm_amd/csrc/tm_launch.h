@@ -43,4 +43,11 @@ void launch_prep(int model, const DevInst* inst, const void* params, const DevSt
 void launch_bump(DevState* state, int n, hipStream_t s);
 size_t const_size(int model);
 
+// Sweep series and replicate statistics (tm_ensemble.hip).
+void launch_gather_series(const double* rows, int64_t rows_cap, int w, const int64_t* steps, int m0, int m1,
+                          int64_t s0, int n_meas, int n_inst, double* series, hipStream_t s);
+void launch_replicate_stats(const double* series, int n_inst, int n_meas, int nf, const int32_t* grp_off,
+                            const int32_t* grp_inst, int n_groups, int flags, int32_t* first_extinct,
+                            double* out, hipStream_t s);
+
 }  // namespace tmk

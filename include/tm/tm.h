@@ -129,6 +129,34 @@ tm_status tm_download_fields(tm_ctx* ctx, int inst, double* host);
  * device from Philox(seed, instance_id; cell, 0xFFFFFFFF, 0xFFFFFFFF, 0), word f. */
 tm_status tm_init_uniform(tm_ctx* ctx, const double* lo, const double* hi);
 
+/* Speckle initial frames: every cell draws u ~ Uniform(0,1) from
+ * Philox(seed, instance_id; cell, 0xFFFFFFFE, 0xFFFFFFFF, 0), word 0, and field f
+ * takes high[p][f] where u <= perc[p], low[p][f] elsewhere, p = the instance's
+ * params_index (perc[n_params], high/low [n_params][nfields]).  Replaces
+ * init_randconstframe (rietkerk_bjork_FKE.cpp:409-431) and, with values from
+ * tm_mft_frame_values or the caller, init_randbistableframe (:434-451),
+ * init_exprtk_homogenousMFTframe (:453-531) and init_exprtk_randbiMFTframe
+ * (:533-642). */
+tm_status tm_init_speckle(tm_ctx* ctx, const double* perc, const double* high, const double* low);
+
+/* Gaussian initial frames (init_gaussframe + init_gaussiantear,
+ * rietkerk_bjork_FKE.cpp:1089-1199): fields f < n_gauss hold
+ * amp[p][f] exp(-((i-cx)^2 + (j-cy)^2) / (2 sd sd)) at cell i*L + j, centred at
+ * the corners (L/8 or 7L/8, integer division) counter-clockwise from the top
+ * left, with sd = L / sd_div[f] (the reference's default is sd_div = {8, 16};
+ * missing entries repeat the last); fields f >= n_gauss hold amp[p][f]. */
+tm_status tm_init_gauss(tm_ctx* ctx, int n_gauss, const double* sd_div, int n_sd, const double* amp);
+
+/* Host only.  Per-field (high, low) values of the reference's mean-field
+ * frames at one parameter point: mft[2*nf] are the caller's mean-field values
+ * at R (the reference's exprtk MFT_Vec_CoexExpr, evaluated), c_spread[2*nf] its
+ * scale factors; the first nf entries apply below R_c, the last nf at or above.
+ * homogeneous != 0: init_exprtk_homogenousMFTframe (rietkerk_bjork_FKE.cpp:453-531),
+ * high == low.  Otherwise init_exprtk_randbiMFTframe (:533-642): field 0 is
+ * max(dP, c_spread E_0) in the speckles.  Returns 0, or -1 on a bad argument. */
+int tm_mft_frame_values(int homogeneous, double R, double R_c, double dP, int nf, const double* c_spread,
+                        const double* mft, double* high, double* low);
+
 /* Advance every instance n_steps steps: Dornic Poisson-Gamma on P (rho), RK4
  * (Rietkerk) or the DP nonlinear sub-step, checks, flush and copy -- the loop
  * body rietkerk_bjork_FKE.cpp:4154-4465 / multiSPDP.cpp:2901-3262.

@@ -91,6 +91,9 @@ def lib():
         L.tmo_reduce.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
         L.tmo_flush.restype = C.c_double
         L.tmo_flush.argtypes = [C.c_double] * 3
+        L.tmo_init_speckle.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_double, _f64p, _f64p,
+                                       _f64p]
+        L.tmo_init_gauss.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, C.c_int, _f64p, _f64p]
         L.tmo_replicate_stats.argtypes = [_f64p, C.c_int, C.c_int, C.c_int,
                                           np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                           C.c_int, C.c_int, _f64p]
@@ -254,3 +257,21 @@ def ref_log10_time_bins(t_max, dt):
     if n:
         R.ref_ssp_log10_time_bins(t_max, dt, out.ctypes.data, n)
     return out
+
+
+def init_speckle(L, perc, high, low, seed, inst):
+    """Speckle frame [nf, L, L] (tmo_init_speckle)."""
+    hi = np.ascontiguousarray(high, np.float64)
+    lo = np.ascontiguousarray(low, np.float64)
+    out = np.empty(hi.size * L * L, np.float64)
+    lib().tmo_init_speckle(L, hi.size, seed, inst, float(perc), hi, lo, out)
+    return out.reshape(hi.size, L, L)
+
+
+def init_gauss(L, n_gauss, sd_div, amp):
+    """Gaussian frame [nf, L, L] (tmo_init_gauss)."""
+    sd = np.ascontiguousarray(sd_div, np.float64)
+    a = np.ascontiguousarray(amp, np.float64)
+    out = np.empty(a.size * L * L, np.float64)
+    lib().tmo_init_gauss(L, a.size, int(n_gauss), sd, sd.size, a, out)
+    return out.reshape(a.size, L, L)

@@ -639,3 +639,42 @@ void tmo_replicate_stats(const double* series, int n_inst, int n_meas, int nf,
     }
     free(x);
 }
+
+/* ---- initial frames -------------------------------------------------------- */
+/* init_randconstframe (FKE:409-431): one uniform per cell decides high or low
+ * for every field together. */
+void tmo_init_speckle(int L, int nf, uint32_t seed, uint32_t inst, double perc,
+                      const double* high, const double* low, double* out)
+{
+    const int64_t N = (int64_t)L * L;
+    for (int64_t i = 0; i < N; ++i) {
+        const uint32_t ctr[4] = {(uint32_t)i, 0xFFFFFFFEu, 0xFFFFFFFFu, 0u}, key[2] = {seed, inst};
+        uint32_t w[4];
+        tmo_philox4x32_10(ctr, key, w);
+        const double r = ((double)w[0] + 0.5) * 0x1p-32;
+        for (int s = 0; s < nf; ++s) out[s * N + i] = r <= perc ? high[s] : low[s];
+    }
+}
+
+/* init_gaussframe (FKE:1089-1183) with init_gaussiantear (FKE:1185-1199):
+ * centres cx = [g/8, 7g/8, 7g/8, g/8, ...], cy = [g/8, g/8, 7g/8, 7g/8, ...]
+ * (integer division), widths g / sd_div, then amp exp(-(di^2 + dj^2) / (2 sd sd))
+ * at cell i*g + j; the remaining fields are the constants amp[s]. */
+void tmo_init_gauss(int L, int nf, int n_gauss, const double* sd_div, int n_sd,
+                    const double* amp, double* out)
+{
+    const int g = L;
+    for (int s = 0; s < nf; ++s) {
+        double* a = out + (int64_t)s * g * g;
+        if (s >= n_gauss) {
+            for (int64_t i = 0; i < (int64_t)g * g; ++i) a[i] = amp[s];
+            continue;
+        }
+        const double cx = (s % 4) % 3 == 0 ? g / 8 : 7 * g / 8;
+        const double cy = (s % 4) < 2 ? g / 8 : 7 * g / 8;
+        const double sd = g / sd_div[s < n_sd ? s : n_sd - 1];
+        for (int i = 0; i < g; ++i)
+            for (int j = 0; j < g; ++j)
+                a[(int64_t)i * g + j] = amp[s] * tmo_exp(-((i - cx) * (i - cx) + (j - cy) * (j - cy)) / (2 * sd * sd));
+    }
+}

@@ -108,6 +108,15 @@ double tmo_dp_nonlinear(int scheme, double b, double dt, double rho);  /* one no
 void tmo_reduce(int L, int nf, const double* fields, double* row);    /* (#>0, mean) per field */
 double tmo_flush(double x, double eps, double neg);                   /* post-step flush rule */
 
+/* Speckle frame (FKE:409-431 rule): u from Philox(seed, inst; cell, 0xFFFFFFFE,
+ * 0xFFFFFFFF, 0) word 0; field f = high[f] where u <= perc, else low[f]. */
+void tmo_init_speckle(int L, int nf, uint32_t seed, uint32_t inst, double perc,
+                      const double* high, const double* low, double* out);
+/* Gaussian frame (FKE:1089-1199): fields f < n_gauss Gaussian bumps at the
+ * corner centres with sd = L / sd_div[min(f, n_sd-1)], the rest amp[f]. */
+void tmo_init_gauss(int L, int nf, int n_gauss, const double* sd_div, int n_sd,
+                    const double* amp, double* out);
+
 /* Across-replicate statistics of a sweep (FKE:4578-4602 first replicate,
  * FKE:1319-1369 / MSP:1080-1125 later ones).  series: [n_inst][n_meas][2*nf]
  * rows (#>0, mean per field); params_index[n_inst] groups the instances, in

@@ -160,6 +160,10 @@ def lib() -> C.CDLL:
         "tm_measure_steps": ([dp, C.c_int, C.c_double, C.POINTER(i64)], C.c_int),
         "tm_run_series": ([vp, C.POINTER(i64), C.c_int, vp], C.c_int),
         "tm_replicate_stats": ([vp, C.c_int, dp], C.c_int),
+        "tm_init_speckle": ([vp, dp, dp, dp], C.c_int),
+        "tm_init_gauss": ([vp, C.c_int, dp, C.c_int, dp], C.c_int),
+        "tm_mft_frame_values": ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp],
+                                C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -209,6 +213,21 @@ def sweep_values(start: float, end: float, num: int) -> np.ndarray:
     """a_space = linspace(a_start, a_end, div) (rietkerk_bjork_FKE.cpp:1374-1398,
     start + delta i with the last point = end, as numpy.linspace)."""
     return np.linspace(start, end, num)
+
+
+def mft_frame_values(homogeneous: bool, R: float, R_c: float, dP: float, c_spread, mft):
+    """(high, low) per field of the reference's mean-field initial frames at one
+    parameter point (rietkerk_bjork_FKE.cpp:453-642); c_spread and mft have 2*nf
+    entries (below R_c, then at or above)."""
+    cs = np.ascontiguousarray(c_spread, np.float64)
+    e = np.ascontiguousarray(mft, np.float64)
+    if cs.size != e.size or cs.size % 2:
+        raise ValueError("c_spread and mft need 2*nf entries each")
+    nf = cs.size // 2
+    hi, lo = np.empty(nf), np.empty(nf)
+    if lib().tm_mft_frame_values(int(homogeneous), R, R_c, dP, nf, _dptr(cs), _dptr(e), _dptr(hi), _dptr(lo)):
+        raise ValueError("tm_mft_frame_values: bad arguments")
+    return hi, lo
 
 
 # Column names of Context.replicate_stats rows, per model (field 0 is P / rho).
@@ -290,6 +309,23 @@ class Context:
         lo = np.ascontiguousarray(lo, np.float64)
         hi = np.ascontiguousarray(hi, np.float64)
         self._check(self._lib.tm_init_uniform(self._h, _dptr(lo), _dptr(hi)))
+
+    def init_speckle(self, perc, high, low):
+        """Speckle frames: per parameter point perc[p], high[p][f], low[p][f] (a single
+        point's values are broadcast to every point)."""
+        n_p, nf = self._n_params, self.nfields
+        pc = np.ascontiguousarray(np.broadcast_to(np.asarray(perc, np.float64), (n_p,)))
+        hi = np.ascontiguousarray(np.broadcast_to(np.asarray(high, np.float64), (n_p, nf)))
+        lo = np.ascontiguousarray(np.broadcast_to(np.asarray(low, np.float64), (n_p, nf)))
+        self._check(self._lib.tm_init_speckle(self._h, _dptr(pc), _dptr(hi), _dptr(lo)))
+
+    def init_gauss(self, amp, n_gauss: int = 1, sd_div=(8.0, 16.0)):
+        """Gaussian frames: fields < n_gauss are bumps of height amp[p][f] and width
+        L / sd_div[f] at the corners, the others the constant amp[p][f]."""
+        n_p, nf = self._n_params, self.nfields
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(amp, np.float64), (n_p, nf)))
+        sd = np.ascontiguousarray(sd_div, np.float64)
+        self._check(self._lib.tm_init_gauss(self._h, int(n_gauss), _dptr(sd), sd.size, _dptr(a)))
 
     def run(self, n_steps: int, rows: bool = True):
         """Advance n_steps; returns rows [n_inst, n_steps, 2*nf] of (#>0, mean) per field."""

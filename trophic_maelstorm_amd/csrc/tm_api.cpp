@@ -594,6 +594,75 @@ tm_status tm_init_uniform(tm_ctx* c, const double* lo, const double* hi)
     return TM_OK;
 }
 
+tm_status tm_init_speckle(tm_ctx* c, const double* perc, const double* high, const double* low)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (!perc || !high || !low) return fail(c, TM_ERR_ARG, "tm_init_speckle: null values");
+    for (size_t i = 0; i < c->inst.size(); ++i) {
+        const tm_instance_desc& d = c->inst[i];
+        const int g = d.params_index;
+        FrameVals v{};
+        v.perc = perc[g];
+        for (int f = 0; f < c->nf; ++f) {
+            v.a[f] = high[(size_t)g * c->nf + f];
+            v.b[f] = low[(size_t)g * c->nf + f];
+        }
+        void* dst = (char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[i];
+        launch_init_speckle(c->prec, c->nf, d.L, d.seed, d.instance_id, v, dst, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TM_OK;
+}
+
+tm_status tm_init_gauss(tm_ctx* c, int n_gauss, const double* sd_div, int n_sd, const double* amp)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (!amp || n_gauss < 0 || n_gauss > c->nf) return fail(c, TM_ERR_ARG, "tm_init_gauss: bad n_gauss or amp");
+    if (n_gauss > 0 && (!sd_div || n_sd < 1)) return fail(c, TM_ERR_ARG, "tm_init_gauss: need sd_div[n_sd >= 1]");
+    for (int k = 0; k < n_sd && k < n_gauss; ++k)
+        if (!(sd_div[k] > 0.0)) return fail(c, TM_ERR_ARG, "tm_init_gauss: sd_div must be > 0");
+    for (size_t i = 0; i < c->inst.size(); ++i) {
+        const tm_instance_desc& d = c->inst[i];
+        const int L = d.L;
+        FrameVals v{};
+        for (int f = 0; f < c->nf; ++f) v.a[f] = amp[(size_t)d.params_index * c->nf + f];
+        // Centres go round the corners from the top left (rietkerk_bjork_FKE.cpp:1110-1115);
+        // missing widths repeat the last one given (:1118-1127).
+        for (int f = 0; f < n_gauss; ++f) {
+            v.cx[f] = (f % 4) % 3 == 0 ? (double)(L / 8) : (double)(7 * L / 8);
+            v.cy[f] = (f % 4) < 2 ? (double)(L / 8) : (double)(7 * L / 8);
+            v.sd[f] = L / sd_div[std::min(f, n_sd - 1)];
+        }
+        void* dst = (char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[i];
+        launch_init_gauss(c->prec, c->nf, L, n_gauss, v, dst, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TM_OK;
+}
+
+int tm_mft_frame_values(int homogeneous, double R, double R_c, double dP, int nf, const double* c_spread,
+                        const double* mft, double* high, double* low)
+{
+    if (nf < 1 || nf > 3 || !c_spread || !mft || !high || !low) return -1;
+    const int o = R < R_c ? 0 : nf;   // below R_c the first nf entries, else the last nf
+    for (int f = 0; f < nf; ++f) {
+        const double e = mft[o + f], cs = c_spread[o + f];
+        high[f] = low[f] = e == 0 ? cs : cs * e;   // a zero MFT value leaves the bare scale factor
+    }
+    if (!homogeneous) {
+        // field 0: the MFT value, raised to max(dP, c_spread[.] E_0) in the speckles; at or
+        // above R_c the background is the bare E_0 (rietkerk_bjork_FKE.cpp:596-630)
+        const double e0 = mft[o], cs0 = c_spread[o];
+        high[0] = std::max(dP, cs0 * e0);
+        low[0] = o == 0 ? cs0 * e0 : e0;
+    }
+    return 0;
+}
+
 tm_status tm_run(tm_ctx* c, int64_t n_steps, double* rows)
 {
     tm_status s = check_ready(c);

@@ -43,6 +43,18 @@ void launch_prep(int model, const DevInst* inst, const void* params, const DevSt
 void launch_bump(DevState* state, int n, hipStream_t s);
 size_t const_size(int model);
 
+// Initial frames (tm_frames.hip).  Per-field values of one instance: speckle
+// frames use perc, a = high, b = low; Gaussian frames use a = amp and the
+// centres and widths cx, cy, sd of the first n_gauss fields.
+struct FrameVals {
+    double perc;
+    double a[3], b[3];
+    double cx[3], cy[3], sd[3];
+};
+void launch_init_speckle(int prec, int nf, int L, uint32_t seed, uint32_t inst_id, const FrameVals& v,
+                         void* fields, hipStream_t s);
+void launch_init_gauss(int prec, int nf, int L, int n_gauss, const FrameVals& v, void* fields, hipStream_t s);
+
 // Sweep series and replicate statistics (tm_ensemble.hip).
 void launch_gather_series(const double* rows, int64_t rows_cap, int w, const int64_t* steps, int m0, int m1,
                           int64_t s0, int n_meas, int n_inst, double* series, hipStream_t s);

@@ -219,6 +219,55 @@ tm_status tm_last_run_time(const tm_ctx* ctx, double* ms, int64_t* n_steps);
 /* The context's HIP stream (hipStream_t) for callers that order their own work. */
 void* tm_stream(tm_ctx* ctx);
 
+/* ---- Output layer: CSV text, snapshots, burn-in input ----------------------
+ * Numbers are printed as the reference's std::ostream prints them: a double
+ * with precision p as printf("%.*g", p, x) -- p = 6 is the reference's default
+ * (its FRAME and PRELIM files), 16 its sweep file -- and integers in decimal.
+ * Host-only functions return 0 on success and a negative value on failure. */
+
+/* FRAME file of one instance (save_frame, rietkerk_bjork_FKE.cpp:2829-2876):
+ * header (may be NULL; the reference's is "a_c,  x,  P(x; t), W(x; t), O(x; t) \n",
+ * rietkerk_bjork_constants_1Sp.h:6), then per cell i "a,i,f0,...\n" with `digits`
+ * significant digits (6 = the reference's). */
+tm_status tm_write_frame_csv(tm_ctx* ctx, int inst, const char* path, double a, const char* header, int digits);
+
+/* Host only: the same file from a host frame [nf][L*L]. */
+int tm_format_frame_csv(const char* path, const double* fields, int nf, int L, double a, const char* header,
+                        int digits);
+
+/* Host only: PRELIM file (save_prelimframe, rietkerk_bjork_FKE.cpp:2723-2765):
+ * header, then per row i "a,r+1,g,t[i]," and each of stats[i][0..n_cols-1]
+ * followed by a comma -- e.g. the rows of tm_replicate_stats. */
+int tm_write_prelim_csv(const char* path, const char* header, double a, int r, int g, const double* t,
+                        const double* stats, int n_rows, int n_cols);
+
+/* Host only: the sweep's final file (rietkerk_bjork_FKE.cpp:4852-4886): the
+ * header line, then rows[n_rows][n_cols] comma-separated with precision 16. */
+int tm_write_sweep_csv(const char* path, const char* header, const double* rows, int n_rows, int n_cols);
+
+/* Binary snapshot of one instance: a 64-byte header ("TMSNAP1", model,
+ * precision, nf, L, seed, instance_id, step, params_index, data bytes) and the
+ * fields in their stored precision, species-major.  A fast stand-in for the
+ * FRAME text at large L; tm_snapshot_to_frame_csv converts it afterwards. */
+tm_status tm_save_snapshot(tm_ctx* ctx, int inst, const char* path);
+
+/* Load a snapshot into instance inst (same model and L; the precision is
+ * converted as tm_upload_fields does).  step (nullable) gets the recorded
+ * instance step, for a resumed run's step0. */
+tm_status tm_load_snapshot(tm_ctx* ctx, int inst, const char* path, int64_t* step);
+
+/* Host only: a snapshot as the FRAME file tm_write_frame_csv would write. */
+int tm_snapshot_to_frame_csv(const char* snap, const char* csv, double a, const char* header, int digits);
+
+/* Host only: burn-in input (init_exprtk_readCSVcolumns_frame,
+ * rietkerk_bjork_FKE.cpp:644-935).  Reads a FRAME file: the header names with
+ * white space removed select the columns cols[n_cols]; "W(x;t)" and "O(x;t)"
+ * go to fields nf-2 and nf-1, any other column to field (its position - 2).
+ * Cells that do not parse as numbers read as 0; fields not named are left as
+ * they are in out[nf][L*L].  Returns the number of rows read (<= L*L), -2 if a
+ * column is missing, -1 on other errors. */
+int tm_read_frame_csv(const char* path, int L, int nf, const char* const* cols, int n_cols, double* out);
+
 /* ---- Sweep measurement and replicate statistics ----------------------------
  * The reference's sweep (first_order_critical_exp_delta_stochastic_MultiSp,
  * rietkerk_bjork_FKE.cpp:4665-4887 / multiSPDP.cpp:3456-3702) runs one OpenMP

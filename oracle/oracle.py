@@ -39,11 +39,15 @@ class RKParams(C.Structure):
                 ("sampler", C.c_int32), ("pad_", C.c_int32)]
 
 
+FMT_PATH = os.path.join(HERE, "libfmt_check.so")
+
+
 def build(force: bool = False) -> str:
-    """Compile liboracle.so (and the reference library when its sources exist)."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True,
-                       stdout=subprocess.DEVNULL)
+    """Compile liboracle.so and libfmt_check.so."""
+    for path in (LIB_PATH, FMT_PATH):
+        if force or not os.path.exists(path):
+            subprocess.run(["make", "-C", HERE, os.path.basename(path)], check=True,
+                           stdout=subprocess.DEVNULL)
     return LIB_PATH
 
 
@@ -275,3 +279,20 @@ def init_gauss(L, n_gauss, sd_div, amp):
     out = np.empty(a.size * L * L, np.float64)
     lib().tmo_init_gauss(L, a.size, int(n_gauss), sd, sd.size, a, out)
     return out.reshape(a.size, L, L)
+
+
+_fmt = None
+
+
+def ostream_text(x: float, prec: int = 0) -> str:
+    """x as std::ostream prints it with setprecision(prec) (0: the default, 6)."""
+    global _fmt
+    if _fmt is None:
+        if not os.path.exists(FMT_PATH):
+            build()
+        _fmt = C.CDLL(FMT_PATH)
+        _fmt.fmt_ostream.argtypes = [C.c_double, C.c_int, C.c_char_p, C.c_int]
+        _fmt.fmt_ostream.restype = C.c_int
+    buf = C.create_string_buffer(64)
+    n = _fmt.fmt_ostream(float(x), int(prec), buf, 64)
+    return buf.raw[:n].decode()

@@ -164,6 +164,15 @@ def lib() -> C.CDLL:
         "tm_init_gauss": ([vp, C.c_int, dp, C.c_int, dp], C.c_int),
         "tm_mft_frame_values": ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp],
                                 C.c_int),
+        "tm_write_frame_csv": ([vp, C.c_int, C.c_char_p, C.c_double, C.c_char_p, C.c_int], C.c_int),
+        "tm_format_frame_csv": ([C.c_char_p, dp, C.c_int, C.c_int, C.c_double, C.c_char_p, C.c_int], C.c_int),
+        "tm_write_prelim_csv": ([C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int, dp, dp, C.c_int,
+                                 C.c_int], C.c_int),
+        "tm_write_sweep_csv": ([C.c_char_p, C.c_char_p, dp, C.c_int, C.c_int], C.c_int),
+        "tm_save_snapshot": ([vp, C.c_int, C.c_char_p], C.c_int),
+        "tm_load_snapshot": ([vp, C.c_int, C.c_char_p, C.POINTER(i64)], C.c_int),
+        "tm_snapshot_to_frame_csv": ([C.c_char_p, C.c_char_p, C.c_double, C.c_char_p, C.c_int], C.c_int),
+        "tm_read_frame_csv": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_int, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -228,6 +237,68 @@ def mft_frame_values(homogeneous: bool, R: float, R_c: float, dP: float, c_sprea
     if lib().tm_mft_frame_values(int(homogeneous), R, R_c, dP, nf, _dptr(cs), _dptr(e), _dptr(hi), _dptr(lo)):
         raise ValueError("tm_mft_frame_values: bad arguments")
     return hi, lo
+
+
+# The reference's file headers (Rietkerk_FPE/rietkerk_bjork_constants_1Sp.h:6-9,
+# Percolation_FD/MultiSPDP_constants_1Sp.h:6-9).
+FRAME_HEADER = {
+    MODEL_RIETKERK: "a_c,  x,  P(x; t), W(x; t), O(x; t) \n",
+    MODEL_DP: "a_c,  x,  P(x; t) \n",
+}
+PRELIM_HEADER = {
+    MODEL_RIETKERK: (" a , r, L, t , <<W(x; t)>_x>_r, <<O(x; t)>_x>_r,  <<P(x; t)>_x>_r, Var[<P(x; t)>_x]_r, "
+                     "# Surviving Runs P(x; t), # Active Sites P(x; t), \n"),
+    MODEL_DP: (" a , r, L, t , <<P(x; t)>_x>_r, Var[<P(x; t)>_x]_r, # Surviving Runs P(x; t),"
+               " # Active Sites P(x; t), \n"),
+}
+
+
+def _b(x):
+    return None if x is None else str(x).encode()
+
+
+def _io_check(rc: int, what: str):
+    if rc < 0:
+        raise OSError(f"{what} failed ({rc})")
+    return rc
+
+
+def format_frame_csv(path, fields, a: float, header: str | None = None, digits: int = 6):
+    """FRAME file from a host frame [nf, L, L] (save_frame's layout)."""
+    f = np.ascontiguousarray(fields, np.float64)
+    nf, L = f.shape[0], f.shape[-1]
+    _io_check(lib().tm_format_frame_csv(_b(path), _dptr(f), nf, L, a, _b(header), digits), "tm_format_frame_csv")
+
+
+def write_prelim_csv(path, header, a: float, r: int, g: int, t, stats):
+    """PRELIM file: rows a, r+1, g, t[i], stats[i, :] (save_prelimframe's layout)."""
+    tt = np.ascontiguousarray(t, np.float64)
+    st = np.ascontiguousarray(stats, np.float64).reshape(tt.size, -1)
+    _io_check(lib().tm_write_prelim_csv(_b(path), _b(header), a, r, g, _dptr(tt), _dptr(st), tt.size, st.shape[1]),
+              "tm_write_prelim_csv")
+
+
+def write_sweep_csv(path, header, rows):
+    """The sweep's final file: header line, then rows with precision 16."""
+    x = np.ascontiguousarray(rows, np.float64)
+    _io_check(lib().tm_write_sweep_csv(_b(path), _b(header), _dptr(x), x.shape[0], x.shape[1]),
+              "tm_write_sweep_csv")
+
+
+def snapshot_to_frame_csv(snap, csv, a: float, header: str | None = None, digits: int = 6):
+    _io_check(lib().tm_snapshot_to_frame_csv(_b(snap), _b(csv), a, _b(header), digits), "tm_snapshot_to_frame_csv")
+
+
+def read_frame_csv(path, L: int, nf: int, cols, out=None):
+    """Burn-in input: the named FRAME-file columns into fields [nf, L, L]; returns
+    (fields, rows read).  Fields not named keep the values of `out` (zeros)."""
+    f = np.zeros((nf, L, L)) if out is None else np.ascontiguousarray(out, np.float64).copy()
+    names = (C.c_char_p * len(cols))(*[c.encode() for c in cols])
+    n = lib().tm_read_frame_csv(_b(path), L, nf, names, len(cols), _dptr(f))
+    if n == -2:
+        raise KeyError(f"a column of {list(cols)} is missing in {path}")
+    _io_check(n, "tm_read_frame_csv")
+    return f, n
 
 
 # Column names of Context.replicate_stats rows, per model (field 0 is P / rho).
@@ -349,6 +420,18 @@ class Context:
         out = np.empty((self._n_params, self._n_series, self.nfields + 3), np.float64)
         self._check(self._lib.tm_replicate_stats(self._h, flags, _dptr(out)))
         return out
+
+    def write_frame_csv(self, i: int, path, a: float, header: str | None = None, digits: int = 6):
+        self._check(self._lib.tm_write_frame_csv(self._h, i, _b(path), a, _b(header), digits))
+
+    def save_snapshot(self, i: int, path):
+        self._check(self._lib.tm_save_snapshot(self._h, i, _b(path)))
+
+    def load_snapshot(self, i: int, path) -> int:
+        """Load instance i's fields from a snapshot; returns the recorded step."""
+        st = C.c_int64()
+        self._check(self._lib.tm_load_snapshot(self._h, i, _b(path), C.byref(st)))
+        return st.value
 
     def measure(self) -> np.ndarray:
         out = np.empty((len(self.instances), 2 * self.nfields), np.float64)

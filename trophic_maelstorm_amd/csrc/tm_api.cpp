@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "tm/tm.h"
+#include "tm_io.h"
 #include "tm_launch.h"
 
 using namespace tmk;
@@ -591,6 +592,70 @@ tm_status tm_init_uniform(tm_ctx* c, const double* lo, const double* hi)
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TM_OK;
+}
+
+// ---- output layer: frames, snapshots ----------------------------------------
+
+tm_status tm_write_frame_csv(tm_ctx* c, int i, const char* path, double a, const char* header, int digits)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (i < 0 || i >= (int)c->inst.size() || !path || digits < 1 || digits > 17)
+        return fail(c, TM_ERR_ARG, "tm_write_frame_csv: bad instance, path or digits");
+    const int L = c->inst[(size_t)i].L;
+    std::vector<double> host((size_t)c->nf * L * L);
+    if ((s = tm_download_fields(c, i, host.data())) != TM_OK) return s;
+    if (tm_format_frame_csv(path, host.data(), c->nf, L, a, header, digits) != 0)
+        return fail(c, TM_ERR_ARG, std::string("tm_write_frame_csv: cannot write ") + path);
+    return TM_OK;
+}
+
+tm_status tm_save_snapshot(tm_ctx* c, int i, const char* path)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (i < 0 || i >= (int)c->inst.size() || !path) return fail(c, TM_ERR_ARG, "tm_save_snapshot: bad instance or path");
+    const tm_instance_desc& d = c->inst[(size_t)i];
+    const size_t bytes = c->esize * (size_t)c->nf * d.L * d.L;
+    std::vector<char> host(bytes);
+    const char* src = (const char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[(size_t)i];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(host.data(), src, bytes, hipMemcpyDeviceToHost));
+    const tmio::SnapInfo info{c->model, c->prec, c->nf, d.L, d.seed, d.instance_id, d.step0 + c->counter,
+                              d.params_index};
+    if (tmio::write_snapshot(path, info, host.data(), bytes) != 0)
+        return fail(c, TM_ERR_ARG, std::string("tm_save_snapshot: cannot write ") + path);
+    return TM_OK;
+}
+
+tm_status tm_load_snapshot(tm_ctx* c, int i, const char* path, int64_t* step)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (i < 0 || i >= (int)c->inst.size() || !path) return fail(c, TM_ERR_ARG, "tm_load_snapshot: bad instance or path");
+    tmio::SnapInfo info;
+    std::vector<char> data;
+    if (tmio::read_snapshot(path, &info, &data) != 0)
+        return fail(c, TM_ERR_ARG, std::string("tm_load_snapshot: not a readable snapshot: ") + path);
+    const tm_instance_desc& d = c->inst[(size_t)i];
+    if (info.model != c->model || info.nf != c->nf || info.L != d.L)
+        return fail(c, TM_ERR_ARG, "tm_load_snapshot: model or lattice size differs from the instance");
+    char* dst = (char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[(size_t)i];
+    const size_t n = (size_t)c->nf * d.L * d.L;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (info.precision == c->prec) {
+        HIPCHK(c, hipMemcpy(dst, data.data(), data.size(), hipMemcpyHostToDevice));
+    } else if (c->prec == TM_F64) {   // FP32 snapshot into FP64 fields: exact
+        std::vector<double> v(n);
+        for (size_t k = 0; k < n; ++k) v[k] = ((const float*)data.data())[k];
+        HIPCHK(c, hipMemcpy(dst, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    } else {                          // FP64 snapshot into FP32 fields: rounded as tm_upload_fields does
+        std::vector<float> v(n);
+        for (size_t k = 0; k < n; ++k) v[k] = (float)((const double*)data.data())[k];
+        HIPCHK(c, hipMemcpy(dst, v.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (step) *step = info.step;
     return TM_OK;
 }
 

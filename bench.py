@@ -143,6 +143,50 @@ def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=N
                 rows=rows, final=final)
 
 
+def sweep_leg(tm, steps=None):
+    """The sweep pieces on the C3 batch (FP32 Rietkerk, 16 R x 16 replicates of
+    512^2): device frames, a measured run over the reference's log-spaced times,
+    the replicate statistics, and the output layer on one instance."""
+    import tempfile
+    b = c3_batch(tm, tm.F32)
+    dt = b["params"][0].dt
+    t_meas = tm.sweep_times(25.0, dt)                       # 5000 steps of dt = 0.005
+    meas = tm.measure_steps(t_meas, dt)
+    if steps:   # a bounded run: the points before `steps`, then `steps` itself
+        meas = np.append(meas[meas < steps], np.int64(steps))
+    cells = cells_of(b)
+    out = {"measurements": int(meas.size), "steps": int(meas[-1])}
+    with tm.Context(0) as ctx:
+        ctx.configure(b["model"], b["precision"], b["params"], b["inst"])
+        t0 = time.perf_counter()
+        ctx.init_gauss([5.0, 1.0, 1.0], 1, [8.0, 16.0])
+        t1 = time.perf_counter()
+        ctx.init_speckle(0.015, [90.0, 1.0, 1.0], [0.0, 0.2, 0.2])
+        t2 = time.perf_counter()
+        out["init_gauss_gbs"] = cells * 3 * 4 / (t1 - t0) / 1e9
+        out["init_speckle_gbs"] = cells * 3 * 4 / (t2 - t1) / 1e9
+        ctx.init_uniform(b["lo"], b["hi"])
+        t0 = time.perf_counter()
+        ctx.run_series(meas, copy=False)
+        t1 = time.perf_counter()
+        out["series_value"] = cells * int(meas[-1]) / (t1 - t0)
+        out["series_step_device_ms"] = ctx.last_run_ms() / int(meas[-1])
+        t0 = time.perf_counter()
+        st = ctx.replicate_stats(tm.STATS_REFERENCE)
+        t1 = time.perf_counter()
+        out["replicate_stats_ms"] = (t1 - t0) * 1e3
+        out["surviving_at_end"] = st[:, -1, 4].tolist()
+        with tempfile.TemporaryDirectory() as d:
+            t0 = time.perf_counter()
+            ctx.save_snapshot(0, os.path.join(d, "s.tms"))
+            t1 = time.perf_counter()
+            ctx.write_frame_csv(0, os.path.join(d, "f.csv"), b["params"][0].R, tm.FRAME_HEADER[b["model"]])
+            t2 = time.perf_counter()
+            out["snapshot_mbs"] = 512 * 512 * 3 * 4 / (t1 - t0) / 1e6
+            out["frame_csv_rows_per_s"] = 512 * 512 / (t2 - t1)
+    return out
+
+
 def rho_error(r32, r64, nf):
     """Errors of FP32 rho(t) against the FP64 run of the same seeds, per field.
     ensemble: mean over instances of the spatial mean, per step -- relative error
@@ -233,7 +277,7 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="headline only")
     ap.add_argument("--variant", default=None,
                     help="kernel variant for the headline (or --only) run instead of the autotuner's choice")
-    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c5"], default=None,
+    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c5", "sweep"], default=None,
                     help="only this side workload, --steps steps (default: its full length), one JSON line")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
@@ -259,6 +303,9 @@ def main():
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    if args.only == "sweep":
+        print(json.dumps({"workload": "c3_sweep", **sweep_leg(tm, args.steps)}), flush=True)
+        return
     if args.only:
         b = {"c2": lambda: c2_batch(tm, tm.F64), "c3": lambda: c3_batch(tm, tm.F32),
              "c3-f64": lambda: c3_batch(tm, tm.F64), "c5": lambda: c5_batch(tm, tm.F32)}[args.only]()
@@ -308,6 +355,10 @@ def main():
         errors["c3"] = dict(rho=rho_error(r3["rows"], r3d["rows"], 3), fields=field_error(r3["final"], r3d["final"]))
         side["ratio_c4_fp32_over_c2_plain_fp64"] = h["value"] / r2["value"]
         side["ratio_c3_fp32_over_c2_plain_fp64"] = r3["value"] / r2["value"]
+        log("[bench] sweep pieces on the C3 batch (frames, series, replicate stats, output)")
+        side["c3_sweep"] = sweep_leg(tm)
+        log(f"[bench]   series {side['c3_sweep']['series_value']:.4g} cell-updates/s, "
+            f"stats {side['c3_sweep']['replicate_stats_ms']:.2f} ms")
         if args.c5_steps > 0:
             c5 = c5_batch(tm, tm.F32, args.c5_steps)
             log(f"[bench] C5 FP32 Rietkerk 8192^2, {args.c5_steps} of its 10^5 steps")

@@ -90,6 +90,10 @@ struct tm_ctx {
     int64_t groups_cap = 0;
     double* d_stats = nullptr;         // [n_params][n_series][nf + 3]
     int64_t stats_cap = 0;
+
+    // initial-frame launch table: instance workgroup starts, per-point values
+    void* d_frame_tab = nullptr;
+    int64_t frame_tab_cap = 0;
 };
 
 namespace {
@@ -153,7 +157,8 @@ void release_config(tm_ctx* c)
     free_dev(c->d_series_steps);
     free_dev(c->d_groups);
     free_dev(c->d_stats);
-    c->series_cap = c->groups_cap = c->stats_cap = 0;
+    free_dev(c->d_frame_tab);
+    c->series_cap = c->groups_cap = c->stats_cap = c->frame_tab_cap = 0;
     c->series_steps_cap = c->n_series = 0;
     c->rows_cap = 0;
     c->nslots = 0;
@@ -579,20 +584,52 @@ tm_status tm_download_fields(tm_ctx* c, int i, double* host)
     return TM_OK;
 }
 
+// One launch of the frame kernel over every instance: the workgroup start of
+// each instance and the values of each parameter point go to a small device table.
+static tm_status enqueue_frames(tm_ctx* c, int mode, int n_gauss, const double* sd_div,
+                                const std::vector<FrameVals>& vals)
+{
+    const int n = (int)c->inst.size();
+    std::vector<int64_t> blk0((size_t)n + 1, 0);
+    for (int k = 0; k < n; ++k) {
+        const int64_t cells = (int64_t)c->inst[(size_t)k].L * c->inst[(size_t)k].L;
+        blk0[(size_t)k + 1] = blk0[(size_t)k] + (cells + kFrameCellsPerBlock - 1) / kFrameCellsPerBlock;
+    }
+    const size_t bytes = sizeof(int64_t) * blk0.size() + sizeof(FrameVals) * vals.size();
+    if ((int64_t)bytes > c->frame_tab_cap) {
+        free_dev(c->d_frame_tab);
+        c->frame_tab_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_frame_tab, bytes));
+        c->frame_tab_cap = (int64_t)bytes;
+    }
+    std::vector<char> host(bytes);
+    std::memcpy(host.data(), blk0.data(), sizeof(int64_t) * blk0.size());
+    std::memcpy(host.data() + sizeof(int64_t) * blk0.size(), vals.data(), sizeof(FrameVals) * vals.size());
+    HIPCHK(c, hipMemcpyAsync(c->d_frame_tab, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
+    const int64_t* d_blk0 = (const int64_t*)c->d_frame_tab;
+    const FrameVals* d_vals = (const FrameVals*)((const char*)c->d_frame_tab + sizeof(int64_t) * blk0.size());
+    launch_init_frames(c->prec, mode, c->d_inst, d_blk0, n, blk0[(size_t)n], c->nf, n_gauss, sd_div, d_vals,
+                       c->d_fields[c->parity], c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // the staging vector lives until the copy is done
+    return TM_OK;
+}
+
 tm_status tm_init_uniform(tm_ctx* c, const double* lo, const double* hi)
 {
     tm_status s = check_ready(c);
     if (s != TM_OK) return s;
     if (!lo || !hi) return fail(c, TM_ERR_ARG, "null bounds");
-    for (size_t i = 0; i < c->inst.size(); ++i) {
-        void* dst = (char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[i];
-        const tm_instance_desc& d = c->inst[i];
-        if (c->prec == TM_F64) launch_init_f64(c->nf, d.L, d.seed, d.instance_id, lo, hi, dst, c->stream);
-        else launch_init_f32(c->nf, d.L, d.seed, d.instance_id, lo, hi, dst, c->stream);
+    std::vector<FrameVals> vals((size_t)c->n_params);
+    for (auto& v : vals) {
+        v = FrameVals{};
+        for (int f = 0; f < c->nf; ++f) {
+            v.a[f] = lo[f];
+            v.b[f] = hi[f];
+        }
     }
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return TM_OK;
+    const double sd_none[3] = {1.0, 1.0, 1.0};
+    return enqueue_frames(c, 2, 0, sd_none, vals);
 }
 
 // ---- output layer: frames, snapshots ----------------------------------------
@@ -664,21 +701,18 @@ tm_status tm_init_speckle(tm_ctx* c, const double* perc, const double* high, con
     tm_status s = check_ready(c);
     if (s != TM_OK) return s;
     if (!perc || !high || !low) return fail(c, TM_ERR_ARG, "tm_init_speckle: null values");
-    for (size_t i = 0; i < c->inst.size(); ++i) {
-        const tm_instance_desc& d = c->inst[i];
-        const int g = d.params_index;
-        FrameVals v{};
+    std::vector<FrameVals> vals((size_t)c->n_params);
+    for (int g = 0; g < c->n_params; ++g) {
+        FrameVals& v = vals[(size_t)g];
+        v = FrameVals{};
         v.perc = perc[g];
         for (int f = 0; f < c->nf; ++f) {
             v.a[f] = high[(size_t)g * c->nf + f];
             v.b[f] = low[(size_t)g * c->nf + f];
         }
-        void* dst = (char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[i];
-        launch_init_speckle(c->prec, c->nf, d.L, d.seed, d.instance_id, v, dst, c->stream);
     }
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return TM_OK;
+    const double sd_none[3] = {1.0, 1.0, 1.0};
+    return enqueue_frames(c, 0, 0, sd_none, vals);
 }
 
 tm_status tm_init_gauss(tm_ctx* c, int n_gauss, const double* sd_div, int n_sd, const double* amp)
@@ -687,26 +721,18 @@ tm_status tm_init_gauss(tm_ctx* c, int n_gauss, const double* sd_div, int n_sd, 
     if (s != TM_OK) return s;
     if (!amp || n_gauss < 0 || n_gauss > c->nf) return fail(c, TM_ERR_ARG, "tm_init_gauss: bad n_gauss or amp");
     if (n_gauss > 0 && (!sd_div || n_sd < 1)) return fail(c, TM_ERR_ARG, "tm_init_gauss: need sd_div[n_sd >= 1]");
-    for (int k = 0; k < n_sd && k < n_gauss; ++k)
-        if (!(sd_div[k] > 0.0)) return fail(c, TM_ERR_ARG, "tm_init_gauss: sd_div must be > 0");
-    for (size_t i = 0; i < c->inst.size(); ++i) {
-        const tm_instance_desc& d = c->inst[i];
-        const int L = d.L;
-        FrameVals v{};
-        for (int f = 0; f < c->nf; ++f) v.a[f] = amp[(size_t)d.params_index * c->nf + f];
-        // Centres go round the corners from the top left (rietkerk_bjork_FKE.cpp:1110-1115);
-        // missing widths repeat the last one given (:1118-1127).
-        for (int f = 0; f < n_gauss; ++f) {
-            v.cx[f] = (f % 4) % 3 == 0 ? (double)(L / 8) : (double)(7 * L / 8);
-            v.cy[f] = (f % 4) < 2 ? (double)(L / 8) : (double)(7 * L / 8);
-            v.sd[f] = L / sd_div[std::min(f, n_sd - 1)];
-        }
-        void* dst = (char*)c->d_fields[c->parity] + c->esize * c->nf * c->cell_off[i];
-        launch_init_gauss(c->prec, c->nf, L, n_gauss, v, dst, c->stream);
+    // Missing widths repeat the last one given (rietkerk_bjork_FKE.cpp:1118-1127).
+    double sd[3] = {1.0, 1.0, 1.0};
+    for (int f = 0; f < n_gauss; ++f) {
+        sd[f] = sd_div[std::min(f, n_sd - 1)];
+        if (!(sd[f] > 0.0)) return fail(c, TM_ERR_ARG, "tm_init_gauss: sd_div must be > 0");
     }
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return TM_OK;
+    std::vector<FrameVals> vals((size_t)c->n_params);
+    for (int g = 0; g < c->n_params; ++g) {
+        vals[(size_t)g] = FrameVals{};
+        for (int f = 0; f < c->nf; ++f) vals[(size_t)g].a[f] = amp[(size_t)g * c->nf + f];
+    }
+    return enqueue_frames(c, 1, n_gauss, sd, vals);
 }
 
 int tm_mft_frame_values(int homogeneous, double R, double R_c, double dP, int nf, const double* c_spread,

@@ -1,4 +1,5 @@
-// tm_frames.hip -- initial frames generated on the device, one thread per cell.
+// tm_frames.hip -- initial frames generated on the device, every instance of a
+// batch in one launch, one thread per cell.
 //
 // * Speckle frames: each cell draws one uniform u from Philox(seed, instance_id;
 //   cell, 0xFFFFFFFE, 0xFFFFFFFF, 0) (word 0, (w + 0.5) 2^-32) and every field
@@ -7,9 +8,17 @@
 //   reference's bistable and mean-field speckle frames reuse with other values.
 // * Gaussian frames: fields f < n_gauss hold amp_f exp(-((p - cx_f)^2 + (q -
 //   cy_f)^2) / (2 sd_f sd_f)) at cell i = p L + q, the others amp_f
-//   (init_gaussiantear, rietkerk_bjork_FKE.cpp:1185-1199).
+//   (init_gaussiantear, rietkerk_bjork_FKE.cpp:1185-1199).  Centres go round the
+//   corners from the top left, cx = [L/8, 7L/8, 7L/8, L/8] and cy = [L/8, L/8,
+//   7L/8, 7L/8] in integer division (:1110-1115); sd_f = L / sd_div_f.
 // Values are formed in FP64, in the reference's operation order with this
 // library's own exp, and stored in the field precision.
+// * Uniform frames (tm_init_uniform): word f of Philox(seed, instance_id; cell,
+//   0xFFFFFFFF, 0xFFFFFFFF, 0) gives field f = lo_f + (hi_f - lo_f) u.
+// Workgroup b covers kFrameCellsPerBlock consecutive cells of the instance k
+// with blk0[k] <= b < blk0[k+1] (its 256 threads stride through them, so every
+// store is a contiguous wave); the values of the instance's parameter point
+// come from vals[params_index].
 #include <hip/hip_runtime.h>
 #include "tm_launch.h"
 
@@ -17,61 +26,76 @@ namespace tmk {
 namespace {
 
 template <typename T>
-__global__ __launch_bounds__(256) void init_speckle_kernel(int L, int nf, uint32_t seed, uint32_t inst_id,
-                                                           FrameVals v, T* __restrict__ fields)
+__global__ __launch_bounds__(256) void init_frames_kernel(int mode, const DevInst* __restrict__ inst,
+                                                          const int64_t* __restrict__ blk0, int n_inst, int nf,
+                                                          int n_gauss, double sd0, double sd1, double sd2,
+                                                          const FrameVals* __restrict__ vals,
+                                                          T* __restrict__ fields)
 {
 #pragma clang fp contract(off)
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = n_inst;            // blk0[lo] <= b < blk0[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (blk0[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const DevInst d = inst[lo];
+    const int L = d.L;
     const int64_t N = (int64_t)L * L;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    const Philox4 w = philox4x32_10((uint32_t)i, 0xFFFFFFFEu, 0xFFFFFFFFu, 0u, seed, inst_id);
-    const double u = ((double)w.v0 + 0.5) * 0x1p-32;
-    const bool high = u <= v.perc;
-    for (int f = 0; f < nf; ++f) fields[f * N + i] = (T)(high ? v.a[f] : v.b[f]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void init_gauss_kernel(int L, int nf, int n_gauss, FrameVals v,
-                                                         T* __restrict__ fields)
-{
-#pragma clang fp contract(off)
-    const int64_t N = (int64_t)L * L;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    const int p = (int)(i / L), q = (int)(i % L);
-    for (int f = 0; f < nf; ++f) {
-        double x = v.a[f];
-        if (f < n_gauss) {
-            const double dp = p - v.cx[f], dq = q - v.cy[f];
-            x = v.a[f] * dexp(-(dp * dp + dq * dq) / (2 * v.sd[f] * v.sd[f]));
+    T* f0 = fields + (int64_t)nf * d.cell_off;
+    const FrameVals v = vals[d.pidx];
+    const double sd_div[3] = {sd0, sd1, sd2};
+    const int64_t base = (b - blk0[lo]) * kFrameCellsPerBlock;
+    for (int r = 0; r < kFrameCellsPerBlock / 256; ++r) {
+        const int64_t i = base + r * 256 + threadIdx.x;
+        if (i >= N) return;
+        if (mode != 1) {
+            const uint32_t c1 = mode == 0 ? 0xFFFFFFFEu : 0xFFFFFFFFu;
+            const Philox4 w = philox4x32_10((uint32_t)i, c1, 0xFFFFFFFFu, 0u, d.seed, d.inst_id);
+            if (mode == 0) {            // speckle: word 0 decides high or low for every field
+                const double u = ((double)w.v0 + 0.5) * 0x1p-32;
+                const bool high = u <= v.perc;
+                for (int f = 0; f < nf; ++f) f0[f * N + i] = (T)(high ? v.a[f] : v.b[f]);
+            } else {                    // uniform: word f gives field f in [a_f, b_f)
+                const uint32_t ws[3] = {w.v0, w.v1, w.v2};
+                for (int f = 0; f < nf; ++f) {
+                    const double u = ((double)ws[f] + 0.5) * 0x1p-32;
+                    f0[f * N + i] = (T)(v.a[f] + (v.b[f] - v.a[f]) * u);
+                }
+            }
+            continue;
         }
-        fields[f * N + i] = (T)x;
+        // cell indices fit 32 bits (the Philox counter word is the 32-bit cell index)
+        const uint32_t iu = (uint32_t)i;
+        const int p = (int)(iu / (uint32_t)L), q = (int)(iu - (uint32_t)p * (uint32_t)L);
+        for (int f = 0; f < nf; ++f) {
+            double x = v.a[f];
+            if (f < n_gauss) {
+                const double cx = (f % 4) % 3 == 0 ? (double)(L / 8) : (double)(7 * L / 8);
+                const double cy = (f % 4) < 2 ? (double)(L / 8) : (double)(7 * L / 8);
+                const double sd = L / sd_div[f];
+                const double dp = p - cx, dq = q - cy;
+                x = v.a[f] * dexp(-(dp * dp + dq * dq) / (2 * sd * sd));
+            }
+            f0[f * N + i] = (T)x;
+        }
     }
 }
 
-inline int cells_grid(int L) { return (int)(((int64_t)L * L + 255) / 256); }
-
 }  // namespace
 
-void launch_init_speckle(int prec, int nf, int L, uint32_t seed, uint32_t inst_id, const FrameVals& v,
-                         void* fields, hipStream_t s)
+void launch_init_frames(int prec, int mode, const DevInst* inst, const int64_t* blk0, int n_inst, int64_t nblocks,
+                        int nf, int n_gauss, const double* sd_div, const FrameVals* vals, void* fields,
+                        hipStream_t s)
 {
+    if (nblocks <= 0) return;
     if (prec == TM_F64)
-        hipLaunchKernelGGL(init_speckle_kernel<double>, dim3(cells_grid(L)), dim3(256), 0, s, L, nf, seed, inst_id,
-                           v, (double*)fields);
+        hipLaunchKernelGGL(init_frames_kernel<double>, dim3((unsigned)nblocks), dim3(256), 0, s, mode, inst, blk0,
+                           n_inst, nf, n_gauss, sd_div[0], sd_div[1], sd_div[2], vals, (double*)fields);
     else
-        hipLaunchKernelGGL(init_speckle_kernel<float>, dim3(cells_grid(L)), dim3(256), 0, s, L, nf, seed, inst_id,
-                           v, (float*)fields);
-}
-
-void launch_init_gauss(int prec, int nf, int L, int n_gauss, const FrameVals& v, void* fields, hipStream_t s)
-{
-    if (prec == TM_F64)
-        hipLaunchKernelGGL(init_gauss_kernel<double>, dim3(cells_grid(L)), dim3(256), 0, s, L, nf, n_gauss, v,
-                           (double*)fields);
-    else
-        hipLaunchKernelGGL(init_gauss_kernel<float>, dim3(cells_grid(L)), dim3(256), 0, s, L, nf, n_gauss, v,
-                           (float*)fields);
+        hipLaunchKernelGGL(init_frames_kernel<float>, dim3((unsigned)nblocks), dim3(256), 0, s, mode, inst, blk0,
+                           n_inst, nf, n_gauss, sd_div[0], sd_div[1], sd_div[2], vals, (float*)fields);
 }
 
 }  // namespace tmk

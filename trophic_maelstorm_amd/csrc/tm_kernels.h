@@ -1030,26 +1030,4 @@ __global__ __launch_bounds__(256) void combine_rows_kernel(CombineArgs a)
 }
 #endif
 
-// ---------------------------------------------------------------------------
-// i.i.d. Uniform(lo, hi) initial fields, one thread per cell of one instance.
-// ---------------------------------------------------------------------------
-template <typename T, int NF>
-__global__ __launch_bounds__(256) void init_uniform_kernel(int L, uint32_t seed, uint32_t inst_id, double lo0,
-                                                           double hi0, double lo1, double hi1, double lo2,
-                                                           double hi2, T* fields)
-{
-#pragma clang fp contract(off)
-    const int64_t N = (int64_t)L * L;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    const Philox4 w = philox4x32_10((uint32_t)i, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, seed, inst_id);
-    const uint32_t ws[3] = {w.v0, w.v1, w.v2};
-    const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        const double u = ((double)ws[f] + 0.5) * 0x1p-32;
-        fields[f * N + i] = (T)(lo[f] + (hi[f] - lo[f]) * u);
-    }
-}
-
 }  // namespace tmk

@@ -57,17 +57,4 @@ void launch_measure_f32(int nf, const DevInst* inst, const DevTile* tiles, int n
                            ty, (const float*)fields, partials);
 }
 
-void launch_init_f32(int nf, int L, uint32_t seed, uint32_t inst_id, const double* lo, const double* hi,
-                     void* fields, hipStream_t s)
-{
-    const int64_t N = (int64_t)L * L;
-    const int grid = (int)((N + 255) / 256);
-    if (nf == 1)
-        hipLaunchKernelGGL((init_uniform_kernel<float, 1>), dim3(grid), dim3(256), 0, s, L, seed, inst_id, lo[0],
-                           hi[0], 0.0, 0.0, 0.0, 0.0, (float*)fields);
-    else
-        hipLaunchKernelGGL((init_uniform_kernel<float, 3>), dim3(grid), dim3(256), 0, s, L, seed, inst_id, lo[0],
-                           hi[0], lo[1], hi[1], lo[2], hi[2], (float*)fields);
-}
-
 }  // namespace tmk

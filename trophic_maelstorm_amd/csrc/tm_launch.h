@@ -33,27 +33,25 @@ void launch_measure_f64(int nf, const DevInst* inst, const DevTile* tiles, int n
                         const void* fields, double2* partials, hipStream_t s);
 void launch_measure_f32(int nf, const DevInst* inst, const DevTile* tiles, int ntiles, int tx, int ty,
                         const void* fields, double2* partials, hipStream_t s);
-void launch_init_f64(int nf, int L, uint32_t seed, uint32_t inst_id, const double* lo, const double* hi,
-                     void* fields, hipStream_t s);
-void launch_init_f32(int nf, int L, uint32_t seed, uint32_t inst_id, const double* lo, const double* hi,
-                     void* fields, hipStream_t s);
 void launch_combine(const CombineArgs& a, int n_inst, int nsteps, hipStream_t s);
 void launch_prep(int model, const DevInst* inst, const void* params, const DevState* state, int n_inst,
                  int nsteps, void* table, hipStream_t s);
 void launch_bump(DevState* state, int n, hipStream_t s);
 size_t const_size(int model);
 
-// Initial frames (tm_frames.hip).  Per-field values of one instance: speckle
-// frames use perc, a = high, b = low; Gaussian frames use a = amp and the
-// centres and widths cx, cy, sd of the first n_gauss fields.
+// Initial frames (tm_frames.hip), one launch for the whole batch.  Values per
+// parameter point: speckle frames use perc, a = high, b = low; Gaussian frames
+// use a = amp; uniform frames a = lo, b = hi.  mode 0 = speckle, 1 = Gaussian,
+// 2 = uniform; blk0[n_inst + 1] = first workgroup of each instance
+// (kFrameCellsPerBlock cells per workgroup).
+constexpr int kFrameCellsPerBlock = 4096;
 struct FrameVals {
     double perc;
     double a[3], b[3];
-    double cx[3], cy[3], sd[3];
 };
-void launch_init_speckle(int prec, int nf, int L, uint32_t seed, uint32_t inst_id, const FrameVals& v,
-                         void* fields, hipStream_t s);
-void launch_init_gauss(int prec, int nf, int L, int n_gauss, const FrameVals& v, void* fields, hipStream_t s);
+void launch_init_frames(int prec, int mode, const DevInst* inst, const int64_t* blk0, int n_inst, int64_t nblocks,
+                        int nf, int n_gauss, const double* sd_div, const FrameVals* vals, void* fields,
+                        hipStream_t s);
 
 // Sweep series and replicate statistics (tm_ensemble.hip).
 void launch_gather_series(const double* rows, int64_t rows_cap, int w, const int64_t* steps, int m0, int m1,

@@ -79,3 +79,56 @@ def test_dp_ensemble_matches_reference_integrator(oracle, a):
         assert zscore < 4.5, (a, t, "occupied", row[6], cnt[alive].mean(), zscore)
         checked += 2
     assert checked == 2 * (10 + len(T_MEAS))
+
+
+# ---------------------------------------------------------------- survival
+SURV_L, SURV_R = 8, 256
+SURV_T = [2.0, 5.0, 8.0, 10.0, 15.0, 20.0, 30.0, 40.0]
+
+
+@pytest.mark.parametrize("a", [0.5, 1.0])
+def test_dp_survival_matches_reference_integrator(oracle, a):
+    """Survival probability of a small lattice: the number of the reference's
+    replicates still occupied at each logged time (its column 5, from
+    var_mean_incremental_surv_runs) against the oracle's, 256 replicates each,
+    as a two-proportion z-test (< 4.5 at each of 8 times).  At a = 0.5 the
+    survival falls from 1 to ~0 by t = 40, at a = 1.0 from 1 to ~0.4.  The
+    reference's second tally (column 7, surv_rep) sits in an array one element
+    shorter than its times (stochasticSPDP.cpp:956), so its last entry is not
+    a count; it is compared at the other times."""
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("reference library not built: /root/reference sources absent")
+    rho0 = oracle.init_uniform(SURV_L, [0.0], [1.0], 77, 0).reshape(-1).copy()
+    out = np.zeros((64, 8), np.float64)
+    tm = np.array(SURV_T)
+    fd = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        n = ref.ref_ssp_percolation_dornic(SURV_L, a, B, D, SIGMA, DT, 1.0, SURV_R, rho0, tm, len(tm), SURV_T[-1],
+                                           out, out.shape[0])
+    finally:
+        os.dup2(fd, 1)
+        os.close(fd)
+        os.close(devnull)
+    rows = {round(r[2], 6): r for r in out[:n]}
+    nsteps = int(round(SURV_T[-1] / DT))
+    p = oracle.dp_params(a=a, b=B, D=D, sigma=SIGMA, dt=DT, scheme=2, sampler=0, flush_eps=0.0, flush_neg=0.0)
+    cnt = np.stack([oracle.dp_run(p, rho0, SURV_L, 4321, k, 0, nsteps)["rows"][:, 0] for k in range(SURV_R)])
+    # the oracle's survivors through the replicate statistics (break at the first empty time)
+    series = np.zeros((SURV_R, len(SURV_T), 2))
+    steps = [int(round(t / DT)) for t in SURV_T]
+    series[:, :, 0] = cnt[:, [s - 1 for s in steps]]
+    series[:, :, 1] = series[:, :, 0]
+    surv_orc = oracle.replicate_stats(series, np.zeros(SURV_R, np.int32), 1, 3)[0, :, 2]
+    for i, t in enumerate(SURV_T):
+        row = rows[round(t, 6)]
+        if i < len(SURV_T) - 1:
+            assert row[5] == row[7]                # the reference's two tallies agree
+        k_ref, k_orc = row[5], surv_orc[i]
+        assert k_orc == np.count_nonzero(cnt[:, steps[i] - 1] > 0)
+        pool = (k_ref + k_orc) / (2 * SURV_R)
+        se = math.sqrt(max(pool * (1 - pool), 1.0 / SURV_R) * 2 / SURV_R)
+        z = abs(k_ref - k_orc) / SURV_R / se
+        assert z < 4.5, (a, t, k_ref, k_orc, z)

@@ -277,7 +277,7 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="headline only")
     ap.add_argument("--variant", default=None,
                     help="kernel variant for the headline (or --only) run instead of the autotuner's choice")
-    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c5", "sweep"], default=None,
+    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4-f64", "c5", "sweep"], default=None,
                     help="only this side workload, --steps steps (default: its full length), one JSON line")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
@@ -308,6 +308,7 @@ def main():
         return
     if args.only:
         b = {"c2": lambda: c2_batch(tm, tm.F64), "c3": lambda: c3_batch(tm, tm.F32),
+             "c4-f64": lambda: c4_batch(tm, tm.F64),
              "c3-f64": lambda: c3_batch(tm, tm.F64), "c5": lambda: c5_batch(tm, tm.F32)}[args.only]()
         n = args.steps or b["steps"]
         log(f"[bench] {args.only}: {cells_of(b)} cells/step x {n} steps")

@@ -209,17 +209,25 @@ def test_absorbing_state_is_exact(tm, oracle):
         assert np.all(rows == 0.0)
 
 
-def test_numeric_fault_is_reported(tm, oracle):
-    rho = np.zeros((1, 16, 16))
+@pytest.mark.parametrize("variant", [None, "dp_f64_64x32_t256_x2", "dp_f64_64x32_t256_x4"])
+def test_numeric_fault_is_reported(tm, oracle, variant):
+    """The first fault (instance, cell, step) is reported.  With a multi-step
+    variant the run covers every step of one launch: the faults of step 0 must
+    win over the later faults their neighbourhoods produce at the next steps."""
+    L = 160
+    rho = np.zeros((1, L, L))
     rho[0, 5, 7] = 100.0     # explicit Euler drives it negative: 100 - 0.1*2*100^2 < 0
+    rho[0, 130, 140] = 100.0
     po = oracle.dp_params()
-    ref = oracle.dp_run(po, rho, 16, 1, 0, 0, 1)
+    ref = oracle.dp_run(po, rho, L, 1, 0, 0, 1)
     assert ref["status"] == 1
     with tm.Context() as ctx:
-        ctx.configure(tm.MODEL_DP, tm.F64, tm.DPParams(), [tm.Instance(16, 0, 1, 0)])
+        ctx.configure(tm.MODEL_DP, tm.F64, tm.DPParams(), [tm.Instance(L, 0, 1, 0)])
+        if variant:
+            ctx.set_launch(tm.variant_names(tm.MODEL_DP, tm.F64).index(variant), 0)
         ctx.upload(0, rho)
         with pytest.raises(tm.NumericFault):
-            ctx.run(1)
+            ctx.run(int(variant[-1]) if variant else 1)
         inst, cell, step = ctx.fault_info()
         assert (inst, step) == (0, 0)
         assert cell == ref["fault_cell"]
@@ -239,3 +247,29 @@ def test_autotune_does_not_change_state(tm, oracle, tmp_path):
         assert ctx.autotune(str(tmp_path / "tune.txt")) == (v, g)
         ms = ctx.time_steps(4)
         assert ms > 0 and ctx.step(1) == 7
+
+
+@pytest.mark.parametrize("variant", ["dp_f64_64x32_t256_x4a", "dp_f64_64x32_t256_x4"])
+def test_multistep_variant_through_extinction_bit_exact(tm, oracle, variant):
+    """A DP batch that dies out (a = 0.3) over 300 steps in blocks of 10: the
+    adaptive variant starts with single steps on the live field and turns to
+    four-step launches once a tenth or less of the cells is occupied; both kinds
+    of launch, and the switch between them, reproduce the oracle bit for bit."""
+    Ls = [32, 40, 17]
+    p = tm.DPParams(a=0.3)
+    po = oracle.dp_params(a=0.3)
+    insts = [tm.Instance(L, 0, 5, 300 + k) for k, L in enumerate(Ls)]
+    print(f"{variant} through extinction", flush=True)
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_DP, tm.F64, p, insts)
+        ctx.set_launch(tm.variant_names(tm.MODEL_DP, tm.F64).index(variant), 10)
+        ctx.init_uniform([0.0], [1.0])
+        init = [ctx.download(k) for k in range(len(Ls))]
+        rows = np.concatenate([ctx.run(150), ctx.run(150)], axis=1)
+        got = [ctx.download(k) for k in range(len(Ls))]
+    for k, d in enumerate(insts):
+        ref = oracle.dp_run(po, init[k][0], d.L, d.seed, d.instance_id, 0, 300)
+        assert np.array_equal(got[k], ref["fields"]), (variant, d.L)
+        assert_rows_close(rows[k], ref["rows"])
+    occupied = rows[:, :, 0].sum(axis=0) / sum(L * L for L in Ls)
+    assert occupied[0] > 0.9 and occupied[-1] <= 0.1      # both modes were used

@@ -73,7 +73,17 @@ struct tm_ctx {
     int parity = 0;
     int64_t counter = 0;
 
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [blocked][parity]
+
+    // Multi-step variants: blocked = launch K steps at a time, else single steps.
+    // Adaptive ones choose per block from the occupied fraction of field 0 at
+    // the end of the previous block (live_frac; < 0: not known yet).
+    bool blocked = true;
+    double live_frac = -1.0;
+    double* d_live = nullptr;   // [n_inst][2 nf] rows of a measure of the current state, then 2 sums
+    hipEvent_t live_ev[2] = {nullptr, nullptr};   // per-block live counts copied to h_pinned[2 + j]
+    bool live_pending[2] = {false, false};
+    int64_t live_block = 0;
 
     // device time of the last tm_run's launches (events on the context stream)
     hipEvent_t run_ev[2] = {nullptr, nullptr};
@@ -115,6 +125,13 @@ tm_status fail(tm_ctx* c, tm_status s, const std::string& msg)
     return s;
 }
 
+// The fields were replaced: the live measure of the old ones no longer applies.
+void invalidate_live(tm_ctx* c)
+{
+    c->live_frac = -1.0;
+    c->live_pending[0] = c->live_pending[1] = false;
+}
+
 const Variant* variant_table(int model, int prec, int* n)
 {
     if (model == TM_MODEL_DP) return prec == TM_F64 ? dp_variants_f64(n) : dp_variants_f32(n);
@@ -125,10 +142,11 @@ const Variant* variant_table(int model, int prec, int* n)
 
 void drop_graphs(tm_ctx* c)
 {
-    for (auto& g : c->graph) {
-        if (g) hipGraphExecDestroy(g);
-        g = nullptr;
-    }
+    for (auto& gp : c->graph)
+        for (auto& g : gp) {
+            if (g) hipGraphExecDestroy(g);
+            g = nullptr;
+        }
 }
 
 template <typename P>
@@ -158,6 +176,8 @@ void release_config(tm_ctx* c)
     free_dev(c->d_groups);
     free_dev(c->d_stats);
     free_dev(c->d_frame_tab);
+    free_dev(c->d_live);
+    invalidate_live(c);
     c->series_cap = c->groups_cap = c->stats_cap = c->frame_tab_cap = 0;
     c->series_steps_cap = c->n_series = 0;
     c->rows_cap = 0;
@@ -265,12 +285,30 @@ int grid_of(tm_ctx* c)
     return std::max(1, std::min(n, c->n_cu * v.wg_per_cu));
 }
 
-void launch_step(tm_ctx* c, int parity, int k)
+// One launch advancing steps k .. k + m - 1: m = the variant's steps per launch,
+// or 1 (a multi-step variant's single-step kernel: single-step mode, or a
+// block's last steps).
+void launch_step(tm_ctx* c, int parity, int k, int m = 1)
 {
     const StepArgs a = step_args(c, parity, k);
     const Variant& v = cur_variant(c);
     if (v.fn_a) v.fn_a(a, (int)c->tiles_a.size(), c->stream);   // split: P* first
-    v.fn(a, grid_of(c), c->stream);
+    const StepLaunch fn = (v.spl > 1 && m == 1) ? v.fn_one : v.fn;
+    fn(a, grid_of(c), c->stream);
+}
+
+// Steps per launch in the current mode.
+int steps_per_launch(tm_ctx* c)
+{
+    const Variant& v = cur_variant(c);
+    return (v.spl > 1 && c->blocked) ? v.spl : 1;
+}
+
+// Number of step launches (= buffer flips) that advance n steps in one block.
+int launches_of(tm_ctx* c, int64_t n)
+{
+    const int spl = steps_per_launch(c);
+    return (int)(n / spl + n % spl);
 }
 
 void launch_fold(tm_ctx* c, int nsteps)
@@ -295,9 +333,12 @@ void launch_fold(tm_ctx* c, int nsteps)
 void enqueue_block(tm_ctx* c, int parity, int n, bool fold)
 {
     launch_prep(c->model, c->d_inst, c->d_params, c->d_state, (int)c->inst.size(), n, c->d_consts, c->stream);
-    for (int k = 0; k < n; ++k) {
-        launch_step(c, parity, k);
+    const int spl = steps_per_launch(c);
+    for (int k = 0; k < n;) {
+        const int m = k + spl <= n ? spl : 1;
+        launch_step(c, parity, k, m);
         parity ^= 1;
+        k += m;
     }
     if (fold) launch_fold(c, n);
     launch_bump(c->d_state, n, c->stream);
@@ -322,8 +363,9 @@ tm_status capture_block(tm_ctx* c, int parity, int n, bool fold, hipGraphExec_t*
 
 tm_status capture(tm_ctx* c, int parity)
 {
-    if (c->graph[parity]) return TM_OK;
-    return capture_block(c, parity, c->graph_steps, true, &c->graph[parity]);
+    hipGraphExec_t& g = c->graph[c->blocked][parity];
+    if (g) return TM_OK;
+    return capture_block(c, parity, c->graph_steps, true, &g);
 }
 
 tm_status ensure_rows(tm_ctx* c, int64_t n_steps)
@@ -353,28 +395,139 @@ tm_status set_run_base(tm_ctx* c)
     return TM_OK;
 }
 
-// Advance n_steps with the rows fold; rows stay on the device.
+// rho(t) row of the current state of every instance into out[i * stride * 2 nf]
+// (device): tile partials of the current fields, then the rows fold.
+void enqueue_measure(tm_ctx* c, double* out, int64_t stride)
+{
+    const Variant& v = cur_variant(c);
+    if (c->prec == TM_F64)
+        launch_measure_f64(c->nf, c->d_inst, c->d_tiles, (int)c->tiles.size(), v.tx, v.ty, c->d_fields[c->parity],
+                           c->d_partials, c->stream);
+    else
+        launch_measure_f32(c->nf, c->d_inst, c->d_tiles, (int)c->tiles.size(), v.tx, v.ty, c->d_fields[c->parity],
+                           c->d_partials, c->stream);
+    CombineArgs a{};
+    a.inst = c->d_inst;
+    a.partials = c->d_partials;
+    a.state = c->d_state;
+    a.rows = out;
+    a.ntiles = (int32_t)c->tiles.size();
+    a.nslots = c->nslots;
+    a.nf = c->nf;
+    a.nsteps = 1;
+    a.rows_cap = stride;
+    a.measure = 1;
+    a.pw = 1;
+    launch_combine(a, (int)c->inst.size(), 1, c->stream);
+}
+
+tm_status ensure_live(tm_ctx* c)
+{
+    if (!c->d_live)
+        HIPCHK(c, hipMalloc((void**)&c->d_live, sizeof(double) * (2 * c->nf * c->inst.size() + 2)));
+    for (auto& e : c->live_ev)
+        if (!e) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return TM_OK;
+}
+
+// The occupied fraction of field 0 of the current state (one measure and a
+// wait), for an adaptive multi-step variant whose mode is not known yet.
+tm_status live_fraction_now(tm_ctx* c, double* frac)
+{
+    tm_status s = ensure_live(c);
+    if (s != TM_OK) return s;
+    const int n = (int)c->inst.size();
+    double* sums = c->d_live + 2 * c->nf * n;
+    enqueue_measure(c, c->d_live, 1);
+    launch_live_count(c->d_live, 1, 2 * c->nf, 0, n, sums, c->stream);
+    double occ = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&occ, sums, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *frac = occ / (double)c->total_cells;
+    return TM_OK;
+}
+
+// An adaptive variant takes single steps while more than this fraction of
+// field-0 cells is occupied: there the K-step kernel's recomputed rings cost
+// more sampling than its saved field traffic is worth (measured on the DP
+// quench, where the 4-step launches win once about 2% of the cells or fewer
+// are occupied).
+constexpr double kLiveThreshold = 0.02;
+
+tm_status update_mode(tm_ctx* c)
+{
+    const Variant& v = cur_variant(c);
+    if (v.spl == 1) return TM_OK;
+    if (!v.adaptive) {
+        c->blocked = true;
+        return TM_OK;
+    }
+    if (c->live_frac < 0.0) {
+        tm_status s = live_fraction_now(c, &c->live_frac);
+        if (s != TM_OK) return s;
+    }
+    c->blocked = c->live_frac <= kLiveThreshold;
+    return TM_OK;
+}
+
+// Advance n_steps with the rows fold; rows stay on the device.  Adaptive
+// multi-step variants re-choose their mode between blocks: after each block a
+// one-workgroup kernel sums field 0's occupied counts of the block's last row
+// and the sum is copied to pinned host memory behind an event; the choice
+// uses the newest sum whose event has completed (one or two blocks old), so
+// the host never waits for the device.
 tm_status advance(tm_ctx* c, int64_t n_steps)
 {
     tm_status s = ensure_rows(c, n_steps);
     if (s != TM_OK) return s;
+    if ((s = update_mode(c)) != TM_OK) return s;
     if ((s = set_run_base(c)) != TM_OK) return s;
+    const bool adaptive = cur_variant(c).spl > 1 && cur_variant(c).adaptive;
+    if (adaptive && (s = ensure_live(c)) != TM_OK) return s;
     int64_t left = n_steps;
     const int B = c->graph_steps;
+    volatile double* h_sum = (volatile double*)c->h_pinned + 2;   // [2], after the run-base staging word
+    auto after_block = [&]() -> tm_status {
+        if (!adaptive) return TM_OK;
+        // The sum of two blocks back, waited for: the host stays at most two
+        // blocks ahead of the device, which keeps the choice at most two blocks
+        // old while the device always has the newest block queued.  The sum of
+        // the block before, if already there, is newer.
+        const int j = (int)(c->live_block & 1);          // slot of two blocks back, reused now
+        if (c->live_pending[j]) {
+            HIPCHK(c, hipEventSynchronize(c->live_ev[j]));
+            c->live_frac = h_sum[j] / (double)c->total_cells;
+            c->live_pending[j] = false;
+        }
+        if (c->live_pending[j ^ 1] && hipEventQuery(c->live_ev[j ^ 1]) == hipSuccess) {
+            c->live_frac = h_sum[j ^ 1] / (double)c->total_cells;
+            c->live_pending[j ^ 1] = false;
+        }
+        double* sums = c->d_live + 2 * c->nf * c->inst.size();
+        launch_live_count(c->d_rows, c->rows_cap, 2 * c->nf, n_steps - left - 1, (int)c->inst.size(), sums + j,
+                          c->stream);
+        HIPCHK(c, hipMemcpyAsync((void*)(h_sum + j), sums + j, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipEventRecord(c->live_ev[j], c->stream));
+        c->live_pending[j] = true;
+        c->live_block += 1;
+        return update_mode(c);
+    };
     while (B > 0 && left >= B) {
         if ((s = capture(c, c->parity)) != TM_OK) return s;
-        HIPCHK(c, hipGraphLaunch(c->graph[c->parity], c->stream));
+        HIPCHK(c, hipGraphLaunch(c->graph[c->blocked][c->parity], c->stream));
         c->counter += B;
-        c->parity ^= (B & 1);
+        c->parity ^= (launches_of(c, B) & 1);
         left -= B;
+        if ((s = after_block()) != TM_OK) return s;
     }
     while (left > 0) {
         const int chunk = (int)std::min<int64_t>(left, c->nslots);
         enqueue_block(c, c->parity, chunk, true);
         HIPCHK(c, hipGetLastError());
-        c->parity ^= (chunk & 1);
+        c->parity ^= (launches_of(c, chunk) & 1);
         c->counter += chunk;
         left -= chunk;
+        if ((s = after_block()) != TM_OK) return s;
     }
     return TM_OK;
 }
@@ -388,7 +541,7 @@ tm_status read_fault(tm_ctx* c, int32_t* inst, int64_t* cell, int64_t* step)
     *cell = -1;
     *step = -1;
     if (st.fault_counter != ~0ull) {
-        *inst = (int32_t)(st.fault_key >> 40);
+        *inst = (int32_t)((st.fault_key >> 40) & 0xFFFFF);   // bits 60..63: the step's place in its launch
         *cell = (int64_t)(st.fault_key & ((1ull << 40) - 1));
         *step = (*inst >= 0 && *inst < (int)c->inst.size() ? c->inst[*inst].step0 : 0) + (int64_t)st.fault_counter;
     }
@@ -487,6 +640,8 @@ void tm_destroy(tm_ctx* c)
     release_config(c);
     free_dev(c->d_state);
     if (c->h_pinned) hipHostFree(c->h_pinned);
+    for (auto e : c->live_ev)
+        if (e) hipEventDestroy(e);
     for (auto e : c->run_ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -551,6 +706,7 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
 
 tm_status tm_upload_fields(tm_ctx* c, int i, const double* host)
 {
+    if (c) invalidate_live(c);   // new fields
     tm_status s = check_ready(c);
     if (s != TM_OK) return s;
     if (i < 0 || i >= (int)c->inst.size() || !host) return fail(c, TM_ERR_ARG, "bad instance or pointer");
@@ -589,6 +745,7 @@ tm_status tm_download_fields(tm_ctx* c, int i, double* host)
 static tm_status enqueue_frames(tm_ctx* c, int mode, int n_gauss, const double* sd_div,
                                 const std::vector<FrameVals>& vals)
 {
+    invalidate_live(c);   // new fields
     const int n = (int)c->inst.size();
     std::vector<int64_t> blk0((size_t)n + 1, 0);
     for (int k = 0; k < n; ++k) {
@@ -668,6 +825,7 @@ tm_status tm_save_snapshot(tm_ctx* c, int i, const char* path)
 
 tm_status tm_load_snapshot(tm_ctx* c, int i, const char* path, int64_t* step)
 {
+    if (c) invalidate_live(c);   // new fields
     tm_status s = check_ready(c);
     if (s != TM_OK) return s;
     if (i < 0 || i >= (int)c->inst.size() || !path) return fail(c, TM_ERR_ARG, "tm_load_snapshot: bad instance or path");
@@ -784,32 +942,6 @@ tm_status tm_last_run_time(const tm_ctx* c, double* ms, int64_t* n_steps)
     return TM_OK;
 }
 
-// rho(t) row of the current state of every instance into out[i * stride * 2 nf]
-// (device): tile partials of the current fields, then the rows fold.
-static void enqueue_measure(tm_ctx* c, double* out, int64_t stride)
-{
-    const Variant& v = cur_variant(c);
-    if (c->prec == TM_F64)
-        launch_measure_f64(c->nf, c->d_inst, c->d_tiles, (int)c->tiles.size(), v.tx, v.ty, c->d_fields[c->parity],
-                           c->d_partials, c->stream);
-    else
-        launch_measure_f32(c->nf, c->d_inst, c->d_tiles, (int)c->tiles.size(), v.tx, v.ty, c->d_fields[c->parity],
-                           c->d_partials, c->stream);
-    CombineArgs a{};
-    a.inst = c->d_inst;
-    a.partials = c->d_partials;
-    a.state = c->d_state;
-    a.rows = out;
-    a.ntiles = (int32_t)c->tiles.size();
-    a.nslots = c->nslots;
-    a.nf = c->nf;
-    a.nsteps = 1;
-    a.rows_cap = stride;
-    a.measure = 1;
-    a.pw = 1;
-    launch_combine(a, (int)c->inst.size(), 1, c->stream);
-}
-
 tm_status tm_measure(tm_ctx* c, double* rows)
 {
     tm_status s = check_ready(c);
@@ -906,10 +1038,12 @@ tm_status tm_time_steps(tm_ctx* c, int64_t n_steps, double* ms_total, int64_t* l
     if (n_steps <= 0 || !ms_total) return fail(c, TM_ERR_ARG, "n_steps must be positive");
     // Step kernels only (no rows fold): graphs of up to 100 step launches, one
     // per starting parity, replayed between two events on the context stream.
+    if ((s = update_mode(c)) != TM_OK) return s;
     const int B = (int)std::min<int64_t>(std::min<int64_t>(n_steps, 100), c->nslots);
     hipGraphExec_t ge[2] = {nullptr, nullptr};
+    const int flip = launches_of(c, B) & 1;
     for (int par = 0; par < 2; ++par) {
-        if (par == 1 && (B & 1) == 0) break;   // even blocks keep the parity
+        if (par == 1 && !flip) break;   // blocks with an even number of launches keep the parity
         if ((s = capture_block(c, c->parity ^ par, B, false, &ge[par])) != TM_OK) return s;
     }
     hipEvent_t e0, e1;
@@ -920,12 +1054,14 @@ tm_status tm_time_steps(tm_ctx* c, int64_t n_steps, double* ms_total, int64_t* l
     int64_t left = n_steps;
     while (left >= B) {
         HIPCHK(c, hipGraphLaunch(ge[(c->parity ^ p0) & 1], c->stream));
-        c->parity ^= (B & 1);
+        c->parity ^= flip;
         left -= B;
     }
+    int64_t nlaunch = (n_steps / B) * launches_of(c, B);
     if (left > 0) {
         enqueue_block(c, c->parity, (int)left, false);
-        c->parity ^= (int)(left & 1);
+        c->parity ^= (launches_of(c, left) & 1);
+        nlaunch += launches_of(c, left);
     }
     HIPCHK(c, hipEventRecord(e1, c->stream));
     HIPCHK(c, hipEventSynchronize(e1));
@@ -937,7 +1073,7 @@ tm_status tm_time_steps(tm_ctx* c, int64_t n_steps, double* ms_total, int64_t* l
     hipEventDestroy(e1);
     c->counter += n_steps;
     *ms_total = ms;
-    if (launches) *launches = n_steps;
+    if (launches) *launches = nlaunch;
     return fault_status(c);
 }
 
@@ -987,20 +1123,22 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
     for (int v = 0; v < nv; ++v) {
         c->variant = v;
         if ((s = build_tiles(c)) != TM_OK) return s;
-        launch_prep(c->model, c->d_inst, c->d_params, c->d_state, (int)c->inst.size(), 1, c->d_consts,
+        if ((s = update_mode(c)) != TM_OK) return s;   // the mode this state would run in
+        const int spl = steps_per_launch(c);           // steps per timed launch
+        launch_prep(c->model, c->d_inst, c->d_params, c->d_state, (int)c->inst.size(), spl, c->d_consts,
                     c->stream);
-        launch_step(c, c->parity, 0);   // warm-up
+        launch_step(c, c->parity, 0, spl);   // warm-up
         // Best of three trials of `reps` launches; reps grows until a trial
         // takes about a millisecond, so that close variants are told apart.
         double vbest = 1e30;
         for (int trial = 0; trial < 3; ++trial) {
             HIPCHK(c, hipEventRecord(e0, c->stream));
-            for (int r = 0; r < reps; ++r) launch_step(c, c->parity, 0);
+            for (int r = 0; r < reps; ++r) launch_step(c, c->parity, 0, spl);
             HIPCHK(c, hipEventRecord(e1, c->stream));
             HIPCHK(c, hipEventSynchronize(e1));
             float ms = 0.f;
             HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-            vbest = std::min(vbest, (double)ms / reps);
+            vbest = std::min(vbest, (double)ms / (reps * spl));   // per step
             if (v == 0 && trial == 0 && ms < 1.0f) reps = std::min(200, (int)(reps * 1.0 / std::max(ms, 1e-3f)) + 1);
         }
         if (v == start_v) start_time = vbest;
@@ -1171,12 +1309,18 @@ tm_status tm_replicate_stats(tm_ctx* c, int flags, double* stats)
 namespace tmk {
 int default_variant(int model, int precision, int64_t total_cells)
 {
-    if (model == TM_MODEL_DP) return precision == TM_F64 ? 2 : 4;   // 64 x 64 / 128 x 32 (<= 80 VGPRs)
-    // Rietkerk: small lattices are latency-bound and want the single fused
-    // 16 x 16 launch; batches want the split step (each P* sampled once).
-    if (total_cells < (1 << 20)) return 0;
+    const char* want;
+    if (model == TM_MODEL_DP) {
+        // adaptive four-step launches: single steps while the field is live,
+        // four steps per launch once it is mostly absorbing
+        want = precision == TM_F64 ? "dp_f64_64x32_t256_x4a" : "dp_f32_128x32_t256_x4a";
+    } else {
+        // Rietkerk: small lattices are latency-bound and want the single fused
+        // 16 x 16 launch; batches want the split step (each P* sampled once).
+        if (total_cells < (1 << 20)) return 0;
+        want = precision == TM_F64 ? "rk_f64_split_32x16_t256" : "rk_f32_split_32x16_t256";
+    }
     const int n = tm_variant_count(model, precision);
-    const char* want = precision == TM_F64 ? "rk_f64_split_32x16_t256" : "rk_f32_split_32x16_t256";
     for (int v = 0; v < n; ++v)
         if (std::string(tm_variant_name(model, precision, v)) == want) return v;
     return 0;

@@ -118,9 +118,33 @@ __global__ __launch_bounds__(256) void replicate_stats_kernel(const double* __re
     o[nf + 2] = act;
 }
 
+// Sum over the instances of field 0's occupied count in row `row` of the rows
+// buffer, in a fixed order (one workgroup): the live measure of the adaptive
+// multi-step launches.
+__global__ __launch_bounds__(256) void live_count_kernel(const double* __restrict__ rows, int64_t rows_cap, int w,
+                                                         int64_t row, int n_inst, double* __restrict__ out)
+{
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n_inst; i += 256) acc += rows[((int64_t)i * rows_cap + row) * w];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
 inline int grid_of(int64_t n) { return (int)((n + 255) / 256); }
 
 }  // namespace
+
+void launch_live_count(const double* rows, int64_t rows_cap, int w, int64_t row, int n_inst, double* out,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(live_count_kernel, dim3(1), dim3(256), 0, s, rows, rows_cap, w, row, n_inst, out);
+}
 
 void launch_gather_series(const double* rows, int64_t rows_cap, int w, const int64_t* steps, int m0, int m1,
                           int64_t s0, int n_meas, int n_inst, double* series, hipStream_t s)

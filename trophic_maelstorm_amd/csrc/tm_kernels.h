@@ -96,11 +96,18 @@ __device__ __forceinline__ int tile_of(int t, int ntiles, int xcd)
     return x * q + (x < r ? x : r) + j;
 }
 
-__device__ __forceinline__ void record_fault(DevState* st, int inst, int64_t cell, int64_t counter)
+// Key = (phase << 60) | (instance << 40) | cell, where phase is the position of
+// the step within a multi-step launch (0 for single-step launches): a fault of
+// an earlier step of the launch, recorded in any order, always has the smaller
+// key, as it has the smaller counter.  The host drops the phase when it decodes
+// the key.
+__device__ __forceinline__ void record_fault(DevState* st, int inst, int64_t cell, int64_t counter,
+                                             int phase = 0)
 {
     if (st->fault_counter < (uint64_t)counter) return;
     atomicMin((unsigned long long*)&st->fault_counter, (unsigned long long)counter);
-    atomicMin((unsigned long long*)&st->fault_key, ((unsigned long long)inst << 40) | (unsigned long long)cell);
+    atomicMin((unsigned long long*)&st->fault_key, ((unsigned long long)phase << 60) |
+                                                       ((unsigned long long)inst << 40) | (unsigned long long)cell);
 }
 
 // Fixed-order block reduction of NF (sum, count) pairs; thread 0 writes them.
@@ -402,6 +409,82 @@ struct DPShape {
     static_assert(NT % CG == 0 && TY % R4 == 0 && TX % 64 == 0, "tile shape");
 };
 
+// The step of one cell from its value and its four neighbours' (old field):
+// the absorbing skip (FKE:4161-4164), the Dornic transition, and for MODE 0 the
+// DP nonlinear sub-step and the flush.  n gets the Poisson integer (-1 where
+// skipped); bad is set when MODE 0 produced a negative or non-finite value.
+template <typename T, int MODE>
+__device__ __forceinline__ T cell_update(const typename StepConst<MODE>::type& c, const DevInst& di,
+                                         uint32_t cell, T cur, T up, T dn, T lf, T rt, int64_t& n, bool& bad)
+{
+    T y = (T)0;
+    n = -1;
+    bad = false;
+    if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
+        y = cur;
+        const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
+        if (!(alpha_i == 0.0 && cur == (T)0)) {
+            Stream st(di.seed, di.inst_id, cell, c.step);
+            y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
+        }
+        if constexpr (MODE == 0) {
+            y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
+            bad = y < (T)0 || !Arith<T>::finite(y);
+            if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
+        }
+    }
+    return y;
+}
+
+// Whether the first NQ dwords of an LDS image are all zero (every thread of the
+// workgroup calls it; one barrier).  An image of +0 only is an absorbing tile:
+// the step leaves every cell at 0 and draws nothing.
+template <int NT, int NQ>
+__device__ __forceinline__ bool image_all_zero(const uint32_t* img, uint32_t* s_flag)
+{
+    static_assert(NQ % 4 == 0, "16-byte reads");
+    uint32_t any = 0;
+    const uint4* v = (const uint4*)img;
+    for (int i = threadIdx.x; i < NQ / 4; i += NT) {
+        const uint4 x = v[i];
+        any |= x.x | x.y | x.z | x.w;
+    }
+    const uint64_t b = __ballot(any != 0);
+    if ((threadIdx.x & 63) == 0) s_flag[threadIdx.x >> 6] = b != 0 ? 1u : 0u;
+    __syncthreads();
+    uint32_t r = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) r |= s_flag[w];
+    return r == 0;
+}
+
+// The absorbing tile's output: zeros in its valid cells (16-byte stores where
+// aligned), draws -1, and zero rho(t) partials in each of the nsl given slots.
+template <typename T, int TX, int TY, int NT>
+__device__ __forceinline__ void zero_tile(const StepArgs& a, int tile, const DevTile& tl, T* fout,
+                                          const int64_t* slots, int nsl)
+{
+    constexpr int CG = TX / 4;
+    const int L = tl.L;
+    const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
+    Quad<T> z;
+    z.v[0] = z.v[1] = z.v[2] = z.v[3] = (T)0;
+    for (int g = threadIdx.x; g < TY * CG; g += NT) {
+        const int gy = tl.y0 + g / CG, gx0 = tl.x0 + 4 * (g % CG);
+        if (gy >= L || gx0 >= L) continue;
+        const int nvalid = min(4, L - gx0);
+        store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, z);
+        if (a.draws) {
+            const int64_t c0 = tl.cell_off + (int64_t)gy * L + gx0;
+#pragma unroll 1
+            for (int q = 0; q < nvalid; ++q) a.draws[c0 + q] = -1;
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < nsl; ++k)
+            a.partials[(slots[k] * a.ntiles + tile) * (NT / 64) + (threadIdx.x >> 6)] = make_double2(0.0, 0.0);
+}
+
 // One tile from its LDS image: the per-cell step on each of this thread's G
 // 4-cell groups (rows ly0 + j R4), then (MODE 0) the wave's rho(t) partial.
 template <typename T, int TX, int TY, int NT, int MODE>
@@ -445,24 +528,12 @@ __device__ __forceinline__ void dp_tile(const StepArgs& a, int tile, const DevTi
 #pragma unroll 1
             for (int q = 0; q < nvalid; ++q) {
                 const int rq = r + q;
-                const T cur = s_rho[rq];
-                const T up = s_rho[rq - RWV], dn = s_rho[rq + RWV], lf = s_rho[rq - 1], rt = s_rho[rq + 1];
                 const uint32_t cell = cell0 + q;
-                T y = (T)0;
-                int64_t n = -1;
-                if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
-                    y = cur;
-                    const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
-                    if (!(alpha_i == 0.0 && cur == (T)0)) {
-                        Stream st(di.seed, di.inst_id, cell, c.step);
-                        y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
-                    }
-                    if constexpr (MODE == 0) {
-                        y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
-                        if (y < (T)0 || !Arith<T>::finite(y)) record_fault(a.state, tl.inst, cell, counter);
-                        if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
-                    }
-                }
+                int64_t n;
+                bool bad;
+                const T y = cell_update<T, MODE>(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
+                                                 s_rho[rq - 1], s_rho[rq + 1], n, bad);
+                if (bad) record_fault(a.state, tl.inst, cell, counter);
                 if (a.draws) a.draws[tl.cell_off + cell] = n;
                 if constexpr (MODE == 0) {
                     sum[0] += (double)y;
@@ -499,7 +570,8 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
     typedef DPShape<T, TX, TY, NT> S;
     typedef typename StepConst<MODE>::type Const;
     constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
-    __shared__ uint32_t s_img[S::NBUF];
+    __shared__ __align__(16) uint32_t s_img[S::NBUF];
+    __shared__ uint32_t s_flag[NT / 64];
 
     const int64_t counter = a.state->counter + a.k;
     const int64_t slot = counter & (int64_t)(a.nslots - 1);   // nslots: a power of two
@@ -513,7 +585,200 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
     tile_to_lds<T, TX, S::RH, NT>((const T*)a.in + NFIN * tl.cell_off, tl.L, tl.x0, tl.y0, s_img);
     __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
     __syncthreads();                 // ... everyone's
+    if (image_all_zero<NT, S::NQ>(s_img, s_flag)) {
+        zero_tile<T, TX, TY, NT>(a, tile, tl, (MODE == 1 ? (T*)a.aux : (T*)a.out) + tl.cell_off, &slot,
+                                 MODE == 0 ? 1 : 0);
+        return;
+    }
     dp_tile<T, TX, TY, NT, MODE>(a, tile, tl, di, c, (const T*)s_img, counter, slot);
+}
+
+// ---------------------------------------------------------------------------
+// K DP steps per launch (temporal blocking): steps k = a.k .. k + K - 1 of one
+// tile from one radius-K image of the field.  Phase j (j = 1 .. K - 1) computes
+// step k + j - 1 on the tile and its radius-(K - j) ring into an LDS frame (two
+// frames, used in turn); the ring's cells belong to the neighbouring tiles,
+// which compute the same values from the same inputs and Philox counters, so
+// the result is the single-step result bit for bit.  The last phase computes
+// step k + K - 1 of the tile from the last frame and writes it.  The field is
+// read and written once per K steps.  Only the tile's own cells count in the
+// rho(t) partials (one set per step) and record faults; draws are recorded for
+// the last step (the last step of a run that ends on this launch).  The image
+// and the frames share one layout: row width RW, tile column lx at HC + lx,
+// tile row ly at (ly + radius) rows from the top.
+// ---------------------------------------------------------------------------
+template <typename T, int TX, int TY, int NT, int K>
+struct DPKShape {
+    static constexpr int HC = LdsFrame<T, K>::HC;            // column halo (whole 16-byte chunks)
+    static constexpr int RW = TX + 2 * HC;                   // row width of the image and the frames
+    static constexpr int RH = TY + 2 * K;                    // image rows: tile rows -K .. TY + K - 1
+    static constexpr int FH = TY + 2 * (K - 1);              // frame rows (the largest frame)
+    static constexpr int CG = TX / 4;                        // 4-cell groups per row
+    static constexpr int NW = NT / 64;
+    static constexpr int NQ = RH * RW * (int)(sizeof(T) / 4);
+    static constexpr int NBUF = ((NQ + 4 * NT - 1) / (4 * NT)) * 4 * NT;   // image dwords (DMA rounds)
+    static_assert(TX % 64 == 0 && NT % 64 == 0 && K >= 2 && K <= HC, "tile shape");
+};
+
+// Wave partial (sum, count) of one step into its slot; an all-zero wave writes zeros.
+template <int NT>
+__device__ __forceinline__ void step_partial(const StepArgs& a, int tile, int64_t counter, double s,
+                                             uint32_t nlive)
+{
+    const int64_t slot = counter & (int64_t)(a.nslots - 1);
+    double2* dst = a.partials + (slot * a.ntiles + tile) * (NT / 64) + (threadIdx.x >> 6);
+    if (__ballot(nlive != 0 || s != 0.0) == 0) {
+        if ((threadIdx.x & 63) == 0) *dst = make_double2(0.0, 0.0);
+    } else {
+        double sum[1] = {s}, cnt[1] = {(double)nlive};
+        wave_partials<1>(sum, cnt, dst);
+    }
+}
+
+template <typename T, int TX, int TY, int NT, int K, int MINW = 1>
+__global__ __launch_bounds__(NT, MINW) void dp_stepk_kernel(StepArgs a)
+{
+    typedef DPKShape<T, TX, TY, NT, K> S;
+    constexpr int HC = S::HC, RW = S::RW, CG = S::CG;
+    __shared__ __align__(16) uint32_t s_img[S::NBUF];
+    // One frame buffer; from phase 2 on, each phase writes into the buffer its
+    // source is not in, so the image's buffer serves as the second frame.
+    __shared__ __align__(16) T s_fr[S::FH * RW];
+    static_assert(S::FH * RW * (int)sizeof(T) <= S::NBUF * 4, "a frame fits the image buffer");
+    __shared__ uint32_t s_flag[NT / 64];
+
+    const int64_t counter = a.state->counter + a.k;            // step k
+    const int tile = blockIdx.x;
+    if (tile >= a.ntiles) return;
+    const DevTile tl = a.tiles[tile];
+    const DevInst di = a.inst[tl.inst];
+    const int L = tl.L;
+    tile_to_lds<T, TX, S::RH, NT, K>((const T*)a.in + tl.cell_off, L, tl.x0, tl.y0, s_img);
+    __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
+    __syncthreads();                 // ... everyone's
+    if (image_all_zero<NT, S::NQ>(s_img, s_flag)) {   // absorbing for all K steps
+        int64_t slots[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) slots[j] = (counter + j) & (int64_t)(a.nslots - 1);
+        zero_tile<T, TX, TY, NT>(a, tile, tl, (T*)a.out + tl.cell_off, slots, K);
+        return;
+    }
+
+    const int tid = threadIdx.x;
+    const T* src = (const T*)s_img;
+    int srad = K;                                              // radius of the source frame
+#pragma unroll 1
+    for (int j = 1; j < K; ++j) {
+        // ---- phase j: step k + j - 1 on tile rows and columns -R .. T + R - 1 ----
+        const int R = K - j;
+        const DPConst c = ((const DPConst*)a.consts)[(int64_t)(a.k + j - 1) * a.n_inst + tl.inst];
+        T* dstf = (j & 1) ? s_fr : (T*)s_img;                   // never the buffer `src` is in
+        double sum = 0.0;
+        uint32_t nlive = 0;
+        const int ngroups = (TY + 2 * R) * CG;
+        for (int g = tid; g < ngroups; g += NT) {
+            const int ly = g / CG - R, cg = g % CG;
+            const int r = (ly + srad) * RW + HC + 4 * cg;          // source index of the group's first cell
+            const int m = (ly + R) * RW + HC + 4 * cg;             // destination index
+            uint32_t bits = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                bits |= as_bits(src[r + q]) | as_bits(src[r - RW + q]) | as_bits(src[r + RW + q]);
+            bits |= as_bits(src[r - 1]) | as_bits(src[r + 4]);
+            Quad<T> yq;
+            yq.v[0] = yq.v[1] = yq.v[2] = yq.v[3] = (T)0;
+            if (bits != 0) {
+                const int gy = wrap(tl.y0 + ly, L);
+                const bool own_row = ly >= 0 && ly < TY && tl.y0 + ly < L;
+#pragma unroll 1
+                for (int q = 0; q < 4; ++q) {
+                    const int rq = r + q;
+                    const int lx = 4 * cg + q;
+                    const uint32_t cell = (uint32_t)gy * (uint32_t)L + (uint32_t)wrap(tl.x0 + lx, L);
+                    int64_t n;
+                    bool bad;
+                    const T y = cell_update<T, 0>(c, di, cell, src[rq], src[rq - RW], src[rq + RW], src[rq - 1],
+                                                  src[rq + 1], n, bad);
+                    if (own_row && tl.x0 + lx < L) {
+                        if (bad) record_fault(a.state, tl.inst, cell, counter + j - 1, j - 1);
+                        sum += (double)y;
+                        nlive += y > (T)0 ? 1u : 0u;
+                    }
+                    yq.v[0] = q == 0 ? y : yq.v[0];
+                    yq.v[1] = q == 1 ? y : yq.v[1];
+                    yq.v[2] = q == 2 ? y : yq.v[2];
+                    yq.v[3] = q == 3 ? y : yq.v[3];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dstf[m + q] = yq.v[q];
+        }
+        // the ring's columns -R .. -1 and TX .. TX + R - 1, one cell per thread
+        const int nedge = 2 * R * (TY + 2 * R);
+        for (int e = tid; e < nedge; e += NT) {
+            const int ly = e / (2 * R) - R, cc = e % (2 * R);
+            const int lx = cc < R ? cc - R : TX + (cc - R);
+            const int rq = (ly + srad) * RW + HC + lx;
+            const uint32_t cell = (uint32_t)wrap(tl.y0 + ly, L) * (uint32_t)L + (uint32_t)wrap(tl.x0 + lx, L);
+            int64_t n;
+            bool bad;
+            dstf[(ly + R) * RW + HC + lx] = cell_update<T, 0>(c, di, cell, src[rq], src[rq - RW], src[rq + RW],
+                                                              src[rq - 1], src[rq + 1], n, bad);
+        }
+        step_partial<NT>(a, tile, counter + j - 1, sum, nlive);
+        __syncthreads();
+        src = dstf;
+        srad = R;
+    }
+
+    // ---- last phase: step k + K - 1 on the tile, from the last frame (radius 1) ----
+    const DPConst c = ((const DPConst*)a.consts)[(int64_t)(a.k + K - 1) * a.n_inst + tl.inst];
+    double sum = 0.0;
+    uint32_t nlive = 0;
+    T* fout = (T*)a.out + tl.cell_off;
+    const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
+    for (int g = tid; g < TY * CG; g += NT) {
+        const int ly = g / CG, cg = g % CG;
+        const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
+        if (gy >= L || gx0 >= L) continue;
+        const int r = (ly + 1) * RW + HC + 4 * cg;
+        const uint32_t cell0 = (uint32_t)gy * (uint32_t)L + (uint32_t)gx0;
+        const bool full = gx0 + 4 <= L;
+        uint32_t bits = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            bits |= as_bits(src[r + q]) | as_bits(src[r - RW + q]) | as_bits(src[r + RW + q]);
+        bits |= as_bits(src[r - 1]) | as_bits(src[r + 4]);
+        const int nvalid = min(4, L - gx0);
+        Quad<T> yq;
+        yq.v[0] = yq.v[1] = yq.v[2] = yq.v[3] = (T)0;
+        if (bits == 0 && full) {
+            if (a.draws) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a.draws[tl.cell_off + cell0 + q] = -1;
+            }
+        } else {
+#pragma unroll 1
+            for (int q = 0; q < nvalid; ++q) {
+                const int rq = r + q;
+                const uint32_t cell = cell0 + q;
+                int64_t n;
+                bool bad;
+                const T y = cell_update<T, 0>(c, di, cell, src[rq], src[rq - RW], src[rq + RW], src[rq - 1],
+                                              src[rq + 1], n, bad);
+                if (bad) record_fault(a.state, tl.inst, cell, counter + K - 1, K - 1);
+                if (a.draws) a.draws[tl.cell_off + cell] = n;
+                sum += (double)y;
+                nlive += y > (T)0 ? 1u : 0u;
+                yq.v[0] = q == 0 ? y : yq.v[0];
+                yq.v[1] = q == 1 ? y : yq.v[1];
+                yq.v[2] = q == 2 ? y : yq.v[2];
+                yq.v[3] = q == 3 ? y : yq.v[3];
+            }
+        }
+        store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
+    }
+    step_partial<NT>(a, tile, counter + K - 1, sum, nlive);
 }
 
 // ---------------------------------------------------------------------------

@@ -12,6 +12,12 @@ static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
     hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, MODE, MINW>), dim3(grid), dim3(NT), 0, s, a);
 }
 
+template <typename T, int TX, int TY, int NT, int K, int MINW = 1>
+static void launch_dpk(const StepArgs& a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((dp_stepk_kernel<T, TX, TY, NT, K, MINW>), dim3(grid), dim3(NT), 0, s, a);
+}
+
 template <typename T, int TX, int TY, int NT, bool SPLIT = false>
 static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
 {
@@ -28,6 +34,26 @@ static const Variant kDP[] = {
     {64, 64, 256, 0, 4, "dp_f32_64x64_t256_w6", &launch_dp<float, 64, 64, 256, 0, 6>},
     {128, 16, 256, 0, 4, "dp_f32_128x16_t256_rm", &launch_dp<float, 128, 16, 256>, 0, 0, nullptr, 1},
     {256, 32, 256, 0, 4, "dp_f32_256x32_t256_rm", &launch_dp<float, 256, 32, 256>, 0, 0, nullptr, 1},
+    // _xK: K steps per launch (temporal blocking); the single-step kernel of the same
+    // tile covers a block's last steps when fewer than K remain
+    {128, 16, 256, 0, 4, "dp_f32_128x16_t256_x2", &launch_dpk<float, 128, 16, 256, 2>, 0, 0, nullptr, 0, 2,
+     &launch_dp<float, 128, 16, 256>},
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x2", &launch_dpk<float, 128, 32, 256, 2>, 0, 0, nullptr, 0, 2,
+     &launch_dp<float, 128, 32, 256>},
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x3", &launch_dpk<float, 128, 32, 256, 3>, 0, 0, nullptr, 0, 3,
+     &launch_dp<float, 128, 32, 256>},
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x4", &launch_dpk<float, 128, 32, 256, 4>, 0, 0, nullptr, 0, 4,
+     &launch_dp<float, 128, 32, 256>},
+    {128, 16, 256, 0, 4, "dp_f32_128x16_t256_x4", &launch_dpk<float, 128, 16, 256, 4>, 0, 0, nullptr, 0, 4,
+     &launch_dp<float, 128, 16, 256>},
+    {128, 24, 256, 0, 4, "dp_f32_128x24_t256_x4", &launch_dpk<float, 128, 24, 256, 4>, 0, 0, nullptr, 0, 4,
+     &launch_dp<float, 128, 24, 256>},
+    // _x4a: adaptive -- single steps (<= 80 VGPRs) while more than a tenth of the cells are
+    // occupied, four-step launches below
+    {128, 24, 256, 0, 4, "dp_f32_128x24_t256_x4a", &launch_dpk<float, 128, 24, 256, 4>, 0, 0, nullptr, 0, 4,
+     &launch_dp<float, 128, 24, 256, 0, 6>, 1},
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x4a", &launch_dpk<float, 128, 32, 256, 4>, 0, 0, nullptr, 0, 4,
+     &launch_dp<float, 128, 32, 256, 0, 6>, 1},
 };
 
 static const Variant kRK[] = {

@@ -12,6 +12,12 @@ static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
     hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, MODE, MINW>), dim3(grid), dim3(NT), 0, s, a);
 }
 
+template <typename T, int TX, int TY, int NT, int K, int MINW = 1>
+static void launch_dpk(const StepArgs& a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((dp_stepk_kernel<T, TX, TY, NT, K, MINW>), dim3(grid), dim3(NT), 0, s, a);
+}
+
 template <typename T, int TX, int TY, int NT, bool SPLIT = false>
 static void launch_rk(const StepArgs& a, int grid, hipStream_t s)
 {
@@ -23,6 +29,17 @@ static const Variant kDP[] = {
     {128, 32, 256, 0, 4, "dp_f64_128x32_t256", &launch_dp<double, 128, 32, 256>},
     {64, 64, 256, 0, 4, "dp_f64_64x64_t256", &launch_dp<double, 64, 64, 256>},
     {256, 32, 256, 0, 4, "dp_f64_256x32_t256", &launch_dp<double, 256, 32, 256>},
+    // _xK: K steps per launch (temporal blocking); the single-step kernel of the same
+    // tile covers a block's last steps when fewer than K remain
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x2", &launch_dpk<double, 64, 32, 256, 2>, 0, 0, nullptr, 0, 2,
+     &launch_dp<double, 64, 32, 256>},
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x4", &launch_dpk<double, 64, 32, 256, 4>, 0, 0, nullptr, 0, 4,
+     &launch_dp<double, 64, 32, 256>},
+    {64, 16, 256, 0, 4, "dp_f64_64x16_t256_x4", &launch_dpk<double, 64, 16, 256, 4>, 0, 0, nullptr, 0, 4,
+     &launch_dp<double, 64, 16, 256>},
+    // _x4a: adaptive -- single steps while more than a tenth of the cells are occupied
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x4a", &launch_dpk<double, 64, 32, 256, 4>, 0, 0, nullptr, 0, 4,
+     &launch_dp<double, 64, 32, 256>, 1},
 };
 
 static const Variant kRK[] = {

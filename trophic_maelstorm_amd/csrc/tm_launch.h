@@ -18,6 +18,9 @@ struct Variant {
     int ax, ay;      // split Rietkerk step: tile shape of the Dornic kernel fn_a,
     StepLaunch fn_a; // launched (one tile per workgroup) before fn; 0 = fused step
     int row_major;   // tile order within an instance: 0 = down vertical strips, 1 = along rows
+    int spl = 1;     // steps per launch of fn (> 1: temporally blocked DP step)
+    StepLaunch fn_one = nullptr;   // spl > 1: the single-step kernel of the same tiles
+    int adaptive = 0;              // spl > 1: single steps while the field is live (see tm_api.cpp)
 };
 
 // Variant tables per (model, precision); defined in tm_kernels_f64.hip / _f32.hip.
@@ -52,6 +55,10 @@ struct FrameVals {
 void launch_init_frames(int prec, int mode, const DevInst* inst, const int64_t* blk0, int n_inst, int64_t nblocks,
                         int nf, int n_gauss, const double* sd_div, const FrameVals* vals, void* fields,
                         hipStream_t s);
+
+// Sum of field 0's occupied counts over the instances in one row (tm_ensemble.hip).
+void launch_live_count(const double* rows, int64_t rows_cap, int w, int64_t row, int n_inst, double* out,
+                       hipStream_t s);
 
 // Sweep series and replicate statistics (tm_ensemble.hip).
 void launch_gather_series(const double* rows, int64_t rows_cap, int w, const int64_t* steps, int m0, int m1,

@@ -35,13 +35,18 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s + % HBM peak: batched FP32 vs plain FP64, ρ(t) err ≤1e-3"
 HBM_PEAK_GBS = 8000.0
 SEED = 20251202
-# HBM bytes per launch of the FP32 DP step kernel on the C4 batch, per kernel
-# variant, from counter runs over the first 10^3 steps
-# (profiles/dp_step_f32_c4_quench_hbm_traffic.md): FETCH_SIZE x 2.000 and
-# WRITE_SIZE x 1.000 KiB, the factors a plain 256 MiB copy needs to read its
-# known size.
+# HBM bytes per launch of the FP32 DP step kernels on the C4 batch, per kernel
+# variant, from counter runs (profiles/dp_step_f32_c4_quench_hbm_traffic.md):
+# FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000 KiB, the factors a plain 256 MiB
+# copy needs to read its known size.
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-C4_F32_TRAFFIC_KIB = {   # variant: (FETCH_SIZE, WRITE_SIZE) per dispatch
+# Multi-step variants: (steps per launch, (FETCH_SIZE, WRITE_SIZE) of a K-step
+# launch, (FETCH_SIZE, WRITE_SIZE) of a single-step launch of the same tiles),
+# means over the absorbing phase of the first 4000 steps.
+C4_F32_BLOCKED_TRAFFIC_KIB = {
+    "dp_f32_128x32_t256_x4a": (4, (87921.6, 141440.2), (75136.4, 139848.7)),
+}
+C4_F32_TRAFFIC_KIB = {   # single-step variant: (FETCH_SIZE, WRITE_SIZE) per dispatch
     "dp_f32_128x32_t256_w6": (75198.8, 139824.1),
     "dp_f32_64x32_t256": (80202.1, 140385.4),
     "dp_f32_128x32_t256": (75199.2, 139824.1),
@@ -90,11 +95,22 @@ def bytes_per_cell(b):
     return b["nf"] * 2 * (8 if b["precision"] == 0 else 4)
 
 
-def traffic_bytes(variant):
+def _kib_to_bytes(t):
+    return (t[0] * FETCH_SCALE + t[1] * WRITE_SCALE) * 1024
+
+
+def traffic_bytes(variant, single_steps=0, blocked_steps=0):
+    """Measured HBM bytes per step of the timed run: per dispatch for a
+    single-step variant; for a multi-step one, its two kinds of launch weighed by
+    the steps each advanced."""
+    if variant in C4_F32_BLOCKED_TRAFFIC_KIB:
+        k, tk, t1 = C4_F32_BLOCKED_TRAFFIC_KIB[variant]
+        n = single_steps + blocked_steps
+        if n == 0:
+            return None
+        return int(round((single_steps * _kib_to_bytes(t1) + blocked_steps / k * _kib_to_bytes(tk)) / n))
     t = C4_F32_TRAFFIC_KIB.get(variant)
-    if t is None:
-        return None
-    return int(round((t[0] * FETCH_SCALE + t[1] * WRITE_SCALE) * 1024))
+    return None if t is None else int(round(_kib_to_bytes(t)))
 
 
 def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=None):
@@ -119,6 +135,7 @@ def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=N
         chunk = max(1, min(steps, b.get("chunk", steps)))
         parts = []
         dev_ms = 0.0
+        m0 = ctx.mode_steps()
         t0 = time.perf_counter()
         done = 0
         while done < steps:
@@ -129,6 +146,7 @@ def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=N
             if done < steps:
                 print(f"[bench]   ... {done}/{steps} steps", file=sys.stderr, flush=True)
         t1 = time.perf_counter()
+        m1 = ctx.mode_steps()
         rows = np.concatenate(parts, axis=1) if want_rows else None
         final = [ctx.download(k) for k in range(min(len(b["inst"]), 4))]
         names = tm.variant_names(b["model"], b["precision"])
@@ -140,7 +158,7 @@ def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=N
     return dict(value=cells * steps / wall, wall_s=wall, ms_per_step=wall / steps * 1e3,
                 kernel_ms=avg_step_s * 1e3, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
                 cells=cells, steps=steps, variant=vname, graph_steps=variant[1] if variant else None,
-                rows=rows, final=final)
+                single_steps=m1[0] - m0[0], blocked_steps=m1[1] - m0[1], rows=rows, final=final)
 
 
 def sweep_leg(tm, steps=None):
@@ -184,6 +202,33 @@ def sweep_leg(tm, steps=None):
             t2 = time.perf_counter()
             out["snapshot_mbs"] = 512 * 512 * 3 * 4 / (t1 - t0) / 1e6
             out["frame_csv_rows_per_s"] = 512 * 512 / (t2 - t1)
+    return out
+
+
+def roofline(h, b):
+    """The headline's roofline: algorithmic bytes (one read and one write of the
+    field per cell-update) per step over the mean device time per step, and the
+    measured HBM bytes per step of the same run.  With temporal blocking (the
+    _xK kernels read and write the field once per K steps) the algorithmic rate
+    exceeds the HBM peak; traffic_frac is then the HBM utilisation."""
+    traffic = traffic_bytes(h["variant"], h["single_steps"], h["blocked_steps"])
+    step_s = h["kernel_ms"] / 1e3
+    out = {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": h["frac"], "traffic": traffic,
+           "kernel": "dp_stepk_kernel<float,128,32,256,4> (4 steps per launch) once <= 2% of cells are "
+                     "occupied, dp_step_kernel<float,128,32,256> single steps before"
+                     if "x4a" in (h["variant"] or "") else "dp_step_kernel<float>",
+           "bytes_per_cell": bytes_per_cell(b), "cells_per_launch": cells_of(b),
+           "step_device_ms": h["kernel_ms"], "single_steps": h["single_steps"],
+           "blocked_steps": h["blocked_steps"],
+           "timing": "HIP events on the context stream around every run() of the timed region; "
+                     "the per-step share also holds the per-block constants, rho(t) fold and "
+                     "counter bump launches"}
+    if traffic is not None:
+        out["traffic_gbs"] = traffic / step_s / 1e9
+        out["traffic_frac"] = out["traffic_gbs"] / HBM_PEAK_GBS
+        out["traffic_note"] = ("traffic: measured HBM bytes per step (FETCH_SIZE x 2 + WRITE_SIZE, per launch "
+                               "kind, weighed by the steps each kind advanced in this run)")
     return out
 
 
@@ -328,7 +373,8 @@ def main():
     if args.headline_only:
         print(json.dumps({"metric": METRIC, "value": h["value"], "unit": "cell-updates/s", "steps": K,
                           "ms_per_step": h["ms_per_step"], "step_device_ms": h["kernel_ms"],
-                          "achieved_gbs": h["achieved_gbs"], "variant": h["variant"]}), flush=True)
+                          "achieved_gbs": h["achieved_gbs"], "variant": h["variant"],
+                          "roofline": roofline(h, c4)}), flush=True)
         return
     c4d = c4_batch(tm, tm.F64)
     log("[bench] C4 FP64 (same seeds)")
@@ -385,13 +431,7 @@ def main():
                                "a 0.72->0.70 over 10^4 steps, rho(t) FP64 every step",
                    "instances": len(c4["inst"]), "cells_per_step": cells_of(c4), "steps": K,
                    "kernel_variant": h["variant"], "graph_steps": h["graph_steps"]},
-        "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": h["frac"], "traffic": traffic_bytes(h["variant"]),
-                     "kernel": "dp_step_kernel<float>", "bytes_per_cell": bytes_per_cell(c4),
-                     "cells_per_launch": cells_of(c4), "step_device_ms": h["kernel_ms"],
-                     "timing": "HIP events on the context stream around every run() of the timed region; "
-                               "the per-step share also holds the per-block constants, rho(t) fold and "
-                               "counter bump launches"},
+        "roofline": roofline(h, c4),
         "cpu_baseline": cpu,
         "side_by_side": side,
         "error_vs_fp64": errors,

@@ -216,6 +216,11 @@ tm_status tm_time_steps(tm_ctx* ctx, int64_t n_steps, double* ms_total, int64_t*
  * the host is outside).  n_steps (nullable) gets that call's step count. */
 tm_status tm_last_run_time(const tm_ctx* ctx, double* ms, int64_t* n_steps);
 
+/* Steps that tm_run / tm_run_series advanced since tm_configure by single-step
+ * launches and by multi-step (temporally blocked) launches of the DP variants
+ * named _xK / _xKa.  A timing aid (e.g. to weigh per-launch traffic). */
+tm_status tm_mode_steps(const tm_ctx* ctx, int64_t* single_steps, int64_t* blocked_steps);
+
 /* The context's HIP stream (hipStream_t) for callers that order their own work. */
 void* tm_stream(tm_ctx* ctx);
 

@@ -156,6 +156,7 @@ def lib() -> C.CDLL:
         "tm_time_steps": ([vp, i64, dp, C.POINTER(i64)], C.c_int),
         "tm_last_run_time": ([vp, dp, C.POINTER(i64)], C.c_int),
         "tm_stream": ([vp], vp),
+        "tm_mode_steps": ([vp, C.POINTER(i64), C.POINTER(i64)], C.c_int),
         "tm_log10_time_bins": ([C.c_double, C.c_double, dp, C.c_int], C.c_int),
         "tm_measure_steps": ([dp, C.c_int, C.c_double, C.POINTER(i64)], C.c_int),
         "tm_run_series": ([vp, C.POINTER(i64), C.c_int, vp], C.c_int),
@@ -476,6 +477,12 @@ class Context:
         ms = C.c_double()
         self._check(self._lib.tm_last_run_time(self._h, C.byref(ms), None))
         return ms.value
+
+    def mode_steps(self):
+        """(single-step, multi-step) steps run() has advanced since configure."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self._lib.tm_mode_steps(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def time_steps(self, n_steps: int) -> float:
         """Device ms of n_steps step-kernel launches (HIP events on the context stream)."""

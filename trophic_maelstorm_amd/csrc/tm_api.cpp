@@ -84,6 +84,7 @@ struct tm_ctx {
     hipEvent_t live_ev[2] = {nullptr, nullptr};   // per-block live counts copied to h_pinned[2 + j]
     bool live_pending[2] = {false, false};
     int64_t live_block = 0;
+    int64_t steps_single = 0, steps_blocked = 0;   // steps tm_run advanced by each kind of launch
 
     // device time of the last tm_run's launches (events on the context stream)
     hipEvent_t run_ev[2] = {nullptr, nullptr};
@@ -295,6 +296,15 @@ void launch_step(tm_ctx* c, int parity, int k, int m = 1)
     if (v.fn_a) v.fn_a(a, (int)c->tiles_a.size(), c->stream);   // split: P* first
     const StepLaunch fn = (v.spl > 1 && m == 1) ? v.fn_one : v.fn;
     fn(a, grid_of(c), c->stream);
+}
+
+// Bookkeeping of tm_mode_steps: n steps of one block in the current mode.
+void count_steps(tm_ctx* c, int64_t n)
+{
+    const int spl = cur_variant(c).spl > 1 && c->blocked ? cur_variant(c).spl : 1;
+    const int64_t blocked = spl > 1 ? (n / spl) * spl : 0;
+    c->steps_blocked += blocked;
+    c->steps_single += n - blocked;
 }
 
 // Steps per launch in the current mode.
@@ -515,6 +525,7 @@ tm_status advance(tm_ctx* c, int64_t n_steps)
     while (B > 0 && left >= B) {
         if ((s = capture(c, c->parity)) != TM_OK) return s;
         HIPCHK(c, hipGraphLaunch(c->graph[c->blocked][c->parity], c->stream));
+        count_steps(c, B);
         c->counter += B;
         c->parity ^= (launches_of(c, B) & 1);
         left -= B;
@@ -524,6 +535,7 @@ tm_status advance(tm_ctx* c, int64_t n_steps)
         const int chunk = (int)std::min<int64_t>(left, c->nslots);
         enqueue_block(c, c->parity, chunk, true);
         HIPCHK(c, hipGetLastError());
+        count_steps(c, chunk);
         c->parity ^= (launches_of(c, chunk) & 1);
         c->counter += chunk;
         left -= chunk;
@@ -932,6 +944,14 @@ tm_status tm_run(tm_ctx* c, int64_t n_steps, double* rows)
     c->last_run_ms = ms;
     c->last_run_steps = n_steps;
     return fault_status(c);
+}
+
+tm_status tm_mode_steps(const tm_ctx* c, int64_t* single_steps, int64_t* blocked_steps)
+{
+    if (!c || !single_steps || !blocked_steps) return TM_ERR_ARG;
+    *single_steps = c->steps_single;
+    *blocked_steps = c->steps_blocked;
+    return TM_OK;
 }
 
 tm_status tm_last_run_time(const tm_ctx* c, double* ms, int64_t* n_steps)

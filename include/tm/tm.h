@@ -42,7 +42,11 @@ typedef enum tm_status {
 
 typedef enum tm_model {
     TM_MODEL_DP = 1,        /* one field rho                         (multiSPDP.cpp, Sp = 1) */
-    TM_MODEL_RIETKERK = 3   /* three fields P, W, O (rietkerk_bjork_constants_1Sp.h:5, Sp = 3) */
+    TM_MODEL_DP2 = 2,       /* vegetation + grazer: fields rho_0, rho_1, gamma_1
+                             * (multiSPDP.cpp built with SPB = 2, MultiSPDP_constants_2Sp.h:5) */
+    TM_MODEL_RIETKERK = 3,  /* three fields P, W, O (rietkerk_bjork_constants_1Sp.h:5, Sp = 3) */
+    TM_MODEL_DP3 = 4        /* vegetation + grazer + predator: fields rho_0, rho_1, rho_2,
+                             * gamma_1, gamma_2 (SPB = 3, MultiSPDP_constants_3Sp.h:5) */
 } tm_model;
 
 typedef enum tm_precision {
@@ -76,6 +80,33 @@ typedef struct tm_dp_params {
     int32_t scheme;     /* tm_dp_scheme */
     int32_t sampler;    /* tm_sampler   */
 } tm_dp_params;
+
+/* Vegetation + consumer parameters: dP_Dornic_2D_MultiSp(..., a, b, c, D[Sp], v[SpB], sigma[], ...,
+ * A[SpB][SpB], H[SpB][SpB], E[SpB], M[SpB], pR[], dtV[SpB], ..., dt, dx, ...)
+ * (Percolation_FD/MultiSPDP.h:344-346).  Species 0 is the vegetation, 1 the grazer, 2 the
+ * predator (TM_MODEL_DP3 only).  a drives the vegetation's Dornic step (ramped as in
+ * tm_dp_params); b is the vegetation's self-limitation in the trophic RK4 -- the reference's
+ * call (multiSPDP.cpp:3222) passes a there, so pass b = a to reproduce its runs.  The
+ * perception disk has radius R = (int)(max pR / dx) sites (1 <= R <= TM_DPC_MAX_RADIUS).
+ * c and dtV have no effect on the reference's step and are not taken. */
+#define TM_DPC_MAX_RADIUS 12
+typedef struct tm_dpc_params {
+    double a, b;
+    double D[3], sigma[3];
+    double v[3];        /* consumer speeds, length/time; v[0] unused    */
+    double M[3];        /* consumer mortality; M[0] unused              */
+    double E[3];        /* consumer efficiency; E[0] unused             */
+    double A[3][3];     /* attack rates: A[0][1] grazer on vegetation, A[1][2] predator on grazer */
+    double H[3][3];     /* handling times, same indices                 */
+    double pR[3];       /* perception radius, length units; pR[0] = 0  */
+    double dt, dx;
+    double a_end;
+    int64_t ramp_steps;
+    double flush_eps;   /* 1e-12 (multiSPDP.cpp:2426)  */
+    double flush_neg;   /* -1e-80 (multiSPDP.cpp:3257) */
+    int32_t sampler;    /* tm_sampler */
+    int32_t reserved;
+} tm_dpc_params;
 
 /* Rietkerk parameters: rietkerk_DorFPE_2D_MultiSp(..., a(=R), c, gmax, alpha, rW, W0, D[Sp], ...,
  * K[3], sigma[], ..., M[](d = M[0]), ..., dt, dx, ...) (Rietkerk_FPE/rietkerk_bjork_basic.h:450-452).
@@ -112,10 +143,12 @@ tm_status tm_create(tm_ctx** out, int device);
 void tm_destroy(tm_ctx* ctx);
 const char* tm_last_error(const tm_ctx* ctx);
 
-/* Declare the batch: model, precision, params[n_params] (tm_dp_params or
- * tm_rietkerk_params), instances[n_inst].  Allocates the double-buffered
- * fields (zeroed) and the per-instance descriptor and tile tables.  Replaces the
- * per-replicate field allocation of rietkerk_bjork_FKE.cpp:3902-3913. */
+/* Declare the batch: model, precision, params[n_params] (tm_dp_params,
+ * tm_rietkerk_params, or tm_dpc_params for TM_MODEL_DP2 / TM_MODEL_DP3, which are
+ * TM_F64 only), instances[n_inst].  Allocates the double-buffered fields (zeroed)
+ * and the per-instance descriptor and tile tables.  Replaces the per-replicate
+ * field allocation of rietkerk_bjork_FKE.cpp:3902-3913 (multiSPDP.cpp:2576-2585
+ * with the perception gamma fields, :2581). */
 tm_status tm_configure(tm_ctx* ctx, int model, int precision, const void* params, int n_params,
                        const tm_instance_desc* instances, int n_inst);
 
@@ -126,7 +159,7 @@ tm_status tm_upload_fields(tm_ctx* ctx, int inst, const double* host);
 tm_status tm_download_fields(tm_ctx* ctx, int inst, double* host);
 
 /* i.i.d. Uniform(lo[f], hi[f]) initial fields for every instance, drawn on the
- * device from Philox(seed, instance_id; cell, 0xFFFFFFFF, 0xFFFFFFFF, 0), word f. */
+ * device from Philox(seed, instance_id; cell, 0xFFFFFFFF, 0xFFFFFFFF, f / 4), word f % 4. */
 tm_status tm_init_uniform(tm_ctx* ctx, const double* lo, const double* hi);
 
 /* Speckle initial frames: every cell draws u ~ Uniform(0,1) from
@@ -289,6 +322,13 @@ int tm_read_frame_csv(const char* path, int L, int nf, const char* const* cols, 
  * 1, duplicates and points closer than dt/2 removed.  Writes at most cap values
  * to out (nullable) and returns how many there are (0 when t_max <= 1). */
 int tm_log10_time_bins(double t_max, double dt, double* out, int cap);
+
+/* Host only.  The perception neighbourhood of the consumer models: offsets
+ * (row, column) of the sites within Euclidean radius R of a site, in the order of
+ * computeNeighboringSitesCentral (multiSPDP.cpp:1442-1461): generated row-major
+ * over [-R, R]^2 and std::sort-ed by squared distance alone, ties as that sort
+ * leaves them.  out[2 k], out[2 k + 1]; returns the count, or -1 if it exceeds cap. */
+int tm_perception_offsets(int R, int32_t* out, int cap);
 
 /* Host only.  The step numbers at which the reference's time loop logs each
  * t_meas[i]: t accumulates t += dt from 0 and the state after n steps is logged

@@ -39,7 +39,18 @@ class RKParams(C.Structure):
                 ("sampler", C.c_int32), ("pad_", C.c_int32)]
 
 
+class DPCParams(C.Structure):
+    """tmo_dpc_params (mirror of tm_dpc_params)."""
+    _fields_ = [("a", C.c_double), ("b", C.c_double), ("D", C.c_double * 3), ("sigma", C.c_double * 3),
+                ("v", C.c_double * 3), ("M", C.c_double * 3), ("E", C.c_double * 3),
+                ("A", (C.c_double * 3) * 3), ("H", (C.c_double * 3) * 3), ("pR", C.c_double * 3),
+                ("dt", C.c_double), ("dx", C.c_double), ("a_end", C.c_double), ("ramp_steps", C.c_int64),
+                ("flush_eps", C.c_double), ("flush_neg", C.c_double), ("sampler", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
 FMT_PATH = os.path.join(HERE, "libfmt_check.so")
+RADII_PATH = os.path.join(HERE, "_ref", "test_perception_radii")
 
 
 def build(force: bool = False) -> str:
@@ -101,6 +112,15 @@ def lib():
         L.tmo_replicate_stats.argtypes = [_f64p, C.c_int, C.c_int, C.c_int,
                                           np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                           C.c_int, C.c_int, _f64p]
+        L.tmo_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.tmo_fixed_sum.restype = C.c_double
+        L.tmo_fixed_sum.argtypes = [_f64p, C.c_int64]
+        L.tmo_dpc_rk4.argtypes = [C.POINTER(DPCParams), C.c_int, _f64p]
+        L.tmo_perception_offsets.restype = C.c_int
+        L.tmo_perception_offsets.argtypes = [C.c_int, C.c_void_p, C.c_int]
+        L.tmo_dpc_run.restype = C.c_int
+        L.tmo_dpc_run.argtypes = [C.POINTER(DPCParams), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_uint32,
+                                  C.c_uint32, C.c_int64, C.c_int64, _f64p, C.c_void_p, C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -197,6 +217,73 @@ def dp_run(p: DPParams, rho, L, seed, inst, step0, nsteps, rows=True, draws=Fals
 
 def rk_run(p: RKParams, fields, L, seed, inst, step0, nsteps, rows=True, draws=False):
     return _run(lib().tmo_rk_run, p, fields, L, seed, inst, step0, nsteps, 3, rows, draws)
+
+
+# ---------------------------------------------------------------- consumer species
+def _v3(v):
+    v = list(v) + [0.0] * (3 - len(v))
+    return (C.c_double * 3)(*v)
+
+
+def _m3(m):
+    return ((C.c_double * 3) * 3)(*[(C.c_double * 3)(*row) for row in np.asarray(m, np.float64)])
+
+
+def dpc_params(p) -> DPCParams:
+    """DPCParams from any object with the tm_dpc_params field names (the product's
+    DPCParams dataclass, or a SimpleNamespace)."""
+    return DPCParams(p.a, p.b, _v3(p.D), _v3(p.sigma), _v3(p.v), _v3(p.M), _v3(p.E), _m3(p.A), _m3(p.H),
+                     _v3(p.pR), p.dt, p.dx, p.a if getattr(p, "a_end", None) is None else p.a_end,
+                     getattr(p, "ramp_steps", 0), getattr(p, "flush_eps", 1e-12),
+                     getattr(p, "flush_neg", -1e-80), getattr(p, "sampler", 0), 0)
+
+
+def sincos2pi(u):
+    s, c = C.c_double(), C.c_double()
+    lib().tmo_sincos2pi(float(u), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def fixed_sum(x):
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    return lib().tmo_fixed_sum(x, x.size)
+
+
+def dpc_rk4(p: DPCParams, ns: int, y):
+    y = np.ascontiguousarray(y, np.float64).copy()
+    lib().tmo_dpc_rk4(C.byref(p), ns, y)
+    return y
+
+
+def perception_offsets(R: int) -> np.ndarray:
+    buf = np.zeros(2 * (2 * R + 1) ** 2, np.int32)
+    n = lib().tmo_perception_offsets(R, buf.ctypes.data, buf.size // 2)
+    return buf[: 2 * n].reshape(n, 2)
+
+
+def dpc_run(p: DPCParams, ns: int, fields, L, seed, inst, step0, nsteps, rows=True):
+    """nsteps of the ns-species step; fields [2 ns - 1, L, L] (densities, then gammas)."""
+    nf = 2 * ns - 1
+    rmax = max(p.pR[s] for s in range(ns)) / p.dx
+    disk = perception_offsets(int(rmax))
+    f = np.ascontiguousarray(fields, np.float64).copy().reshape(-1)
+    r = np.zeros((nsteps, 2 * nf), np.float64) if rows else None
+    fc = C.c_int64(-1)
+    st = lib().tmo_dpc_run(C.byref(p), ns, L, disk.ctypes.data, disk.shape[0], seed, inst, step0, nsteps, f,
+                           r.ctypes.data if r is not None else None, C.byref(fc))
+    return {"fields": f.reshape(nf, L, L), "rows": r, "status": st, "fault_index": fc.value}
+
+
+def ref_perception_sites():
+    """The site list printed by the reference's Test/test_perception_radii.cpp (built
+    from source into oracle/_ref): the sites within radius 8 of (2, 2) on a 20 x 20
+    lattice, in its computeNeighboringSitesCentral order; None when not built."""
+    import re
+    if not os.path.exists(RADII_PATH):
+        return None
+    out = subprocess.run([RADII_PATH], capture_output=True, text=True, check=True).stdout
+    body = out.split(":", 1)[1]        # drop the "Neighboring sites for (2, 2)" title
+    return [(int(a), int(b)) for a, b in re.findall(r"\((\d+), (\d+)\)", body)]
 
 
 def laplacian(x):

@@ -343,11 +343,13 @@ void tmo_init_uniform(int L, int nf, uint32_t seed, uint32_t inst,
     const int64_t N = (int64_t)L * L;
     uint32_t key[2] = {seed, inst};
     for (int64_t i = 0; i < N; ++i) {
-        uint32_t ctr[4] = {(uint32_t)i, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u};
         uint32_t w[4];
-        tmo_philox4x32_10(ctr, key, w);
-        for (int f = 0; f < nf; ++f) {
-            double u = ((double)w[f] + 0.5) * 0x1p-32;
+        for (int f = 0; f < nf; ++f) {     /* field f: word f % 4 of block f / 4 */
+            if (f % 4 == 0) {
+                uint32_t ctr[4] = {(uint32_t)i, 0xFFFFFFFFu, 0xFFFFFFFFu, (uint32_t)(f / 4)};
+                tmo_philox4x32_10(ctr, key, w);
+            }
+            double u = ((double)w[f % 4] + 0.5) * 0x1p-32;
             out[f * N + i] = lo[f] + (hi[f] - lo[f]) * u;
         }
     }
@@ -677,4 +679,246 @@ void tmo_init_gauss(int L, int nf, int n_gauss, const double* sd_div, int n_sd,
             for (int j = 0; j < g; ++j)
                 a[(int64_t)i * g + j] = amp[s] * tmo_exp(-((i - cx) * (i - cx) + (j - cy) * (j - cy)) / (2 * sd * sd));
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* Consumer species: vegetation + grazer (SPB = 2) and + predator      */
+/* (SPB = 3), the time-loop body of dP_Dornic_2D_MultiSp               */
+/* (MSP:2901-2958 vegetation Dornic, :2962-2978 spatial means and the  */
+/* cover factor, :1699-1786 / :2166-2284 perception gamma, :3041-3190   */
+/* consumer Dornic with upwind advection, :1670-1697 / :1789-1819 +     */
+/* :1821-1898 RK4 of the trophic terms, :3228-3262 checks and flush).  */
+/* ------------------------------------------------------------------ */
+
+/* sin and cos of 2 pi u: t = 4u (exact), q = nearest integer to t, x = (t - q) pi/2
+ * in [-pi/4, pi/4]; Taylor polynomials of sin x to x^17 and cos x to x^18 in x^2
+ * (next terms below 1e-19), then the quarter-turn rotation by q. */
+void tmo_sincos2pi(double u, double* s_out, double* c_out)
+{
+    const double t = 4.0 * u;
+    const double q = floor(t + 0.5);
+    const double x = (t - q) * 0x1.921fb54442d18p+0;
+    const double z = x * x;
+    double s = 0x1.952c77030ad4ap-49;
+    s = s * z + -0x1.ae7f3e733b81fp-41;
+    s = s * z + 0x1.6124613a86d09p-33;
+    s = s * z + -0x1.ae64567f544e4p-26;
+    s = s * z + 0x1.71de3a556c734p-19;
+    s = s * z + -0x1.a01a01a01a01ap-13;
+    s = s * z + 0x1.1111111111111p-7;
+    s = s * z + -0x1.5555555555555p-3;
+    s = x + x * (z * s);
+    double c = -0x1.6827863b97d97p-53;
+    c = c * z + 0x1.ae7f3e733b81fp-45;
+    c = c * z + -0x1.93974a8c07c9dp-37;
+    c = c * z + 0x1.1eed8eff8d898p-29;
+    c = c * z + -0x1.27e4fb7789f5cp-22;
+    c = c * z + 0x1.a01a01a01a01ap-16;
+    c = c * z + -0x1.6c16c16c16c17p-10;
+    c = c * z + 0x1.5555555555555p-5;
+    c = c * z + -0x1.0000000000000p-1;
+    c = 1.0 + z * c;
+    const int k = ((int)q) & 3;
+    if (k == 0) { *s_out = s; *c_out = c; }
+    else if (k == 1) { *s_out = c; *c_out = -s; }
+    else if (k == 2) { *s_out = -s; *c_out = -c; }
+    else { *s_out = -c; *c_out = s; }
+}
+
+/* Fixed-order sum of x[0..n): chunks of 4096 cells; in a chunk lane t (0..255)
+ * adds cells t, t + 256, ... from 0.0, the 256 lane sums are folded pairwise
+ * (s[t] += s[t + o] for o = 128, 64, ..., 1), and the chunk sums are added in
+ * chunk order from 0.0.  It replaces mean_of_vector's sequential sum
+ * (MSP:997-1005) wherever the mean steers the dynamics, so that a parallel
+ * reduction can reproduce it exactly. */
+double tmo_fixed_sum(const double* x, int64_t n)
+{
+    double total = 0.0;
+    for (int64_t c0 = 0; c0 < n; c0 += 4096) {
+        double s[256];
+        for (int t = 0; t < 256; ++t) {
+            double a = 0.0;
+            for (int64_t i = c0 + t; i < c0 + 4096 && i < n; i += 256) a += x[i];
+            s[t] = a;
+        }
+        for (int o = 128; o > 0; o >>= 1)
+            for (int t = 0; t < o; ++t) s[t] += s[t + o];
+        total += s[0];
+    }
+    return total;
+}
+
+static int64_t wrapi(int64_t v, int L) { int64_t m = v % L; return m < 0 ? m + L : m; }
+
+/* f_DP_Dor_2Sp (MSP:1670-1697) and f_DP_Dor_3Sp (MSP:1789-1819) at one cell. */
+static void dpc_f(const tmo_dpc_params* p, int ns, const double* y, double* f)
+{
+    const double A01H01 = p->A[0][1] * p->H[0][1];
+    const double EA1 = p->E[1] * p->A[0][1];
+    f[0] = -p->b * y[0] * y[0] - (p->A[0][1] * y[0] / (1 + A01H01 * y[0])) * y[1];
+    if (ns == 2) {
+        f[1] = (EA1 * y[0] / (1 + A01H01 * y[0])) * y[1];
+        return;
+    }
+    const double A12H12 = p->A[1][2] * p->H[1][2];
+    const double EA2 = p->E[2] * p->A[1][2];
+    f[1] = (EA1 * y[0] / (1 + A01H01 * y[0])) * y[1] - p->A[1][2] * y[1] / (1 + A12H12 * y[1]) * y[2];
+    f[2] = (EA2 * y[1] / (1 + A12H12 * y[1])) * y[2];
+}
+
+/* RK4_Integrate_Stochastic_MultiSp (MSP:1821-1898) at one cell: the trophic
+ * terms have no spatial coupling, so the four stages are pointwise. */
+void tmo_dpc_rk4(const tmo_dpc_params* p, int ns, double* y)
+{
+    const double dt = p->dt, dt2 = dt / 2.0, dt6 = dt / 6.0;
+    double k1[3], k2[3], k3[3], k4[3], ys[3] = {0.0, 0.0, 0.0};
+    dpc_f(p, ns, y, k1);
+    for (int s = 0; s < ns; ++s) ys[s] = y[s] + (dt2) * k1[s];
+    dpc_f(p, ns, ys, k2);
+    for (int s = 0; s < ns; ++s) ys[s] = y[s] + (dt2) * k2[s];
+    dpc_f(p, ns, ys, k3);
+    for (int s = 0; s < ns; ++s) ys[s] = y[s] + (dt) * k3[s];
+    dpc_f(p, ns, ys, k4);
+    for (int s = 0; s < ns; ++s) y[s] += (dt6) * (k1[s] + 2.0 * k2[s] + 2.0 * k3[s] + k4[s]);
+}
+
+static double dpc_ramp_a(const tmo_dpc_params* p, int64_t step)
+{
+    if (p->ramp_steps <= 0) return p->a;
+    int64_t m = step < p->ramp_steps ? step : p->ramp_steps;
+    return p->a + (p->a_end - p->a) * ((double)m / (double)p->ramp_steps);
+}
+
+int tmo_dpc_run(const tmo_dpc_params* p, int ns, int L, const int32_t* disk, int disk_len,
+                uint32_t seed, uint32_t inst, int64_t step0, int64_t nsteps, double* fields,
+                double* rows, int64_t* fault_index)
+{
+    const int nf = 2 * ns - 1;                 /* rho_0 .. rho_{ns-1}, gamma_1 .. gamma_{ns-1} */
+    const int64_t N = (int64_t)L * L;
+    double* old = (double*)malloc(sizeof(double) * N * nf);
+    const double eps_g = 1.0e-12;              /* calc_gamma eps, MSP:1715 / :2182 */
+    const double dx = p->dx, dt = p->dt;
+    const double dx2 = dx * dx, dx1 = 1 / dx, dx1_2 = 1 / (dx2), dxb2 = dx / 2.0;   /* MSP:2470 */
+    const double Dc0 = p->D[0] / (dx2);        /* diff_coefficient[0], MSP:2479 */
+    double sigma2_1[3];
+    for (int s = 0; s < ns; ++s) sigma2_1[s] = 1 / (p->sigma[s] * p->sigma[s]);
+    double rmax = p->pR[0];                    /* r_max_effective, MSP:2429 */
+    for (int s = 1; s < ns; ++s) if (p->pR[s] > rmax) rmax = p->pR[s];
+    rmax = rmax / dx;
+    double fr[3];
+    for (int s = 0; s < ns; ++s) fr[s] = p->pR[s] / (dx * rmax);   /* MSP:2434-2435 */
+    const int compat = p->sampler == 1;
+    int fault = 0;
+    if (fault_index) *fault_index = -1;
+    for (int64_t n = 0; n < nsteps; ++n) {
+        const int64_t step = step0 + n;
+        memcpy(old, fields, sizeof(double) * N * nf);
+        /* spatial means and vegetation cover of the step's start (MSP:2962-2975) */
+        double mean[3], inv[3];
+        for (int s = 0; s < ns; ++s) mean[s] = tmo_fixed_sum(old + s * N, N) / (double)N;
+        double occ = 0.0;
+        for (int64_t i = 0; i < N; ++i) if (old[i] > 0) occ += 1;
+        double nfac = 1 - occ / (double)N;
+        if (nfac < 0.35) nfac = 0.35;
+        const int new_len = (int)(nfac * nfac * (double)disk_len);   /* MSP:1716 / :2183 */
+        for (int s = 0; s < ns; ++s) inv[s] = mean[s] >= eps_g ? 1 / mean[s] : 1 / eps_g;
+        int eff[3];
+        for (int s = 0; s < ns; ++s) {
+            eff[s] = (int)(fr[s] * fr[s] * (double)new_len);
+            if (ns == 3 && eff[s] < 1 && fr[s] > 0) eff[s] = 1;    /* MSP:2215-2216 */
+        }
+        double cst[3];
+        tmo_dornic_constants(dpc_ramp_a(p, step), p->D[0], p->sigma[0], dt, dx, cst);
+        const double lam0 = cst[2], ll0 = cst[2] * cst[1];
+        for (int pp = 0; pp < L; ++pp)
+            for (int q = 0; q < L; ++q) {
+                const int64_t i = (int64_t)pp * L + q;
+                const int64_t up = nb_up(L, pp, q), dn = nb_dn(L, pp, q), lf = nb_lf(L, pp, q), rt = nb_rt(L, pp, q);
+                double y[3];
+                /* vegetation Dornic (MSP:2901-2945) */
+                const double* r0 = old;
+                const double alpha0 = Dc0 * (r0[up] + r0[dn] + r0[lf] + r0[rt]);
+                y[0] = r0[i];
+                if (!(alpha0 == 0 && r0[i] == 0)) {
+                    stream_t st;
+                    int64_t k;
+                    st_init(&st, seed, inst, (uint32_t)i, step);
+                    y[0] = dornic_cell(&st, alpha0, r0[i], ll0, lam0, sigma2_1[0], compat, &k);
+                }
+                /* perception gamma (calc_gamma_2Sp_NonRefugia / calc_gamma_3Sp_NonRefugia) */
+                double gam[3] = {0.0, 0.0, 0.0};
+                for (int s = 1; s < ns; ++s) {
+                    double g = old[(ns + s - 1) * N + i];
+                    if (fr[s] == 0.0) { gam[s] = g; continue; }
+                    if (mean[s] < eps_g) { gam[s] = 0.0; continue; }
+                    if (mean[s - 1] < eps_g) { gam[s] = 0.0; continue; }
+                    const double* src = old + (s - 1) * N;
+                    if (ns == 2) {                                 /* MSP:1756-1760: accumulates */
+                        for (int k = 0; k < eff[s]; ++k)
+                            g += src[wrapi(pp + disk[2 * k], L) * L + wrapi(q + disk[2 * k + 1], L)];
+                        g = (g * inv[s - 1]) / (eff[s]);
+                    } else if (s == 1) {                           /* MSP:2228-2249 */
+                        double sum = 0.0;
+                        const double* pr = old + 2 * N;
+                        for (int k = 0; k < eff[s]; ++k) {
+                            const int64_t j = wrapi(pp + disk[2 * k], L) * L + wrapi(q + disk[2 * k + 1], L);
+                            if (mean[2] < eps_g) sum += src[j];
+                            else sum += src[j] * (1 - pr[j] * inv[2]);
+                        }
+                        g = (sum * inv[s - 1]) / (eff[s]);
+                    } else {                                       /* MSP:2251-2258 */
+                        double sum = 0.0;
+                        for (int k = 0; k < eff[s]; ++k)
+                            sum += src[wrapi(pp + disk[2 * k], L) * L + wrapi(q + disk[2 * k + 1], L)];
+                        g = (sum / (eff[s])) * inv[s - 1];
+                    }
+                    if (g > 1) g = 1.0;                            /* MSP:1763-1766 */
+                    else if (g < 0) g = 0.0;
+                    gam[s] = g;
+                }
+                /* consumer Dornic with advection (MSP:3041-3190) */
+                for (int s = 1; s < ns; ++s) {
+                    const double* rs = old + s * N;
+                    y[s] = rs[i];
+                    if (mean[s] == 0.0) continue;                  /* MSP:3044-3045 */
+                    if (gam[s] > 1) gam[s] = 1;
+                    stream_t st;
+                    st_init(&st, seed, inst, (uint32_t)i, step);
+                    st.ctr[3] = (uint32_t)s << 24;
+                    double sn, cs;
+                    tmo_sincos2pi(st_u(&st), &sn, &cs);
+                    const double vx = p->v[s] * cs, vy = p->v[s] * sn;
+                    const double vxa = vx * (double)((0 < vx) - (vx < 0));
+                    const double vya = vy * (double)((0 < vy) - (vy < 0));
+                    const double gp = (1 - gam[s]);
+                    double Dx = (p->D[s] - (dxb2) * (gp * vxa * (1 - dt * dx1 * gp * vxa))) * dx1_2;
+                    double Dy = (p->D[s] - (dxb2) * (gp * vya * (1 - dt * dx1 * gp * vya))) * dx1_2;
+                    if (Dx < Dc0) Dx = Dc0;
+                    if (Dy < Dc0) Dy = Dc0;
+                    const int64_t ux = vx > 0 ? lf : rt, uy = vy > 0 ? up : dn;
+                    const double alpha = gam[s] * (Dy * (rs[up] + rs[dn]) + Dx * (rs[lf] + rs[rt]))
+                                         + dx1 * (gp) * (vxa * rs[ux] + vya * rs[uy]);
+                    if (alpha == 0 && rs[i] == 0) continue;
+                    const double beta = -p->M[s] - 2 * gam[s] * (Dx + Dy) - (1 - gam[s]) * (vxa + vya) * dx1;
+                    const double lexp = tmo_exp((beta) * dt);
+                    const double lam = 2 * (beta) * (sigma2_1[s]) / (lexp - 1.0);
+                    int64_t k;
+                    y[s] = dornic_cell(&st, alpha, rs[i], lam * lexp, lam, sigma2_1[s], compat, &k);
+                }
+                tmo_dpc_rk4(p, ns, y);
+                for (int s = 0; s < ns; ++s) {
+                    if (y[s] < 0 || !isfinite(y[s])) {
+                        const int64_t key = (int64_t)s * N + i;
+                        if (fault_index && (*fault_index < 0 || key < *fault_index)) *fault_index = key;
+                        fault = 1;
+                    }
+                    fields[s * N + i] = tmo_flush(y[s], p->flush_eps, p->flush_neg);
+                }
+                for (int s = 1; s < ns; ++s) fields[(ns + s - 1) * N + i] = gam[s];
+            }
+        if (rows) tmo_reduce(L, nf, fields, rows + 2 * nf * n);
+        if (fault) break;
+    }
+    free(old);
+    return fault;
 }

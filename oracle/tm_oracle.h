@@ -62,6 +62,27 @@ typedef struct tmo_rk_params {
     int32_t pad_;
 } tmo_rk_params;
 
+/* Vegetation + consumers, the multi-species DP step (multiSPDP.cpp with SPB = 2 or 3,
+ * MultiSPDP.h:344-346 argument list).  Species 0 is the vegetation, 1 the grazer,
+ * 2 the predator.  b is the vegetation's self-limitation in the trophic RK4; the
+ * reference's call (multiSPDP.cpp:3222) passes a there, so pass b = a to reproduce it. */
+typedef struct tmo_dpc_params {
+    double a, b;
+    double D[3], sigma[3];
+    double v[3];           /* consumer speeds; v[0] unused */
+    double M[3];           /* consumer mortality; M[0] unused */
+    double E[3];           /* consumer efficiency; E[0] unused */
+    double A[3][3];        /* attack rates; A[0][1] grazer on vegetation, A[1][2] predator on grazer */
+    double H[3][3];        /* handling times, same indices */
+    double pR[3];          /* perception radius, length units; pR[0] = 0 */
+    double dt, dx;
+    double a_end;
+    int64_t ramp_steps;
+    double flush_eps, flush_neg;
+    int32_t sampler;
+    int32_t pad_;
+} tmo_dpc_params;
+
 /* ---- primitives exposed for the oracle's own pins ---- */
 void tmo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double tmo_exp(double x);
@@ -127,6 +148,30 @@ void tmo_init_gauss(int L, int nf, int n_gauss, const double* sd_div, int n_sd,
  * occupied count of field 0 over the surviving replicates. */
 void tmo_replicate_stats(const double* series, int n_inst, int n_meas, int nf,
                          const int32_t* params_index, int n_params, int flags, double* out);
+
+/* ---- consumer species (multiSPDP.cpp, SPB = 2, 3) ---- */
+/* sin and cos of 2 pi u (quarter-turn reduction + Taylor polynomials); the
+ * direction of a consumer's advection draw (MSP:3100). */
+void tmo_sincos2pi(double u, double* s, double* c);
+/* The fixed-order sum that stands for mean_of_vector's sequential sum where the
+ * means steer the step (chunks of 4096, 256 lanes, pairwise lane fold). */
+double tmo_fixed_sum(const double* x, int64_t n);
+/* RK4 of the trophic terms at one cell, y[ns] in place (MSP:1670-1697, :1789-1819, :1821-1898). */
+void tmo_dpc_rk4(const tmo_dpc_params* p, int ns, double* y);
+/* Offsets (row, column) of the sites within radius R of a site, in the order of
+ * computeNeighboringSitesCentral (MSP:1442-1461): x-major over [-R, R]^2, then
+ * std::sort by squared distance.  out[2 k], out[2 k + 1]; returns the count, or
+ * -1 if cap is too small (perception.cpp). */
+int tmo_perception_offsets(int R, int32_t* out, int cap);
+/* nsteps of the ns-species step on host fields [2 ns - 1][L*L] = rho_0 .. rho_{ns-1},
+ * gamma_1 .. gamma_{ns-1}.  disk: perception offsets of R = (int)(max pR / dx),
+ * disk_len of them.  Consumer s draws from the stream of counter block s << 24:
+ * word 0 is its advection direction, the rest its Dornic sampler.  rows as
+ * tmo_dp_run, over all 2 ns - 1 fields.  Returns 1 on a negative or non-finite
+ * density (the smallest species-major index s L^2 + cell in *fault_index). */
+int tmo_dpc_run(const tmo_dpc_params* p, int ns, int L, const int32_t* disk, int disk_len,
+                uint32_t seed, uint32_t inst, int64_t step0, int64_t nsteps, double* fields,
+                double* rows, int64_t* fault_index);
 
 #ifdef __cplusplus
 }

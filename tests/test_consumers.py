@@ -1,0 +1,278 @@
+"""The consumer-species step (multiSPDP.cpp built with SPB = 2, 3) in the CPU
+oracle, pinned against the reference's own neighbourhood program, closed forms,
+reductions to the pinned single-species step, conservation laws, brute force and
+the transition moments; and the product's host-side perception table against
+the oracle's.  No GPU needed."""
+import math
+import types
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+
+def params(**kw):
+    """Test parameters with rates of order one (the field values of
+    order_2stocDP.cpp make nothing happen in a few steps)."""
+    p = dict(a=0.5, b=0.5, D=(0.1 * 0.01, 0.2 * 0.01, 0.3 * 0.01), sigma=(0.5, 0.7, 0.9),
+             v=(0.0, 0.4, 0.3), M=(0.0, 0.2, 0.3), E=(1.0, 0.6, 0.5),
+             A=((0.0, 0.8, 0.0), (0.8, 0.0, 0.6), (0.0, 0.6, 0.0)),
+             H=((0.0, 0.5, 0.0), (0.5, 0.0, 0.4), (0.0, 0.4, 0.0)),
+             pR=(0.0, 0.3, 0.2), dt=0.1, dx=0.1, a_end=None, ramp_steps=0, flush_eps=1e-12,
+             flush_neg=-1e-80, sampler=0)
+    p.update(kw)
+    return types.SimpleNamespace(**p)
+
+
+# ------------------------------------------------------------------ perception neighbourhood
+def test_perception_order_matches_the_reference_program(oracle):
+    """The reference's Test/test_perception_radii.cpp, built from source, prints
+    the sites within radius 8 of (2, 2) on a 20 x 20 lattice in
+    computeNeighboringSitesCentral's order, ties included."""
+    sites = oracle.ref_perception_sites()
+    if sites is None:
+        pytest.skip("oracle/_ref/test_perception_radii not built (reference sources absent)")
+    off = oracle.perception_offsets(8)
+    mine = [((2 + int(dx)) % 20, (2 + int(dy)) % 20) for dx, dy in off]
+    assert len(mine) == 197 and mine == sites
+
+
+@pytest.mark.parametrize("R", [0, 1, 2, 3, 5, 8, 12])
+def test_perception_offsets_are_the_disk_by_distance(oracle, R):
+    off = oracle.perception_offsets(R)
+    disk = {(x, y) for x in range(-R, R + 1) for y in range(-R, R + 1) if x * x + y * y <= R * R}
+    assert len(off) == len(disk) and {tuple(map(int, o)) for o in off} == disk
+    d2 = off[:, 0] ** 2 + off[:, 1] ** 2
+    assert np.all(np.diff(d2) >= 0) and tuple(off[0]) == (0, 0)
+
+
+@pytest.mark.parametrize("R", [1, 5, 8, 12])
+def test_product_perception_table_equals_oracle(oracle, R):
+    import trophic_maelstorm_amd as tm
+    from trophic_maelstorm_amd import build as b
+    import os
+    if not os.path.exists(b.LIB):
+        b.build()
+    assert np.array_equal(tm.perception_offsets(R), oracle.perception_offsets(R))
+
+
+# ------------------------------------------------------------------ primitives
+def test_sincos2pi_against_libm(oracle):
+    rng = np.random.default_rng(1)
+    for u in np.concatenate([rng.random(20000), (np.arange(64) + 0.5) / 64]):
+        s, c = oracle.sincos2pi(u)
+        f = u - round(u)                      # exact; libm's own argument rounding stays below 3e-16
+        assert abs(s - math.sin(2 * math.pi * f)) < 1e-15 and abs(c - math.cos(2 * math.pi * f)) < 1e-15
+    assert oracle.sincos2pi(0.0) == (0.0, 1.0)
+    assert oracle.sincos2pi(0.25)[0] == 1.0 and oracle.sincos2pi(0.5)[1] == -1.0
+    assert oracle.sincos2pi(0.75)[0] == -1.0
+
+
+def test_fixed_sum_order(oracle):
+    # five cells: lanes 0..4 hold one cell each; the fold adds lane 4 to 0 (o = 4),
+    # then 2 to 0 and 3 to 1 (o = 2), then 1 to 0: ((x0 + x4) + x2) + (x1 + x3)
+    x = np.array([1e16, 1.0, -1e16, 1.0, 1.0])
+    assert oracle.fixed_sum(x) == ((x[0] + x[4]) + x[2]) + (x[1] + x[3])
+    # integers are exact in any order; random data within rounding of the exact sum
+    n = 3 * 4096 + 17
+    k = np.arange(n, dtype=np.float64)
+    assert oracle.fixed_sum(k) == n * (n - 1) / 2
+    r = np.random.default_rng(2).random(n)
+    assert abs(oracle.fixed_sum(r) - math.fsum(r)) < 1e-12 * math.fsum(r)
+
+
+# ------------------------------------------------------------------ trophic RK4
+def _rhs(p, ns):
+    A, H, E, b = np.asarray(p.A), np.asarray(p.H), np.asarray(p.E), p.b
+
+    def f(_, y):
+        g01 = A[0, 1] * y[0] / (1 + A[0, 1] * H[0, 1] * y[0])
+        out = [-b * y[0] ** 2 - g01 * y[1], E[1] * g01 * y[1]]
+        if ns == 3:
+            g12 = A[1, 2] * y[1] / (1 + A[1, 2] * H[1, 2] * y[1])
+            out[1] -= g12 * y[2]
+            out.append(E[2] * g12 * y[2])
+        return out
+    return f
+
+
+@pytest.mark.parametrize("ns", [2, 3])
+def test_trophic_rk4_against_the_ode(oracle, ns):
+    """One RK4 step of the Holling type II chain equals the ODE flow to O(dt^5)
+    (dt = 0.01: a wrong term shows up at 1e-4, the RK4 error stays below 1e-10)."""
+    p = params(dt=0.01)
+    po = oracle.dpc_params(p)
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        y = rng.uniform(0.1, 2.0, ns)
+        got = oracle.dpc_rk4(po, ns, y)
+        ref = solve_ivp(_rhs(p, ns), (0, p.dt), y, rtol=1e-12, atol=1e-14).y[:, -1]
+        assert np.max(np.abs(got - ref)) < 1e-10
+
+
+@pytest.mark.parametrize("ns", [2, 3])
+def test_trophic_rk4_conserves_the_linear_invariant(oracle, ns):
+    """b = 0 and H = 0 (mass action): E1 E2 rho0 + E2 rho1 + rho2 is conserved, and
+    RK4 keeps linear invariants exactly up to rounding."""
+    p = params(b=0.0, H=np.zeros((3, 3)))
+    po = oracle.dpc_params(p)
+    w = np.array([p.E[1], 1.0]) if ns == 2 else np.array([p.E[1] * p.E[2], p.E[2], 1.0])
+    y = np.array([1.3, 0.7, 0.4])[:ns]
+    got = oracle.dpc_rk4(po, ns, y)
+    assert abs(w @ got - w @ y) < 1e-14 and not np.allclose(got, y)
+
+
+def test_vegetation_alone_is_the_dp_step(oracle):
+    """Without grazing (A = 0) the vegetation field is the single-species DP step
+    with the RK4 scheme -- the same Philox stream -- bit for bit, whatever the
+    consumers do."""
+    L, steps = 24, 12
+    p = params(A=np.zeros((3, 3)))
+    rng = np.random.default_rng(4)
+    f = np.concatenate([rng.uniform(0, 1.0, (1, L, L)), rng.uniform(0, 0.8, (1, L, L)), np.zeros((1, L, L))])
+    got = oracle.dpc_run(oracle.dpc_params(p), 2, f, L, 9, 3, 5, steps)
+    dp = oracle.dp_params(a=p.a, b=p.b, D=p.D[0], sigma=p.sigma[0], dt=p.dt, dx=p.dx, scheme=1)
+    ref = oracle.dp_run(dp, f[0], L, 9, 3, 5, steps)
+    assert np.array_equal(got["fields"][0], ref["fields"][0])
+    assert np.any(got["fields"][1] != f[1])
+
+
+# ------------------------------------------------------------------ perception gamma
+def _gamma_brute(p, ns, old, L):
+    """calc_gamma_2Sp_NonRefugia / calc_gamma_3Sp_NonRefugia written directly
+    from the reference's statements, site by site, with numpy means."""
+    eps = 1e-12
+    rmax = max(p.pR[:ns]) / p.dx
+    R = int(rmax)
+    # the order is pinned above against the reference's own program
+    from oracle import oracle as o
+    disk = o.perception_offsets(R)
+    mean = [old[s].mean() for s in range(ns)]
+    nfac = max(1 - np.count_nonzero(old[0] > 0) / (L * L), 0.35)
+    new_len = int(nfac * nfac * len(disk))
+    inv = [1 / m if m >= eps else 1 / eps for m in mean]
+    out = np.zeros((ns, L, L))
+    for s in range(1, ns):
+        fr = p.pR[s] / (p.dx * rmax)
+        eff = int(fr * fr * new_len)
+        if ns == 3 and eff < 1 and fr > 0:
+            eff = 1
+        for i in range(L):
+            for j in range(L):
+                g = old[ns + s - 1, i, j] if ns == 2 else 0.0
+                if mean[s] < eps or mean[s - 1] < eps:
+                    out[s, i, j] = 0.0
+                    continue
+                nb = [((i + dx) % L, (j + dy) % L) for dx, dy in disk[:eff]]
+                if ns == 2:
+                    g = (g + sum(old[0][a] for a in nb)) * inv[0] / eff
+                elif s == 1:
+                    if mean[2] < eps:
+                        g = sum(old[0][a] for a in nb) * inv[0] / eff
+                    else:
+                        g = sum(old[0][a] * (1 - old[2][a] * inv[2]) for a in nb) * inv[0] / eff
+                else:
+                    g = sum(old[1][a] for a in nb) / eff * inv[1]
+                out[s, i, j] = min(max(g, 0.0), 1.0)
+    return out
+
+
+@pytest.mark.parametrize("ns", [2, 3])
+def test_gamma_against_brute_force(oracle, ns):
+    """The perception fields after one step equal a direct site-by-site
+    evaluation of the reference's formulas (2 species: onto the previous gamma),
+    with 40% vegetation cover so the neighbourhood is truncated."""
+    L = 14
+    p = params(pR=(0.0, 0.35, 0.25))                       # R = 3, predator at fr = 5/7
+    rng = np.random.default_rng(5)
+    f = np.zeros((2 * ns - 1, L, L))
+    f[0] = rng.uniform(0, 2.0, (L, L)) * (rng.random((L, L)) < 0.4)
+    f[1] = rng.uniform(0, 0.5, (L, L))
+    if ns == 3:
+        f[2] = rng.uniform(0, 0.3, (L, L))
+    f[ns:] = rng.uniform(0, 0.2, (ns - 1, L, L))          # previous gamma
+    got = oracle.dpc_run(oracle.dpc_params(p), ns, f, L, 2, 0, 0, 1)["fields"]
+    want = _gamma_brute(p, ns, f, L)
+    for s in range(1, ns):
+        assert np.allclose(got[ns + s - 1], want[s], rtol=1e-12, atol=1e-15), s
+        assert 0.0 < got[ns + s - 1].mean() < 1.0
+
+
+def test_extinct_consumer_stays_extinct(oracle):
+    L = 16
+    p = params()
+    f = np.zeros((3, L, L))
+    f[0] = 1.0
+    f[2] = 0.3          # a stale perception value
+    out = oracle.dpc_run(oracle.dpc_params(p), 2, f, L, 1, 0, 0, 3)
+    assert np.all(out["fields"][1] == 0.0) and np.all(out["fields"][2] == 0.0)
+    assert np.all(out["rows"][:, 2] == 0.0)
+
+
+# ------------------------------------------------------------------ consumer Dornic
+def _cir(alpha, rho, beta, dt, sigma):
+    e = math.exp(beta * dt)
+    mean = rho * e + alpha / beta * (e - 1)
+    var = sigma ** 2 / beta * (rho * e * (e - 1) + alpha / (2 * beta) * (e - 1) ** 2)
+    return mean, var
+
+
+def test_grazer_with_full_perception_is_a_dornic_cir_step(oracle):
+    """Uniform vegetation 1.0 makes gamma exactly 1 (the sum of eff ones over eff):
+    no advection, alpha = 4 D1/dx^2 rho and beta = -M1 - 4 D1/dx^2.  With A = 0 the
+    RK4 leaves the grazer alone, so its one-step moments are the CIR transition's."""
+    L, r = 256, 0.6
+    p = params(A=np.zeros((3, 3)))
+    f = np.zeros((3, L, L))
+    f[0], f[1] = 1.0, r
+    out = oracle.dpc_run(oracle.dpc_params(p), 2, f, L, 11, 0, 0, 1)["fields"]
+    assert np.all(out[2] == 1.0)
+    Dc = p.D[1] / p.dx ** 2
+    mean, var = _cir(4 * Dc * r, r, -p.M[1] - 4 * Dc, p.dt, p.sigma[1])
+    x = out[1].ravel()
+    se = math.sqrt(var / x.size)
+    assert abs(x.mean() - mean) < 5 * se, (x.mean() - mean) / se
+    assert abs(x.var() - var) < 0.05 * var
+
+
+def test_pure_advection_conserves_uniform_mass(oracle):
+    """No vegetation makes gamma 0: pure upwind advection, alpha = (|vx| + |vy|) rho / dx
+    and beta = -M - (|vx| + |vy|)/dx.  With M = 0 and uniform rho the transition mean
+    is rho in every direction."""
+    L, r = 256, 0.5
+    p = params(M=(0.0, 0.0, 0.0), A=np.zeros((3, 3)), v=(0.0, 0.4, 0.3))
+    f = np.zeros((3, L, L))
+    f[1] = r
+    out = oracle.dpc_run(oracle.dpc_params(p), 2, f, L, 12, 0, 0, 1)["fields"]
+    assert np.all(out[2] == 0.0)
+    x = out[1].ravel()
+    se = x.std() / math.sqrt(x.size)
+    assert abs(x.mean() - r) < 5 * se, (x.mean() - r) / se
+
+
+def test_upwind_neighbour_and_direction_stream(oracle):
+    """One occupied grazer cell and no vegetation (gamma 0): a neighbour becomes
+    occupied exactly when its own drawn direction points away from the source --
+    vx > 0 takes from the left (column - 1), vy > 0 from above (row - 1).  The
+    direction is word 0 of Philox(seed, inst; cell, step, block 1 << 24),
+    evaluated here with numpy's cos and sin."""
+    L, c0 = 16, (7, 9)
+    p = params(A=np.zeros((3, 3)))
+    hits = {"right": 0, "left": 0, "down": 0, "up": 0}
+    for seed in range(1, 41):
+        f = np.zeros((3, L, L))
+        f[1][c0] = 2.0
+        out = oracle.dpc_run(oracle.dpc_params(p), 2, f, L, seed, 4, 0, 1)["fields"][1]
+        occupied = {tuple(x) for x in np.argwhere(out > 0)}
+        assert occupied <= {c0, (7, 10), (7, 8), (8, 9), (6, 9)}
+
+        def direction(cell):
+            w = oracle.philox([cell[0] * L + cell[1], 0, 0, 1 << 24], [seed, 4])[0]
+            ang = 2 * math.pi * ((int(w) + 0.5) * 2.0 ** -32)
+            return math.cos(ang), math.sin(ang)
+        for name, cell, want in [("right", (7, 10), lambda vx, vy: vx > 0), ("left", (7, 8), lambda vx, vy: vx < 0),
+                                 ("down", (8, 9), lambda vx, vy: vy > 0), ("up", (6, 9), lambda vx, vy: vy < 0)]:
+            vx, vy = direction(cell)
+            assert (cell in occupied) == want(vx, vy), (seed, name, vx, vy)
+            hits[name] += cell in occupied
+    assert min(hits.values()) > 5

@@ -21,7 +21,11 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libtm.so")
 
-MODEL_DP, MODEL_RIETKERK = 1, 3
+MODEL_DP, MODEL_DP2, MODEL_RIETKERK, MODEL_DP3 = 1, 2, 3, 4
+# fields per instance; the consumer models hold the densities, then the perception fields
+NFIELDS = {MODEL_DP: 1, MODEL_DP2: 3, MODEL_RIETKERK: 3, MODEL_DP3: 5}
+NSPECIES = {MODEL_DP: 1, MODEL_DP2: 2, MODEL_RIETKERK: 3, MODEL_DP3: 3}
+DPC_MAX_RADIUS = 12
 F64, F32 = 0, 1
 SAMPLER_EXACT, SAMPLER_COMPAT = 0, 1
 DP_EULER, DP_RK4, DP_EXACT = 0, 1, 2
@@ -57,6 +61,15 @@ class _RK(C.Structure):
                 ("D", C.c_double * 3), ("K", C.c_double * 3), ("sigma", C.c_double),
                 ("dt", C.c_double), ("dx", C.c_double), ("flush_eps", C.c_double),
                 ("sampler", C.c_int32), ("reserved", C.c_int32)]
+
+
+class _DPC(C.Structure):
+    _fields_ = [("a", C.c_double), ("b", C.c_double), ("D", C.c_double * 3), ("sigma", C.c_double * 3),
+                ("v", C.c_double * 3), ("M", C.c_double * 3), ("E", C.c_double * 3),
+                ("A", (C.c_double * 3) * 3), ("H", (C.c_double * 3) * 3), ("pR", C.c_double * 3),
+                ("dt", C.c_double), ("dx", C.c_double), ("a_end", C.c_double), ("ramp_steps", C.c_int64),
+                ("flush_eps", C.c_double), ("flush_neg", C.c_double), ("sampler", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class _Inst(C.Structure):
@@ -110,6 +123,66 @@ class RietkerkParams:
         return _RK(self.R, self.c, self.gmax, self.alpha, self.rW, self.W0, self.d,
                    (C.c_double * 3)(*self.D), (C.c_double * 3)(*self.K), self.sigma, self.dt,
                    self.dx, self.flush_eps, self.sampler, 0)
+
+
+def _mat3(m):
+    m = np.zeros((3, 3)) if m is None else np.asarray(m, np.float64)
+    return ((C.c_double * 3) * 3)(*[(C.c_double * 3)(*row) for row in m])
+
+
+def _vec3(v):
+    v = list(v) + [0.0] * (3 - len(v))
+    return (C.c_double * 3)(*v)
+
+
+@dataclass
+class DPCParams:
+    """Vegetation + consumers: dP_Dornic_2D_MultiSp(..., a, b, c, D[Sp], v[SpB], sigma[], ...,
+    A[SpB][SpB], H[SpB][SpB], E[SpB], M[SpB], pR[], dtV[SpB], ..., dt, dx, ...)
+    (Percolation_FD/MultiSPDP.h:344-346) for MODEL_DP2 (vegetation, grazer) and MODEL_DP3
+    (and predator).  Defaults: the grazer and predator values of
+    Percolation_FD/order_2stocDP.cpp:29-67 with dx = 0.1 and a perception radius
+    of 5 sites.  b is the vegetation self-limitation of the trophic RK4 (the
+    reference passes a there: b = a reproduces its runs)."""
+    a: float = 0.03
+    b: float = 1e-6
+    D: tuple = (0.00025 / 24.0, 0.0298, 0.05221)
+    sigma: tuple = ((0.00025 / 24.0) ** 0.5 / 0.1, 1.0, 3.0)
+    v: tuple = (0.0, 0.45427, 0.40587)
+    M: tuple = (0.0, 0.061609 * 20.0 ** -0.25 / 8760.0, 0.01)
+    E: tuple = (1.0, 0.45, 0.3)
+    A: tuple | None = None
+    H: tuple | None = None
+    pR: tuple = (0.0, 0.521425, 0.521425)
+    dt: float = 0.1
+    dx: float = 0.1
+    a_end: float | None = None
+    ramp_steps: int = 0
+    flush_eps: float = 1e-12
+    flush_neg: float = -1e-80
+    sampler: int = SAMPLER_EXACT
+
+    def __post_init__(self):
+        if self.A is None:
+            aij = 3.6 * 10.0 ** -6.08 * 20.0 ** -0.37
+            self.A = ((0.0, aij, 0.0), (aij, 0.0, 0.05), (0.0, 0.05, 0.0))
+        if self.H is None:
+            self.H = ((0.0, 1.0, 0.0), (1.0, 0.0, 1.0), (0.0, 1.0, 0.0))
+
+    def _c(self) -> _DPC:
+        return _DPC(self.a, self.b, _vec3(self.D), _vec3(self.sigma), _vec3(self.v), _vec3(self.M),
+                    _vec3(self.E), _mat3(self.A), _mat3(self.H), _vec3(self.pR), self.dt, self.dx,
+                    self.a if self.a_end is None else self.a_end, self.ramp_steps, self.flush_eps,
+                    self.flush_neg, self.sampler, 0)
+
+
+def perception_offsets(R: int) -> np.ndarray:
+    """Offsets (row, column) within radius R in computeNeighboringSitesCentral's order."""
+    buf = np.zeros(2 * (2 * R + 1) ** 2, np.int32)
+    n = lib().tm_perception_offsets(int(R), buf.ctypes.data_as(C.POINTER(C.c_int32)), buf.size // 2)
+    if n < 0:
+        raise ValueError(f"bad radius {R}")
+    return buf[: 2 * n].reshape(n, 2)
 
 
 @dataclass
@@ -174,6 +247,7 @@ def lib() -> C.CDLL:
         "tm_load_snapshot": ([vp, C.c_int, C.c_char_p, C.POINTER(i64)], C.c_int),
         "tm_snapshot_to_frame_csv": ([C.c_char_p, C.c_char_p, C.c_double, C.c_char_p, C.c_int], C.c_int),
         "tm_read_frame_csv": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_int, dp], C.c_int),
+        "tm_perception_offsets": ([C.c_int, C.POINTER(i32), C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -350,11 +424,15 @@ class Context:
 
     @property
     def nfields(self) -> int:
-        return int(self.model)
+        return NFIELDS[self.model]
+
+    @property
+    def nspecies(self) -> int:
+        return NSPECIES[self.model]
 
     def configure(self, model: int, precision: int, params, instances):
         plist = params if isinstance(params, (list, tuple)) else [params]
-        ctype = _DP if model == MODEL_DP else _RK
+        ctype = _DP if model == MODEL_DP else (_DPC if model in (MODEL_DP2, MODEL_DP3) else _RK)
         parr = (ctype * len(plist))(*[p._c() for p in plist])
         iarr = (_Inst * len(instances))(*[_Inst(i.L, i.params_index, i.seed, i.instance_id, i.step0)
                                           for i in instances])

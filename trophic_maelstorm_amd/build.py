@@ -28,6 +28,7 @@ UNITS = {
     "tm_kernels_f32.hip": ["-ffp-contract=fast"],
     "tm_ensemble.hip": ["-ffp-contract=off"],
     "tm_frames.hip": ["-ffp-contract=off"],
+    "tm_consumers.hip": ["-ffp-contract=off"],
     "tm_api.cpp": ["-ffp-contract=off"],
     "tm_io.cpp": ["-ffp-contract=off"],
 }

@@ -105,6 +105,11 @@ struct tm_ctx {
     // initial-frame launch table: instance workgroup starts, per-point values
     void* d_frame_tab = nullptr;
     int64_t frame_tab_cap = 0;
+
+    // consumer models: DPCWork, per-instance tables, chunk sums, perception offsets
+    void* d_dpc = nullptr;
+    int dpc_nchunks = 0;
+    int dpc_halo = 1;
 };
 
 namespace {
@@ -137,8 +142,59 @@ const Variant* variant_table(int model, int prec, int* n)
 {
     if (model == TM_MODEL_DP) return prec == TM_F64 ? dp_variants_f64(n) : dp_variants_f32(n);
     if (model == TM_MODEL_RIETKERK) return prec == TM_F64 ? rk_variants_f64(n) : rk_variants_f32(n);
+    if (model == TM_MODEL_DP2 && prec == TM_F64) return dpc2_variants_f64(n);
+    if (model == TM_MODEL_DP3 && prec == TM_F64) return dpc3_variants_f64(n);
     *n = 0;
     return nullptr;
+}
+
+bool is_consumer_model(int model) { return model == TM_MODEL_DP2 || model == TM_MODEL_DP3; }
+
+// Fields per instance: the densities, then the consumers' perception fields.
+int nfields_of(int model)
+{
+    switch (model) {
+    case TM_MODEL_DP: return 1;
+    case TM_MODEL_DP2: return 3;
+    case TM_MODEL_DP3: return 5;
+    default: return 3;
+    }
+}
+
+// Density fields (species) per instance: what a FRAME file holds.
+int species_of(int model)
+{
+    switch (model) {
+    case TM_MODEL_DP2: return 2;
+    case TM_MODEL_DP3: return 3;
+    default: return nfields_of(model);
+    }
+}
+
+// computeNeighboringSitesCentral (multiSPDP.cpp:1442-1461): the offsets (row,
+// column) of [-R, R]^2 within radius R, generated row-major and sorted by
+// std::sort on squared distance alone.  The sort is not stable; the order of
+// equal distances is the one that call makes of this input sequence, which
+// the reference's truncated perception sums depend on.
+std::vector<std::pair<int, int>> perception_offsets(int R)
+{
+    std::vector<std::pair<int, int>> v;
+    for (int x = -R; x <= R; ++x)
+        for (int y = -R; y <= R; ++y)
+            if (x * x + y * y <= R * R) v.emplace_back(x, y);
+    std::sort(v.begin(), v.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+        return (a.first * a.first + a.second * a.second) < (b.first * b.first + b.second * b.second);
+    });
+    return v;
+}
+
+// r_max_effective = max pR / dx (multiSPDP.cpp:2429) of one parameter point.
+double perception_rmax(const tm_dpc_params& p, int ns)
+{
+    double m = p.pR[0];
+    for (int s = 1; s < ns; ++s)
+        if (p.pR[s] > m) m = p.pR[s];
+    return m / p.dx;
 }
 
 void drop_graphs(tm_ctx* c)
@@ -178,6 +234,9 @@ void release_config(tm_ctx* c)
     free_dev(c->d_stats);
     free_dev(c->d_frame_tab);
     free_dev(c->d_live);
+    free_dev(c->d_dpc);
+    c->dpc_nchunks = 0;
+    c->dpc_halo = 1;
     invalidate_live(c);
     c->series_cap = c->groups_cap = c->stats_cap = c->frame_tab_cap = 0;
     c->series_steps_cap = c->n_series = 0;
@@ -273,6 +332,9 @@ StepArgs step_args(tm_ctx* c, int parity, int k)
     a.k = k;
     a.n_inst = (int32_t)c->inst.size();
     a.xcd = c->xcd;
+    a.work = c->d_dpc;
+    a.nchunks = c->dpc_nchunks;
+    a.halo = c->dpc_halo;
     return a;
 }
 
@@ -577,6 +639,71 @@ tm_status fault_status(tm_ctx* c)
 }
 
 // Zero state with no fault recorded.
+// Consumer models: perception tables per parameter point, statistics chunks
+// and per-instance tables, in one device allocation behind a DPCWork header.
+tm_status setup_dpc(tm_ctx* c)
+{
+    const int ns = species_of(c->model);
+    const tm_dpc_params* P = (const tm_dpc_params*)c->params.data();
+    std::vector<int2> disk;
+    std::vector<int> p_off((size_t)c->n_params), p_len((size_t)c->n_params), p_R((size_t)c->n_params);
+    int halo = 1;
+    for (int g = 0; g < c->n_params; ++g) {
+        const double rmax = perception_rmax(P[g], ns);
+        const int R = (int)rmax;
+        const std::vector<std::pair<int, int>> o = perception_offsets(R);
+        p_off[(size_t)g] = (int)disk.size();
+        p_len[(size_t)g] = (int)o.size();
+        p_R[(size_t)g] = R;
+        for (const auto& q : o) disk.push_back(make_int2(q.first, q.second));
+        halo = std::max(halo, R);
+    }
+    std::vector<DPCInstInfo> info(c->inst.size());
+    std::vector<int2> chunks;
+    for (size_t i = 0; i < c->inst.size(); ++i) {
+        const tm_instance_desc& d = c->inst[i];
+        const tm_dpc_params& p = P[d.params_index];
+        const double rmax = perception_rmax(p, ns);
+        DPCInstInfo& in = info[i];
+        in = DPCInstInfo{};
+        in.R = p_R[(size_t)d.params_index];
+        in.disk_off = p_off[(size_t)d.params_index];
+        in.disk_len = p_len[(size_t)d.params_index];
+        for (int s = 0; s < ns; ++s) in.fr[s] = p.pR[s] / (p.dx * rmax);   // multiSPDP.cpp:2434-2435
+        const int64_t cells = (int64_t)d.L * d.L;
+        const int nch = (int)((cells + kDpcChunkCells - 1) / kDpcChunkCells);
+        in.chunk0 = (int)chunks.size();
+        in.nchunks = nch;
+        for (int k = 0; k < nch; ++k) chunks.push_back(make_int2((int)i, k));
+    }
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_info = up(sizeof(DPCWork));
+    const size_t o_st = o_info + up(sizeof(DPCInstInfo) * info.size());
+    const size_t o_chunk = o_st + up(sizeof(DPCStepState) * info.size());
+    const size_t o_chunks = o_chunk + up(sizeof(double) * 4 * chunks.size());
+    const size_t o_disk = o_chunks + up(sizeof(int2) * chunks.size());
+    const size_t total = o_disk + up(sizeof(int2) * disk.size());
+    free_dev(c->d_dpc);
+    if (hipMalloc(&c->d_dpc, total) != hipSuccess) return fail(c, TM_ERR_NOMEM, "consumer tables allocation failed");
+    char* base = (char*)c->d_dpc;
+    DPCWork w{};
+    w.params = (const tm_dpc_params*)c->d_params;
+    w.info = (const DPCInstInfo*)(base + o_info);
+    w.st = (DPCStepState*)(base + o_st);
+    w.chunk = (double*)(base + o_chunk);
+    w.chunks = (const int2*)(base + o_chunks);
+    w.disk = (const int2*)(base + o_disk);
+    std::vector<char> host(total, 0);
+    std::memcpy(host.data(), &w, sizeof(w));
+    std::memcpy(host.data() + o_info, info.data(), sizeof(DPCInstInfo) * info.size());
+    std::memcpy(host.data() + o_chunks, chunks.data(), sizeof(int2) * chunks.size());
+    std::memcpy(host.data() + o_disk, disk.data(), sizeof(int2) * disk.size());
+    HIPCHK(c, hipMemcpy(c->d_dpc, host.data(), total, hipMemcpyHostToDevice));
+    c->dpc_nchunks = (int)chunks.size();
+    c->dpc_halo = halo;
+    return TM_OK;
+}
+
 tm_status reset_state(tm_ctx* c)
 {
     DevState st{};
@@ -666,9 +793,31 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
                        const tm_instance_desc* instances, int n_inst)
 {
     if (!c) return TM_ERR_ARG;
-    if (model != TM_MODEL_DP && model != TM_MODEL_RIETKERK) return fail(c, TM_ERR_ARG, "unknown model");
+    if (model != TM_MODEL_DP && model != TM_MODEL_RIETKERK && !is_consumer_model(model))
+        return fail(c, TM_ERR_ARG, "unknown model");
     if (precision != TM_F64 && precision != TM_F32) return fail(c, TM_ERR_ARG, "unknown precision");
+    if (is_consumer_model(model) && precision != TM_F64)
+        return fail(c, TM_ERR_ARG, "the consumer models run in TM_F64 only");
     if (!params || n_params <= 0 || !instances || n_inst <= 0) return fail(c, TM_ERR_ARG, "empty batch");
+    if (is_consumer_model(model)) {
+        const tm_dpc_params* P = (const tm_dpc_params*)params;
+        const int ns = species_of(model);
+        for (int g = 0; g < n_params; ++g) {
+            if (!(P[g].dx > 0.0) || !(P[g].dt > 0.0)) return fail(c, TM_ERR_ARG, "dt and dx must be positive");
+            for (int s = 0; s < ns; ++s)
+                if (!(P[g].pR[s] >= 0.0)) return fail(c, TM_ERR_ARG, "perception radii must be >= 0");
+            const double rmax = perception_rmax(P[g], ns);
+            if (!(rmax > 0.0)) return fail(c, TM_ERR_ARG, "the largest perception radius must be positive");
+            if (!(rmax < TM_DPC_MAX_RADIUS + 1.0))
+                return fail(c, TM_ERR_ARG, "perception radius above TM_DPC_MAX_RADIUS sites");
+        }
+        for (int i = 0; i < n_inst; ++i) {
+            const tm_instance_desc& d = instances[i];
+            if (d.params_index < 0 || d.params_index >= n_params) continue;   // reported below
+            if ((int)perception_rmax(P[d.params_index], ns) >= d.L)
+                return fail(c, TM_ERR_ARG, "perception radius must be smaller than L");
+        }
+    }
     for (int i = 0; i < n_inst; ++i) {
         const tm_instance_desc& d = instances[i];
         if (d.L < 1 || d.L > 65536) return fail(c, TM_ERR_ARG, "instance L out of range [1, 65536]");
@@ -681,9 +830,10 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
     release_config(c);
     c->model = model;
     c->prec = precision;
-    c->nf = model;   // enum value = number of fields
+    c->nf = nfields_of(model);
     c->esize = precision == TM_F64 ? sizeof(double) : sizeof(float);
-    c->param_size = model == TM_MODEL_DP ? sizeof(tm_dp_params) : sizeof(tm_rietkerk_params);
+    c->param_size = model == TM_MODEL_DP ? sizeof(tm_dp_params)
+                    : is_consumer_model(model) ? sizeof(tm_dpc_params) : sizeof(tm_rietkerk_params);
     c->params.assign((const unsigned char*)params, (const unsigned char*)params + c->param_size * n_params);
     c->n_params = n_params;
     c->inst.assign(instances, instances + n_inst);
@@ -712,6 +862,7 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
     c->counter = 0;
     tm_status s = build_tiles(c);
     if (s != TM_OK) return s;
+    if (is_consumer_model(model) && (s = setup_dpc(c)) != TM_OK) return s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return TM_OK;
 }
@@ -797,7 +948,7 @@ tm_status tm_init_uniform(tm_ctx* c, const double* lo, const double* hi)
             v.b[f] = hi[f];
         }
     }
-    const double sd_none[3] = {1.0, 1.0, 1.0};
+    const double sd_none[kMaxFields] = {1.0, 1.0, 1.0, 1.0, 1.0};
     return enqueue_frames(c, 2, 0, sd_none, vals);
 }
 
@@ -812,7 +963,8 @@ tm_status tm_write_frame_csv(tm_ctx* c, int i, const char* path, double a, const
     const int L = c->inst[(size_t)i].L;
     std::vector<double> host((size_t)c->nf * L * L);
     if ((s = tm_download_fields(c, i, host.data())) != TM_OK) return s;
-    if (tm_format_frame_csv(path, host.data(), c->nf, L, a, header, digits) != 0)
+    // a FRAME file holds the densities (multiSPDP.cpp:1596-1645 writes Rho_dt)
+    if (tm_format_frame_csv(path, host.data(), species_of(c->model), L, a, header, digits) != 0)
         return fail(c, TM_ERR_ARG, std::string("tm_write_frame_csv: cannot write ") + path);
     return TM_OK;
 }
@@ -881,7 +1033,7 @@ tm_status tm_init_speckle(tm_ctx* c, const double* perc, const double* high, con
             v.b[f] = low[(size_t)g * c->nf + f];
         }
     }
-    const double sd_none[3] = {1.0, 1.0, 1.0};
+    const double sd_none[kMaxFields] = {1.0, 1.0, 1.0, 1.0, 1.0};
     return enqueue_frames(c, 0, 0, sd_none, vals);
 }
 
@@ -892,7 +1044,7 @@ tm_status tm_init_gauss(tm_ctx* c, int n_gauss, const double* sd_div, int n_sd, 
     if (!amp || n_gauss < 0 || n_gauss > c->nf) return fail(c, TM_ERR_ARG, "tm_init_gauss: bad n_gauss or amp");
     if (n_gauss > 0 && (!sd_div || n_sd < 1)) return fail(c, TM_ERR_ARG, "tm_init_gauss: need sd_div[n_sd >= 1]");
     // Missing widths repeat the last one given (rietkerk_bjork_FKE.cpp:1118-1127).
-    double sd[3] = {1.0, 1.0, 1.0};
+    double sd[kMaxFields] = {1.0, 1.0, 1.0, 1.0, 1.0};
     for (int f = 0; f < n_gauss; ++f) {
         sd[f] = sd_div[std::min(f, n_sd - 1)];
         if (!(sd[f] > 0.0)) return fail(c, TM_ERR_ARG, "tm_init_gauss: sd_div must be > 0");
@@ -1188,6 +1340,18 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
 
 // ---- Sweep measurement and replicate statistics ----------------------------
 
+int tm_perception_offsets(int R, int32_t* out, int cap)
+{
+    if (R < 0 || (R > 0 && !out)) return -1;
+    const std::vector<std::pair<int, int>> v = perception_offsets(R);
+    if ((int)v.size() > cap) return -1;
+    for (size_t k = 0; k < v.size(); ++k) {
+        out[2 * k] = v[k].first;
+        out[2 * k + 1] = v[k].second;
+    }
+    return (int)v.size();
+}
+
 int tm_log10_time_bins(double t_max, double dt, double* out, int cap)
 {
     // 10^p for p = 0, 0.04, 0.08, ... (p accumulated) up to floor(10 log10 t_max)/10,
@@ -1330,6 +1494,7 @@ namespace tmk {
 int default_variant(int model, int precision, int64_t total_cells)
 {
     const char* want;
+    if (is_consumer_model(model)) return 0;
     if (model == TM_MODEL_DP) {
         // adaptive four-step launches: single steps while the field is live,
         // four steps per launch once it is mostly absorbing

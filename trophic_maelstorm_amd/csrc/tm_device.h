@@ -164,6 +164,41 @@ TM_HD double dlog(double x)
 
 TM_HD double dsqrt(double x) { return __builtin_sqrt(x); }
 
+// sin and cos of 2 pi u, for a consumer's advection direction: t = 4u (exact),
+// q = nearest integer to t, x = (t - q) pi/2 in [-pi/4, pi/4], Taylor
+// polynomials of sin x to x^17 and cos x to x^18 in x^2 (the next terms are
+// below 1e-19), then the quarter-turn rotation by q.
+TM_HD void dsincos2pi(double u, double& sn, double& cs)
+{
+#pragma clang fp contract(off)
+    const double t = 4.0 * u;
+    const double q = __builtin_floor(t + 0.5);
+    const double x = (t - q) * 0x1.921fb54442d18p+0;
+    const double z = x * x;
+    double s = 0x1.952c77030ad4ap-49;
+    s = s * z + -0x1.ae7f3e733b81fp-41;
+    s = s * z + 0x1.6124613a86d09p-33;
+    s = s * z + -0x1.ae64567f544e4p-26;
+    s = s * z + 0x1.71de3a556c734p-19;
+    s = s * z + -0x1.a01a01a01a01ap-13;
+    s = s * z + 0x1.1111111111111p-7;
+    s = s * z + -0x1.5555555555555p-3;
+    s = x + x * (z * s);
+    double c = -0x1.6827863b97d97p-53;
+    c = c * z + 0x1.ae7f3e733b81fp-45;
+    c = c * z + -0x1.93974a8c07c9dp-37;
+    c = c * z + 0x1.1eed8eff8d898p-29;
+    c = c * z + -0x1.27e4fb7789f5cp-22;
+    c = c * z + 0x1.a01a01a01a01ap-16;
+    c = c * z + -0x1.6c16c16c16c17p-10;
+    c = c * z + 0x1.5555555555555p-5;
+    c = c * z + -0x1.0000000000000p-1;
+    c = 1.0 + z * c;
+    const int k = ((int)q) & 3;
+    sn = k == 0 ? s : (k == 1 ? c : (k == 2 ? -s : -c));
+    cs = k == 0 ? c : (k == 1 ? -s : (k == 2 ? -c : s));
+}
+
 // Acklam's rational approximation of the standard normal quantile (|rel err|
 // < 1.15e-9); one uniform in, one deviate out, no rejection.
 TM_HD double norm_icdf_d(double p)

@@ -13,8 +13,8 @@
 //   7L/8, 7L/8] in integer division (:1110-1115); sd_f = L / sd_div_f.
 // Values are formed in FP64, in the reference's operation order with this
 // library's own exp, and stored in the field precision.
-// * Uniform frames (tm_init_uniform): word f of Philox(seed, instance_id; cell,
-//   0xFFFFFFFF, 0xFFFFFFFF, 0) gives field f = lo_f + (hi_f - lo_f) u.
+// * Uniform frames (tm_init_uniform): word f % 4 of Philox(seed, instance_id;
+//   cell, 0xFFFFFFFF, 0xFFFFFFFF, f / 4) gives field f = lo_f + (hi_f - lo_f) u.
 // Workgroup b covers kFrameCellsPerBlock consecutive cells of the instance k
 // with blk0[k] <= b < blk0[k+1] (its 256 threads stride through them, so every
 // store is a contiguous wave); the values of the instance's parameter point
@@ -28,7 +28,7 @@ namespace {
 template <typename T>
 __global__ __launch_bounds__(256) void init_frames_kernel(int mode, const DevInst* __restrict__ inst,
                                                           const int64_t* __restrict__ blk0, int n_inst, int nf,
-                                                          int n_gauss, double sd0, double sd1, double sd2,
+                                                          int n_gauss, FrameSd sd_div,
                                                           const FrameVals* __restrict__ vals,
                                                           T* __restrict__ fields)
 {
@@ -45,7 +45,6 @@ __global__ __launch_bounds__(256) void init_frames_kernel(int mode, const DevIns
     const int64_t N = (int64_t)L * L;
     T* f0 = fields + (int64_t)nf * d.cell_off;
     const FrameVals v = vals[d.pidx];
-    const double sd_div[3] = {sd0, sd1, sd2};
     const int64_t base = (b - blk0[lo]) * kFrameCellsPerBlock;
     for (int r = 0; r < kFrameCellsPerBlock / 256; ++r) {
         const int64_t i = base + r * 256 + threadIdx.x;
@@ -57,10 +56,13 @@ __global__ __launch_bounds__(256) void init_frames_kernel(int mode, const DevIns
                 const double u = ((double)w.v0 + 0.5) * 0x1p-32;
                 const bool high = u <= v.perc;
                 for (int f = 0; f < nf; ++f) f0[f * N + i] = (T)(high ? v.a[f] : v.b[f]);
-            } else {                    // uniform: word f gives field f in [a_f, b_f)
-                const uint32_t ws[3] = {w.v0, w.v1, w.v2};
+            } else {                    // uniform: word f % 4 of block f / 4 gives field f in [a_f, b_f)
+                Philox4 wb = w;
                 for (int f = 0; f < nf; ++f) {
-                    const double u = ((double)ws[f] + 0.5) * 0x1p-32;
+                    if (f > 0 && f % 4 == 0)
+                        wb = philox4x32_10((uint32_t)i, c1, 0xFFFFFFFFu, (uint32_t)(f / 4), d.seed, d.inst_id);
+                    const uint32_t wf = f % 4 == 0 ? wb.v0 : (f % 4 == 1 ? wb.v1 : (f % 4 == 2 ? wb.v2 : wb.v3));
+                    const double u = ((double)wf + 0.5) * 0x1p-32;
                     f0[f * N + i] = (T)(v.a[f] + (v.b[f] - v.a[f]) * u);
                 }
             }
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(256) void init_frames_kernel(int mode, const DevIns
             if (f < n_gauss) {
                 const double cx = (f % 4) % 3 == 0 ? (double)(L / 8) : (double)(7 * L / 8);
                 const double cy = (f % 4) < 2 ? (double)(L / 8) : (double)(7 * L / 8);
-                const double sd = L / sd_div[f];
+                const double sd = L / sd_div.v[f];
                 const double dp = p - cx, dq = q - cy;
                 x = v.a[f] * dexp(-(dp * dp + dq * dq) / (2 * sd * sd));
             }
@@ -90,12 +92,14 @@ void launch_init_frames(int prec, int mode, const DevInst* inst, const int64_t* 
                         hipStream_t s)
 {
     if (nblocks <= 0) return;
+    FrameSd sd;
+    for (int f = 0; f < kMaxFields; ++f) sd.v[f] = sd_div[f];
     if (prec == TM_F64)
         hipLaunchKernelGGL(init_frames_kernel<double>, dim3((unsigned)nblocks), dim3(256), 0, s, mode, inst, blk0,
-                           n_inst, nf, n_gauss, sd_div[0], sd_div[1], sd_div[2], vals, (double*)fields);
+                           n_inst, nf, n_gauss, sd, vals, (double*)fields);
     else
         hipLaunchKernelGGL(init_frames_kernel<float>, dim3((unsigned)nblocks), dim3(256), 0, s, mode, inst, blk0,
-                           n_inst, nf, n_gauss, sd_div[0], sd_div[1], sd_div[2], vals, (float*)fields);
+                           n_inst, nf, n_gauss, sd, vals, (float*)fields);
 }
 
 }  // namespace tmk

@@ -26,6 +26,8 @@
 
 namespace tmk {
 
+constexpr int kMaxFields = 5;   // fields per instance: 1 (DP), 3 (DP2, Rietkerk), 5 (DP3)
+
 struct DevInst {
     int32_t L;
     int32_t pidx;
@@ -75,6 +77,9 @@ struct StepArgs {
     int32_t n_inst;
     int32_t xcd;         // XCD-aware tile order
     int32_t ntiles_a;
+    const void* work;    // consumer models: DPCWork (device) of the batch
+    int32_t nchunks;     // consumer models: statistics chunks of the batch
+    int32_t halo;        // consumer models: LDS halo (largest perception radius, >= 1)
 };
 
 // Periodic index: v mod L in [0, L).
@@ -1162,6 +1167,34 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
 // prep_consts: table[k][i] = constants of instance i at step counter + k.
 // bump_counter: counter += n (one thread; runs after the block's rows fold).
 // ---------------------------------------------------------------------------
+// Vegetation + consumers (multiSPDP.cpp with SPB = 2, 3): the per-step
+// constants of the vegetation's Dornic step (its linear rate a ramped as in
+// the DP model); the consumers' constants depend on each cell's perception and
+// draw and are formed in the step kernel.
+struct DPCConst {
+    DornicConst k;
+    double eps, neg;
+    int compat;
+    int64_t step;
+};
+
+__device__ __forceinline__ DPCConst dpc_constants(const tm_dpc_params& p, int64_t step)
+{
+#pragma clang fp contract(off)
+    double a = p.a;
+    if (p.ramp_steps > 0) {
+        const int64_t m = step < p.ramp_steps ? step : p.ramp_steps;
+        a = p.a + (p.a_end - p.a) * ((double)m / (double)p.ramp_steps);
+    }
+    DPCConst c;
+    c.k = dornic_constants(a, p.D[0], p.sigma[0], p.dt, p.dx);
+    c.eps = p.flush_eps;
+    c.neg = p.flush_neg;
+    c.compat = p.sampler == TM_SAMPLER_COMPAT;
+    c.step = step;
+    return c;
+}
+
 template <int MODEL>
 __global__ __launch_bounds__(256) void prep_consts_kernel(const DevInst* inst, const void* params,
                                                           const DevState* state, int n_inst, int nsteps,
@@ -1174,6 +1207,8 @@ __global__ __launch_bounds__(256) void prep_consts_kernel(const DevInst* inst, c
     const int64_t step = di.step0 + state->counter + k;
     if (MODEL == TM_MODEL_DP)
         ((DPConst*)table)[idx] = dp_constants(((const tm_dp_params*)params)[di.pidx], step);
+    else if (MODEL == TM_MODEL_DP2)   // both consumer models
+        ((DPCConst*)table)[idx] = dpc_constants(((const tm_dpc_params*)params)[di.pidx], step);
     else
         ((RKConstD*)table)[idx] = rk_constants(((const tm_rietkerk_params*)params)[di.pidx], step);
 }
@@ -1245,7 +1280,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
 __global__ __launch_bounds__(256) void combine_rows_kernel(CombineArgs a)
 {
 #pragma clang fp contract(off)
-    __shared__ double s_s[3][256], s_e[3][256], s_n[3][256];
+    __shared__ double s_s[kMaxFields][256], s_e[kMaxFields][256], s_n[kMaxFields][256];
     const int i = blockIdx.x, j = blockIdx.y;
     const DevInst di = a.inst[i];
     int64_t slot = 0, row = 0;

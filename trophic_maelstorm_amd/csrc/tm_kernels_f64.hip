@@ -64,8 +64,11 @@ void launch_measure_f64(int nf, const DevInst* inst, const DevTile* tiles, int n
     if (nf == 1)
         hipLaunchKernelGGL((measure_kernel<double, 1>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
                            ty, (const double*)fields, partials);
-    else
+    else if (nf == 3)
         hipLaunchKernelGGL((measure_kernel<double, 3>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
+                           ty, (const double*)fields, partials);
+    else
+        hipLaunchKernelGGL((measure_kernel<double, 5>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
                            ty, (const double*)fields, partials);
 }
 
@@ -82,6 +85,9 @@ void launch_prep(int model, const DevInst* inst, const void* params, const DevSt
     if (model == TM_MODEL_DP)
         hipLaunchKernelGGL(prep_consts_kernel<TM_MODEL_DP>, dim3(grid), dim3(256), 0, s, inst, params, state,
                            n_inst, nsteps, table);
+    else if (model == TM_MODEL_DP2 || model == TM_MODEL_DP3)
+        hipLaunchKernelGGL(prep_consts_kernel<TM_MODEL_DP2>, dim3(grid), dim3(256), 0, s, inst, params, state,
+                           n_inst, nsteps, table);
     else
         hipLaunchKernelGGL(prep_consts_kernel<TM_MODEL_RIETKERK>, dim3(grid), dim3(256), 0, s, inst, params,
                            state, n_inst, nsteps, table);
@@ -92,6 +98,11 @@ void launch_bump(DevState* state, int n, hipStream_t s)
     hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(64), 0, s, state, n);
 }
 
-size_t const_size(int model) { return model == TM_MODEL_DP ? sizeof(DPConst) : sizeof(RKConstD); }
+size_t const_size(int model)
+{
+    if (model == TM_MODEL_DP) return sizeof(DPConst);
+    if (model == TM_MODEL_DP2 || model == TM_MODEL_DP3) return sizeof(DPCConst);
+    return sizeof(RKConstD);
+}
 
 }  // namespace tmk

@@ -29,6 +29,44 @@ const Variant* rk_variants_f64(int* n);
 const Variant* dp_variants_f32(int* n);
 const Variant* rk_variants_f32(int* n);
 
+// Vegetation + consumer models (tm_consumers.hip), FP64 only.
+const Variant* dpc2_variants_f64(int* n);
+const Variant* dpc3_variants_f64(int* n);
+
+// Consumer models: per-instance tables of the step (device), set at configure
+// (DPCInstInfo) or rewritten at the start of every step (DPCStepState).
+constexpr int kDpcChunkCells = 4096;   // cells per statistics chunk
+struct DPCInstInfo {
+    int32_t R;          // perception radius in sites, (int)(max pR / dx)
+    int32_t disk_off;   // first offset of its perception table in DPCWork::disk
+    int32_t disk_len;   // offsets within radius R
+    int32_t chunk0;     // first statistics chunk of the instance
+    int32_t nchunks;
+    int32_t pad;
+    double fr[3];       // pR[s] / (dx r_max): perception radius relative to the largest
+};
+struct DPCStepState {
+    double mean[3];     // spatial means of the densities at the step's start
+    double inv[3];      // 1 / mean, or 1 / eps below eps
+    int32_t eff[3];     // perception neighbours each species uses this step
+    int32_t pad;
+};
+struct DPCWork {
+    const tm_dpc_params* params;
+    const DPCInstInfo* info;
+    DPCStepState* st;
+    double* chunk;        // [nchunks][4]: density sums, vegetation occupied count
+    const int2* chunks;   // [nchunks]: (instance, chunk index within it)
+    const int2* disk;     // perception offsets (row, column) of every table
+};
+// LDS of the consumer step kernel: NS density images with the halo, plus the
+// perception offsets of the largest radius.
+inline size_t dpc_lds_bytes(int ns, int tx, int ty, int halo)
+{
+    return sizeof(double) * (size_t)ns * (tx + 2 * halo) * (ty + 2 * halo) +
+           sizeof(int) * (size_t)(2 * halo + 1) * (2 * halo + 1);
+}
+
 // Default variant index per (model, precision, batch size in cells).
 int default_variant(int model, int precision, int64_t total_cells);
 
@@ -50,7 +88,10 @@ size_t const_size(int model);
 constexpr int kFrameCellsPerBlock = 4096;
 struct FrameVals {
     double perc;
-    double a[3], b[3];
+    double a[kMaxFields], b[kMaxFields];
+};
+struct FrameSd {   // Gaussian widths L / v[f], per field (sd_div[kMaxFields] at launch)
+    double v[kMaxFields];
 };
 void launch_init_frames(int prec, int mode, const DevInst* inst, const int64_t* blk0, int n_inst, int64_t nblocks,
                         int nf, int n_gauss, const double* sd_div, const FrameVals* vals, void* fields,

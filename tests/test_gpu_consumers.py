@@ -54,7 +54,7 @@ def _run_both(tm, oracle, model, plist, insts, init, steps, variant=None, graph=
 def test_consumers_bit_exact_mixed_batch(tm, oracle, model):
     m = getattr(tm, "MODEL_" + model)
     ns = tm.NSPECIES[m]
-    plist = [params(), params(pR=(0.0, 0.52, 0.31), v=(0.0, 0.6, 0.2), a=0.3)]   # R = 3 and R = 5
+    plist = [params(), params(pR=(0.0, 0.52, 0.31), v=(0.0, 0.6, 0.2), a=0.3)]   # R = 2 and R = 5
     Ls = [16, 37, 24, 61]
     insts = [tm.Instance(L, k % 2, 7, 40 + k, 3 * k) for k, L in enumerate(Ls)]
     rng = np.random.default_rng(10)
@@ -117,8 +117,8 @@ def test_consumer_models_reject_bad_configurations(tm):
     with tm.Context() as ctx:
         with pytest.raises(tm.TMError):
             ctx.configure(tm.MODEL_DP2, tm.F32, _tm_params(tm, params()), [tm.Instance(16)])
-        with pytest.raises(tm.TMError):          # R = 3 is not below L = 3
-            ctx.configure(tm.MODEL_DP2, tm.F64, _tm_params(tm, params()), [tm.Instance(3)])
+        with pytest.raises(tm.TMError):          # R = (int)(0.3 / 0.1) = 2 is not below L = 2
+            ctx.configure(tm.MODEL_DP2, tm.F64, _tm_params(tm, params()), [tm.Instance(2)])
         with pytest.raises(tm.TMError):          # R = 13 sites
             ctx.configure(tm.MODEL_DP3, tm.F64, _tm_params(tm, params(pR=(0.0, 1.35, 0.2))), [tm.Instance(64)])
         with pytest.raises(tm.TMError):          # no perception at all

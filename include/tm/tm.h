@@ -367,7 +367,12 @@ tm_status tm_run_series(tm_ctx* ctx, const int64_t* steps, int n_meas, double* s
  * 1319-1369, multiSPDP.cpp:1080-1125; later replicates, in order).  A replicate
  * survives at a measurement when field 0 is occupied there and at every earlier
  * one.  Without TM_STATS_INTDIV_VARIANCE the variance is the unbiased sample
- * variance.  Parameter points with no instance get zero rows. */
+ * variance.  Parameter points with no instance get zero rows.
+ * TM_MODEL_DP2 / TM_MODEL_DP3: stats [n_params][n_meas][4 * species] = per
+ * species (vegetation, grazer, predator) those four statistics of that species,
+ * each over the replicates it still occupies (multiSPDP.cpp:3376-3391 first
+ * replicate, :1080-1123 later ones); with TM_STATS_STOP_AT_EXTINCTION a
+ * replicate's rows after the vegetation's first extinct measurement read as zero. */
 tm_status tm_replicate_stats(tm_ctx* ctx, int flags, double* stats);
 
 #ifdef __cplusplus

@@ -112,6 +112,9 @@ def lib():
         L.tmo_replicate_stats.argtypes = [_f64p, C.c_int, C.c_int, C.c_int,
                                           np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                           C.c_int, C.c_int, _f64p]
+        L.tmo_species_stats.argtypes = [_f64p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
+                                        C.c_int, C.c_int, _f64p]
         L.tmo_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tmo_fixed_sum.restype = C.c_double
         L.tmo_fixed_sum.argtypes = [_f64p, C.c_int64]
@@ -272,6 +275,16 @@ def dpc_run(p: DPCParams, ns: int, fields, L, seed, inst, step0, nsteps, rows=Tr
     st = lib().tmo_dpc_run(C.byref(p), ns, L, disk.ctypes.data, disk.shape[0], seed, inst, step0, nsteps, f,
                            r.ctypes.data if r is not None else None, C.byref(fc))
     return {"fields": f.reshape(nf, L, L), "rows": r, "status": st, "fault_index": fc.value}
+
+
+def species_stats(series, params_index, n_params, nsp, flags):
+    """series [n_inst, n_meas, 2 nf] -> [n_params, n_meas, 4 nsp] (multi-species form)."""
+    x = np.ascontiguousarray(series, np.float64)
+    n_inst, n_meas, w = x.shape
+    pidx = np.ascontiguousarray(params_index, np.int32)
+    out = np.zeros((n_params, n_meas, 4 * nsp), np.float64)
+    lib().tmo_species_stats(x.reshape(-1), n_inst, n_meas, w // 2, nsp, pidx, n_params, flags, out.reshape(-1))
+    return out
 
 
 def ref_perception_sites():

@@ -922,3 +922,69 @@ int tmo_dpc_run(const tmo_dpc_params* p, int ns, int L, const int32_t* disk, int
     free(old);
     return fault;
 }
+
+/* ---- multi-species replicate statistics -----------------------------------
+ * multiSPDP.cpp:3376-3391 (first replicate: per species mean, 0, surviving
+ * 1 or 0, active) and var_mean_incremental_surv_runs (MSP:1080-1123: per
+ * species, the recurrence over the measurements until that species' first
+ * empty one, where its loop breaks).  With flag 2 the replicate's rows after
+ * the vegetation's first empty measurement are zero (its time loop ended).
+ * out: [n_params][n_meas][4 nsp]. */
+void tmo_species_stats(const double* series, int n_inst, int n_meas, int nf, int nsp,
+                       const int32_t* params_index, int n_params, int flags, double* out)
+{
+    const int w = 2 * nf, nc = 4 * nsp;
+    const int intdiv = (flags & 1) != 0, stop = (flags & 2) != 0;
+    double* x = (double*)malloc(sizeof(double) * (size_t)(n_meas > 0 ? n_meas : 1) * w);
+    memset(out, 0, sizeof(double) * (size_t)n_params * n_meas * nc);
+    for (int g = 0; g < n_params; ++g) {
+        double* tab = out + (int64_t)g * n_meas * nc;
+        int j = 0;
+        for (int k = 0; k < n_inst; ++k) {
+            if (params_index[k] != g) continue;
+            int ended = 0;
+            for (int i = 0; i < n_meas; ++i) {
+                const double* src = series + ((int64_t)k * n_meas + i) * w;
+                for (int c = 0; c < w; ++c) x[(int64_t)i * w + c] = ended ? 0.0 : src[c];
+                if (stop && src[0] == 0.0) ended = 1;
+            }
+            for (int s = 0; s < nsp; ++s) {
+                if (j == 0) {                                     /* MSP:3380-3390 */
+                    for (int i = 0; i < n_meas; ++i) {
+                        double* row = tab + (int64_t)i * nc + 4 * s;
+                        const double* xi = x + (int64_t)i * w + 2 * s;
+                        row[0] = xi[1];
+                        row[1] = 0.0;
+                        row[3] = xi[0];
+                        if (xi[0] > 0) row[2] = 1;
+                        else if (xi[0] == 0) row[2] = 0;
+                    }
+                    continue;
+                }
+                for (int i = 0; i < n_meas; ++i) {                /* MSP:1087-1120 */
+                    double* row = tab + (int64_t)i * nc + 4 * s;
+                    const double* xi = x + (int64_t)i * w + 2 * s;
+                    const int r = row[2] + 1;
+                    if (xi[0] == 0.0) break;
+                    const double mean_prev = row[0];
+                    if (row[2] == 0.0) {
+                        row[0] = xi[1];
+                        row[1] = 0.0;
+                        row[3] = xi[0];
+                    } else {
+                        row[0] = (xi[1] + (r - 1) * row[0]) / r;
+                        row[3] = (xi[0] + (r - 1) * row[3]) / r;
+                        const double inner = (r - 2) * row[1] + (xi[1] - row[0]) * (xi[1] - row[0]);
+                        if (intdiv)
+                            row[1] = (row[0] - mean_prev) * (row[0] - mean_prev) + (1 / (r - 1)) * inner;
+                        else
+                            row[1] = (row[0] - mean_prev) * (row[0] - mean_prev) + inner / (r - 1);
+                    }
+                    row[2] = r;
+                }
+            }
+            ++j;
+        }
+    }
+    free(x);
+}

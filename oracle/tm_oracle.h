@@ -173,6 +173,13 @@ int tmo_dpc_run(const tmo_dpc_params* p, int ns, int L, const int32_t* disk, int
                 uint32_t seed, uint32_t inst, int64_t step0, int64_t nsteps, double* fields,
                 double* rows, int64_t* fault_index);
 
+/* Multi-species replicate statistics (multiSPDP.cpp:3376-3391, :1080-1123):
+ * out [n_params][n_meas][4 nsp] = per species (mean, variance, surviving count,
+ * mean occupied count) over the replicates it still occupies; flags as above,
+ * flag 2 ending a replicate at the vegetation's (field 0's) first empty row. */
+void tmo_species_stats(const double* series, int n_inst, int n_meas, int nf, int nsp,
+                       const int32_t* params_index, int n_params, int flags, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -276,3 +276,46 @@ def test_upwind_neighbour_and_direction_stream(oracle):
             assert (cell in occupied) == want(vx, vy), (seed, name, vx, vy)
             hits[name] += cell in occupied
     assert min(hits.values()) > 5
+
+
+# ------------------------------------------------------------------ multi-species replicate statistics
+def _series(rng, n_inst, n_meas, nsp, extinct_at=None):
+    """Random rows (count, mean) per field, with species dying out at given rows."""
+    x = np.zeros((n_inst, n_meas, 2 * (2 * nsp - 1)))
+    for i in range(n_inst):
+        for f in range(2 * nsp - 1):
+            x[i, :, 2 * f] = rng.integers(1, 400, n_meas)
+            x[i, :, 2 * f + 1] = rng.random(n_meas)
+        for f in range(nsp):
+            e = (extinct_at or {}).get((i, f))
+            if e is not None:
+                x[i, e:, 2 * f:2 * f + 2] = 0.0
+    return x
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_species_stats_per_species_is_the_single_species_recurrence(oracle, flags):
+    """Each species' columns are the pinned single-field statistics of that
+    species' rows alone (MSP:1080-1123 treats species independently); with flag 2
+    those rows are first zeroed after the vegetation's end."""
+    rng = np.random.default_rng(20)
+    nsp, n_meas = 3, 9
+    pidx = np.array([0, 1, 0, 0, 1, 0])
+    x = _series(rng, len(pidx), n_meas, nsp, {(0, 1): 4, (2, 0): 6, (3, 2): 0, (4, 1): 2, (5, 0): 3})
+    got = oracle.species_stats(x, pidx, 2, nsp, flags)
+    for s in range(nsp):
+        xs = x[:, :, 2 * s:2 * s + 2].copy()
+        if flags & 2:
+            for i in range(len(pidx)):
+                e = np.argmax(x[i, :, 0] == 0.0) if np.any(x[i, :, 0] == 0.0) else n_meas
+                xs[i, e + 1:] = 0.0
+        want = oracle.replicate_stats(xs, pidx, 2, flags & 1)
+        assert np.array_equal(got[:, :, 4 * s:4 * s + 4], want), s
+
+
+def test_species_stats_single_species_equals_the_dp_statistics(oracle):
+    rng = np.random.default_rng(21)
+    pidx = np.array([0, 0, 0, 0])
+    x = _series(rng, 4, 7, 1, {(1, 0): 3})
+    for flags in (0, 1, 2, 3):
+        assert np.array_equal(oracle.species_stats(x, pidx, 1, 1, flags), oracle.replicate_stats(x, pidx, 1, flags))

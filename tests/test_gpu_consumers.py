@@ -123,3 +123,27 @@ def test_consumer_models_reject_bad_configurations(tm):
             ctx.configure(tm.MODEL_DP3, tm.F64, _tm_params(tm, params(pR=(0.0, 1.35, 0.2))), [tm.Instance(64)])
         with pytest.raises(tm.TMError):          # no perception at all
             ctx.configure(tm.MODEL_DP2, tm.F64, _tm_params(tm, params(pR=(0.0, 0.0, 0.0))), [tm.Instance(64)])
+
+
+@pytest.mark.parametrize("model", ["DP2", "DP3"])
+def test_consumer_species_statistics(tm, oracle, model):
+    """Per-species replicate statistics on the device equal the oracle's on the
+    same device series, bit for bit, for every flag; the second parameter point's
+    consumers die out within the run."""
+    m = getattr(tm, "MODEL_" + model)
+    ns = tm.NSPECIES[m]
+    plist = [params(), params(M=(0.0, 3.0, 4.0))]
+    insts = [tm.Instance(20 + 3 * (k % 3), k % 2, 5, k) for k in range(8)]
+    pidx = np.array([d.params_index for d in insts])
+    rng = np.random.default_rng(14)
+    print(f"consumers {model} statistics", flush=True)
+    with tm.Context() as ctx:
+        ctx.configure(m, tm.F64, [_tm_params(tm, p) for p in plist], insts)
+        for k, d in enumerate(insts):
+            ctx.upload(k, _initial(rng, ns, d.L))
+        series = ctx.run_series([1, 4, 9, 20, 45, 70])
+        assert np.any(series[pidx == 1, -1, 2] == 0.0)
+        for flags in (0, 1, 2, 3):
+            got = ctx.replicate_stats(flags)
+            assert got.shape == (2, 6, 4 * ns)
+            assert np.array_equal(got, oracle.species_stats(series, pidx, 2, ns, flags)), flags

@@ -319,12 +319,22 @@ def mft_frame_values(homogeneous: bool, R: float, R_c: float, dP: float, c_sprea
 FRAME_HEADER = {
     MODEL_RIETKERK: "a_c,  x,  P(x; t), W(x; t), O(x; t) \n",
     MODEL_DP: "a_c,  x,  P(x; t) \n",
+    MODEL_DP2: "a_c,  x,  P(x; t), G(x; t) \n",               # MultiSPDP_constants_2Sp.h:6
+    MODEL_DP3: "a_c,  x,  P(x; t), G(x; t), Pr(x; t)\n",      # MultiSPDP_constants_3Sp.h:6
 }
 PRELIM_HEADER = {
     MODEL_RIETKERK: (" a , r, L, t , <<W(x; t)>_x>_r, <<O(x; t)>_x>_r,  <<P(x; t)>_x>_r, Var[<P(x; t)>_x]_r, "
                      "# Surviving Runs P(x; t), # Active Sites P(x; t), \n"),
     MODEL_DP: (" a , r, L, t , <<P(x; t)>_x>_r, Var[<P(x; t)>_x]_r, # Surviving Runs P(x; t),"
                " # Active Sites P(x; t), \n"),
+    # MultiSPDP_constants_2Sp.h:10-11, MultiSPDP_constants_3Sp.h:10-12
+    MODEL_DP2: (" a , r, L, t , <<P(x; t)>_x>_r, Var[<P(x; t)>_x]_r, # Surviving Runs P(x; t),"
+                " # Active Sites P(x; t), <<G(x; t)>_x>_r, Var[<G(x; t)>_x]_r, # Surviving Runs G(x; t),"
+                " # Active Sites G(x; t), \n"),
+    MODEL_DP3: (" a , r, L, t , <<P(x; t)>_x>_r, Var[<P(x; t)>_x]_r, # Surviving Runs P(x; t),"
+                " # Active Sites P(x; t), <<G(x; t)>_x>_r, Var[<G(x; t)>_x]_r, # Surviving Runs G(x; t),"
+                " # Active Sites G(x; t),<<Pr(x; t)>_x>_r, Var[<Pr(x; t)>_x]_r, # Surviving Runs Pr(x; t),"
+                " # Active Sites Pr(x; t), \n"),
 }
 
 
@@ -380,6 +390,8 @@ def read_frame_csv(path, L: int, nf: int, cols, out=None):
 STATS_COLUMNS = {
     MODEL_DP: ["mean", "var", "surviving", "active"],
     MODEL_RIETKERK: ["mean_W", "mean_O", "mean", "var", "surviving", "active"],
+    MODEL_DP2: [f"{c}_{s}" for s in ("P", "G") for c in ("mean", "var", "surviving", "active")],
+    MODEL_DP3: [f"{c}_{s}" for s in ("P", "G", "Pr") for c in ("mean", "var", "surviving", "active")],
 }
 
 
@@ -495,8 +507,10 @@ class Context:
 
     def replicate_stats(self, flags: int = STATS_REFERENCE) -> np.ndarray:
         """Across-replicate statistics of the last run_series, computed on the device:
-        [n_params, n_meas, nf+3], columns STATS_COLUMNS[model]."""
-        out = np.empty((self._n_params, self._n_series, self.nfields + 3), np.float64)
+        [n_params, n_meas, nf+3] (consumer models: [.., 4 * species]), columns
+        STATS_COLUMNS[model]."""
+        ncol = 4 * self.nspecies if self.model in (MODEL_DP2, MODEL_DP3) else self.nfields + 3
+        out = np.empty((self._n_params, self._n_series, ncol), np.float64)
         self._check(self._lib.tm_replicate_stats(self._h, flags, _dptr(out)))
         return out
 

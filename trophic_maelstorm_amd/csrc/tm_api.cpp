@@ -99,7 +99,7 @@ struct tm_ctx {
     int series_steps_cap = 0;
     int32_t* d_groups = nullptr;       // [n_params + 1] offsets, [n_inst] members, [n_inst] first extinct
     int64_t groups_cap = 0;
-    double* d_stats = nullptr;         // [n_params][n_series][nf + 3]
+    double* d_stats = nullptr;         // [n_params][n_series][nf + 3], consumer models [4 species]
     int64_t stats_cap = 0;
 
     // initial-frame launch table: instance workgroup starts, per-point values
@@ -1458,8 +1458,11 @@ tm_status tm_replicate_stats(tm_ctx* c, int flags, double* stats)
     if (flags & ~TM_STATS_REFERENCE) return fail(c, TM_ERR_ARG, "tm_replicate_stats: unknown flags");
     if (c->n_series <= 0) return fail(c, TM_ERR_STATE, "tm_run_series has not been called");
     const int n_inst = (int)c->inst.size(), ng = c->n_params, n_meas = c->n_series;
-    // Instances grouped by parameter point, in instance (= replicate) order.
-    std::vector<int32_t> grp((size_t)ng + 1 + 2 * (size_t)n_inst, 0);
+    const int nsp = is_consumer_model(c->model) ? species_of(c->model) : 0;   // multi-species rows
+    const int ncol = nsp > 0 ? 4 * nsp : c->nf + 3;
+    // Instances grouped by parameter point, in instance (= replicate) order; then
+    // the first extinct measurement per (instance, species).
+    std::vector<int32_t> grp((size_t)ng + 1 + (size_t)n_inst * (1 + std::max(nsp, 1)), 0);
     for (const tm_instance_desc& d : c->inst) ++grp[(size_t)d.params_index + 1];
     for (int g = 0; g < ng; ++g) grp[(size_t)g + 1] += grp[(size_t)g];
     std::vector<int32_t> cur(grp.begin(), grp.begin() + ng);
@@ -1471,7 +1474,7 @@ tm_status tm_replicate_stats(tm_ctx* c, int flags, double* stats)
         HIPCHK(c, hipMalloc((void**)&c->d_groups, sizeof(int32_t) * (size_t)gsize));
         c->groups_cap = gsize;
     }
-    const int64_t nout = (int64_t)ng * n_meas * (c->nf + 3);
+    const int64_t nout = (int64_t)ng * n_meas * ncol;
     if (nout > c->stats_cap) {
         free_dev(c->d_stats);
         c->stats_cap = 0;
@@ -1480,7 +1483,7 @@ tm_status tm_replicate_stats(tm_ctx* c, int flags, double* stats)
     }
     HIPCHK(c, hipMemcpyAsync(c->d_groups, grp.data(), sizeof(int32_t) * (size_t)(ng + 1 + n_inst),
                              hipMemcpyHostToDevice, c->stream));
-    launch_replicate_stats(c->d_series, n_inst, n_meas, c->nf, c->d_groups, c->d_groups + ng + 1, ng, flags,
+    launch_replicate_stats(c->d_series, n_inst, n_meas, c->nf, nsp, c->d_groups, c->d_groups + ng + 1, ng, flags,
                            c->d_groups + ng + 1 + n_inst, c->d_stats, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(stats, c->d_stats, sizeof(double) * (size_t)nout, hipMemcpyDeviceToHost, c->stream));

@@ -104,7 +104,9 @@ void launch_live_count(const double* rows, int64_t rows_cap, int w, int64_t row,
 // Sweep series and replicate statistics (tm_ensemble.hip).
 void launch_gather_series(const double* rows, int64_t rows_cap, int w, const int64_t* steps, int m0, int m1,
                           int64_t s0, int n_meas, int n_inst, double* series, hipStream_t s);
-void launch_replicate_stats(const double* series, int n_inst, int n_meas, int nf, const int32_t* grp_off,
+// nsp > 0: the multi-species form, [4 nsp] columns per row (species_stats_kernel);
+// nsp = 0: field 0's statistics plus the other fields' means, [nf + 3] columns.
+void launch_replicate_stats(const double* series, int n_inst, int n_meas, int nf, int nsp, const int32_t* grp_off,
                             const int32_t* grp_inst, int n_groups, int flags, int32_t* first_extinct,
                             double* out, hipStream_t s);
 

@@ -87,6 +87,21 @@ def c5_batch(tm, precision, steps=100_000):
                 steps=steps, nf=3, chunk=500)
 
 
+def c6_batch(tm, model):
+    """Vegetation + consumers (FP64): 16 a-values x 4 replicates of 256^2 with the
+    grazer (and predator) values of order_2stocDP.cpp (dx = 0.1 km, dt = 0.1 hr,
+    perception radius 0.521 km = 5 sites), vegetation from Uniform(0, 20), grazers
+    Uniform(0, 2), predators Uniform(0, 0.5), perception fields 0."""
+    ps = [tm.DPCParams(a=float(a), b=1e-6) for a in np.linspace(5e-6, 2e-5, 16)]
+    inst = [tm.Instance(256, k // 4, SEED, k) for k in range(64)]
+    if model == tm.MODEL_DP2:
+        lo, hi, steps = [0.0, 0.0, 0.0], [20.0, 2.0, 0.0], 2000
+    else:
+        lo, hi, steps = [0.0, 0.0, 0.0, 0.0, 0.0], [20.0, 2.0, 0.5, 0.0, 0.0], 1000
+    return dict(model=model, precision=tm.F64, params=ps, inst=inst, lo=lo, hi=hi, steps=steps,
+                nf=tm.NFIELDS[model], chunk=500)
+
+
 def cells_of(b):
     return sum(i.L * i.L for i in b["inst"])
 
@@ -322,7 +337,7 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="headline only")
     ap.add_argument("--variant", default=None,
                     help="kernel variant for the headline (or --only) run instead of the autotuner's choice")
-    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4-f64", "c5", "sweep"], default=None,
+    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4-f64", "c5", "sweep", "dp2", "dp3"], default=None,
                     help="only this side workload, --steps steps (default: its full length), one JSON line")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
@@ -354,7 +369,8 @@ def main():
     if args.only:
         b = {"c2": lambda: c2_batch(tm, tm.F64), "c3": lambda: c3_batch(tm, tm.F32),
              "c4-f64": lambda: c4_batch(tm, tm.F64),
-             "c3-f64": lambda: c3_batch(tm, tm.F64), "c5": lambda: c5_batch(tm, tm.F32)}[args.only]()
+             "c3-f64": lambda: c3_batch(tm, tm.F64), "c5": lambda: c5_batch(tm, tm.F32),
+             "dp2": lambda: c6_batch(tm, tm.MODEL_DP2), "dp3": lambda: c6_batch(tm, tm.MODEL_DP3)}[args.only]()
         n = args.steps or b["steps"]
         log(f"[bench] {args.only}: {cells_of(b)} cells/step x {n} steps")
         r = run_workload(tm, b, n, args.warmup, variant_name=args.variant)
@@ -406,6 +422,13 @@ def main():
         side["c3_sweep"] = sweep_leg(tm)
         log(f"[bench]   series {side['c3_sweep']['series_value']:.4g} cell-updates/s, "
             f"stats {side['c3_sweep']['replicate_stats_ms']:.2f} ms")
+        for name, model in (("c6_consumers_dp2_fp64", tm.MODEL_DP2), ("c6_consumers_dp3_fp64", tm.MODEL_DP3)):
+            c6 = c6_batch(tm, model)
+            log(f"[bench] {name}: 64 x 256^2, {c6['steps']} steps")
+            r6 = run_workload(tm, c6, c6["steps"], args.warmup, want_rows=True)
+            side[name] = {k: r6[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant",
+                                            "steps")}
+            log(f"[bench]   {r6['value']:.4g} cell-updates/s, kernel {r6['kernel_ms']:.4f} ms")
         if args.c5_steps > 0:
             c5 = c5_batch(tm, tm.F32, args.c5_steps)
             log(f"[bench] C5 FP32 Rietkerk 8192^2, {args.c5_steps} of its 10^5 steps")

@@ -45,6 +45,7 @@ FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 # means over the absorbing phase of the first 4000 steps.
 C4_F32_BLOCKED_TRAFFIC_KIB = {
     "dp_f32_128x32_t256_x4a": (4, (87921.6, 141440.2), (75136.4, 139848.7)),
+    "dp_f32_128x24_t256_x8a": (8, (175845.5, 145216.1), (114941.6, 140040.5)),
 }
 C4_F32_TRAFFIC_KIB = {   # single-step variant: (FETCH_SIZE, WRITE_SIZE) per dispatch
     "dp_f32_128x32_t256_w6": (75198.8, 139824.1),
@@ -220,6 +221,18 @@ def sweep_leg(tm, steps=None):
     return out
 
 
+def _kernel_desc(variant):
+    """The kernels an adaptive multi-step variant runs, from its name (dp_f32_TXxTY_tNT_xKa)."""
+    v = variant or ""
+    if v in C4_F32_BLOCKED_TRAFFIC_KIB:
+        shape = v.split("_")[2]
+        tx, ty = shape.split("x")
+        k = C4_F32_BLOCKED_TRAFFIC_KIB[v][0]
+        return (f"dp_stepk_kernel<float,{tx},{ty},256,{k}> ({k} steps per launch) once <= 2% of cells are "
+                f"occupied, dp_step_kernel<float,{tx},{ty},256> single steps before")
+    return "dp_step_kernel<float>"
+
+
 def roofline(h, b):
     """The headline's roofline: algorithmic bytes (one read and one write of the
     field per cell-update) per step over the mean device time per step, and the
@@ -230,9 +243,7 @@ def roofline(h, b):
     step_s = h["kernel_ms"] / 1e3
     out = {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": h["frac"], "traffic": traffic,
-           "kernel": "dp_stepk_kernel<float,128,32,256,4> (4 steps per launch) once <= 2% of cells are "
-                     "occupied, dp_step_kernel<float,128,32,256> single steps before"
-                     if "x4a" in (h["variant"] or "") else "dp_step_kernel<float>",
+           "kernel": _kernel_desc(h["variant"]),
            "bytes_per_cell": bytes_per_cell(b), "cells_per_launch": cells_of(b),
            "step_device_ms": h["kernel_ms"], "single_steps": h["single_steps"],
            "blocked_steps": h["blocked_steps"],

@@ -223,7 +223,9 @@ tm_status tm_reset(tm_ctx* ctx);
 
 /* Launch shape.  variant: index into the kernel table for the configured
  * model/precision (-1 = default); graph_steps: steps per captured hipGraph
- * block (0 = eager launches).  tm_variant_count / tm_variant_name describe the table. */
+ * block (0 = eager launches), rounded up to a whole number of launches for a
+ * variant that advances several steps per launch.  tm_variant_count /
+ * tm_variant_name describe the table. */
 tm_status tm_set_launch(tm_ctx* ctx, int variant, int graph_steps);
 int tm_variant_count(int model, int precision);
 const char* tm_variant_name(int model, int precision, int variant);

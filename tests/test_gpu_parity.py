@@ -209,7 +209,7 @@ def test_absorbing_state_is_exact(tm, oracle):
         assert np.all(rows == 0.0)
 
 
-@pytest.mark.parametrize("variant", [None, "dp_f64_64x32_t256_x2", "dp_f64_64x32_t256_x4"])
+@pytest.mark.parametrize("variant", [None, "dp_f64_64x32_t256_x2", "dp_f64_64x32_t256_x4", "dp_f64_64x32_t256_x8"])
 def test_numeric_fault_is_reported(tm, oracle, variant):
     """The first fault (instance, cell, step) is reported.  With a multi-step
     variant the run covers every step of one launch: the faults of step 0 must
@@ -249,11 +249,12 @@ def test_autotune_does_not_change_state(tm, oracle, tmp_path):
         assert ms > 0 and ctx.step(1) == 7
 
 
-@pytest.mark.parametrize("variant", ["dp_f64_64x32_t256_x4a", "dp_f64_64x32_t256_x4"])
+@pytest.mark.parametrize("variant", ["dp_f64_64x32_t256_x4a", "dp_f64_64x32_t256_x4", "dp_f64_64x32_t256_x8a",
+                                     "dp_f64_64x32_t256_x8", "dp_f64_64x8_t128_x16"])
 def test_multistep_variant_through_extinction_bit_exact(tm, oracle, variant):
     """A DP batch that dies out (a = 0.3) over 300 steps in blocks of 10: the
     adaptive variant starts with single steps on the live field and turns to
-    four-step launches once a tenth or less of the cells is occupied; both kinds
+    K-step launches once 2% or less of the cells are occupied (step 133); both kinds
     of launch, and the switch between them, reproduce the oracle bit for bit."""
     Ls = [32, 40, 17]
     p = tm.DPParams(a=0.3)
@@ -272,4 +273,4 @@ def test_multistep_variant_through_extinction_bit_exact(tm, oracle, variant):
         assert np.array_equal(got[k], ref["fields"]), (variant, d.L)
         assert_rows_close(rows[k], ref["rows"])
     occupied = rows[:, :, 0].sum(axis=0) / sum(L * L for L in Ls)
-    assert occupied[0] > 0.9 and occupied[-1] <= 0.1      # both modes were used
+    assert occupied[0] > 0.9 and occupied[-1] <= 0.02     # both modes were used

@@ -519,13 +519,11 @@ tm_status live_fraction_now(tm_ctx* c, double* frac)
     return TM_OK;
 }
 
-// An adaptive variant takes single steps while more than this fraction of
-// field-0 cells is occupied: there the K-step kernel's recomputed rings cost
-// more sampling than its saved field traffic is worth (measured on the DP
-// quench, where the 4-step launches win once about 2% of the cells or fewer
-// are occupied).
-constexpr double kLiveThreshold = 0.02;
-
+// An adaptive variant takes single steps while more than its threshold
+// (Variant::adaptive, per mille) of the field-0 cells is occupied: there the
+// K-step kernel's recomputed rings cost more sampling than its saved field
+// traffic is worth (measured on the DP quench, where the 4-step launches win
+// once about 2% of the cells or fewer are occupied).
 tm_status update_mode(tm_ctx* c)
 {
     const Variant& v = cur_variant(c);
@@ -538,7 +536,7 @@ tm_status update_mode(tm_ctx* c)
         tm_status s = live_fraction_now(c, &c->live_frac);
         if (s != TM_OK) return s;
     }
-    c->blocked = c->live_frac <= kLiveThreshold;
+    c->blocked = c->live_frac <= v.adaptive * 1e-3;
     return TM_OK;
 }
 
@@ -1197,7 +1195,10 @@ tm_status tm_set_launch(tm_ctx* c, int variant, int graph_steps)
     if (variant < -1 || variant >= n) return fail(c, TM_ERR_ARG, "variant out of range");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->variant = variant < 0 ? default_variant(c->model, c->prec, c->total_cells) : variant;
-    c->graph_steps = graph_steps;
+    // A graph block of a multi-step variant is a whole number of its launches:
+    // a remainder would run as single steps in every block.
+    const int spl = cur_variant(c).spl;
+    c->graph_steps = spl > 1 ? (graph_steps + spl - 1) / spl * spl : graph_steps;
     return build_tiles(c);
 }
 
@@ -1274,7 +1275,7 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
             if (k == key && v >= 0 && v < tm_variant_count(c->model, c->prec)) {
                 if ((s = tm_set_launch(c, v, g)) != TM_OK) return s;
                 if (out_variant) *out_variant = v;
-                if (out_graph_steps) *out_graph_steps = g;
+                if (out_graph_steps) *out_graph_steps = c->graph_steps;
                 return TM_OK;
             }
         }
@@ -1327,9 +1328,10 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
     hipEventDestroy(e1);
     // Graph block: short steps are launch-bound and want long graphs.
     const double per_step_ms = best;
-    const int g = per_step_ms < 0.05 ? 200 : (graph_steps > 0 ? graph_steps : 100);
+    int g = per_step_ms < 0.05 ? 200 : (graph_steps > 0 ? graph_steps : 100);
     c->variant = best_v;
     if ((s = tm_set_launch(c, best_v, g)) != TM_OK) return s;
+    g = c->graph_steps;
     mkdirs(path);
     std::ofstream f(path, std::ios::app);
     if (f) f << key << " " << best_v << " " << g << "\n";
@@ -1499,9 +1501,9 @@ int default_variant(int model, int precision, int64_t total_cells)
     const char* want;
     if (is_consumer_model(model)) return 0;
     if (model == TM_MODEL_DP) {
-        // adaptive four-step launches: single steps while the field is live,
-        // four steps per launch once it is mostly absorbing
-        want = precision == TM_F64 ? "dp_f64_64x32_t256_x4a" : "dp_f32_128x32_t256_x4a";
+        // adaptive eight-step launches: single steps while more than 2% of the
+        // cells are occupied, eight steps per launch once the field is mostly absorbing
+        want = precision == TM_F64 ? "dp_f64_64x32_t256_x8a" : "dp_f32_128x24_t256_x8a";
     } else {
         // Rietkerk: small lattices are latency-bound and want the single fused
         // 16 x 16 launch; batches want the split step (each P* sampled once).

@@ -48,12 +48,24 @@ static const Variant kDP[] = {
      &launch_dp<float, 128, 16, 256>},
     {128, 24, 256, 0, 4, "dp_f32_128x24_t256_x4", &launch_dpk<float, 128, 24, 256, 4>, 0, 0, nullptr, 0, 4,
      &launch_dp<float, 128, 24, 256>},
-    // _x4a: adaptive -- single steps (<= 80 VGPRs) while more than a tenth of the cells are
-    // occupied, four-step launches below
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x8", &launch_dpk<float, 128, 32, 256, 8>, 0, 0, nullptr, 0, 8,
+     &launch_dp<float, 128, 32, 256>},
+    // _xKa: adaptive -- single steps (<= 80 VGPRs) while more than the given per-mille of
+    // the cells are occupied, K-step launches below
     {128, 24, 256, 0, 4, "dp_f32_128x24_t256_x4a", &launch_dpk<float, 128, 24, 256, 4>, 0, 0, nullptr, 0, 4,
-     &launch_dp<float, 128, 24, 256, 0, 6>, 1},
+     &launch_dp<float, 128, 24, 256, 0, 6>, 20},
     {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x4a", &launch_dpk<float, 128, 32, 256, 4>, 0, 0, nullptr, 0, 4,
-     &launch_dp<float, 128, 32, 256, 0, 6>, 1},
+     &launch_dp<float, 128, 32, 256, 0, 6>, 20},
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x8a", &launch_dpk<float, 128, 32, 256, 8>, 0, 0, nullptr, 0, 8,
+     &launch_dp<float, 128, 32, 256, 0, 6>, 20},
+    {128, 24, 256, 0, 4, "dp_f32_128x24_t256_x8a", &launch_dpk<float, 128, 24, 256, 8>, 0, 0, nullptr, 0, 8,
+     &launch_dp<float, 128, 24, 256, 0, 6>, 20},
+    {128, 16, 256, 0, 4, "dp_f32_128x16_t256_x8a", &launch_dpk<float, 128, 16, 256, 8>, 0, 0, nullptr, 0, 8,
+     &launch_dp<float, 128, 16, 256, 0, 6>, 20},
+    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x6a", &launch_dpk<float, 128, 32, 256, 6>, 0, 0, nullptr, 0, 6,
+     &launch_dp<float, 128, 32, 256, 0, 6>, 20},
+    {128, 16, 256, 0, 4, "dp_f32_128x16_t256_x16a", &launch_dpk<float, 128, 16, 256, 16>, 0, 0, nullptr, 0, 16,
+     &launch_dp<float, 128, 16, 256, 0, 6>, 20},
 };
 
 static const Variant kRK[] = {

@@ -37,9 +37,16 @@ static const Variant kDP[] = {
      &launch_dp<double, 64, 32, 256>},
     {64, 16, 256, 0, 4, "dp_f64_64x16_t256_x4", &launch_dpk<double, 64, 16, 256, 4>, 0, 0, nullptr, 0, 4,
      &launch_dp<double, 64, 16, 256>},
-    // _x4a: adaptive -- single steps while more than a tenth of the cells are occupied
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x8", &launch_dpk<double, 64, 32, 256, 8>, 0, 0, nullptr, 0, 8,
+     &launch_dp<double, 64, 32, 256>},
+    {64, 8, 128, 0, 2, "dp_f64_64x8_t128_x16", &launch_dpk<double, 64, 8, 128, 16>, 0, 0, nullptr, 0, 16,
+     &launch_dp<double, 64, 8, 128>},
+    // _xKa: adaptive -- single steps while more than the given per-mille of the cells are
+    // occupied, K-step launches below
     {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x4a", &launch_dpk<double, 64, 32, 256, 4>, 0, 0, nullptr, 0, 4,
-     &launch_dp<double, 64, 32, 256>, 1},
+     &launch_dp<double, 64, 32, 256>, 20},
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x8a", &launch_dpk<double, 64, 32, 256, 8>, 0, 0, nullptr, 0, 8,
+     &launch_dp<double, 64, 32, 256>, 20},
 };
 
 static const Variant kRK[] = {

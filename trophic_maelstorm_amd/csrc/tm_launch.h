@@ -20,7 +20,8 @@ struct Variant {
     int row_major;   // tile order within an instance: 0 = down vertical strips, 1 = along rows
     int spl = 1;     // steps per launch of fn (> 1: temporally blocked DP step)
     StepLaunch fn_one = nullptr;   // spl > 1: the single-step kernel of the same tiles
-    int adaptive = 0;              // spl > 1: single steps while the field is live (see tm_api.cpp)
+    int adaptive = 0;              // spl > 1, > 0: single steps while more than this per-mille of
+                                   // the cells are occupied, spl-step launches below (tm_api.cpp)
 };
 
 // Variant tables per (model, precision); defined in tm_kernels_f64.hip / _f32.hip.

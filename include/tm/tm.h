@@ -325,6 +325,43 @@ int tm_read_frame_csv(const char* path, int L, int nf, const char* const* cols, 
  * to out (nullable) and returns how many there are (0 when t_max <= 1). */
 int tm_log10_time_bins(double t_max, double dt, double* out, int cap);
 
+/* ---- Rietkerk consumers' movement (the reference's CUDA seam) ----------------
+ * advdiff_FKE_MultiSp (Rietkerk_FPE/rietkerk_bjork_FKE.cpp:3614-3695) and its CUDA
+ * twin updateMovement (kernel_advdiff_FKE2D.cu:92-180) move each consumer species
+ * one analytic Fokker-Planck step: every cell with rho >= 1e-10 spreads rho through
+ * the product of a row and a column kernel, each gamma x (Gaussian centred on the
+ * cell) + (1 - gamma) x (Gaussian centred on the advected mean cell + v mu_prefactor,
+ * rounded), the two Gaussians being the stencils below. */
+typedef struct tm_fke_move_species {
+    int32_t rD;            /* sig_D_ScaledBounds[s]: diffusion stencil half-width, sites   */
+    int32_t rV;            /* sig_vD_ScaledBounds[s]: advection stencil half-width, sites  */
+    double mu_prefactor;   /* mu_vel_prefactor[s] (FKE:248)                                 */
+} tm_fke_move_species;
+
+/* Host only.  The constants set_global_FKE_params derives for one consumer species
+ * (FKE:222-250: beta = 3 / dt_analytic, or 3 / dt_adv with mv_invariance == 1;
+ * sigma_D = sqrt(2 D dt_analytic); the velocity OU spread; CI = 4 by default) and
+ * its two stencils, precompute_gaussian(sigma, rD) and (sigma, rV) (FKE:1735-1749,
+ * :3764-3772 -- the reference passes the species' noise sigma).  stencil_D gets
+ * rD + 1 values, stencil_V rV + 1.  Returns 0, or -1 if a cap is too small. */
+int tm_fke_move_constants(double D, double v, double sigma, double dt_analytic, double dt_adv, int mv_invariance,
+                          double dx, int CI, tm_fke_move_species* out, double* stencil_D, int cap_D,
+                          double* stencil_V, int cap_V);
+
+/* The movement step of ns consumer species on one L x L lattice, on `device`:
+ * host arrays rho, gamma [ns][L*L], v_eff [ns][L*L][2] (vx, vy as the reference's
+ * std::pair), stencils concatenated by species (rD_s + 1 and rV_s + 1 values);
+ * out [ns][L*L] receives the moved fields (it may alias rho).  Replaces the
+ * reference's host copies, cudaMemset, launchAdvDiffKernel_MultiSp and copy back
+ * (FKE:4276-4301, rietkerk_bjork_basic.h:461-467) and the CPU path (FKE:4303-4309).
+ * The kernel gathers (each cell pulls from the cells whose stencils cover it), so
+ * it needs no atomics; stencils and advection shifts must fit the lattice
+ * (2 max(rD, rV + max |v| mu_prefactor + 1) + 1 <= L).  Returns 0 or a negative
+ * tm_status. */
+int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* sp, const double* stencil_D,
+                         const double* stencil_V, const double* rho, const double* gamma, const double* v_eff,
+                         double* out);
+
 /* Host only.  The perception neighbourhood of the consumer models: offsets
  * (row, column) of the sites within Euclidean radius R of a site, in the order of
  * computeNeighboringSitesCentral (multiSPDP.cpp:1442-1461): generated row-major

@@ -115,6 +115,12 @@ def lib():
         L.tmo_species_stats.argtypes = [_f64p, C.c_int, C.c_int, C.c_int, C.c_int,
                                         np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                         C.c_int, C.c_int, _f64p]
+        L.tmo_fke_move_constants.restype = C.c_int
+        L.tmo_fke_move_constants.argtypes = [C.c_double] * 5 + [C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int32),
+                                                                 C.POINTER(C.c_int32), C.POINTER(C.c_double), _f64p,
+                                                                 C.c_int, _f64p, C.c_int]
+        L.tmo_fke_movement.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_double, _f64p,
+                                       _f64p, _f64p]
         L.tmo_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tmo_fixed_sum.restype = C.c_double
         L.tmo_fixed_sum.argtypes = [_f64p, C.c_int64]
@@ -285,6 +291,27 @@ def species_stats(series, params_index, n_params, nsp, flags):
     out = np.zeros((n_params, n_meas, 4 * nsp), np.float64)
     lib().tmo_species_stats(x.reshape(-1), n_inst, n_meas, w // 2, nsp, pidx, n_params, flags, out.reshape(-1))
     return out
+
+
+def fke_move_constants(D, v, sigma, dt_analytic, dx, dt_adv=0.0, mv_invariance=0, CI=4):
+    rD, rV, pref = C.c_int32(), C.c_int32(), C.c_double()
+    sd, sv = np.zeros(4096), np.zeros(4096)
+    rc = lib().tmo_fke_move_constants(D, v, sigma, dt_analytic, dt_adv, mv_invariance, dx, CI, C.byref(rD), C.byref(rV),
+                                      C.byref(pref), sd, sd.size, sv, sv.size)
+    assert rc == 0
+    return (rD.value, rV.value, pref.value), sd[: rD.value + 1].copy(), sv[: rV.value + 1].copy()
+
+
+def fke_movement(rho, gamma, vx, vy, rD, rV, pref, SD, SV):
+    """One species' movement step (advdiff_FKE_MultiSp restated): fields [L, L]."""
+    r = np.ascontiguousarray(rho, np.float64)
+    L = r.shape[-1]
+    out = np.empty(L * L)
+    lib().tmo_fke_movement(L, r.reshape(-1), np.ascontiguousarray(gamma, np.float64).reshape(-1),
+                           np.ascontiguousarray(vx, np.float64).reshape(-1),
+                           np.ascontiguousarray(vy, np.float64).reshape(-1), rD, rV, pref,
+                           np.ascontiguousarray(SD, np.float64), np.ascontiguousarray(SV, np.float64), out)
+    return out.reshape(L, L)
 
 
 def ref_perception_sites():

@@ -988,3 +988,90 @@ void tmo_species_stats(const double* series, int n_inst, int n_meas, int nf, int
     }
     free(x);
 }
+
+/* ------------------------------------------------------------------ */
+/* Consumer movement of the Rietkerk model: the analytic Fokker-Planck  */
+/* step of advdiff_FKE_MultiSp (FKE:3614-3695, the CPU twin of the      */
+/* reference's CUDA kernel updateMovement, kernel_advdiff_FKE2D.cu:92). */
+/* ------------------------------------------------------------------ */
+
+/* precompute_gaussian (FKE:1735-1749): a normalised half-stencil of R + 1 values. */
+static void fke_gauss(double sigma, int R, double* st)
+{
+    const double PI_ = 3.141592653589793;
+    double sum = 0.0;
+    const double factor = 1.0 / (sqrt(2 * PI_) * sigma);
+    for (int x = 0; x <= R; ++x) {
+        st[x] = factor * exp(-0.5 * (x * x) / (sigma * sigma));
+        sum += (x > 0) ? 2 * st[x] : st[0];
+    }
+    for (int x = 0; x <= R; ++x) st[x] /= sum;
+}
+
+/* set_global_FKE_params (FKE:195-250) for one consumer species, and its two
+ * stencils (FKE:3764-3772; both are Gaussians of the species' noise sigma). */
+int tmo_fke_move_constants(double D, double v, double sigma, double dt_analytic, double dt_adv,
+                           int mv_invariance, double dx, int CI, int32_t* rD, int32_t* rV,
+                           double* mu_pref, double* stencil_D, int cap_D, double* stencil_V, int cap_V)
+{
+    const double Dt_ay = dt_analytic;
+    const double beta_vel = mv_invariance == 1 ? 3.0 / dt_adv : 3.0 / dt_analytic;
+    const double exp_beta_Dt = exp(-beta_vel * dt_analytic);
+    const double exp_2beta_Dt = exp(-2 * beta_vel * dt_analytic);
+    const double sigma_v = sqrt(2 * beta_vel * 1 * fabs(v));
+    const double sigma_D = sqrt(2 * D * dt_analytic);
+    const double sigma_vx = (sigma_v / beta_vel) * sqrt((Dt_ay + (1 / (2 * beta_vel)) * (1 - exp_2beta_Dt)
+                                                         - (2 / beta_vel) * (1 - exp_beta_Dt)));
+    *rD = (int)(CI * sigma_D / dx);
+    *rV = (int)(CI * sigma_vx / dx);
+    *mu_pref = (1 / dx) * (Dt_ay - ((1 - exp_beta_Dt) / beta_vel));
+    if (*rD + 1 > cap_D || *rV + 1 > cap_V) return -1;
+    fke_gauss(sigma, *rD, stencil_D);
+    fke_gauss(sigma, *rV, stencil_V);
+    return 0;
+}
+
+/* One species: out = the scatter of every cell with rho >= 1e-10 (FKE:3626-3628;
+ * epsilon at rietkerk_bjork_basic.h:119) through the product of its row and
+ * column kernels, gamma^2 (diffusion x diffusion), gamma - gamma^2 (each mixed
+ * pair) and (1 - gamma)^2 (advection x advection), centred on the cell for
+ * diffusion and on its advected mean for advection.  The reference's advection
+ * bounds read .first below the mean and .second above (FKE:3640-3641); both
+ * are int(CI sigma_v / dx) of the same sigma, one rV here. */
+void tmo_fke_movement(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
+                      int rD, int rV, double mu_pref, const double* SD, const double* SV, double* out)
+{
+    const int64_t L2 = (int64_t)L * L;
+    for (int64_t i = 0; i < L2; ++i) out[i] = 0.0;
+    for (int64_t i = 0; i < L2; ++i) {
+        if (rho[i] < 1.0e-10) continue;
+        const int c_i = (int)(i / L), c_j = (int)(i % L);
+        const double g = gamma[i], g2 = g * g, r = rho[i];
+        const int min_p = c_i - rD, max_p = c_i + rD, min_q = c_j - rD, max_q = c_j + rD;
+        const int mu_x = (int)(round(c_i + vx[i] * (mu_pref)) + L) % L;
+        const int mu_y = (int)(round(c_j + vy[i] * (mu_pref)) + L) % L;
+        const int min_p_vx = mu_x - rV, max_p_vx = mu_x + rV, min_q_vy = mu_y - rV, max_q_vy = mu_y + rV;
+        for (int p = min_p; p <= max_p; ++p) {
+            const int dxx = abs(p - c_i);
+            for (int q = min_q; q <= max_q; ++q) {
+                const int dyy = abs(q - c_j);
+                out[(int64_t)((p + L) % L) * L + (q + L) % L] += (g2) * r * (SD[dxx] * SD[dyy]);
+            }
+            for (int q = min_q_vy; q <= max_q_vy; ++q) {
+                const int dyy = abs(q - mu_y);
+                out[(int64_t)((p + L) % L) * L + (q + L) % L] += (g - g2) * r * (SD[dxx] * SV[dyy]);
+            }
+        }
+        for (int p = min_p_vx; p <= max_p_vx; ++p) {
+            const int dxx = abs(p - mu_x);
+            for (int q = min_q; q <= max_q; ++q) {
+                const int dyy = abs(q - c_j);
+                out[(int64_t)((p + L) % L) * L + (q + L) % L] += (g - g2) * r * (SV[dxx] * SD[dyy]);
+            }
+            for (int q = min_q_vy; q <= max_q_vy; ++q) {
+                const int dyy = abs(q - mu_y);
+                out[(int64_t)((p + L) % L) * L + (q + L) % L] += (1 - g) * (1 - g) * r * (SV[dxx] * SV[dyy]);
+            }
+        }
+    }
+}

@@ -180,6 +180,16 @@ int tmo_dpc_run(const tmo_dpc_params* p, int ns, int L, const int32_t* disk, int
 void tmo_species_stats(const double* series, int n_inst, int n_meas, int nf, int nsp,
                        const int32_t* params_index, int n_params, int flags, double* out);
 
+/* Rietkerk consumers' movement (advdiff_FKE_MultiSp, FKE:3614-3695): the
+ * per-species constants of set_global_FKE_params (FKE:195-250) with the stencils
+ * of precompute_gaussian (FKE:1735-1749; returns -1 if a stencil exceeds its cap),
+ * and the scatter of one species' field.  out is overwritten. */
+int tmo_fke_move_constants(double D, double v, double sigma, double dt_analytic, double dt_adv,
+                           int mv_invariance, double dx, int CI, int32_t* rD, int32_t* rV,
+                           double* mu_pref, double* stencil_D, int cap_D, double* stencil_V, int cap_V);
+void tmo_fke_movement(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
+                      int rD, int rV, double mu_pref, const double* SD, const double* SV, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -185,6 +185,40 @@ def perception_offsets(R: int) -> np.ndarray:
     return buf[: 2 * n].reshape(n, 2)
 
 
+class _MoveSpecies(C.Structure):
+    _fields_ = [("rD", C.c_int32), ("rV", C.c_int32), ("mu_prefactor", C.c_double)]
+
+
+def fke_move_constants(D, v, sigma, dt_analytic, dx, dt_adv=0.0, mv_invariance=0, CI=4):
+    """One consumer species' movement constants and stencils (set_global_FKE_params,
+    precompute_gaussian): ((rD, rV, mu_prefactor), stencil_D, stencil_V)."""
+    sp = _MoveSpecies()
+    sd, sv = np.zeros(4096), np.zeros(4096)
+    rc = lib().tm_fke_move_constants(D, v, sigma, dt_analytic, dt_adv, mv_invariance, dx, CI, C.byref(sp), _dptr(sd),
+                                     sd.size, _dptr(sv), sv.size)
+    if rc != 0:
+        raise ValueError("tm_fke_move_constants: bad arguments or stencil too wide")
+    return (sp.rD, sp.rV, sp.mu_prefactor), sd[: sp.rD + 1].copy(), sv[: sp.rV + 1].copy()
+
+
+def fke_movement(rho, gamma, v_eff, species, stencils_D, stencils_V, device: int = 0):
+    """The Rietkerk consumers' movement step on the GPU: rho, gamma [ns, L, L],
+    v_eff [ns, L, L, 2]; species [(rD, rV, mu_prefactor)] and their stencils."""
+    r = np.ascontiguousarray(rho, np.float64)
+    ns, L = r.shape[0], r.shape[-1]
+    g = np.ascontiguousarray(gamma, np.float64)
+    v = np.ascontiguousarray(v_eff, np.float64)
+    sp = (_MoveSpecies * ns)(*[_MoveSpecies(int(a), int(b), float(c)) for a, b, c in species])
+    sd = np.ascontiguousarray(np.concatenate(stencils_D), np.float64)
+    sv = np.ascontiguousarray(np.concatenate(stencils_V), np.float64)
+    out = np.empty_like(r)
+    rc = lib().tm_fke_movement_host(device, ns, L, C.cast(sp, C.c_void_p), _dptr(sd), _dptr(sv), _dptr(r), _dptr(g),
+                                    _dptr(v), _dptr(out))
+    if rc != 0:
+        raise TMError(rc, "tm_fke_movement_host failed")
+    return out
+
+
 @dataclass
 class Instance:
     L: int
@@ -248,6 +282,9 @@ def lib() -> C.CDLL:
         "tm_snapshot_to_frame_csv": ([C.c_char_p, C.c_char_p, C.c_double, C.c_char_p, C.c_int], C.c_int),
         "tm_read_frame_csv": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_int, dp], C.c_int),
         "tm_perception_offsets": ([C.c_int, C.POINTER(i32), C.c_int], C.c_int),
+        "tm_fke_move_constants": ([C.c_double] * 5 + [C.c_int, C.c_double, C.c_int, vp, dp, C.c_int, dp, C.c_int],
+                                  C.c_int),
+        "tm_fke_movement_host": ([C.c_int, C.c_int, C.c_int, vp, dp, dp, dp, dp, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -29,6 +29,7 @@ UNITS = {
     "tm_ensemble.hip": ["-ffp-contract=off"],
     "tm_frames.hip": ["-ffp-contract=off"],
     "tm_consumers.hip": ["-ffp-contract=off"],
+    "tm_movement.hip": ["-ffp-contract=off"],
     "tm_api.cpp": ["-ffp-contract=off"],
     "tm_io.cpp": ["-ffp-contract=off"],
 }

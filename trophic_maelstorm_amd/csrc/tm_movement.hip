@@ -1,0 +1,205 @@
+// tm_movement.hip -- movement of the Rietkerk model's consumer species: the
+// analytic Fokker-Planck step of advdiff_FKE_MultiSp
+// (Rietkerk_FPE/rietkerk_bjork_FKE.cpp:3614-3695), whose CUDA twin
+// updateMovement (kernel_advdiff_FKE2D.cu:92-180) is the reference's only GPU
+// code.
+//
+// Each source cell i with rho_i >= 1e-10 spreads rho_i over the lattice with
+// weight A_i(x) B_i(y), where the row kernel is
+//   A_i(x) = gamma_i SD(x - c_i) + (1 - gamma_i) SV(x - mu_x(i))
+// (SD, SV: the diffusion and advection half-stencils, mu_x(i) the rounded
+// advected mean) and B_i the same in columns.  Expanded, that is the
+// reference's four scatter terms (gamma^2, twice gamma - gamma^2, (1 - gamma)^2).
+// The reference scatters with atomicAdd; this kernel gathers instead: every
+// output cell loops over the sources within reach -- those with |x - c_i| <=
+// W and |y - c_j| <= W, W = max(rD, rV + largest shift) -- and adds their
+// terms in FP64, in a fixed order.  No atomics, and the result is
+// deterministic.  The sums differ from the reference's sequential scatter only
+// in their order.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <vector>
+
+#include "tm/tm.h"
+
+namespace {
+
+constexpr double kMoveEps = 1.0e-10;   // epsilon, rietkerk_bjork_basic.h:119
+
+struct MoveSpecies {
+    int rD, rV, W, offD, offV;
+    double pref;
+};
+struct MoveArgs {
+    MoveSpecies sp[3];
+    int L;
+};
+
+// x - m (both in [0, L)) as its representative in (-L/2, L/2]: the one offset that
+// matters while every reach stays below L/2.
+__device__ __forceinline__ int signed_gap(int x, int m, int L)
+{
+    int e = x - m;
+    if (2 * e > L) e -= L;
+    if (2 * e <= -L) e += L;
+    return e;
+}
+
+__device__ __forceinline__ int wrapi(int v, int L)
+{
+    v %= L;
+    return v < 0 ? v + L : v;
+}
+
+__global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double* __restrict__ rho,
+                                                       const double* __restrict__ gamma,
+                                                       const double2* __restrict__ veff,
+                                                       const double* __restrict__ SDall,
+                                                       const double* __restrict__ SVall, double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int s = blockIdx.y;
+    const int L = a.L;
+    const int64_t L2 = (int64_t)L * L;
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= L2) return;
+    const MoveSpecies sp = a.sp[s];
+    const double* SD = SDall + sp.offD;
+    const double* SV = SVall + sp.offV;
+    const double* rs = rho + s * L2;
+    const double* gs = gamma + s * L2;
+    const double2* vs = veff + s * L2;
+    const int x = (int)(o / L), y = (int)(o % L);
+    double acc = 0.0;
+    for (int dr = -sp.W; dr <= sp.W; ++dr) {
+        const int ci = wrapi(x - dr, L);
+        for (int dc = -sp.W; dc <= sp.W; ++dc) {
+            const int cj = wrapi(y - dc, L);
+            const int64_t i = (int64_t)ci * L + cj;
+            const double r = rs[i];
+            if (r < kMoveEps) continue;
+            const double2 v = vs[i];
+            const int mux = (int)(__builtin_round(ci + v.x * (sp.pref)) + L) % L;
+            const int muy = (int)(__builtin_round(cj + v.y * (sp.pref)) + L) % L;
+            const int ex = signed_gap(x, mux, L), ey = signed_gap(y, muy, L);
+            const int adr = dr < 0 ? -dr : dr, adc = dc < 0 ? -dc : dc;
+            const int aex = ex < 0 ? -ex : ex, aey = ey < 0 ? -ey : ey;
+            const bool rowD = adr <= sp.rD, rowV = aex <= sp.rV, colD = adc <= sp.rD, colV = aey <= sp.rV;
+            if (!((rowD || rowV) && (colD || colV))) continue;
+            const double g = gs[i], g2 = g * g;
+            double t = 0.0;
+            if (rowD && colD) t += (g2) * r * (SD[adr] * SD[adc]);
+            if (rowD && colV) t += (g - g2) * r * (SD[adr] * SV[aey]);
+            if (rowV && colD) t += (g - g2) * r * (SV[aex] * SD[adc]);
+            if (rowV && colV) t += (1 - g) * (1 - g) * r * (SV[aex] * SV[aey]);
+            acc += t;
+        }
+    }
+    out[s * L2 + o] = acc;
+}
+
+// precompute_gaussian (FKE:1735-1749): R + 1 normalised half-stencil values.
+void gauss_stencil(double sigma, int R, double* st)
+{
+    const double pi = 3.141592653589793;
+    double sum = 0.0;
+    const double factor = 1.0 / (std::sqrt(2 * pi) * sigma);
+    for (int x = 0; x <= R; ++x) {
+        st[x] = factor * std::exp(-0.5 * (x * x) / (sigma * sigma));
+        sum += (x > 0) ? 2 * st[x] : st[0];
+    }
+    for (int x = 0; x <= R; ++x) st[x] /= sum;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int tm_fke_move_constants(double D, double v, double sigma, double dt_analytic, double dt_adv, int mv_invariance,
+                          double dx, int CI, tm_fke_move_species* out, double* stencil_D, int cap_D,
+                          double* stencil_V, int cap_V)
+{
+    if (!out || !stencil_D || !stencil_V || !(dx > 0.0) || !(dt_analytic > 0.0) || !(sigma > 0.0)) return -1;
+    if (mv_invariance == 1 && !(dt_adv > 0.0)) return -1;
+    // set_global_FKE_params, FKE:222-250
+    const double Dt_ay = dt_analytic;
+    const double beta_vel = mv_invariance == 1 ? 3.0 / dt_adv : 3.0 / dt_analytic;
+    const double exp_beta_Dt = std::exp(-beta_vel * dt_analytic);
+    const double exp_2beta_Dt = std::exp(-2 * beta_vel * dt_analytic);
+    const double sigma_v = std::sqrt(2 * beta_vel * 1 * std::fabs(v));
+    const double sigma_D = std::sqrt(2 * D * dt_analytic);
+    const double sigma_vx = (sigma_v / beta_vel) * std::sqrt((Dt_ay + (1 / (2 * beta_vel)) * (1 - exp_2beta_Dt)
+                                                               - (2 / beta_vel) * (1 - exp_beta_Dt)));
+    out->rD = (int)(CI * sigma_D / dx);
+    out->rV = (int)(CI * sigma_vx / dx);
+    out->mu_prefactor = (1 / dx) * (Dt_ay - ((1 - exp_beta_Dt) / beta_vel));
+    if (out->rD < 0 || out->rV < 0 || out->rD + 1 > cap_D || out->rV + 1 > cap_V) return -1;
+    gauss_stencil(sigma, out->rD, stencil_D);
+    gauss_stencil(sigma, out->rV, stencil_V);
+    return 0;
+}
+
+int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* sp, const double* stencil_D,
+                         const double* stencil_V, const double* rho, const double* gamma, const double* v_eff,
+                         double* out)
+{
+    if (ns < 1 || ns > 3 || L < 1 || !sp || !stencil_D || !stencil_V || !rho || !gamma || !v_eff || !out)
+        return TM_ERR_ARG;
+    const int64_t L2 = (int64_t)L * L;
+    if (L2 > 0x7FFFFFFFll) return TM_ERR_ARG;
+    MoveArgs a{};
+    a.L = L;
+    int offD = 0, offV = 0;
+    for (int s = 0; s < ns; ++s) {
+        if (sp[s].rD < 0 || sp[s].rV < 0) return TM_ERR_ARG;
+        double vmax = 0.0;   // the largest shift any source of the species can make
+        for (int64_t i = 0; i < L2; ++i) {
+            const double vx = std::fabs(v_eff[2 * (s * L2 + i)]), vy = std::fabs(v_eff[2 * (s * L2 + i) + 1]);
+            if (!(vx <= 1e300) || !(vy <= 1e300)) return TM_ERR_ARG;   // NaN or inf velocity
+            vmax = std::max(vmax, std::max(vx, vy));
+        }
+        const double shift = vmax * std::fabs(sp[s].mu_prefactor);
+        if (!(shift < (double)L)) return TM_ERR_ARG;
+        const int S = (int)std::floor(shift + 0.5) + 1;
+        MoveSpecies& m = a.sp[s];
+        m.rD = sp[s].rD;
+        m.rV = sp[s].rV;
+        m.pref = sp[s].mu_prefactor;
+        m.W = std::max(m.rD, m.rV + S);
+        m.offD = offD;
+        m.offV = offV;
+        if (2 * m.W + 1 > L) return TM_ERR_ARG;   // reaches must not wrap onto themselves
+        offD += m.rD + 1;
+        offV += m.rV + 1;
+    }
+    if (hipSetDevice(device) != hipSuccess) return TM_ERR_NODEV;
+    DevBuf d_rho, d_gam, d_v, d_out, d_sd, d_sv;
+    const size_t fb = sizeof(double) * (size_t)ns * L2;
+    if (hipMalloc(&d_rho.p, fb) != hipSuccess || hipMalloc(&d_gam.p, fb) != hipSuccess ||
+        hipMalloc(&d_v.p, 2 * fb) != hipSuccess || hipMalloc(&d_out.p, fb) != hipSuccess ||
+        hipMalloc(&d_sd.p, sizeof(double) * offD) != hipSuccess || hipMalloc(&d_sv.p, sizeof(double) * offV) != hipSuccess)
+        return TM_ERR_NOMEM;
+    if (hipMemcpy(d_rho.p, rho, fb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_gam.p, gamma, fb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_v.p, v_eff, 2 * fb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_sd.p, stencil_D, sizeof(double) * offD, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_sv.p, stencil_V, sizeof(double) * offV, hipMemcpyHostToDevice) != hipSuccess)
+        return TM_ERR_HIP;
+    const dim3 grid((unsigned)((L2 + 255) / 256), (unsigned)ns);
+    hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, nullptr, a, (const double*)d_rho.p,
+                       (const double*)d_gam.p, (const double2*)d_v.p, (const double*)d_sd.p,
+                       (const double*)d_sv.p, (double*)d_out.p);
+    if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
+    if (hipMemcpy(out, d_out.p, fb, hipMemcpyDeviceToHost) != hipSuccess) return TM_ERR_HIP;
+    return TM_OK;
+}
+
+}  // extern "C"

@@ -362,6 +362,17 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
                          const double* stencil_V, const double* rho, const double* gamma, const double* v_eff,
                          double* out);
 
+/* The same step on device arrays (hipMalloc'd by the caller, same layouts), on
+ * `stream` (a hipStream_t, or null): the drop-in for launchAdvDiffKernel_MultiSp
+ * (kernel_advdiff_FKE2D.cu:224-236) with the constants passed in place of the
+ * reference's __constant__ copies (copyToDeviceConstantMemory_AdvTerms, :20-86).
+ * out is overwritten, so the reference's cudaMemset of Rho_tsar is not needed;
+ * out must not alias rho.  A first small kernel finds the largest speed, which
+ * the host reads back (one stream synchronisation) to size the gather window.
+ * Returns 0 or a negative tm_status. */
+int tm_fke_movement(int ns, int L, const tm_fke_move_species* sp, const double* stencil_D, const double* stencil_V,
+                    const double* rho, const double* gamma, const double* v_eff, double* out, void* stream);
+
 /* Host only.  The perception neighbourhood of the consumer models: offsets
  * (row, column) of the sites within Euclidean radius R of a site, in the order of
  * computeNeighboringSitesCentral (multiSPDP.cpp:1442-1461): generated row-major

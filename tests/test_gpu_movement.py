@@ -40,3 +40,46 @@ def test_movement_refuses_reaches_beyond_the_lattice(tm, oracle):
     z = np.zeros((1, L, L))
     with pytest.raises(tm.TMError):
         tm.fke_movement(z + 1.0, z, np.zeros((1, L, L, 2)), [(rD, rV, pref)], [SD], [SV])
+
+
+def test_device_entry_point_equals_host_entry_point(tm, oracle):
+    """tm_fke_movement on device arrays (the drop-in for launchAdvDiffKernel_MultiSp)
+    gives the host entry point's result bit for bit.  Device memory comes from the
+    HIP runtime the library itself links."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    H2D, D2H = 1, 2
+    L = 40
+    consts = [_species(oracle, 0.5, 0.9, 0.8, 0.5, 0.25)]
+    r, g, vx, vy = _case(21, L)
+    rho, gam = r[None].copy(), g[None].copy()
+    v = np.ascontiguousarray(np.stack([vx, vy], axis=-1)[None])
+    want = tm.fke_movement(rho, gam, v, [c[0] for c in consts], [c[1] for c in consts], [c[2] for c in consts])
+    host = dict(rho=rho, gam=gam, v=v, sd=np.ascontiguousarray(consts[0][1]), sv=np.ascontiguousarray(consts[0][2]),
+                out=np.zeros_like(rho))
+    dev = {}
+    try:
+        for k, x in host.items():
+            p = C.c_void_p()
+            assert hip.hipMalloc(C.byref(p), x.nbytes) == 0
+            dev[k] = p
+            assert hip.hipMemcpy(p, x.ctypes.data, x.nbytes, H2D) == 0
+        sp = (tm._MoveSpecies * 1)(tm._MoveSpecies(*consts[0][0]))
+        print("movement device entry", flush=True)
+
+        def run():
+            return tm.lib().tm_fke_movement(1, L, C.cast(sp, C.c_void_p), dev["sd"], dev["sv"], dev["rho"],
+                                            dev["gam"], dev["v"], dev["out"], None)
+        assert run() == 0
+        got = np.empty_like(rho)
+        assert hip.hipMemcpy(got.ctypes.data, dev["out"], got.nbytes, D2H) == 0
+        assert np.array_equal(got, want)
+        v[0, 3, 4, 0] = float("nan")                 # a NaN velocity is refused
+        assert hip.hipMemcpy(dev["v"], v.ctypes.data, v.nbytes, H2D) == 0
+        assert run() < 0
+    finally:
+        for p in dev.values():
+            hip.hipFree(p)

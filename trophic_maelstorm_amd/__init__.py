@@ -285,6 +285,7 @@ def lib() -> C.CDLL:
         "tm_fke_move_constants": ([C.c_double] * 5 + [C.c_int, C.c_double, C.c_int, vp, dp, C.c_int, dp, C.c_int],
                                   C.c_int),
         "tm_fke_movement_host": ([C.c_int, C.c_int, C.c_int, vp, dp, dp, dp, dp, dp, dp], C.c_int),
+        "tm_fke_movement": ([C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -98,6 +98,59 @@ __global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double*
     out[s * L2 + o] = acc;
 }
 
+// The largest |vx|, |vy| of each species (one workgroup per species); a NaN or
+// infinite velocity gives +inf, which the host refuses.
+__global__ __launch_bounds__(256) void vmax_kernel(const double2* __restrict__ veff, int64_t L2, double* out)
+{
+    __shared__ double red[256];
+    const int s = blockIdx.x;
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < L2; i += 256) {
+        const double2 v = veff[s * L2 + i];
+        const double a = __builtin_fabs(v.x), b = __builtin_fabs(v.y);
+        double c = a > b ? a : b;
+        if (!(a <= 1e300) || !(b <= 1e300)) c = __builtin_inf();   // NaN or infinite
+        m = c > m ? c : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + o] ? red[threadIdx.x]
+                                                                                         : red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = red[0];
+}
+
+// Kernel arguments of ns species from their constants and largest speeds; the
+// stencils' offsets in the concatenated arrays.  TM_ERR_ARG when a reach does
+// not fit the lattice.
+int move_args(int ns, int L, const tm_fke_move_species* sp, const double* vmax, MoveArgs* a, int* nD, int* nV)
+{
+    *a = MoveArgs{};
+    a->L = L;
+    int offD = 0, offV = 0;
+    for (int s = 0; s < ns; ++s) {
+        if (sp[s].rD < 0 || sp[s].rV < 0) return TM_ERR_ARG;
+        const double shift = vmax[s] * std::fabs(sp[s].mu_prefactor);   // the largest shift of any source
+        if (!(shift < (double)L)) return TM_ERR_ARG;                    // NaN and inf included
+        const int S = (int)std::floor(shift + 0.5) + 1;
+        MoveSpecies& m = a->sp[s];
+        m.rD = sp[s].rD;
+        m.rV = sp[s].rV;
+        m.pref = sp[s].mu_prefactor;
+        m.W = std::max(m.rD, m.rV + S);
+        m.offD = offD;
+        m.offV = offV;
+        if (2 * m.W + 1 > L) return TM_ERR_ARG;   // reaches must not wrap onto themselves
+        offD += m.rD + 1;
+        offV += m.rV + 1;
+    }
+    *nD = offD;
+    *nV = offV;
+    return TM_OK;
+}
+
 // precompute_gaussian (FKE:1735-1749): R + 1 normalised half-stencil values.
 void gauss_stencil(double sigma, int R, double* st)
 {
@@ -147,6 +200,31 @@ int tm_fke_move_constants(double D, double v, double sigma, double dt_analytic, 
     return 0;
 }
 
+int tm_fke_movement(int ns, int L, const tm_fke_move_species* sp, const double* stencil_D, const double* stencil_V,
+                    const double* rho, const double* gamma, const double* v_eff, double* out, void* stream)
+{
+    if (ns < 1 || ns > 3 || L < 1 || !sp || !stencil_D || !stencil_V || !rho || !gamma || !v_eff || !out)
+        return TM_ERR_ARG;
+    const int64_t L2 = (int64_t)L * L;
+    if (L2 > 0x7FFFFFFFll) return TM_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf d_vmax;
+    if (hipMalloc(&d_vmax.p, sizeof(double) * ns) != hipSuccess) return TM_ERR_NOMEM;
+    hipLaunchKernelGGL(vmax_kernel, dim3(ns), dim3(256), 0, st, (const double2*)v_eff, L2, (double*)d_vmax.p);
+    double vmax[3] = {0.0, 0.0, 0.0};
+    if (hipMemcpyAsync(vmax, d_vmax.p, sizeof(double) * ns, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return TM_ERR_HIP;
+    MoveArgs a;
+    int nD, nV;
+    const int rc = move_args(ns, L, sp, vmax, &a, &nD, &nV);
+    if (rc != TM_OK) return rc;
+    const dim3 grid((unsigned)((L2 + 255) / 256), (unsigned)ns);
+    hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, st, a, rho, gamma, (const double2*)v_eff, stencil_D,
+                       stencil_V, out);
+    return hipGetLastError() == hipSuccess ? TM_OK : TM_ERR_HIP;
+}
+
 int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* sp, const double* stencil_D,
                          const double* stencil_V, const double* rho, const double* gamma, const double* v_eff,
                          double* out)
@@ -155,43 +233,29 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
         return TM_ERR_ARG;
     const int64_t L2 = (int64_t)L * L;
     if (L2 > 0x7FFFFFFFll) return TM_ERR_ARG;
-    MoveArgs a{};
-    a.L = L;
-    int offD = 0, offV = 0;
-    for (int s = 0; s < ns; ++s) {
-        if (sp[s].rD < 0 || sp[s].rV < 0) return TM_ERR_ARG;
-        double vmax = 0.0;   // the largest shift any source of the species can make
+    double vmax[3] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < ns; ++s)
         for (int64_t i = 0; i < L2; ++i) {
             const double vx = std::fabs(v_eff[2 * (s * L2 + i)]), vy = std::fabs(v_eff[2 * (s * L2 + i) + 1]);
             if (!(vx <= 1e300) || !(vy <= 1e300)) return TM_ERR_ARG;   // NaN or inf velocity
-            vmax = std::max(vmax, std::max(vx, vy));
+            vmax[s] = std::max(vmax[s], std::max(vx, vy));
         }
-        const double shift = vmax * std::fabs(sp[s].mu_prefactor);
-        if (!(shift < (double)L)) return TM_ERR_ARG;
-        const int S = (int)std::floor(shift + 0.5) + 1;
-        MoveSpecies& m = a.sp[s];
-        m.rD = sp[s].rD;
-        m.rV = sp[s].rV;
-        m.pref = sp[s].mu_prefactor;
-        m.W = std::max(m.rD, m.rV + S);
-        m.offD = offD;
-        m.offV = offV;
-        if (2 * m.W + 1 > L) return TM_ERR_ARG;   // reaches must not wrap onto themselves
-        offD += m.rD + 1;
-        offV += m.rV + 1;
-    }
+    MoveArgs a;
+    int nD, nV;
+    int rc = move_args(ns, L, sp, vmax, &a, &nD, &nV);
+    if (rc != TM_OK) return rc;
     if (hipSetDevice(device) != hipSuccess) return TM_ERR_NODEV;
     DevBuf d_rho, d_gam, d_v, d_out, d_sd, d_sv;
     const size_t fb = sizeof(double) * (size_t)ns * L2;
     if (hipMalloc(&d_rho.p, fb) != hipSuccess || hipMalloc(&d_gam.p, fb) != hipSuccess ||
         hipMalloc(&d_v.p, 2 * fb) != hipSuccess || hipMalloc(&d_out.p, fb) != hipSuccess ||
-        hipMalloc(&d_sd.p, sizeof(double) * offD) != hipSuccess || hipMalloc(&d_sv.p, sizeof(double) * offV) != hipSuccess)
+        hipMalloc(&d_sd.p, sizeof(double) * nD) != hipSuccess || hipMalloc(&d_sv.p, sizeof(double) * nV) != hipSuccess)
         return TM_ERR_NOMEM;
     if (hipMemcpy(d_rho.p, rho, fb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_gam.p, gamma, fb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_v.p, v_eff, 2 * fb, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_sd.p, stencil_D, sizeof(double) * offD, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_sv.p, stencil_V, sizeof(double) * offV, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(d_sd.p, stencil_D, sizeof(double) * nD, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_sv.p, stencil_V, sizeof(double) * nV, hipMemcpyHostToDevice) != hipSuccess)
         return TM_ERR_HIP;
     const dim3 grid((unsigned)((L2 + 255) / 256), (unsigned)ns);
     hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, nullptr, a, (const double*)d_rho.p,

@@ -65,17 +65,16 @@ def test_consumers_bit_exact_mixed_batch(tm, oracle, model):
 
 
 @pytest.mark.parametrize("model", ["DP2", "DP3"])
-@pytest.mark.parametrize("variant_index", [0, 1])
-def test_consumers_variants_and_eager(tm, oracle, model, variant_index):
+def test_consumers_every_variant_eager_and_graph(tm, oracle, model):
     m = getattr(tm, "MODEL_" + model)
     ns = tm.NSPECIES[m]
-    name = tm.variant_names(m, tm.F64)[variant_index]
     Ls = [29, 40]
     insts = [tm.Instance(L, 0, 3, k) for k, L in enumerate(Ls)]
     rng = np.random.default_rng(11)
     init = [_initial(rng, ns, L) for L in Ls]
-    print(f"consumers {model} {name}", flush=True)
-    _run_both(tm, oracle, m, [params()], insts, init, 9, variant=name, graph=0 if variant_index else 4)
+    for k, name in enumerate(tm.variant_names(m, tm.F64)):
+        print(f"consumers {model} {name}", flush=True)
+        _run_both(tm, oracle, m, [params()], insts, init, 9, variant=name, graph=0 if k % 2 else 4)
 
 
 @pytest.mark.parametrize("model", ["DP2", "DP3"])

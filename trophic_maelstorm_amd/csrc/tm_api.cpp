@@ -1499,8 +1499,11 @@ namespace tmk {
 int default_variant(int model, int precision, int64_t total_cells)
 {
     const char* want;
-    if (is_consumer_model(model)) return 0;
-    if (model == TM_MODEL_DP) {
+    if (model == TM_MODEL_DP2) {
+        want = "dpc2_f64_32x16_t256_w5";   // measured fastest on the consumer sweep batch
+    } else if (model == TM_MODEL_DP3) {
+        want = "dpc3_f64_32x16_t256_w4";
+    } else if (model == TM_MODEL_DP) {
         // adaptive eight-step launches: single steps while more than 2% of the
         // cells are occupied, eight steps per launch once the field is mostly absorbing
         want = precision == TM_F64 ? "dp_f64_64x32_t256_x8a" : "dp_f32_128x24_t256_x8a";

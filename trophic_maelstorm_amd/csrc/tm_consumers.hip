@@ -142,8 +142,8 @@ __device__ __forceinline__ void dpc_rk4(const tm_dpc_params& p, double* y)
     for (int s = 0; s < NS; ++s) y[s] += (dt6) * (k1[s] + 2.0 * k2[s] + 2.0 * k3[s] + k4[s]);
 }
 
-template <int NS, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void dpc_step_kernel(StepArgs a)
+template <int NS, int TX, int TY, int NT, int MINW>
+__global__ __launch_bounds__(NT, MINW) void dpc_step_kernel(StepArgs a)
 {
 #pragma clang fp contract(off)
     constexpr int NF = 2 * NS - 1;
@@ -294,29 +294,35 @@ __global__ __launch_bounds__(NT) void dpc_step_kernel(StepArgs a)
     block_partials<NT, NF>(sum, cnt, a.partials + (slot * a.ntiles + tile) * NF, s_red);
 }
 
-template <int NS, int TX, int TY, int NT>
+template <int NS, int TX, int TY, int NT, int MINW = 1>
 void launch_dpc(const StepArgs& a, int grid, hipStream_t s)
 {
     const size_t lds = dpc_lds_bytes(NS, TX, TY, a.halo);
     if (lds > 65536) {   // above the default dynamic-LDS limit (CDNA4 has 160 KiB per CU)
         static size_t granted = 0;
         if (lds > granted &&
-            hipFuncSetAttribute((const void*)dpc_step_kernel<NS, TX, TY, NT>,
+            hipFuncSetAttribute((const void*)dpc_step_kernel<NS, TX, TY, NT, MINW>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess)
             granted = lds;
     }
     hipLaunchKernelGGL(dpc_chunk_kernel<NS>, dim3(a.nchunks), dim3(256), 0, s, a);
     hipLaunchKernelGGL(dpc_state_kernel<NS>, dim3((a.n_inst + 63) / 64), dim3(64), 0, s, a);
-    hipLaunchKernelGGL((dpc_step_kernel<NS, TX, TY, NT>), dim3(grid), dim3(NT), lds, s, a);
+    hipLaunchKernelGGL((dpc_step_kernel<NS, TX, TY, NT, MINW>), dim3(grid), dim3(NT), lds, s, a);
 }
 
+// _wN: at most 512/N VGPRs, N waves per SIMD
 const Variant kDPC2[] = {
     {32, 16, 256, 0, 1, "dpc2_f64_32x16_t256", &launch_dpc<2, 32, 16, 256>, 0, 0, nullptr, 1},
     {64, 8, 256, 0, 1, "dpc2_f64_64x8_t256", &launch_dpc<2, 64, 8, 256>, 0, 0, nullptr, 1},
+    {32, 16, 256, 0, 1, "dpc2_f64_32x16_t256_w4", &launch_dpc<2, 32, 16, 256, 4>, 0, 0, nullptr, 1},
+    {32, 16, 256, 0, 1, "dpc2_f64_32x16_t256_w5", &launch_dpc<2, 32, 16, 256, 5>, 0, 0, nullptr, 1},
+    {32, 16, 256, 0, 1, "dpc2_f64_32x16_t256_w6", &launch_dpc<2, 32, 16, 256, 6>, 0, 0, nullptr, 1},
 };
 const Variant kDPC3[] = {
     {32, 16, 256, 0, 1, "dpc3_f64_32x16_t256", &launch_dpc<3, 32, 16, 256>, 0, 0, nullptr, 1},
     {64, 8, 256, 0, 1, "dpc3_f64_64x8_t256", &launch_dpc<3, 64, 8, 256>, 0, 0, nullptr, 1},
+    {32, 16, 256, 0, 1, "dpc3_f64_32x16_t256_w4", &launch_dpc<3, 32, 16, 256, 4>, 0, 0, nullptr, 1},
+    {32, 16, 256, 0, 1, "dpc3_f64_32x16_t256_w5", &launch_dpc<3, 32, 16, 256, 5>, 0, 0, nullptr, 1},
 };
 
 }  // namespace

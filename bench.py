@@ -103,6 +103,29 @@ def c6_batch(tm, model):
                 nf=tm.NFIELDS[model], chunk=500)
 
 
+def movement_leg(tm, L=1024, reps=20):
+    """The Rietkerk consumers' movement step (two species, 1024^2, the grazer and
+    predator values of the reference's 3-species driver: D = 0.0298 and 0.05221
+    km^2/hr, v = 0.45427 and 0.40587 km/hr, sigma = 1 and 3, dt = 0.1 hr, dx = 0.1
+    km) through the host-array entry point: wall time per call, PCIe copies
+    included; the kernel's own time is in the rocprof trace."""
+    rng = np.random.default_rng(SEED)
+    consts = [tm.fke_move_constants(0.0298, 0.45427, 1.0, 0.1, 0.1), tm.fke_move_constants(0.05221, 0.40587, 3.0, 0.1, 0.1)]
+    rho = rng.uniform(0, 2, (2, L, L)) * (rng.random((2, L, L)) < 0.5)
+    gam = rng.random((2, L, L))
+    th = 2 * np.pi * rng.random((2, L, L))
+    speed = np.array([0.45427, 0.40587])[:, None, None]
+    v = np.stack([speed * np.cos(th), speed * np.sin(th)], axis=-1)
+    args = ([c[0] for c in consts], [c[1] for c in consts], [c[2] for c in consts])
+    tm.fke_movement(rho, gam, v, *args)       # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tm.fke_movement(rho, gam, v, *args)
+    wall = (time.perf_counter() - t0) / reps
+    return {"cells": 2 * L * L, "calls": reps, "ms_per_call": wall * 1e3, "species": [c[0] for c in consts],
+            "cell_moves_per_s": 2 * L * L / wall}
+
+
 def cells_of(b):
     return sum(i.L * i.L for i in b["inst"])
 
@@ -348,7 +371,8 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="headline only")
     ap.add_argument("--variant", default=None,
                     help="kernel variant for the headline (or --only) run instead of the autotuner's choice")
-    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4-f64", "c5", "sweep", "dp2", "dp3"], default=None,
+    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4-f64", "c5", "sweep", "dp2", "dp3", "movement"],
+                    default=None,
                     help="only this side workload, --steps steps (default: its full length), one JSON line")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
@@ -374,6 +398,9 @@ def main():
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    if args.only == "movement":
+        print(json.dumps({"workload": "fke_movement", **movement_leg(tm)}), flush=True)
+        return
     if args.only == "sweep":
         print(json.dumps({"workload": "c3_sweep", **sweep_leg(tm, args.steps)}), flush=True)
         return

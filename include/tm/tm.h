@@ -373,6 +373,19 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
 int tm_fke_movement(int ns, int L, const tm_fke_move_species* sp, const double* stencil_D, const double* stencil_V,
                     const double* rho, const double* gamma, const double* v_eff, double* out, void* stream);
 
+/* Perception gamma and velocities of the Rietkerk model's consumers, one step:
+ * calc_gamma_vel_2Sp_NonRefugia (rietkerk_bjork_FKE.cpp:2534-2606) for ns = 2 and
+ * calc_gamma_vel_3Sp_NonRefugia (:3371-3482) for ns = 3, on `device`, host arrays.
+ * rho [ns][L*L]: the old densities (vegetation, grazer, predator); gamma [ns-1][L*L]:
+ * the consumers' previous gamma, updated in place (both forms add onto it);
+ * v_eff [ns][L*L][2]: with ns = 3 every species draws v[s] (cos, sin)(2 pi u), u =
+ * word 0 of Philox(seed, inst; cell, step, block (s << 24) | 0x800000); with ns = 2 it
+ * is left as it is (the reference draws none).  pR [ns] and dx set the perception
+ * radius as in tm_dpc_params; an extinct consumer gets gamma 1.  The spatial means
+ * use the consumer models' fixed-order sum.  Returns 0 or a negative tm_status. */
+int tm_fke_perception_host(int device, int ns, int L, const double* pR, double dx, const double* v, uint32_t seed,
+                           uint32_t inst, int64_t step, const double* rho, double* gamma, double* v_eff);
+
 /* Host only.  The perception neighbourhood of the consumer models: offsets
  * (row, column) of the sites within Euclidean radius R of a site, in the order of
  * computeNeighboringSitesCentral (multiSPDP.cpp:1442-1461): generated row-major

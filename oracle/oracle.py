@@ -121,6 +121,8 @@ def lib():
                                                                  C.c_int, _f64p, C.c_int]
         L.tmo_fke_movement.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_double, _f64p,
                                        _f64p, _f64p]
+        L.tmo_fke_perception.argtypes = [C.c_int, C.c_int, _f64p, C.c_double, _f64p, C.c_void_p, C.c_int, C.c_uint32,
+                                         C.c_uint32, C.c_int64, _f64p, _f64p, _f64p]
         L.tmo_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tmo_fixed_sum.restype = C.c_double
         L.tmo_fixed_sum.argtypes = [_f64p, C.c_int64]
@@ -312,6 +314,20 @@ def fke_movement(rho, gamma, vx, vy, rD, rV, pref, SD, SV):
                            np.ascontiguousarray(vy, np.float64).reshape(-1), rD, rV, pref,
                            np.ascontiguousarray(SD, np.float64), np.ascontiguousarray(SV, np.float64), out)
     return out.reshape(L, L)
+
+
+def fke_perception(rho, gamma, v_eff, pR, dx, v, seed=1, inst=0, step=0):
+    """tmo_fke_perception: returns (gamma, v_eff) updated copies."""
+    r = np.ascontiguousarray(rho, np.float64)
+    ns, L = r.shape[0], r.shape[-1]
+    rmax = max(pR[:ns]) / dx
+    disk = perception_offsets(int(rmax))
+    g = np.ascontiguousarray(gamma, np.float64).copy()
+    ve = np.ascontiguousarray(v_eff, np.float64).copy()
+    lib().tmo_fke_perception(ns, L, np.ascontiguousarray(list(pR)[:ns], np.float64), dx,
+                             np.ascontiguousarray(list(v)[:ns], np.float64), disk.ctypes.data, disk.shape[0],
+                             seed, inst, step, r.reshape(-1), g.reshape(-1), ve.reshape(-1))
+    return g, ve
 
 
 def ref_perception_sites():

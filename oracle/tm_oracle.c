@@ -1075,3 +1075,68 @@ void tmo_fke_movement(int L, const double* rho, const double* gamma, const doubl
         }
     }
 }
+
+/* Perception and velocities of the Rietkerk model's consumers:
+ * calc_gamma_vel_2Sp_NonRefugia (FKE:2534-2606) and calc_gamma_vel_3Sp_NonRefugia
+ * (FKE:3371-3482), from the old densities rho[ns][L*L] (vegetation first).
+ * gamma[ns-1][L*L] (consumers 1 .. ns-1) is updated in place; both forms add onto
+ * the previous gamma.  With three species every species s (vegetation
+ * included) draws a velocity v[s] (cos, sin)(2 pi u) per cell, u = word 0 of
+ * Philox(seed, inst; cell, step, block (s << 24) | 0x800000), into veff[s][cell];
+ * the two-species form draws none.  Spatial means by tmo_fixed_sum; the cover
+ * factor and neighbourhood counts as in the DP consumer step, with the
+ * three-species form's floor of one (FKE:3386, :3416). */
+void tmo_fke_perception(int ns, int L, const double* pR, double dx, const double* v, const int32_t* disk,
+                        int disk_len, uint32_t seed, uint32_t inst, int64_t step, const double* rho,
+                        double* gamma, double* veff)
+{
+    const int64_t N = (int64_t)L * L;
+    const double eps_g = 1.0e-12;
+    double mean[3], inv[3], fr[3];
+    for (int s = 0; s < ns; ++s) mean[s] = tmo_fixed_sum(rho + s * N, N) / (double)N;
+    double occ = 0.0;
+    for (int64_t i = 0; i < N; ++i) if (rho[i] > 0) occ += 1;
+    double nfac = 1 - occ / (double)N;
+    if (nfac < 0.35) nfac = 0.35;
+    int new_len = (int)(nfac * nfac * (double)disk_len);
+    if (ns == 3 && new_len < 1) new_len = 1;
+    for (int s = 0; s < ns; ++s) inv[s] = mean[s] >= eps_g ? 1 / mean[s] : 1 / eps_g;
+    double rmax = pR[0];
+    for (int s = 1; s < ns; ++s) if (pR[s] > rmax) rmax = pR[s];
+    rmax = rmax / dx;
+    for (int s = 0; s < ns; ++s) fr[s] = pR[s] / (dx * rmax);
+    for (int p = 0; p < L; ++p)
+        for (int q = 0; q < L; ++q) {
+            const int64_t i = (int64_t)p * L + q;
+            if (ns == 3)
+                for (int s = 0; s < ns; ++s) {
+                    stream_t st;
+                    st_init(&st, seed, inst, (uint32_t)i, step);
+                    st.ctr[3] = ((uint32_t)s << 24) | 0x800000u;
+                    double sn, cs;
+                    tmo_sincos2pi(st_u(&st), &sn, &cs);
+                    veff[2 * (s * N + i)] = v[s] * cs;
+                    veff[2 * (s * N + i) + 1] = v[s] * sn;
+                }
+            for (int s = 1; s < ns; ++s) {
+                double* g = gamma + (s - 1) * N + i;
+                if (fr[s] == 0.0) continue;
+                int eff = (int)(fr[s] * fr[s] * (double)new_len);
+                if (ns == 3 && eff < 1 && fr[s] > 0) eff = 1;
+                if (mean[s] < eps_g) { *g = 1.0; continue; }
+                if (mean[s - 1] < eps_g) { *g = 0.0; continue; }
+                const double* src = rho + (s - 1) * N;
+                double acc = *g;
+                for (int k = 0; k < eff; ++k) {
+                    const int64_t j = wrapi(p + disk[2 * k], L) * L + wrapi(q + disk[2 * k + 1], L);
+                    if (ns == 3 && s == 1 && !(mean[2] < eps_g)) acc += src[j] * (1 - rho[2 * N + j] * inv[2]);
+                    else acc += src[j];
+                }
+                if (ns == 2 || s == 1) acc = (acc * inv[s - 1]) / (eff);
+                else acc = (acc / (eff)) * inv[s - 1];
+                if (acc > 1) acc = 1.0;
+                else if (acc < 0) acc = 0.0;
+                *g = acc;
+            }
+        }
+}

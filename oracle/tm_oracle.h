@@ -190,6 +190,13 @@ int tmo_fke_move_constants(double D, double v, double sigma, double dt_analytic,
 void tmo_fke_movement(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
                       int rD, int rV, double mu_pref, const double* SD, const double* SV, double* out);
 
+/* Perception gamma and velocities of the Rietkerk consumers (FKE:2534-2606,
+ * :3371-3482); gamma[ns-1][L*L] updated in place, veff[ns][L*L][2] drawn with
+ * three species. */
+void tmo_fke_perception(int ns, int L, const double* pR, double dx, const double* v, const int32_t* disk,
+                        int disk_len, uint32_t seed, uint32_t inst, int64_t step, const double* rho,
+                        double* gamma, double* veff);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,3 +83,18 @@ def test_device_entry_point_equals_host_entry_point(tm, oracle):
     finally:
         for p in dev.values():
             hip.hipFree(p)
+
+
+@pytest.mark.parametrize("ns", [2, 3])
+def test_fke_perception_bit_exact(tm, oracle, ns):
+    L, dx, pR, v = 37, 0.1, (0.0, 0.52, 0.31), (0.1, 0.45, 0.4)
+    rng = np.random.default_rng(40 + ns)
+    rho = np.zeros((ns, L, L))
+    rho[0] = rng.uniform(0, 2, (L, L)) * (rng.random((L, L)) < 0.5)
+    rho[1:] = rng.uniform(0, 0.4, (ns - 1, L, L))
+    gam = rng.uniform(0, 0.1, (ns - 1, L, L))
+    veff = rng.uniform(-1, 1, (ns, L, L, 2))
+    print(f"fke perception ns={ns}", flush=True)
+    g1, v1 = tm.fke_perception(rho, gam, veff, pR, dx, v, seed=9, inst=3, step=17)
+    g2, v2 = oracle.fke_perception(rho, gam, veff, pR, dx, v, seed=9, inst=3, step=17)
+    assert np.array_equal(g1, g2) and np.array_equal(v1, v2)

@@ -108,3 +108,76 @@ def test_mixed_perception_is_the_product_kernel(oracle):
         return k
     want = 1.5 * np.outer(kern(10, 12), kern(20, 19))
     assert np.allclose(out, want, rtol=1e-13, atol=1e-16)
+
+
+# ------------------------------------------------------------------ perception and velocities (FKE)
+def _fke_gamma_brute(pR, dx, ns, rho, gam_old):
+    """calc_gamma_vel_2Sp/3Sp_NonRefugia's gamma, site by site from the reference's
+    statements, with numpy means."""
+    from oracle import oracle as o
+    L = rho.shape[-1]
+    eps = 1e-12
+    rmax = max(pR[:ns]) / dx
+    disk = o.perception_offsets(int(rmax))
+    mean = [rho[s].mean() for s in range(ns)]
+    nfac = max(1 - np.count_nonzero(rho[0] > 0) / (L * L), 0.35)
+    new_len = int(nfac * nfac * len(disk))
+    if ns == 3:
+        new_len = max(new_len, 1)
+    inv = [1 / m if m >= eps else 1 / eps for m in mean]
+    out = gam_old.copy()
+    for s in range(1, ns):
+        fr = pR[s] / (dx * rmax)
+        eff = int(fr * fr * new_len)
+        if ns == 3 and eff < 1 and fr > 0:
+            eff = 1
+        for i in range(L):
+            for j in range(L):
+                if mean[s] < eps:
+                    out[s - 1, i, j] = 1.0
+                    continue
+                if mean[s - 1] < eps:
+                    out[s - 1, i, j] = 0.0
+                    continue
+                nb = [((i + a) % L, (j + b) % L) for a, b in disk[:eff]]
+                g = gam_old[s - 1, i, j]
+                if ns == 3 and s == 1 and mean[2] >= eps:
+                    g += sum(rho[0][k] * (1 - rho[2][k] * inv[2]) for k in nb)
+                else:
+                    g += sum(rho[s - 1][k] for k in nb)
+                g = g * inv[s - 1] / eff if (ns == 2 or s == 1) else g / eff * inv[s - 1]
+                out[s - 1, i, j] = min(max(g, 0.0), 1.0)
+    return out
+
+
+@pytest.mark.parametrize("ns", [2, 3])
+def test_fke_perception_against_brute_force(oracle, ns):
+    L, dx, pR, v = 14, 0.1, (0.0, 0.35, 0.25), (0.1, 0.45, 0.4)
+    rng = np.random.default_rng(30 + ns)
+    rho = np.zeros((ns, L, L))
+    rho[0] = rng.uniform(0, 2, (L, L)) * (rng.random((L, L)) < 0.4)
+    rho[1:] = rng.uniform(0, 0.4, (ns - 1, L, L))
+    gam = rng.uniform(0, 0.1, (ns - 1, L, L))
+    veff = np.full((ns, L, L, 2), 7.0)
+    g, ve = oracle.fke_perception(rho, gam, veff, pR, dx, v, seed=5, inst=2, step=11)
+    assert np.allclose(g, _fke_gamma_brute(pR, dx, ns, rho, gam), rtol=1e-12, atol=1e-15)
+    if ns == 2:
+        assert np.all(ve == 7.0)                     # the two-species form draws no velocity
+    else:
+        for s in range(3):
+            for (i, j) in [(0, 0), (3, 9), (13, 13)]:
+                w = oracle.philox([i * L + j, 11, 0, (s << 24) | 0x800000], [5, 2])[0]
+                ang = 2 * math.pi * ((int(w) + 0.5) * 2.0 ** -32)
+                assert ve[s, i, j, 0] == pytest.approx(v[s] * math.cos(ang), abs=1e-15)
+                assert ve[s, i, j, 1] == pytest.approx(v[s] * math.sin(ang), abs=1e-15)
+
+
+def test_fke_extinct_consumer_has_full_perception(oracle):
+    L = 12
+    rho = np.zeros((2, L, L))
+    rho[0] = 1.0
+    g, _ = oracle.fke_perception(rho, np.full((1, L, L), 0.3), np.zeros((2, L, L, 2)), (0.0, 0.35), 0.1, (0.0, 0.45))
+    assert np.all(g == 1.0)
+    rho[0], rho[1] = 0.0, 0.5                        # no resource: gamma 0
+    g, _ = oracle.fke_perception(rho, np.full((1, L, L), 0.3), np.zeros((2, L, L, 2)), (0.0, 0.35), 0.1, (0.0, 0.45))
+    assert np.all(g == 0.0)

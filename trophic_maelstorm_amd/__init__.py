@@ -219,6 +219,22 @@ def fke_movement(rho, gamma, v_eff, species, stencils_D, stencils_V, device: int
     return out
 
 
+def fke_perception(rho, gamma, v_eff, pR, dx, v, seed=1, inst=0, step=0, device: int = 0):
+    """One step of the Rietkerk consumers' perception (and, with three species,
+    velocity draws) on the GPU: returns (gamma, v_eff), updated copies."""
+    r = np.ascontiguousarray(rho, np.float64)
+    ns, L = r.shape[0], r.shape[-1]
+    g = np.ascontiguousarray(gamma, np.float64).copy()
+    ve = np.ascontiguousarray(v_eff, np.float64).copy()
+    pr = np.ascontiguousarray(list(pR)[:ns], np.float64)
+    vv = np.ascontiguousarray(list(v)[:ns], np.float64)
+    rc = lib().tm_fke_perception_host(device, ns, L, _dptr(pr), dx, _dptr(vv), seed, inst, step, _dptr(r), _dptr(g),
+                                      _dptr(ve))
+    if rc != 0:
+        raise TMError(rc, "tm_fke_perception_host failed")
+    return g, ve
+
+
 @dataclass
 class Instance:
     L: int
@@ -286,6 +302,8 @@ def lib() -> C.CDLL:
                                   C.c_int),
         "tm_fke_movement_host": ([C.c_int, C.c_int, C.c_int, vp, dp, dp, dp, dp, dp, dp], C.c_int),
         "tm_fke_movement": ([C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
+        "tm_fke_perception_host": ([C.c_int, C.c_int, C.c_int, dp, C.c_double, dp, C.c_uint32, C.c_uint32, i64, dp,
+                                    dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "tm/tm.h"
+#include "tm_device.h"
 
 namespace {
 
@@ -151,6 +152,78 @@ int move_args(int ns, int L, const tm_fke_move_species* sp, const double* vmax, 
     return TM_OK;
 }
 
+// Perception gamma and velocities (calc_gamma_vel_2Sp/3Sp_NonRefugia,
+// FKE:2534-2606, :3371-3482): one thread per cell, the perception offsets
+// gathered from the old densities.  Per-species numbers of the step (means,
+// inverses, neighbour counts) come from the host.
+struct PerceptArgs {
+    int ns, L;
+    double v[3], mean[3], inv[3], fr[3];
+    int eff[3];
+    uint32_t seed, inst;
+    int64_t step;
+};
+
+__global__ __launch_bounds__(256) void fke_perception_kernel(PerceptArgs a, const int2* __restrict__ disk,
+                                                             const double* __restrict__ rho,
+                                                             double* __restrict__ gamma, double2* __restrict__ veff)
+{
+#pragma clang fp contract(off)
+    const int L = a.L;
+    const int64_t N = (int64_t)L * L;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const int p = (int)(i / L), q = (int)(i % L);
+    if (a.ns == 3)
+        for (int s = 0; s < 3; ++s) {
+            tmk::Stream st(a.seed, a.inst, (uint32_t)i, a.step);
+            st.blk = ((uint32_t)s << 24) | 0x800000u;
+            double sn, cs;
+            tmk::dsincos2pi(st.u64(), sn, cs);
+            veff[s * N + i] = make_double2(a.v[s] * cs, a.v[s] * sn);
+        }
+    for (int s = 1; s < a.ns; ++s) {
+        double* g = gamma + (s - 1) * N + i;
+        if (a.fr[s] == 0.0) continue;
+        if (a.mean[s] < 1.0e-12) { *g = 1.0; continue; }
+        if (a.mean[s - 1] < 1.0e-12) { *g = 0.0; continue; }
+        const double* src = rho + (s - 1) * N;
+        const bool weigh = a.ns == 3 && s == 1 && !(a.mean[2] < 1.0e-12);
+        double acc = *g;
+        for (int k = 0; k < a.eff[s]; ++k) {
+            const int2 o = disk[k];
+            const int64_t j = (int64_t)wrapi(p + o.x, L) * L + wrapi(q + o.y, L);
+            if (weigh) acc += src[j] * (1 - rho[2 * N + j] * a.inv[2]);
+            else acc += src[j];
+        }
+        if (a.ns == 2 || s == 1) acc = (acc * a.inv[s - 1]) / (double)a.eff[s];
+        else acc = (acc / (double)a.eff[s]) * a.inv[s - 1];
+        if (acc > 1) acc = 1.0;
+        else if (acc < 0) acc = 0.0;
+        *g = acc;
+    }
+}
+
+// The fixed-order sum the consumer models use for their spatial means (chunks
+// of 4096 cells, 256 lane sums folded pairwise, chunk sums in order), so that
+// the same means come out on the host and in the device reduction.
+double fixed_sum(const double* x, int64_t n)
+{
+    double total = 0.0;
+    for (int64_t c0 = 0; c0 < n; c0 += 4096) {
+        double s[256];
+        for (int t = 0; t < 256; ++t) {
+            double a = 0.0;
+            for (int64_t i = c0 + t; i < c0 + 4096 && i < n; i += 256) a += x[i];
+            s[t] = a;
+        }
+        for (int o = 128; o > 0; o >>= 1)
+            for (int t = 0; t < o; ++t) s[t] += s[t + o];
+        total += s[0];
+    }
+    return total;
+}
+
 // precompute_gaussian (FKE:1735-1749): R + 1 normalised half-stencil values.
 void gauss_stencil(double sigma, int R, double* st)
 {
@@ -263,6 +336,61 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
                        (const double*)d_sv.p, (double*)d_out.p);
     if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
     if (hipMemcpy(out, d_out.p, fb, hipMemcpyDeviceToHost) != hipSuccess) return TM_ERR_HIP;
+    return TM_OK;
+}
+
+int tm_fke_perception_host(int device, int ns, int L, const double* pR, double dx, const double* v, uint32_t seed,
+                           uint32_t inst, int64_t step, const double* rho, double* gamma, double* v_eff)
+{
+    if ((ns != 2 && ns != 3) || L < 1 || !pR || !v || !rho || !gamma || !v_eff || !(dx > 0.0)) return TM_ERR_ARG;
+    const int64_t N = (int64_t)L * L;
+    if (N > 0x7FFFFFFFll) return TM_ERR_ARG;
+    double rmax = pR[0];
+    for (int s = 1; s < ns; ++s) rmax = pR[s] > rmax ? pR[s] : rmax;
+    rmax = rmax / dx;
+    if (!(rmax > 0.0) || !(rmax < TM_DPC_MAX_RADIUS + 1.0) || (int)rmax >= L) return TM_ERR_ARG;
+    const int R = (int)rmax;
+    std::vector<int32_t> off(2 * (size_t)(2 * R + 1) * (2 * R + 1));
+    const int disk_len = tm_perception_offsets(R, off.data(), (int)off.size() / 2);
+    if (disk_len < 1) return TM_ERR_ARG;
+    PerceptArgs a{};
+    a.ns = ns;
+    a.L = L;
+    a.seed = seed;
+    a.inst = inst;
+    a.step = step;
+    double occ = 0.0;
+    for (int64_t i = 0; i < N; ++i) occ += rho[i] > 0 ? 1.0 : 0.0;
+    double nfac = 1 - occ / (double)N;                 // FKE:4109-4112
+    if (nfac < 0.35) nfac = 0.35;
+    int new_len = (int)(nfac * nfac * (double)disk_len);
+    if (ns == 3 && new_len < 1) new_len = 1;           // FKE:3386
+    for (int s = 0; s < ns; ++s) {
+        a.v[s] = v[s];
+        a.mean[s] = fixed_sum(rho + s * N, N) / (double)N;
+        a.inv[s] = a.mean[s] >= 1.0e-12 ? 1 / a.mean[s] : 1 / 1.0e-12;
+        a.fr[s] = pR[s] / (dx * rmax);
+        int e = (int)(a.fr[s] * a.fr[s] * (double)new_len);
+        if (ns == 3 && e < 1 && a.fr[s] > 0) e = 1;     // FKE:3415-3416
+        a.eff[s] = e;
+    }
+    if (hipSetDevice(device) != hipSuccess) return TM_ERR_NODEV;
+    DevBuf d_rho, d_gam, d_v, d_disk;
+    const size_t fb = sizeof(double) * (size_t)N;
+    if (hipMalloc(&d_rho.p, fb * ns) != hipSuccess || hipMalloc(&d_gam.p, fb * (ns - 1)) != hipSuccess ||
+        hipMalloc(&d_v.p, 2 * fb * ns) != hipSuccess || hipMalloc(&d_disk.p, sizeof(int2) * disk_len) != hipSuccess)
+        return TM_ERR_NOMEM;
+    if (hipMemcpy(d_rho.p, rho, fb * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_gam.p, gamma, fb * (ns - 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_v.p, v_eff, 2 * fb * ns, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_disk.p, off.data(), sizeof(int2) * disk_len, hipMemcpyHostToDevice) != hipSuccess)
+        return TM_ERR_HIP;
+    hipLaunchKernelGGL(fke_perception_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, nullptr, a,
+                       (const int2*)d_disk.p, (const double*)d_rho.p, (double*)d_gam.p, (double2*)d_v.p);
+    if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
+    if (hipMemcpy(gamma, d_gam.p, fb * (ns - 1), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(v_eff, d_v.p, 2 * fb * ns, hipMemcpyDeviceToHost) != hipSuccess)
+        return TM_ERR_HIP;
     return TM_OK;
 }
 

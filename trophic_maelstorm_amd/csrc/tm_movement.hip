@@ -10,7 +10,8 @@
 // (SD, SV: the diffusion and advection half-stencils, mu_x(i) the rounded
 // advected mean) and B_i the same in columns.  Expanded, that is the
 // reference's four scatter terms (gamma^2, twice gamma - gamma^2, (1 - gamma)^2).
-// The reference scatters with atomicAdd; this kernel gathers instead: every
+// The reference scatters with atomicAdd; this kernel gathers instead: a first
+// pass forms each source's term coefficients and advected mean once, then every
 // output cell loops over the sources within reach -- those with |x - c_i| <=
 // W and |y - c_j| <= W, W = max(rD, rV + largest shift) -- and adds their
 // terms in FP64, in a fixed order.  No atomics, and the result is
@@ -52,9 +53,44 @@ __device__ __forceinline__ int wrapi(int v, int L)
     return v < 0 ? v + L : v;
 }
 
-__global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double* __restrict__ rho,
-                                                       const double* __restrict__ gamma,
-                                                       const double2* __restrict__ veff,
+// Per source cell: its three term coefficients (gamma^2 rho, (gamma - gamma^2) rho,
+// (1 - gamma)^2 rho, formed as the reference forms them) and its advected mean
+// cell, once per call; mu.x = -1 marks a source below the 1e-10 floor.
+__global__ __launch_bounds__(256) void fke_move_prep_kernel(MoveArgs a, const double* __restrict__ rho,
+                                                            const double* __restrict__ gamma,
+                                                            const double2* __restrict__ veff,
+                                                            double4* __restrict__ coef, int2* __restrict__ mu)
+{
+#pragma clang fp contract(off)
+    const int s = blockIdx.y;
+    const int L = a.L;
+    const int64_t L2 = (int64_t)L * L;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= L2) return;
+    const double r = rho[s * L2 + i];
+    if (r < kMoveEps) {
+        mu[s * L2 + i] = make_int2(-1, -1);
+        return;
+    }
+    const double pref = a.sp[s].pref;
+    const int ci = (int)(i / L), cj = (int)(i % L);
+    const double2 v = veff[s * L2 + i];
+    const int mux = (int)(__builtin_round(ci + v.x * (pref)) + L) % L;
+    const int muy = (int)(__builtin_round(cj + v.y * (pref)) + L) % L;
+    const double g = gamma[s * L2 + i], g2 = g * g;
+    coef[s * L2 + i] = make_double4((g2) * r, (g - g2) * r, (1 - g) * (1 - g) * r, 0.0);
+    mu[s * L2 + i] = make_int2(mux, muy);
+}
+
+// v in [-L, 2L) -> v mod L
+__device__ __forceinline__ int wrap1(int v, int L)
+{
+    v += v < 0 ? L : 0;
+    return v >= L ? v - L : v;
+}
+
+__global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double4* __restrict__ coef,
+                                                       const int2* __restrict__ mu,
                                                        const double* __restrict__ SDall,
                                                        const double* __restrict__ SVall, double* __restrict__ out)
 {
@@ -67,32 +103,30 @@ __global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double*
     const MoveSpecies sp = a.sp[s];
     const double* SD = SDall + sp.offD;
     const double* SV = SVall + sp.offV;
-    const double* rs = rho + s * L2;
-    const double* gs = gamma + s * L2;
-    const double2* vs = veff + s * L2;
+    const double4* cs = coef + s * L2;
+    const int2* ms = mu + s * L2;
     const int x = (int)(o / L), y = (int)(o % L);
     double acc = 0.0;
     for (int dr = -sp.W; dr <= sp.W; ++dr) {
-        const int ci = wrapi(x - dr, L);
+        const int ci = wrap1(x - dr, L);
+        const int adr = dr < 0 ? -dr : dr;
+        const bool rowD = adr <= sp.rD;
         for (int dc = -sp.W; dc <= sp.W; ++dc) {
-            const int cj = wrapi(y - dc, L);
+            const int cj = wrap1(y - dc, L);
             const int64_t i = (int64_t)ci * L + cj;
-            const double r = rs[i];
-            if (r < kMoveEps) continue;
-            const double2 v = vs[i];
-            const int mux = (int)(__builtin_round(ci + v.x * (sp.pref)) + L) % L;
-            const int muy = (int)(__builtin_round(cj + v.y * (sp.pref)) + L) % L;
-            const int ex = signed_gap(x, mux, L), ey = signed_gap(y, muy, L);
-            const int adr = dr < 0 ? -dr : dr, adc = dc < 0 ? -dc : dc;
+            const int2 m = ms[i];
+            if (m.x < 0) continue;
+            const int ex = signed_gap(x, m.x, L), ey = signed_gap(y, m.y, L);
+            const int adc = dc < 0 ? -dc : dc;
             const int aex = ex < 0 ? -ex : ex, aey = ey < 0 ? -ey : ey;
-            const bool rowD = adr <= sp.rD, rowV = aex <= sp.rV, colD = adc <= sp.rD, colV = aey <= sp.rV;
+            const bool rowV = aex <= sp.rV, colD = adc <= sp.rD, colV = aey <= sp.rV;
             if (!((rowD || rowV) && (colD || colV))) continue;
-            const double g = gs[i], g2 = g * g;
+            const double4 c = cs[i];
             double t = 0.0;
-            if (rowD && colD) t += (g2) * r * (SD[adr] * SD[adc]);
-            if (rowD && colV) t += (g - g2) * r * (SD[adr] * SV[aey]);
-            if (rowV && colD) t += (g - g2) * r * (SV[aex] * SD[adc]);
-            if (rowV && colV) t += (1 - g) * (1 - g) * r * (SV[aex] * SV[aey]);
+            if (rowD && colD) t += c.x * (SD[adr] * SD[adc]);
+            if (rowD && colV) t += c.y * (SD[adr] * SV[aey]);
+            if (rowV && colD) t += c.y * (SV[aex] * SD[adc]);
+            if (rowV && colV) t += c.z * (SV[aex] * SV[aey]);
             acc += t;
         }
     }
@@ -292,10 +326,18 @@ int tm_fke_movement(int ns, int L, const tm_fke_move_species* sp, const double* 
     int nD, nV;
     const int rc = move_args(ns, L, sp, vmax, &a, &nD, &nV);
     if (rc != TM_OK) return rc;
+    DevBuf d_coef, d_mu;
+    if (hipMalloc(&d_coef.p, sizeof(double4) * (size_t)ns * L2) != hipSuccess ||
+        hipMalloc(&d_mu.p, sizeof(int2) * (size_t)ns * L2) != hipSuccess)
+        return TM_ERR_NOMEM;
     const dim3 grid((unsigned)((L2 + 255) / 256), (unsigned)ns);
-    hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, st, a, rho, gamma, (const double2*)v_eff, stencil_D,
-                       stencil_V, out);
-    return hipGetLastError() == hipSuccess ? TM_OK : TM_ERR_HIP;
+    hipLaunchKernelGGL(fke_move_prep_kernel, grid, dim3(256), 0, st, a, rho, gamma, (const double2*)v_eff,
+                       (double4*)d_coef.p, (int2*)d_mu.p);
+    hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, st, a, (const double4*)d_coef.p, (const int2*)d_mu.p,
+                       stencil_D, stencil_V, out);
+    if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
+    // the scratch is released when this returns: wait for the kernels that use it
+    return hipStreamSynchronize(st) == hipSuccess ? TM_OK : TM_ERR_HIP;
 }
 
 int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* sp, const double* stencil_D,
@@ -318,11 +360,12 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
     int rc = move_args(ns, L, sp, vmax, &a, &nD, &nV);
     if (rc != TM_OK) return rc;
     if (hipSetDevice(device) != hipSuccess) return TM_ERR_NODEV;
-    DevBuf d_rho, d_gam, d_v, d_out, d_sd, d_sv;
+    DevBuf d_rho, d_gam, d_v, d_out, d_sd, d_sv, d_coef, d_mu;
     const size_t fb = sizeof(double) * (size_t)ns * L2;
     if (hipMalloc(&d_rho.p, fb) != hipSuccess || hipMalloc(&d_gam.p, fb) != hipSuccess ||
         hipMalloc(&d_v.p, 2 * fb) != hipSuccess || hipMalloc(&d_out.p, fb) != hipSuccess ||
-        hipMalloc(&d_sd.p, sizeof(double) * nD) != hipSuccess || hipMalloc(&d_sv.p, sizeof(double) * nV) != hipSuccess)
+        hipMalloc(&d_sd.p, sizeof(double) * nD) != hipSuccess || hipMalloc(&d_sv.p, sizeof(double) * nV) != hipSuccess ||
+        hipMalloc(&d_coef.p, 4 * fb) != hipSuccess || hipMalloc(&d_mu.p, sizeof(int2) * (size_t)ns * L2) != hipSuccess)
         return TM_ERR_NOMEM;
     if (hipMemcpy(d_rho.p, rho, fb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_gam.p, gamma, fb, hipMemcpyHostToDevice) != hipSuccess ||
@@ -331,9 +374,10 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
         hipMemcpy(d_sv.p, stencil_V, sizeof(double) * nV, hipMemcpyHostToDevice) != hipSuccess)
         return TM_ERR_HIP;
     const dim3 grid((unsigned)((L2 + 255) / 256), (unsigned)ns);
-    hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, nullptr, a, (const double*)d_rho.p,
-                       (const double*)d_gam.p, (const double2*)d_v.p, (const double*)d_sd.p,
-                       (const double*)d_sv.p, (double*)d_out.p);
+    hipLaunchKernelGGL(fke_move_prep_kernel, grid, dim3(256), 0, nullptr, a, (const double*)d_rho.p,
+                       (const double*)d_gam.p, (const double2*)d_v.p, (double4*)d_coef.p, (int2*)d_mu.p);
+    hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, nullptr, a, (const double4*)d_coef.p,
+                       (const int2*)d_mu.p, (const double*)d_sd.p, (const double*)d_sv.p, (double*)d_out.p);
     if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
     if (hipMemcpy(out, d_out.p, fb, hipMemcpyDeviceToHost) != hipSuccess) return TM_ERR_HIP;
     return TM_OK;

@@ -89,7 +89,13 @@ def test_fp32_one_step_tracks_fp64(tm, model):
 def ensemble_checks(r32, r64, field=0, floor=0.1, rel_tol=1e-3):
     """FP32 vs FP64 ensemble rho(t) on the same seeds: within rel_tol while the
     ensemble mean is at least `floor` of its initial value, and no bias over the
-    whole run (paired per-instance differences within 4 standard errors)."""
+    whole run: paired per-instance differences within 4 standard errors, or below
+    2e-5 of the initial mean.  Pairs on the same seeds share nearly all their
+    draws, so their differences are mostly deterministic FP32 rounding rather than
+    noise, and the standard error alone flags rounding-sized offsets: on the
+    Rietkerk P field (values near 45 moved by small RK4 increments) the offset
+    reaches 6e-6 of the mean at z above 4 for some seed sets, 170 times inside
+    rel_tol."""
     c = 2 * field + 1
     m32, m64 = r32[:, :, c].mean(0), r64[:, :, c].mean(0)
     live = m64 >= floor * m64[0]
@@ -98,7 +104,7 @@ def ensemble_checks(r32, r64, field=0, floor=0.1, rel_tol=1e-3):
     d = r32[:, :, c] - r64[:, :, c]
     se = d.std(0, ddof=1) / np.sqrt(d.shape[0])
     z = np.abs(d.mean(0)) / np.maximum(se, 1e-300)
-    assert np.all((z < 4.0) | (np.abs(d.mean(0)) <= 1e-6 * np.abs(m64[0]))), z.max()
+    assert np.all((z < 4.0) | (np.abs(d.mean(0)) <= 2e-5 * np.abs(m64[0]))), z.max()
 
 
 def test_fp32_dp_ensemble_matches_fp64(tm):
@@ -155,3 +161,29 @@ def test_fp32_dp_variants_agree(tm):
     for v, (f, r) in enumerate(out[1:], 1):
         assert np.array_equal(f, f0), names[v]
         np.testing.assert_allclose(r, r0, rtol=1e-12, err_msg=names[v])
+
+
+@pytest.mark.parametrize("hi", [1.0, 3.0, 4.0])
+def test_fp32_deferring_step_is_identical(tm, hi):
+    """The two-pass single step (rejected Gamma trials and PTRS-sized Poisson means
+    deferred to a queue in LDS) computes every cell as the one-pass step does: the
+    same field and the same draws.  hi = 3 mixes PTRS cells in; hi = 4 defers
+    nearly every live cell, more than the queue holds, so the in-place fallback runs."""
+    names = tm.variant_names(tm.MODEL_DP, tm.F32)
+    insts = [tm.Instance(L, 0, 5, k) for k, L in enumerate([96, 130, 64, 50, 257])]
+    out = {}
+    for want in ("dp_f32_128x32_t256_w6", "dp_f32_128x24_t256_w6q"):
+        with tm.Context() as ctx:
+            # the exact nonlinear sub-step keeps large densities positive
+            ctx.configure(tm.MODEL_DP, tm.F32, tm.DPParams(a=0.9, scheme=tm.DP_EXACT), insts)
+            ctx.set_launch(names.index(want), 4)
+            ctx.init_uniform([0.0], [hi])
+            ctx.record_draws(True)
+            rows = ctx.run(4)
+            out[want] = (np.concatenate([ctx.download(k).ravel() for k in range(len(insts))]),
+                         np.concatenate([ctx.draws(k).ravel() for k in range(len(insts))]), rows)
+    (f_a, d_a, r_a), (f_b, d_b, r_b) = out.values()
+    assert np.array_equal(f_a, f_b)
+    assert np.array_equal(d_a, d_b)
+    assert (d_a >= 0).sum() > 0.5 * d_a.size
+    np.testing.assert_allclose(r_a, r_b, rtol=1e-12)

@@ -224,26 +224,30 @@ TM_HD double norm_icdf_d(double p)
     return num / den;
 }
 
-// Same approximation in FP32 for the FP32 Gamma sampler.
+// Same approximation in FP32 for the FP32 Gamma sampler, on the hardware
+// log2, square root and reciprocal (about 1 ulp each; the IEEE forms of the
+// divide and square root cost ten instructions more apiece).  Both rational
+// forms are evaluated and one is selected: the tail form is needed somewhere in
+// almost every wavefront, and a branch would add the mask bookkeeping to both.
+// p comes from Stream::u32(), so p and 1 - p are at least 2^-24: the tail form's
+// logarithm is finite for every p.
 __device__ __forceinline__ float norm_icdf_f(float p)
 {
     const float plow = 0.02425f;
-    if (p < plow || p > 1.0f - plow) {
-        const float q = __builtin_sqrtf(-2.0f * __logf(p < plow ? p : 1.0f - p));
-        const float num = ((((-7.784894002430293e-03f * q - 3.223964580411365e-01f) * q - 2.400758277161838e+00f) * q
-                            - 2.549732539343734e+00f) * q + 4.374664141464968e+00f) * q + 2.938163982698783e+00f;
-        const float den = (((7.784695709041462e-03f * q + 3.224671290700398e-01f) * q + 2.445134137142996e+00f) * q
-                           + 3.754408661907416e+00f) * q + 1.0f;
-        const float x = __fdividef(num, den);
-        return p < plow ? x : -x;
-    }
-    const float q = p - 0.5f;
-    const float r = q * q;
-    const float num = (((((-3.969683028665376e+01f * r + 2.209460984245205e+02f) * r - 2.759285104469687e+02f) * r
-                         + 1.383577518672690e+02f) * r - 3.066479806614716e+01f) * r + 2.506628277459239e+00f) * q;
-    const float den = ((((-5.447609879822406e+01f * r + 1.615858368580409e+02f) * r - 1.556989798598866e+02f) * r
-                        + 6.680131188771972e+01f) * r - 1.328068155288572e+01f) * r + 1.0f;
-    return __fdividef(num, den);
+    const bool lo = p < plow, tail = lo || p > 1.0f - plow;
+    const float qt = __builtin_amdgcn_sqrtf(-1.38629436f * __builtin_amdgcn_logf(lo ? p : 1.0f - p));  // sqrt(-2 ln)
+    const float nt = ((((-7.784894002430293e-03f * qt - 3.223964580411365e-01f) * qt - 2.400758277161838e+00f) * qt
+                       - 2.549732539343734e+00f) * qt + 4.374664141464968e+00f) * qt + 2.938163982698783e+00f;
+    const float dt = (((7.784695709041462e-03f * qt + 3.224671290700398e-01f) * qt + 2.445134137142996e+00f) * qt
+                      + 3.754408661907416e+00f) * qt + 1.0f;
+    const float qc = p - 0.5f;
+    const float r = qc * qc;
+    const float nc = (((((-3.969683028665376e+01f * r + 2.209460984245205e+02f) * r - 2.759285104469687e+02f) * r
+                        + 1.383577518672690e+02f) * r - 3.066479806614716e+01f) * r + 2.506628277459239e+00f) * qc;
+    const float dc = ((((-5.447609879822406e+01f * r + 1.615858368580409e+02f) * r - 1.556989798598866e+02f) * r
+                       + 6.680131188771972e+01f) * r - 1.328068155288572e+01f) * r + 1.0f;
+    const float num = tail ? (lo ? nt : -nt) : nc;
+    return num * __builtin_amdgcn_rcpf(tail ? dt : dc);
 }
 
 // log(k!): exact table below 16, Stirling series (4 terms) above.

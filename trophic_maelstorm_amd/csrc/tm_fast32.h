@@ -22,22 +22,20 @@ __device__ __forceinline__ float log2_hw(float x) { return __builtin_amdgcn_logf
 __device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }  // v_exp_f32
 __device__ __forceinline__ float logf_hw(float x) { return log2_hw(x) * 0.693147181f; }
 
-// -expm1(-z) for z >= 0 in FP32.
-__device__ __forceinline__ float neg_expm1_neg(float z)
+// -expm1(-z) for z >= 0 in FP32, given e = e^{-z}: the series below 1/2, where
+// 1 - e would cancel, and 1 - e above (both formed, one selected).
+__device__ __forceinline__ float neg_expm1_neg(float z, float e)
 {
-    if (z < 0.5f) {
-        // z - z^2/2 + z^3/6 - ... to z^8 (truncation < 3e-8 relative at z = 1/2)
-        float p = -2.48015873e-5f;
-        p = p * z + 1.98412698e-4f;
-        p = p * z - 1.38888889e-3f;
-        p = p * z + 8.33333333e-3f;
-        p = p * z - 4.16666667e-2f;
-        p = p * z + 1.66666667e-1f;
-        p = p * z - 0.5f;
-        p = p * z + 1.0f;
-        return p * z;
-    }
-    return 1.0f - __expf(-z);
+    // z - z^2/2 + z^3/6 - ... to z^8 (truncation < 3e-8 relative at z = 1/2)
+    float p = -2.48015873e-5f;
+    p = p * z + 1.98412698e-4f;
+    p = p * z - 1.38888889e-3f;
+    p = p * z + 8.33333333e-3f;
+    p = p * z - 4.16666667e-2f;
+    p = p * z + 1.66666667e-1f;
+    p = p * z - 0.5f;
+    p = p * z + 1.0f;
+    return z < 0.5f ? p * z : 1.0f - e;
 }
 
 // log(k!) - [(k + 1/2) log k - k + log(2 pi)/2]  (Stirling's error), k >= 1.
@@ -78,31 +76,22 @@ __device__ __forceinline__ int64_t poisson_f(Stream& s, double zd, bool compat)
         return (int64_t)((double)(norm_icdf_f(s.u32()) * __builtin_sqrtf(zf)) + zd);
     }
     if (zd < 12.0) {
-        // Both searches stop once the next term can no longer move the running
-        // sum at FP32 resolution: the remaining tail mass is below 2^-24, and a
-        // uniform that falls in the rounding gap left between the sum's limit and
-        // 1 (or 0) maps to that extreme quantile instead of running on.
+        // Search on the upper tail S_k = P(N > k) against v = 1 - u, for every
+        // mean: S_0 = -expm1(-z) stays resolved as z -> 0, and one loop (not one
+        // per range of z) runs in a wavefront whatever its lanes' means.  The
+        // search stops once the next term can no longer move S at FP32
+        // resolution; a v that falls in the rounding gap left below S maps to
+        // that extreme quantile instead of running on.
         const float z = (float)zd;
+        const float v = s.v32();
         float p = __expf(-z);                   // P(N = 0)
+        float S = neg_expm1_neg(z, p);          // P(N > 0)
         int k = 0;
-        if (z < 0.5f) {
-            const float v = s.v32();            // 1 - u, fine near 0
-            float S = neg_expm1_neg(z);         // P(N > 0), resolved as z -> 0
-            while (v < S && k < 200) {
-                k += 1;
-                p = p * z * rcpf((float)k);
-                if (p <= S * 0x1p-24f) break;
-                S = S - p;
-            }
-        } else {
-            const float u = s.u32();
-            float F = p;                        // P(N <= 0)
-            while (u > F && k < 200) {
-                k += 1;
-                p = p * z * rcpf((float)k);
-                if (p <= 0x1p-25f) break;
-                F = F + p;
-            }
+        while (v < S && k < 200) {
+            k += 1;
+            p = p * z * rcpf((float)k);
+            if (p <= S * 0x1p-24f) break;
+            S = S - p;
         }
         return k;
     }
@@ -136,15 +125,21 @@ __device__ __forceinline__ int64_t poisson_f(Stream& s, double zd, bool compat)
     return (int64_t)__builtin_floor(zd);
 }
 
-// Gamma(shape, 1), Marsaglia-Tsang in FP32.
-__device__ __forceinline__ float gamma_f(Stream& s, float shape)
+// Gamma(shape, 1), Marsaglia-Tsang in FP32, at most TRIALS trials.  With
+// TRIALS = 64 it always returns true (64 rejections leave g = d, as the FP64
+// sampler does).  With fewer it returns false, and leaves s and g unspecified,
+// where the sampler would have drawn one more trial: the caller then starts the
+// transition again from its first word with the full sampler.
+template <int TRIALS>
+__device__ __forceinline__ bool gamma_f_try(Stream& s, float shape, float& g)
 {
-    if (!(shape > 0.0f)) return 0.0f;
+    if (!(shape > 0.0f)) { g = 0.0f; return true; }
     const float a = shape < 1.0f ? shape + 1.0f : shape;
     const float d = a - 0.333333343f;
     const float c = __builtin_amdgcn_rsqf(9.0f * d);
-    float g = d;
-    for (int it = 0; it < 64; ++it) {
+    g = d;
+    bool ok = false;
+    for (int it = 0; it < TRIALS; ++it) {
         const float x = norm_icdf_f(s.u32());
         const float y = c * x;
         float v = 1.0f + y;
@@ -152,18 +147,25 @@ __device__ __forceinline__ float gamma_f(Stream& s, float shape)
         v = v * v * v;
         const float u = s.u32();
         const float x2 = x * x;
-        if (u < 1.0f - 0.0331f * x2 * x2) { g = d * v; break; }
-        float acc;
-        if (__builtin_fabsf(y) < 0.1f) {
-            // x^2/2 + d(1 - v + log v) = d y^4 (-3/4 + 3/5 y - 1/2 y^2 + 3/7 y^3 - 3/8 y^4)
-            const float y2 = y * y;
-            acc = d * y2 * y2 * (-0.75f + y * (0.6f + y * (-0.5f + y * (0.428571429f - 0.375f * y))));
-        } else {
-            acc = 0.5f * x2 + d * (1.0f - v + logf_hw(v));
-        }
-        if (logf_hw(u) < acc) { g = d * v; break; }
+        if (u < 1.0f - 0.0331f * x2 * x2) { g = d * v; ok = true; break; }
+        // x^2/2 + d(1 - v + log v); below |y| = 0.1 its series in y, which does not
+        // cancel: d y^4 (-3/4 + 3/5 y - 1/2 y^2 + 3/7 y^3 - 3/8 y^4).  Both are
+        // formed and one selected (v > 0 here, so the logarithm is finite).
+        const float y2 = y * y;
+        const float ser = d * y2 * y2 * (-0.75f + y * (0.6f + y * (-0.5f + y * (0.428571429f - 0.375f * y))));
+        const float dir = 0.5f * x2 + d * (1.0f - v + logf_hw(v));
+        const float acc = __builtin_fabsf(y) < 0.1f ? ser : dir;
+        if (logf_hw(u) < acc) { g = d * v; ok = true; break; }
     }
+    if (TRIALS < 64 && !ok) return false;
     if (shape < 1.0f) g = g * exp2_hw(log2_hw(s.u32()) * rcpf(shape));
+    return true;
+}
+
+__device__ __forceinline__ float gamma_f(Stream& s, float shape)
+{
+    float g;
+    gamma_f_try<64>(s, shape, g);
     return g;
 }
 

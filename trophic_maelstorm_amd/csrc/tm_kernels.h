@@ -187,24 +187,39 @@ __device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double 
 // FP32 storage: alpha_i, the Poisson mean and the Gamma shape in FP64 (the same
 // values the FP64 path forms from the same fields); the Poisson and Gamma
 // deviates from the FP32 samplers of tm_fast32.h.
-__device__ __forceinline__ float dornic_cell(Stream& s, double alpha_i, float cur, const DornicConst& k,
-                                             bool compat, int64_t& n)
+// ONE = true is the first pass of the deferring step (dp_tile_deferred): it
+// returns false, leaving n and y unspecified, where the transition needs the
+// PTRS sampler (Poisson mean >= 12) or a second Gamma trial; otherwise it draws
+// the same words and computes the same value as ONE = false.
+template <bool ONE>
+__device__ __forceinline__ bool dornic_cell_f(Stream& s, double alpha_i, float cur, const DornicConst& k,
+                                              bool compat, int64_t& n, float& y)
 {
 #pragma clang fp contract(off)
     const double mu = -1.0 + 2.0 * alpha_i * k.sigma2_1;
     const double ziggy = k.ll * (double)cur;
+    if (ONE && ziggy >= 12.0) return false;
     n = poisson_f(s, ziggy, compat);
     const double gru = mu + 1.0 + (double)n;
     float g;
     if (compat && gru > 100.0) {
         const float gf = (float)gru;
         g = norm_icdf_f(s.u32()) * __builtin_sqrtf(gf) + gf;
-    } else {
-        g = gamma_f(s, (float)gru);
+    } else if (!gamma_f_try<ONE ? 1 : 64>(s, (float)gru, g)) {
+        return false;
     }
     // g / lambda with 1/lambda carried as hi + lo: a single FP32 rounding of 1/lambda
     // would bias every cell by the same relative amount every step.
-    return __builtin_fmaf(g, k.inv_lambda_hi, g * k.inv_lambda_lo);
+    y = __builtin_fmaf(g, k.inv_lambda_hi, g * k.inv_lambda_lo);
+    return true;
+}
+
+__device__ __forceinline__ float dornic_cell(Stream& s, double alpha_i, float cur, const DornicConst& k,
+                                             bool compat, int64_t& n)
+{
+    float y;
+    dornic_cell_f<false>(s, alpha_i, cur, k, compat, n, y);
+    return y;
 }
 
 // alpha_i = (D/dx^2)(up + down + left + right) in FP64, in the reference's order
@@ -441,6 +456,30 @@ __device__ __forceinline__ T cell_update(const typename StepConst<MODE>::type& c
     return y;
 }
 
+// First pass of the deferring FP32 DP step: cell_update<float, 0> where the
+// one-trial transition succeeds (returns true, same y, n and bad); false where
+// the cell has to be computed again by cell_update.
+__device__ __forceinline__ bool cell_update_first(const DPConst& c, const DevInst& di, uint32_t cell, float cur,
+                                                  float up, float dn, float lf, float rt, float& y, int64_t& n,
+                                                  bool& bad)
+{
+    y = 0.0f;
+    n = -1;
+    bad = false;
+    if (!(cur == 0.0f && up == 0.0f && dn == 0.0f && lf == 0.0f && rt == 0.0f)) {
+        y = cur;
+        const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
+        if (!(alpha_i == 0.0 && cur == 0.0f)) {
+            Stream st(di.seed, di.inst_id, cell, c.step);
+            if (!dornic_cell_f<true>(st, alpha_i, cur, c.k, c.compat, n, y)) return false;
+        }
+        y = dp_nonlinear<float>(c.scheme, (float)c.b, (float)c.dt, y);
+        bad = y < 0.0f || !Arith<float>::finite(y);
+        if (y < (float)c.eps && y >= (float)c.neg) y = 0.0f;
+    }
+    return true;
+}
+
 // Whether the first NQ dwords of an LDS image are all zero (every thread of the
 // workgroup calls it; one barrier).  An image of +0 only is an absorbing tile:
 // the step leaves every cell at 0 and draws nothing.
@@ -563,20 +602,134 @@ __device__ __forceinline__ void dp_tile(const StepArgs& a, int tile, const DevTi
     }
 }
 
+// The FP32 DP step of one tile in two passes.  A Gamma trial is rejected in a
+// few percent of the cells, but a wavefront runs the next trial, and the second
+// Philox block it needs, whenever any of its 64 lanes needs it, which is nearly
+// always.  So the first pass stops each cell's transition after one trial: a
+// cell whose trial was rejected, or whose Poisson mean needs the PTRS sampler,
+// goes to a queue in LDS and gets a placeholder.  After a barrier the second
+// pass runs the unchanged full step (cell_update) on the queued cells, packed
+// densely into the lanes, and overwrites the placeholders.  Every cell is
+// computed with the same arithmetic from the same Philox words as in dp_tile,
+// so the field and the draws are identical; only the order of the rho(t)
+// partial sums differs.  Deferred cells beyond QCAP are computed in place.
+template <int TX, int TY, int NT, int QCAP>
+__device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, const DevTile& tl, const DevInst& di,
+                                                 const DPConst& c, const float* s_rho, int64_t counter,
+                                                 int64_t slot, uint32_t* s_q)
+{
+    typedef DPShape<float, TX, TY, NT> S;
+    constexpr int V = S::V, RWV = S::RWV, R4 = S::R4;
+    const int tid = threadIdx.x;
+    const int cg = tid % S::CG, ly0 = tid / S::CG;
+    const int L = tl.L;
+    float* fout = (float*)a.out + tl.cell_off;
+    const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
+    double sum[1] = {0.0};
+    uint32_t nlive = 0;
+    for (int j = 0; j < S::G; ++j) {
+        const int ly = ly0 + j * R4;
+        const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
+        if (gy >= L || gx0 >= L) continue;
+        const int r = (ly + 1) * RWV + V + 4 * cg;
+        const uint32_t cell0 = (uint32_t)gy * (uint32_t)L + (uint32_t)gx0;
+        const bool full = gx0 + 4 <= L;
+        uint32_t bits = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            bits |= as_bits(s_rho[r + q]) | as_bits(s_rho[r - RWV + q]) | as_bits(s_rho[r + RWV + q]);
+        bits |= as_bits(s_rho[r - 1]) | as_bits(s_rho[r + 4]);
+        const int nvalid = min(4, L - gx0);
+        Quad<float> yq;
+        yq.v[0] = yq.v[1] = yq.v[2] = yq.v[3] = 0.0f;
+        if (bits == 0 && full) {
+            if (a.draws) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a.draws[tl.cell_off + cell0 + q] = -1;
+            }
+        } else {
+#pragma unroll 1
+            for (int q = 0; q < nvalid; ++q) {
+                const int rq = r + q;
+                const uint32_t cell = cell0 + q;
+                float y;
+                int64_t n;
+                bool bad;
+                bool done = cell_update_first(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
+                                              s_rho[rq - 1], s_rho[rq + 1], y, n, bad);
+                if (!done) {
+                    const uint32_t i = atomicAdd(&s_q[QCAP], 1u);
+                    if (i < (uint32_t)QCAP) {
+                        s_q[i] = (uint32_t)(ly * TX + 4 * cg + q);
+                        y = 0.0f;
+                        n = -1;
+                    } else {
+                        y = cell_update<float, 0>(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
+                                                  s_rho[rq - 1], s_rho[rq + 1], n, bad);
+                        done = true;
+                    }
+                }
+                if (done) {
+                    if (bad) record_fault(a.state, tl.inst, cell, counter);
+                    sum[0] += (double)y;
+                    nlive += y > 0.0f ? 1u : 0u;
+                }
+                if (a.draws) a.draws[tl.cell_off + cell] = n;
+                yq.v[0] = q == 0 ? y : yq.v[0];
+                yq.v[1] = q == 1 ? y : yq.v[1];
+                yq.v[2] = q == 2 ? y : yq.v[2];
+                yq.v[3] = q == 3 ? y : yq.v[3];
+            }
+        }
+        store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
+    }
+    // The placeholders are written (and acknowledged) before any thread of the
+    // workgroup overwrites them.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const uint32_t nq = min(s_q[QCAP], (uint32_t)QCAP);
+#pragma unroll 1
+    for (uint32_t i = tid; i < nq; i += NT) {
+        const uint32_t e = s_q[i];
+        const int ly = (int)(e / TX), lx = (int)(e % TX);
+        const int r = (ly + 1) * RWV + V + lx;
+        const uint32_t cell = (uint32_t)(tl.y0 + ly) * (uint32_t)L + (uint32_t)(tl.x0 + lx);
+        int64_t n;
+        bool bad;
+        const float y = cell_update<float, 0>(c, di, cell, s_rho[r], s_rho[r - RWV], s_rho[r + RWV], s_rho[r - 1],
+                                              s_rho[r + 1], n, bad);
+        if (bad) record_fault(a.state, tl.inst, cell, counter);
+        if (a.draws) a.draws[tl.cell_off + cell] = n;
+        sum[0] += (double)y;
+        nlive += y > 0.0f ? 1u : 0u;
+        fout[cell] = y;
+    }
+    double2* dst = a.partials + (slot * a.ntiles + tile) * S::NW + (tid >> 6);
+    if (__ballot(nlive != 0 || sum[0] != 0.0) == 0) {
+        if ((tid & 63) == 0) *dst = make_double2(0.0, 0.0);
+    } else {
+        double cnt[1] = {(double)nlive};
+        wave_partials<1>(sum, cnt, dst);
+    }
+}
+
 // MODE 0: the DP step.  MODE 1: the Dornic sub-step of the Rietkerk P field
 // alone (rietkerk_bjork_FKE.cpp:4154-4211), writing P* to a.aux for the RK4
 // kernel -- same staging, absorbing-group skip and sampler, on its own tiles.
-// One tile per workgroup: its descriptor, instance and constants are read
-// before the kernel writes anything, so they are scalar loads issued
+// QCAP > 0 (FP32 DP step only): the two-pass tile with a queue of QCAP deferred
+// cells.  One tile per workgroup: its descriptor, instance and constants are
+// read before the kernel writes anything, so they are scalar loads issued
 // together, and the image copy follows at once.
-template <typename T, int TX, int TY, int NT, int MODE, int MINW = 1>
+template <typename T, int TX, int TY, int NT, int MODE, int MINW = 1, int QCAP = 0>
 __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
 {
     typedef DPShape<T, TX, TY, NT> S;
     typedef typename StepConst<MODE>::type Const;
     constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
+    static_assert(QCAP == 0 || (sizeof(T) == 4 && MODE == 0), "deferring step: FP32 DP only");
     __shared__ __align__(16) uint32_t s_img[S::NBUF];
     __shared__ uint32_t s_flag[NT / 64];
+    __shared__ uint32_t s_q[QCAP + 1];                // deferred cells, then their count
 
     const int64_t counter = a.state->counter + a.k;
     const int64_t slot = counter & (int64_t)(a.nslots - 1);   // nslots: a power of two
@@ -587,6 +740,7 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
     const DevTile tl = tiles[tile];
     const DevInst di = a.inst[tl.inst];
     const Const c = ((const Const*)a.consts)[(int64_t)a.k * a.n_inst + tl.inst];
+    if (QCAP > 0 && threadIdx.x == 0) s_q[QCAP] = 0;
     tile_to_lds<T, TX, S::RH, NT>((const T*)a.in + NFIN * tl.cell_off, tl.L, tl.x0, tl.y0, s_img);
     __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
     __syncthreads();                 // ... everyone's
@@ -595,7 +749,10 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
                                  MODE == 0 ? 1 : 0);
         return;
     }
-    dp_tile<T, TX, TY, NT, MODE>(a, tile, tl, di, c, (const T*)s_img, counter, slot);
+    if constexpr (QCAP > 0)
+        dp_tile_deferred<TX, TY, NT, QCAP>(a, tile, tl, di, c, (const float*)s_img, counter, slot, s_q);
+    else
+        dp_tile<T, TX, TY, NT, MODE>(a, tile, tl, di, c, (const T*)s_img, counter, slot);
 }
 
 // ---------------------------------------------------------------------------

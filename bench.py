@@ -46,6 +46,7 @@ FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 C4_F32_BLOCKED_TRAFFIC_KIB = {
     "dp_f32_128x32_t256_x4a": (4, (87921.6, 141440.2), (75136.4, 139848.7)),
     "dp_f32_128x24_t256_x8a": (8, (175845.5, 145216.1), (114941.6, 140040.5)),
+    "dp_f32_128x24_t256_x8aq": (8, (175843.5, 145216.1), (115013.0, 149042.5)),
 }
 C4_F32_TRAFFIC_KIB = {   # single-step variant: (FETCH_SIZE, WRITE_SIZE) per dispatch
     "dp_f32_128x32_t256_w6": (75198.8, 139824.1),
@@ -251,8 +252,9 @@ def _kernel_desc(variant):
         shape = v.split("_")[2]
         tx, ty = shape.split("x")
         k = C4_F32_BLOCKED_TRAFFIC_KIB[v][0]
+        one = "two-pass (deferred Gamma rejections) " if v.endswith("q") else ""
         return (f"dp_stepk_kernel<float,{tx},{ty},256,{k}> ({k} steps per launch) once <= 2% of cells are "
-                f"occupied, dp_step_kernel<float,{tx},{ty},256> single steps before")
+                f"occupied, {one}dp_step_kernel<float,{tx},{ty},256> single steps before")
     return "dp_step_kernel<float>"
 
 

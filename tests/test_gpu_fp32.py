@@ -168,7 +168,7 @@ def test_fp32_deferring_step_is_identical(tm, hi):
     """The two-pass single step (rejected Gamma trials and PTRS-sized Poisson means
     deferred to a queue in LDS) computes every cell as the one-pass step does: the
     same field and the same draws.  hi = 3 mixes PTRS cells in; hi = 4 defers
-    nearly every live cell, more than the queue holds, so the in-place fallback runs."""
+    nearly every live cell, filling the queue to most of the tile."""
     names = tm.variant_names(tm.MODEL_DP, tm.F32)
     insts = [tm.Instance(L, 0, 5, k) for k, L in enumerate([96, 130, 64, 50, 257])]
     out = {}
@@ -187,3 +187,28 @@ def test_fp32_deferring_step_is_identical(tm, hi):
     assert np.array_equal(d_a, d_b)
     assert (d_a >= 0).sum() > 0.5 * d_a.size
     np.testing.assert_allclose(r_a, r_b, rtol=1e-12)
+
+
+@pytest.mark.parametrize("variant", ["dp_f32_128x32_t256_w6", "dp_f32_128x24_t256_w6q"])
+def test_fp32_numeric_fault_is_reported(tm, variant):
+    """The first fault (instance, cell, step) is reported by the FP32 step as by the
+    FP64 step.  With b = 0.3 the explicit Euler sub-step y - 0.1 b y^2 turns negative
+    only above y = 33: the planted densities (about 72 after the transition) fault,
+    their neighbours (about 8.5) do not.  A density of 100 has a Poisson mean far
+    above 12, so the two-pass step defers these cells and records the fault from
+    its second pass."""
+    L = 160
+    rho = np.zeros((1, L, L))
+    rho[0, 5, 7] = 100.0
+    rho[0, 130, 140] = 100.0
+    info = {}
+    for prec, name in ((tm.F32, variant), (tm.F64, None)):
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_DP, prec, tm.DPParams(b=0.3), [tm.Instance(L, 0, 1, 0)])
+            if name:
+                ctx.set_launch(tm.variant_names(tm.MODEL_DP, prec).index(name), 0)
+            ctx.upload(0, rho)
+            with pytest.raises(tm.NumericFault):
+                ctx.run(1)
+            info[prec] = ctx.fault_info()
+    assert info[tm.F32] == info[tm.F64] == (0, 5 * L + 7, 0)

@@ -612,12 +612,17 @@ __device__ __forceinline__ void dp_tile(const StepArgs& a, int tile, const DevTi
 // densely into the lanes, and overwrites the placeholders.  Every cell is
 // computed with the same arithmetic from the same Philox words as in dp_tile,
 // so the field and the draws are identical; only the order of the rho(t)
-// partial sums differs.  Deferred cells beyond QCAP are computed in place.
-template <int TX, int TY, int NT, int QCAP>
+// partial sums differs.  The queue is filled in thread order (a bit mask per
+// thread and a workgroup prefix sum), so which thread finishes which cell, and
+// with it the rho(t) partials, is the same in every run.  It holds every cell
+// of the tile (16-bit tile offsets), so it cannot overflow.
+template <int TX, int TY, int NT>
 __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, const DevTile& tl, const DevInst& di,
                                                  const DPConst& c, const float* s_rho, int64_t counter,
-                                                 int64_t slot, uint32_t* s_q)
+                                                 int64_t slot, uint16_t* s_q, uint32_t* s_wn)
 {
+    static_assert(TX * TY <= 65536, "16-bit tile offsets");
+    static_assert(DPShape<float, TX, TY, NT>::G * 4 <= 32, "one mask bit per cell of a thread");
     typedef DPShape<float, TX, TY, NT> S;
     constexpr int V = S::V, RWV = S::RWV, R4 = S::R4;
     const int tid = threadIdx.x;
@@ -627,6 +632,7 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
     const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
     double sum[1] = {0.0};
     uint32_t nlive = 0;
+    uint32_t dmask = 0;                                       // bit 4 j + q: deferred
     for (int j = 0; j < S::G; ++j) {
         const int ly = ly0 + j * R4;
         const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
@@ -655,21 +661,13 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
                 float y;
                 int64_t n;
                 bool bad;
-                bool done = cell_update_first(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
-                                              s_rho[rq - 1], s_rho[rq + 1], y, n, bad);
+                const bool done = cell_update_first(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
+                                                    s_rho[rq - 1], s_rho[rq + 1], y, n, bad);
                 if (!done) {
-                    const uint32_t i = atomicAdd(&s_q[QCAP], 1u);
-                    if (i < (uint32_t)QCAP) {
-                        s_q[i] = (uint32_t)(ly * TX + 4 * cg + q);
-                        y = 0.0f;
-                        n = -1;
-                    } else {
-                        y = cell_update<float, 0>(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
-                                                  s_rho[rq - 1], s_rho[rq + 1], n, bad);
-                        done = true;
-                    }
-                }
-                if (done) {
+                    dmask |= 1u << (4 * j + q);
+                    y = 0.0f;
+                    n = -1;
+                } else {
                     if (bad) record_fault(a.state, tl.inst, cell, counter);
                     sum[0] += (double)y;
                     nlive += y > 0.0f ? 1u : 0u;
@@ -683,11 +681,33 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
         }
         store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
     }
+    // Queue offsets: an inclusive scan of the per-thread counts within the wave,
+    // then the totals of the waves before it.
+    const uint32_t cnt = (uint32_t)__builtin_popcount(dmask);
+    const int lane = tid & 63;
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d *= 2) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_wn[tid >> 6] = incl;
     // The placeholders are written (and acknowledged) before any thread of the
     // workgroup overwrites them.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    const uint32_t nq = min(s_q[QCAP], (uint32_t)QCAP);
+    uint32_t base = incl - cnt, nq = 0;
+#pragma unroll
+    for (int w = 0; w < S::NW; ++w) {
+        base += w < (tid >> 6) ? s_wn[w] : 0u;
+        nq += s_wn[w];
+    }
+#pragma unroll 1
+    for (uint32_t m = dmask; m != 0; m &= m - 1) {
+        const int b = __builtin_ctz(m);
+        s_q[base++] = (uint16_t)((ly0 + (b >> 2) * R4) * TX + 4 * cg + (b & 3));
+    }
+    __syncthreads();
 #pragma unroll 1
     for (uint32_t i = tid; i < nq; i += NT) {
         const uint32_t e = s_q[i];
@@ -716,20 +736,21 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
 // MODE 0: the DP step.  MODE 1: the Dornic sub-step of the Rietkerk P field
 // alone (rietkerk_bjork_FKE.cpp:4154-4211), writing P* to a.aux for the RK4
 // kernel -- same staging, absorbing-group skip and sampler, on its own tiles.
-// QCAP > 0 (FP32 DP step only): the two-pass tile with a queue of QCAP deferred
+// Q = 1 (FP32 DP step only): the two-pass tile with its queue of deferred
 // cells.  One tile per workgroup: its descriptor, instance and constants are
 // read before the kernel writes anything, so they are scalar loads issued
 // together, and the image copy follows at once.
-template <typename T, int TX, int TY, int NT, int MODE, int MINW = 1, int QCAP = 0>
+template <typename T, int TX, int TY, int NT, int MODE, int MINW = 1, int Q = 0>
 __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
 {
     typedef DPShape<T, TX, TY, NT> S;
     typedef typename StepConst<MODE>::type Const;
     constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
-    static_assert(QCAP == 0 || (sizeof(T) == 4 && MODE == 0), "deferring step: FP32 DP only");
+    static_assert(Q == 0 || (sizeof(T) == 4 && MODE == 0), "deferring step: FP32 DP only");
     __shared__ __align__(16) uint32_t s_img[S::NBUF];
     __shared__ uint32_t s_flag[NT / 64];
-    __shared__ uint32_t s_q[QCAP + 1];                // deferred cells, then their count
+    __shared__ uint16_t s_q[Q ? TX * TY : 1];         // deferred cells (tile offsets)
+    __shared__ uint32_t s_wn[NT / 64];                // ... and their count per wave
 
     const int64_t counter = a.state->counter + a.k;
     const int64_t slot = counter & (int64_t)(a.nslots - 1);   // nslots: a power of two
@@ -740,7 +761,6 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
     const DevTile tl = tiles[tile];
     const DevInst di = a.inst[tl.inst];
     const Const c = ((const Const*)a.consts)[(int64_t)a.k * a.n_inst + tl.inst];
-    if (QCAP > 0 && threadIdx.x == 0) s_q[QCAP] = 0;
     tile_to_lds<T, TX, S::RH, NT>((const T*)a.in + NFIN * tl.cell_off, tl.L, tl.x0, tl.y0, s_img);
     __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
     __syncthreads();                 // ... everyone's
@@ -749,8 +769,8 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
                                  MODE == 0 ? 1 : 0);
         return;
     }
-    if constexpr (QCAP > 0)
-        dp_tile_deferred<TX, TY, NT, QCAP>(a, tile, tl, di, c, (const float*)s_img, counter, slot, s_q);
+    if constexpr (Q != 0)
+        dp_tile_deferred<TX, TY, NT>(a, tile, tl, di, c, (const float*)s_img, counter, slot, s_q, s_wn);
     else
         dp_tile<T, TX, TY, NT, MODE>(a, tile, tl, di, c, (const T*)s_img, counter, slot);
 }

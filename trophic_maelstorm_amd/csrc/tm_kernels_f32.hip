@@ -6,10 +6,10 @@
 
 namespace tmk {
 
-template <typename T, int TX, int TY, int NT, int MODE = 0, int MINW = 1, int QCAP = 0>
+template <typename T, int TX, int TY, int NT, int MODE = 0, int MINW = 1, int Q = 0>
 static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, MODE, MINW, QCAP>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, MODE, MINW, Q>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 template <typename T, int TX, int TY, int NT, int K, int MINW = 1>
@@ -67,12 +67,10 @@ static const Variant kDP[] = {
     {128, 16, 256, 0, 4, "dp_f32_128x16_t256_x16a", &launch_dpk<float, 128, 16, 256, 16>, 0, 0, nullptr, 0, 16,
      &launch_dp<float, 128, 16, 256, 0, 6>, 20},
     // _q: the single steps defer cells whose Gamma trial is rejected to a second
-    // pass over a queue of 511 cells per tile (dp_tile_deferred)
-    {128, 24, 256, 0, 4, "dp_f32_128x24_t256_w6q", &launch_dp<float, 128, 24, 256, 0, 6, 511>},
+    // pass over a queue in LDS (dp_tile_deferred)
+    {128, 24, 256, 0, 4, "dp_f32_128x24_t256_w6q", &launch_dp<float, 128, 24, 256, 0, 6, 1>},
     {128, 24, 256, 0, 4, "dp_f32_128x24_t256_x8aq", &launch_dpk<float, 128, 24, 256, 8>, 0, 0, nullptr, 0, 8,
-     &launch_dp<float, 128, 24, 256, 0, 6, 511>, 20},
-    {128, 32, 256, 0, 4, "dp_f32_128x32_t256_x8aq", &launch_dpk<float, 128, 32, 256, 8>, 0, 0, nullptr, 0, 8,
-     &launch_dp<float, 128, 32, 256, 0, 6, 511>, 20},
+     &launch_dp<float, 128, 24, 256, 0, 6, 1>, 20},
 };
 
 static const Variant kRK[] = {

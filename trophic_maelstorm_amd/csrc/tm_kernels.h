@@ -683,9 +683,9 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
     }
     // Queue offsets: an inclusive scan of the per-thread counts within the wave,
     // then the totals of the waves before it.
-    const uint32_t cnt = (uint32_t)__builtin_popcount(dmask);
+    const uint32_t ndef = (uint32_t)__builtin_popcount(dmask);
     const int lane = tid & 63;
-    uint32_t incl = cnt;
+    uint32_t incl = ndef;
 #pragma unroll
     for (int d = 1; d < 64; d *= 2) {
         const uint32_t o = __shfl_up(incl, d, 64);
@@ -696,7 +696,7 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
     // workgroup overwrites them.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    uint32_t base = incl - cnt, nq = 0;
+    uint32_t base = incl - ndef, nq = 0;
 #pragma unroll
     for (int w = 0; w < S::NW; ++w) {
         base += w < (tid >> 6) ? s_wn[w] : 0u;

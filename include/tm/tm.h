@@ -386,6 +386,36 @@ int tm_fke_movement(int ns, int L, const tm_fke_move_species* sp, const double* 
 int tm_fke_perception_host(int device, int ns, int L, const double* pR, double dx, const double* v, uint32_t seed,
                            uint32_t inst, int64_t step, const double* rho, double* gamma, double* v_eff);
 
+/* Parameters of the Rietkerk consumer model's RK4 (f_DorFKE_2Sp / f_DorFKE_3Sp,
+ * rietkerk_bjork_FKE.cpp:2449-2474, :2611-2650).  Names follow the reference's
+ * arguments: a is the rainfall; K[1] and K[2] enter (K[0] does not); A, H, E, M
+ * are indexed [prey][consumer] as there.  D_W and D_O are the diffusion constants
+ * of W and O (the reference's Dxd2 = D / dx^2, FKE:3849-3852).  dt is the step the
+ * RK4 receives, used by its third stage; dt_rk is Dt_Analytic, which sets the
+ * half and sixth steps dt2 = dt_rk / 2 and dt6 = dt_rk / 6 (set_global_FKE_params,
+ * FKE:206-210).  dt_rk = dt gives the textbook RK4. */
+typedef struct tm_fke_rk_params {
+    double a, c, gmax, alpha, rW, W0;
+    double K[3];
+    double A[3][3], H[3][3], E[3], M[3];
+    double D_W, D_O;
+    double dt, dt_rk, dx;
+} tm_fke_rk_params;
+
+/* One RK4 step of the Rietkerk consumer model's reaction and water terms, in
+ * place: RK4_Integrate_Stochastic_2Sp (rietkerk_bjork_FKE.cpp:2476-2533) for
+ * ns = 1 consumer and RK4_Integrate_Stochastic_MultiSp (:2652-2720) for ns = 2.
+ * fields [ns + 3][L*L] in the reference's order: P, G, (Pr,) W, O; periodic
+ * lattice.  Device arrays (hipMalloc'd), on `stream` (a hipStream_t, or null); the
+ * library allocates its scratch (3 (ns + 3) L^2 doubles) and waits for the stream
+ * before returning.  n_bad (may be null) gets the number of values that ended
+ * negative or non-finite, which the reference logs and carries on from
+ * (FKE:2511-2529).  Returns 0 or a negative tm_status. */
+int tm_fke_rk4(int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t* n_bad, void* stream);
+
+/* The same step on host arrays, on `device`. */
+int tm_fke_rk4_host(int device, int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t* n_bad);
+
 /* Host only.  The perception neighbourhood of the consumer models: offsets
  * (row, column) of the sites within Euclidean radius R of a site, in the order of
  * computeNeighboringSitesCentral (multiSPDP.cpp:1442-1461): generated row-major

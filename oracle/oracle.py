@@ -124,6 +124,8 @@ def lib():
         L.tmo_fke_perception.argtypes = [C.c_int, C.c_int, _f64p, C.c_double, _f64p, C.c_void_p, C.c_int, C.c_uint32,
                                          C.c_uint32, C.c_int64, _f64p, _f64p, _f64p]
         L.tmo_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.tmo_fke_rk4.restype = C.c_int64
+        L.tmo_fke_rk4.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p]
         L.tmo_fixed_sum.restype = C.c_double
         L.tmo_fixed_sum.argtypes = [_f64p, C.c_int64]
         L.tmo_dpc_rk4.argtypes = [C.POINTER(DPCParams), C.c_int, _f64p]
@@ -439,3 +441,28 @@ def ostream_text(x: float, prec: int = 0) -> str:
     buf = C.create_string_buffer(64)
     n = _fmt.fmt_ostream(float(x), int(prec), buf, 64)
     return buf.raw[:n].decode()
+
+
+# ---------------------------------------------------------------- consumer RK4 (FKE)
+class FkeRkParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("a", "c", "gmax", "alpha", "rW", "W0")] + [
+        ("K", C.c_double * 3), ("A", (C.c_double * 3) * 3), ("H", (C.c_double * 3) * 3), ("E", C.c_double * 3),
+        ("M", C.c_double * 3), ("D_W", C.c_double), ("D_O", C.c_double), ("dt", C.c_double), ("dt_rk", C.c_double),
+        ("dx", C.c_double)]
+
+
+def fke_rk_params(a=1.0, c=10.0, gmax=0.05, alpha=0.2, rW=0.2, W0=0.2, K=(0.0, 5.0, 5.0), A=None, H=None,
+                  E=(1.0, 0.45, 0.3), M=(0.0, 0.01, 0.01), D_W=0.1, D_O=100.0, dt=0.005, dt_rk=None,
+                  dx=2.0) -> FkeRkParams:
+    A = np.zeros((3, 3)) if A is None else np.asarray(A, np.float64)
+    H = np.zeros((3, 3)) if H is None else np.asarray(H, np.float64)
+    m3 = lambda m: ((C.c_double * 3) * 3)(*[(C.c_double * 3)(*row) for row in m])
+    return FkeRkParams(a, c, gmax, alpha, rW, W0, (C.c_double * 3)(*K), m3(A), m3(H), (C.c_double * 3)(*E),
+                       (C.c_double * 3)(*M), D_W, D_O, dt, dt if dt_rk is None else dt_rk, dx)
+
+
+def fke_rk4(p: FkeRkParams, y):
+    """tmo_fke_rk4: one step of y [ns + 3, L, L] (P, G, (Pr,) W, O); returns (y, n_bad)."""
+    y = np.ascontiguousarray(y, np.float64).copy()
+    n = lib().tmo_fke_rk4(C.cast(C.byref(p), C.c_void_p), y.shape[0] - 3, y.shape[-1], y.reshape(-1))
+    return y, int(n)

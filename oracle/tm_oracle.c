@@ -1140,3 +1140,72 @@ void tmo_fke_perception(int ns, int L, const double* pR, double dx, const double
             }
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* The Rietkerk consumer model's RK4 (FKE:2449-2533, :2611-2720).            */
+/* ------------------------------------------------------------------------ */
+
+/* f_DorFKE_2Sp (ns = 1, fields P G W O) and f_DorFKE_3Sp (ns = 2, P G Pr W O). */
+static void fke_rhs(const tmo_fke_rk_params* p, int ns, int L, const double* R, double* f)
+{
+    const int64_t L2 = (int64_t)L * L;
+    const double dx2 = p->dx * p->dx;
+    const double dW = p->D_W / (dx2), dO = p->D_O / (dx2);   /* diff_coefficient, FKE:3849-3852 */
+    const double cgmax = p->c * p->gmax, K2W0 = p->K[2] * p->W0, A01H01 = p->A[0][1] * p->H[0][1];
+    const double A12H12 = p->A[1][2] * p->H[1][2];
+    const double EA1 = p->E[1] * p->A[0][1], EA2 = p->E[2] * p->A[1][2];
+    const int w = ns + 1, o = ns + 2;
+    const double* Pv = R;
+    const double* Gv = R + L2;
+    const double* Wv = R + w * L2;
+    const double* Ov = R + o * L2;
+    for (int64_t i = 0; i < L2; ++i) {
+        const int y = (int)(i / L), x = (int)(i % L);
+        /* nR2[i]: i - g, i + g, i - 1, i + 1 with periodic wrap (FKE:3821) */
+        const int64_t n00 = (int64_t)((y + L - 1) % L) * L + x, n01 = (int64_t)((y + 1) % L) * L + x;
+        const int64_t n10 = (int64_t)y * L + (x + L - 1) % L, n11 = (int64_t)y * L + (x + 1) % L;
+        f[i] = cgmax * Wv[i] * Pv[i] / (Wv[i] + p->K[1]) - (p->A[0][1] * Pv[i] / (1 + A01H01 * Pv[i])) * Gv[i];
+        if (ns == 1) {
+            f[L2 + i] = (EA1 * Pv[i] / (1 + A01H01 * Pv[i])) * Gv[i] - p->M[1] * Gv[i];
+        } else {
+            const double* Qv = R + 2 * L2;
+            f[L2 + i] = (EA1 * Pv[i] / (1 + A01H01 * Pv[i])) * Gv[i] - p->M[1] * Gv[i]
+                        - p->A[1][2] * Gv[i] / (1 + A12H12 * Gv[i]) * Qv[i];
+            f[2 * L2 + i] = (EA2 * Gv[i] / (1 + A12H12 * Gv[i])) * Qv[i] - p->M[2] * Qv[i];
+        }
+        f[w * L2 + i] = p->alpha * (Pv[i] + K2W0) / (Pv[i] + p->K[2]) * Ov[i]
+                        - p->gmax * Wv[i] * Pv[i] / (Wv[i] + p->K[1]) - p->rW * Wv[i]
+                        + (dW) * (Wv[n00] + Wv[n01] + Wv[n10] + Wv[n11] - 4 * Wv[i]);
+        f[o * L2 + i] = p->a - p->alpha * (Pv[i] + K2W0) / (Pv[i] + p->K[2]) * Ov[i]
+                        + (dO) * (Ov[n00] + Ov[n01] + Ov[n10] + Ov[n11] - 4 * Ov[i]);
+    }
+}
+
+int64_t tmo_fke_rk4(const tmo_fke_rk_params* p, int ns, int L, double* y)
+{
+    const int64_t n = (int64_t)(ns + 3) * L * L;
+    const double dt2 = p->dt_rk / 2.0, dt6 = p->dt_rk / 6.0, dt = p->dt;
+    double* K1 = (double*)malloc(sizeof(double) * n);
+    double* K2 = (double*)malloc(sizeof(double) * n);
+    double* K3 = (double*)malloc(sizeof(double) * n);
+    double* K4 = (double*)malloc(sizeof(double) * n);
+    double* T = (double*)malloc(sizeof(double) * n);
+    int64_t bad = 0;
+    fke_rhs(p, ns, L, y, K1);
+    for (int64_t j = 0; j < n; ++j) T[j] = y[j] + (dt2) * K1[j];
+    fke_rhs(p, ns, L, T, K2);
+    for (int64_t j = 0; j < n; ++j) T[j] = y[j] + (dt2) * K2[j];
+    fke_rhs(p, ns, L, T, K3);
+    for (int64_t j = 0; j < n; ++j) T[j] = y[j] + (dt) * K3[j];
+    fke_rhs(p, ns, L, T, K4);
+    for (int64_t j = 0; j < n; ++j) {
+        y[j] += (dt6) * (K1[j] + 2.0 * K2[j] + 2.0 * K3[j] + K4[j]);
+        if (y[j] < 0 || !isfinite(y[j])) ++bad;
+    }
+    free(K1);
+    free(K2);
+    free(K3);
+    free(K4);
+    free(T);
+    return bad;
+}

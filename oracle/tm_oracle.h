@@ -30,7 +30,10 @@
  * comparison with the reference's own percolationDornic_2D built from source;
  * the replicate statistics against the reference's own
  * var_mean_incremental_surv_runs (stochasticSPDP.cpp:173-207) built from
- * source, and against numpy means and unbiased variances.
+ * source, and against numpy means and unbiased variances; the consumer RK4
+ * (tmo_fke_rk4) by the single-species RK4 bit for bit with no grazers, the
+ * two-species form bit for bit with no predators, and the ODE (solve_ivp) on
+ * uniform fields.
  */
 #ifndef TM_ORACLE_H
 #define TM_ORACLE_H
@@ -196,6 +199,21 @@ void tmo_fke_movement(int L, const double* rho, const double* gamma, const doubl
 void tmo_fke_perception(int ns, int L, const double* pR, double dx, const double* v, const int32_t* disk,
                         int disk_len, uint32_t seed, uint32_t inst, int64_t step, const double* rho,
                         double* gamma, double* veff);
+
+/* The Rietkerk consumer model's RK4 (RK4_Integrate_Stochastic_2Sp, FKE:2476-2533,
+ * and _MultiSp, :2652-2720, with f_DorFKE_2Sp, :2449-2474, and f_DorFKE_3Sp,
+ * :2611-2650): one step of y[ns + 3][L*L] = P, G, (Pr,) W, O in place; returns the
+ * number of values left negative or non-finite.  dt2 = dt_rk / 2 and
+ * dt6 = dt_rk / 6 as set_global_FKE_params sets them (FKE:206-210); dt is the
+ * RK4's own step (its third stage). */
+typedef struct tmo_fke_rk_params {
+    double a, c, gmax, alpha, rW, W0;
+    double K[3];
+    double A[3][3], H[3][3], E[3], M[3];
+    double D_W, D_O;
+    double dt, dt_rk, dx;
+} tmo_fke_rk_params;
+int64_t tmo_fke_rk4(const tmo_fke_rk_params* p, int ns, int L, double* y);
 
 #ifdef __cplusplus
 }

@@ -235,6 +235,65 @@ def fke_perception(rho, gamma, v_eff, pR, dx, v, seed=1, inst=0, step=0, device:
     return g, ve
 
 
+class _FKERK(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("a", "c", "gmax", "alpha", "rW", "W0")] + [
+        ("K", C.c_double * 3), ("A", (C.c_double * 3) * 3), ("H", (C.c_double * 3) * 3), ("E", C.c_double * 3),
+        ("M", C.c_double * 3), ("D_W", C.c_double), ("D_O", C.c_double), ("dt", C.c_double), ("dt_rk", C.c_double),
+        ("dx", C.c_double)]
+
+
+@dataclass
+class FKERKParams:
+    """The Rietkerk consumer model's RK4 (f_DorFKE_2Sp / f_DorFKE_3Sp arguments,
+    Rietkerk_FPE/rietkerk_bjork_FKE.cpp:2449-2474, :2611-2650).  a is the rainfall;
+    D_W and D_O the diffusion constants of W and O; dt the RK4's step (its third
+    stage) and dt_rk Dt_Analytic, which sets its half and sixth steps (None: dt).
+    Defaults: the Rietkerk constants of RietkerkParams and the trophic constants
+    of DPCParams."""
+    a: float = 1.0
+    c: float = 10.0
+    gmax: float = 0.05
+    alpha: float = 0.2
+    rW: float = 0.2
+    W0: float = 0.2
+    K: tuple = (0.0, 5.0, 5.0)
+    A: tuple | None = None
+    H: tuple | None = None
+    E: tuple = (1.0, 0.45, 0.3)
+    M: tuple = (0.0, 0.061609 * 20.0 ** -0.25 / 8760.0, 0.01)
+    D_W: float = 0.1
+    D_O: float = 100.0
+    dt: float = 0.005
+    dt_rk: float | None = None
+    dx: float = 2.0
+
+    def __post_init__(self):
+        d = DPCParams()
+        if self.A is None:
+            self.A = d.A
+        if self.H is None:
+            self.H = d.H
+
+    def _c(self) -> _FKERK:
+        return _FKERK(self.a, self.c, self.gmax, self.alpha, self.rW, self.W0, _vec3(self.K), _mat3(self.A),
+                      _mat3(self.H), _vec3(self.E), _vec3(self.M), self.D_W, self.D_O, self.dt,
+                      self.dt if self.dt_rk is None else self.dt_rk, self.dx)
+
+
+def fke_rk4(fields, params: FKERKParams, device: int = 0):
+    """One RK4 step of the Rietkerk consumer model on the GPU (tm_fke_rk4_host):
+    fields [ns + 3, L, L] in the order P, G, (Pr,) W, O.  Returns (fields, n_bad):
+    an updated copy and the number of values left negative or non-finite."""
+    f = np.ascontiguousarray(fields, np.float64).copy()
+    ns, L = f.shape[0] - 3, f.shape[-1]
+    p = params._c()
+    bad = C.c_int64(0)
+    rc = lib().tm_fke_rk4_host(device, ns, L, C.cast(C.byref(p), C.c_void_p), _dptr(f), C.byref(bad))
+    if rc != 0:
+        raise TMError(rc, "tm_fke_rk4_host failed")
+    return f, int(bad.value)
+
+
 @dataclass
 class Instance:
     L: int
@@ -304,6 +363,8 @@ def lib() -> C.CDLL:
         "tm_fke_movement": ([C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
         "tm_fke_perception_host": ([C.c_int, C.c_int, C.c_int, dp, C.c_double, dp, C.c_uint32, C.c_uint32, i64, dp,
                                     dp, dp], C.c_int),
+        "tm_fke_rk4_host": ([C.c_int, C.c_int, C.c_int, C.c_void_p, dp, C.POINTER(C.c_int64)], C.c_int),
+        "tm_fke_rk4": ([C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

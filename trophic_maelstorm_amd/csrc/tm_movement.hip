@@ -279,6 +279,135 @@ struct DevBuf {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Trophic RK4 of the Rietkerk consumer model: RK4_Integrate_Stochastic_2Sp /
+// _MultiSp with f_DorFKE_2Sp / f_DorFKE_3Sp (rietkerk_bjork_FKE.cpp:2449-2533,
+// :2611-2720).  Fields in the reference's order: P, G, (Pr,) W, O.  W and O
+// diffuse, so every stage needs its neighbours' stage values: one grid-wide
+// pass per stage.  The K's are folded into one accumulator in the reference's
+// order ((K1 + 2 K2) + 2 K3) + K4, so no K array is kept.
+// ---------------------------------------------------------------------------
+struct FkeRk {
+    int ns, L;                       // consumer species (1 or 2), lattice side
+    double a, cgmax, gmax, alpha, rW, K1, K2, K2W0;
+    double A01, A01H01, EA1, M1, A12, A12H12, EA2, M2;
+    double dW, dO;                   // D / dx^2 of W and O
+    double dt, dt2, dt6;             // the RK4's dt; dt2 and dt6 from Dt_Analytic (FKE:206-210)
+};
+
+// f at cell i of the NF fields y (stride L^2), in the reference's operation order.
+template <int NS>
+__device__ __forceinline__ void fke_rhs(const FkeRk& c, const double* y, int64_t L2, int i, int up, int dn, int lf,
+                                        int rt, double (&f)[NS + 3])
+{
+#pragma clang fp contract(off)
+    constexpr int W = NS + 1, O = NS + 2;
+    const double P = y[i], G = y[L2 + i], Wi = y[W * L2 + i], Oi = y[O * L2 + i];
+    const double* yW = y + W * L2;
+    const double* yO = y + O * L2;
+    const double A = c.alpha * (P + c.K2W0) / (P + c.K2) * Oi;
+    const double X = c.EA1 * P / (1 + c.A01H01 * P) * G;
+    f[0] = c.cgmax * Wi * P / (Wi + c.K1) - (c.A01 * P / (1 + c.A01H01 * P)) * G;
+    if constexpr (NS == 1) {
+        f[1] = X - c.M1 * G;
+    } else {
+        const double Pr = y[2 * L2 + i];
+        f[1] = X - c.M1 * G - c.A12 * G / (1 + c.A12H12 * G) * Pr;
+        f[2] = (c.EA2 * G / (1 + c.A12H12 * G)) * Pr - c.M2 * Pr;
+    }
+    f[W] = A - c.gmax * Wi * P / (Wi + c.K1) - c.rW * Wi + c.dW * (yW[up] + yW[dn] + yW[lf] + yW[rt] - 4 * Wi);
+    f[O] = c.a - A + c.dO * (yO[up] + yO[dn] + yO[lf] + yO[rt] - 4 * Oi);
+}
+
+// Stage s (1..4) from the stage values src: K_s = f(src).  Stages 1-3 fold K_s
+// into acc and write the next stage's values rho + h K_s to nxt; stage 4 writes
+// rho + dt6 (acc + K4) over rho and counts the cells that turned negative or
+// non-finite (the reference logs them, FKE:2511-2529, and carries on).
+template <int NS>
+__global__ __launch_bounds__(256) void fke_rk_stage_kernel(FkeRk c, int stage, double* rho, const double* src,
+                                                           double* nxt, double* acc, unsigned long long* n_bad)
+{
+#pragma clang fp contract(off)
+    constexpr int NF = NS + 3;
+    const int L = c.L;
+    const int64_t L2 = (int64_t)L * L;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L2) return;
+    const int y = (int)(i / L), x = (int)(i % L);
+    const int up = (y == 0 ? L - 1 : y - 1) * L + x, dn = (y == L - 1 ? 0 : y + 1) * L + x;
+    const int lf = y * L + (x == 0 ? L - 1 : x - 1), rt = y * L + (x == L - 1 ? 0 : x + 1);
+    double f[NF];
+    fke_rhs<NS>(c, src, L2, (int)i, up, dn, lf, rt, f);
+    uint32_t bad = 0;
+#pragma unroll
+    for (int s = 0; s < NF; ++s) {
+        const int64_t j = s * L2 + i;
+        if (stage == 1) {
+            acc[j] = f[s];
+            nxt[j] = rho[j] + c.dt2 * f[s];
+        } else if (stage < 4) {
+            acc[j] = acc[j] + 2.0 * f[s];
+            nxt[j] = rho[j] + (stage == 2 ? c.dt2 : c.dt) * f[s];
+        } else {
+            const double r = rho[j] + c.dt6 * (acc[j] + f[s]);
+            rho[j] = r;
+            bad += (r < 0 || !__builtin_isfinite(r)) ? 1u : 0u;
+        }
+    }
+    if (stage == 4 && bad) atomicAdd(n_bad, (unsigned long long)bad);
+}
+
+bool fke_rk_args(int ns, int L, const tm_fke_rk_params* p, FkeRk* c)
+{
+#pragma clang fp contract(off)
+    if ((ns != 1 && ns != 2) || L < 1 || !p || (int64_t)L * L > 0x7FFFFFFFll) return false;
+    if (!(p->dx > 0.0) || !(p->dt_rk > 0.0) || !std::isfinite(p->dt)) return false;
+    const double dx2 = p->dx * p->dx;
+    c->ns = ns;
+    c->L = L;
+    c->a = p->a;
+    c->cgmax = p->c * p->gmax;
+    c->gmax = p->gmax;
+    c->alpha = p->alpha;
+    c->rW = p->rW;
+    c->K1 = p->K[1];
+    c->K2 = p->K[2];
+    c->K2W0 = p->K[2] * p->W0;
+    c->A01 = p->A[0][1];
+    c->A01H01 = p->A[0][1] * p->H[0][1];
+    c->EA1 = p->E[1] * p->A[0][1];
+    c->M1 = p->M[1];
+    c->A12 = p->A[1][2];
+    c->A12H12 = p->A[1][2] * p->H[1][2];
+    c->EA2 = p->E[2] * p->A[1][2];
+    c->M2 = p->M[2];
+    c->dW = p->D_W / dx2;
+    c->dO = p->D_O / dx2;
+    c->dt = p->dt;
+    c->dt2 = p->dt_rk / 2.0;
+    c->dt6 = p->dt_rk / 6.0;
+    return true;
+}
+
+// The four stage launches on stream st; scratch holds 3 NF L^2 doubles.
+void fke_rk_launch(const FkeRk& c, double* fields, double* scratch, unsigned long long* n_bad, hipStream_t st)
+{
+    const int64_t L2 = (int64_t)c.L * c.L;
+    const int64_t nf = (int64_t)(c.ns + 3) * L2;
+    double *sa = scratch, *sb = scratch + nf, *acc = scratch + 2 * nf;
+    const dim3 grid((unsigned)((L2 + 255) / 256));
+    const double* src[4] = {fields, sa, sb, sa};
+    double* nxt[4] = {sa, sb, sa, nullptr};
+    for (int s = 1; s <= 4; ++s) {
+        if (c.ns == 1)
+            hipLaunchKernelGGL(fke_rk_stage_kernel<1>, grid, dim3(256), 0, st, c, s, fields, src[s - 1], nxt[s - 1], acc,
+                               n_bad);
+        else
+            hipLaunchKernelGGL(fke_rk_stage_kernel<2>, grid, dim3(256), 0, st, c, s, fields, src[s - 1], nxt[s - 1], acc,
+                               n_bad);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -435,6 +564,52 @@ int tm_fke_perception_host(int device, int ns, int L, const double* pR, double d
     if (hipMemcpy(gamma, d_gam.p, fb * (ns - 1), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(v_eff, d_v.p, 2 * fb * ns, hipMemcpyDeviceToHost) != hipSuccess)
         return TM_ERR_HIP;
+    return TM_OK;
+}
+
+int tm_fke_rk4(int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t* n_bad, void* stream)
+{
+    FkeRk c;
+    if (!fields || !fke_rk_args(ns, L, p, &c)) return TM_ERR_ARG;
+    const hipStream_t st = (hipStream_t)stream;
+    const size_t nf = (size_t)(ns + 3) * L * L;
+    DevBuf d_s, d_bad;
+    if (hipMalloc(&d_s.p, sizeof(double) * 3 * nf) != hipSuccess ||
+        hipMalloc(&d_bad.p, sizeof(unsigned long long)) != hipSuccess)
+        return TM_ERR_NOMEM;
+    unsigned long long bad = 0;
+    if (hipMemcpyAsync(d_bad.p, &bad, sizeof(bad), hipMemcpyHostToDevice, st) != hipSuccess) return TM_ERR_HIP;
+    fke_rk_launch(c, fields, (double*)d_s.p, (unsigned long long*)d_bad.p, st);
+    if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
+    // the scratch is freed on return: wait for the stages that use it
+    if (hipMemcpyAsync(&bad, d_bad.p, sizeof(bad), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return TM_ERR_HIP;
+    if (n_bad) *n_bad = (int64_t)bad;
+    return TM_OK;
+}
+
+int tm_fke_rk4_host(int device, int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t* n_bad)
+{
+    FkeRk c;
+    if (!fields || !fke_rk_args(ns, L, p, &c)) return TM_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return TM_ERR_NODEV;
+    const size_t nf = (size_t)(ns + 3) * L * L;
+    DevBuf d_f, d_s, d_bad;
+    if (hipMalloc(&d_f.p, sizeof(double) * nf) != hipSuccess ||
+        hipMalloc(&d_s.p, sizeof(double) * 3 * nf) != hipSuccess ||
+        hipMalloc(&d_bad.p, sizeof(unsigned long long)) != hipSuccess)
+        return TM_ERR_NOMEM;
+    unsigned long long bad = 0;
+    if (hipMemcpy(d_f.p, fields, sizeof(double) * nf, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_bad.p, &bad, sizeof(bad), hipMemcpyHostToDevice) != hipSuccess)
+        return TM_ERR_HIP;
+    fke_rk_launch(c, (double*)d_f.p, (double*)d_s.p, (unsigned long long*)d_bad.p, nullptr);
+    if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
+    if (hipMemcpy(fields, d_f.p, sizeof(double) * nf, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&bad, d_bad.p, sizeof(bad), hipMemcpyDeviceToHost) != hipSuccess)
+        return TM_ERR_HIP;
+    if (n_bad) *n_bad = (int64_t)bad;
     return TM_OK;
 }
 

@@ -416,6 +416,19 @@ int tm_fke_rk4(int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t
 /* The same step on host arrays, on `device`. */
 int tm_fke_rk4_host(int device, int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t* n_bad);
 
+/* The Rietkerk consumer model's vegetation Dornic sub-step (rietkerk_bjork_FKE.cpp:
+ * 4150-4211): Pstar [L*L] from the old vegetation P [L*L], periodic lattice, with
+ * the constants of FKE:3847-3849 -- pass the vegetation's mortality M[0] as p->d,
+ * and p->D[0], p->sigma, p->dt, p->dx, p->sampler; the other fields are unused.
+ * The same transition, Philox stream (seed, inst; cell, step) and absorbing skip as
+ * the single-species step's P Dornic.  No flush (the reference flushes after the
+ * movement step, FKE:4453-4462).  Device arrays on `stream` (Pstar must not alias
+ * P), or host arrays on `device`.  Returns 0 or a negative tm_status. */
+int tm_fke_veg_dornic(int L, const tm_rietkerk_params* p, uint32_t seed, uint32_t inst, int64_t step,
+                      const double* P, double* Pstar, void* stream);
+int tm_fke_veg_dornic_host(int device, int L, const tm_rietkerk_params* p, uint32_t seed, uint32_t inst,
+                           int64_t step, const double* P, double* Pstar);
+
 /* Host only.  The perception neighbourhood of the consumer models: offsets
  * (row, column) of the sites within Euclidean radius R of a site, in the order of
  * computeNeighboringSitesCentral (multiSPDP.cpp:1442-1461): generated row-major

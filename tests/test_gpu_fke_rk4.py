@@ -73,3 +73,27 @@ def test_rejects_bad_arguments(tm):
         tm.fke_rk4(np.ones((6, 8, 8)), pg)          # ns = 3 consumers is not a model
     with pytest.raises(tm.TMError, match="TM_ERR_ARG"):
         tm.fke_rk4(np.ones((4, 8, 8)), tm.FKERKParams(dx=0.0))
+
+
+@pytest.mark.parametrize("sampler,L", [(0, 40), (1, 33)])
+def test_vegetation_dornic_matches_oracle(tm, oracle, sampler, L):
+    """tm_fke_veg_dornic against the oracle's Rietkerk step with no plant growth
+    (c = 0): its RK4 then leaves P at P* exactly, so its P after one step is P* with
+    the step's flush (below 1e-10 to 0), which the check applies to the operator's
+    output.  Includes an absorbing patch, large densities (the compat sampler's
+    Gaussian shortcuts above 100) and a step counter above 2^32."""
+    rng = np.random.default_rng(L)
+    f = np.stack([rng.uniform(0, 90, (L, L)), rng.random((L, L)), rng.random((L, L))])
+    f[0, 3:9, 4:12] = 0.0
+    f[0, 20:24, 2:6] = 400.0
+    d, D, sigma, dt, dx = 0.25, (0.1, 0.1, 100.0), 0.5, 0.005, 2.0
+    step = (1 << 32) + 17
+    print(f"veg dornic sampler={sampler} L={L}", flush=True)
+    po = oracle.rk_params(c=0.0, d=d, D=D, sigma=sigma, dt=dt, dx=dx, sampler=sampler)
+    ref = oracle.rk_run(po, f, L, 7, 3, step, 1, rows=False)
+    assert ref["status"] == 0
+    pg = tm.RietkerkParams(c=0.0, d=d, D=D, sigma=sigma, dt=dt, dx=dx, sampler=sampler)
+    got = tm.fke_veg_dornic(f[0], pg, 7, 3, step)
+    got = np.where((got < 1e-10) & (got >= 0.0), 0.0, got)
+    assert np.array_equal(got, ref["fields"][0])
+    assert np.all(got[4:8, 5:11] == 0.0)

@@ -294,6 +294,20 @@ def fke_rk4(fields, params: FKERKParams, device: int = 0):
     return f, int(bad.value)
 
 
+def fke_veg_dornic(P, params: RietkerkParams, seed: int, inst: int, step: int, device: int = 0):
+    """The Rietkerk consumer model's vegetation Dornic sub-step on the GPU
+    (tm_fke_veg_dornic_host): P* from P [L, L].  params.d is the vegetation's
+    mortality M[0]; D[0], sigma, dt, dx and sampler enter."""
+    p = np.ascontiguousarray(P, np.float64)
+    out = np.empty_like(p)
+    c = params._c()
+    rc = lib().tm_fke_veg_dornic_host(device, p.shape[-1], C.cast(C.byref(c), C.c_void_p), seed, inst, step,
+                                      _dptr(p), _dptr(out))
+    if rc != 0:
+        raise TMError(rc, "tm_fke_veg_dornic_host failed")
+    return out
+
+
 @dataclass
 class Instance:
     L: int
@@ -365,6 +379,9 @@ def lib() -> C.CDLL:
                                     dp, dp], C.c_int),
         "tm_fke_rk4_host": ([C.c_int, C.c_int, C.c_int, C.c_void_p, dp, C.POINTER(C.c_int64)], C.c_int),
         "tm_fke_rk4": ([C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p], C.c_int),
+        "tm_fke_veg_dornic_host": ([C.c_int, C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, i64, dp, dp], C.c_int),
+        "tm_fke_veg_dornic": ([C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, i64, C.c_void_p, C.c_void_p, C.c_void_p],
+                              C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -23,6 +23,7 @@
 
 #include "tm/tm.h"
 #include "tm_device.h"
+#include "tm_kernels.h"
 
 namespace {
 
@@ -408,6 +409,37 @@ void fke_rk_launch(const FkeRk& c, double* fields, double* scratch, unsigned lon
     }
 }
 
+// ---------------------------------------------------------------------------
+// Vegetation Dornic sub-step of the Rietkerk consumer model (FKE:4150-4211):
+// the same transition, constants and Philox stream as the single-species
+// step's P Dornic (cell_update MODE 1), with the vegetation's mortality M[0] as
+// d (beta_0 = -M[0] - 4 D_0 / dx^2, FKE:3847-3849).  No flush: the reference
+// flushes after the movement step.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fke_veg_dornic_kernel(tm_rietkerk_params p, int L, uint32_t seed,
+                                                             uint32_t inst, int64_t step, const double* P,
+                                                             double* Pstar)
+{
+    const int64_t L2 = (int64_t)L * L;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L2) return;
+    const tmk::RKConstD c = tmk::rk_constants(p, step);
+    tmk::DevInst di = {};
+    di.seed = seed;
+    di.inst_id = inst;
+    const int y = (int)(i / L), x = (int)(i % L);
+    const int up = (y == 0 ? L - 1 : y - 1) * L + x, dn = (y == L - 1 ? 0 : y + 1) * L + x;
+    const int lf = y * L + (x == 0 ? L - 1 : x - 1), rt = y * L + (x == L - 1 ? 0 : x + 1);
+    int64_t n;
+    bool bad;
+    Pstar[i] = tmk::cell_update<double, 1>(c, di, (uint32_t)i, P[i], P[up], P[dn], P[lf], P[rt], n, bad);
+}
+
+bool veg_dornic_ok(int L, const tm_rietkerk_params* p)
+{
+    return L >= 1 && (int64_t)L * L <= 0x7FFFFFFFll && p && p->dx > 0.0 && p->sigma > 0.0 && p->dt > 0.0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -611,6 +643,30 @@ int tm_fke_rk4_host(int device, int ns, int L, const tm_fke_rk_params* p, double
         return TM_ERR_HIP;
     if (n_bad) *n_bad = (int64_t)bad;
     return TM_OK;
+}
+
+int tm_fke_veg_dornic(int L, const tm_rietkerk_params* p, uint32_t seed, uint32_t inst, int64_t step,
+                      const double* P, double* Pstar, void* stream)
+{
+    if (!P || !Pstar || P == Pstar || !veg_dornic_ok(L, p)) return TM_ERR_ARG;
+    const int64_t L2 = (int64_t)L * L;
+    hipLaunchKernelGGL(fke_veg_dornic_kernel, dim3((unsigned)((L2 + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       *p, L, seed, inst, step, P, Pstar);
+    return hipGetLastError() == hipSuccess ? TM_OK : TM_ERR_HIP;
+}
+
+int tm_fke_veg_dornic_host(int device, int L, const tm_rietkerk_params* p, uint32_t seed, uint32_t inst,
+                           int64_t step, const double* P, double* Pstar)
+{
+    if (!P || !Pstar || !veg_dornic_ok(L, p)) return TM_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return TM_ERR_NODEV;
+    const size_t fb = sizeof(double) * (size_t)L * L;
+    DevBuf d_p, d_s;
+    if (hipMalloc(&d_p.p, fb) != hipSuccess || hipMalloc(&d_s.p, fb) != hipSuccess) return TM_ERR_NOMEM;
+    if (hipMemcpy(d_p.p, P, fb, hipMemcpyHostToDevice) != hipSuccess) return TM_ERR_HIP;
+    int rc = tm_fke_veg_dornic(L, p, seed, inst, step, (const double*)d_p.p, (double*)d_s.p, nullptr);
+    if (rc != TM_OK) return rc;
+    return hipMemcpy(Pstar, d_s.p, fb, hipMemcpyDeviceToHost) == hipSuccess ? TM_OK : TM_ERR_HIP;
 }
 
 }  // extern "C"

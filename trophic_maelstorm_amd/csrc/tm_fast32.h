@@ -86,12 +86,19 @@ __device__ __forceinline__ int64_t poisson_f(Stream& s, double zd, bool compat)
         const float v = s.v32();
         float p = __expf(-z);                   // P(N = 0)
         float S = neg_expm1_neg(z, p);          // P(N > 0)
+        // Two terms per trip with one exit test: the same terms and decisions as
+        // one term per trip, with half the loop's mask bookkeeping.  A trip that
+        // ends after its first term returns k + 1.
         int k = 0;
         while (v < S && k < 200) {
-            k += 1;
-            p = p * z * rcpf((float)k);
-            if (p <= S * 0x1p-24f) break;
-            S = S - p;
+            const float p1 = p * z * rcpf((float)(k + 1));
+            const float S1 = S - p1;
+            const bool go2 = !(p1 <= S * 0x1p-24f) && v < S1 && k + 1 < 200;
+            const float p2 = p1 * z * rcpf((float)(k + 2));
+            k += go2 ? 2 : 1;
+            if (!go2 || p2 <= S1 * 0x1p-24f) break;
+            p = p2;
+            S = S1 - p2;
         }
         return k;
     }

@@ -84,6 +84,29 @@ struct Stream {
     __device__ float v32() { return ((float)(~word()) + 0.5f) * 0x1p-32f; }
 };
 
+// The first four words of a cell's stream, computed at once: block 0 of the
+// same Philox stream, handed out in the same order, with no refill check.  For
+// callers that provably draw at most four words (the first pass of the
+// deferring FP32 step); a fifth word would repeat the fourth.
+struct Stream4 {
+    uint32_t b0, b1, b2, b3;
+
+    __device__ Stream4(uint32_t seed, uint32_t inst, uint32_t cell, int64_t step)
+    {
+        const Philox4 r = philox4x32_10(cell, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), 0u, seed,
+                                        inst);
+        b0 = r.v0; b1 = r.v1; b2 = r.v2; b3 = r.v3;
+    }
+    __device__ uint32_t word()
+    {
+        const uint32_t w = b0;
+        b0 = b1; b1 = b2; b2 = b3;
+        return w;
+    }
+    __device__ float u32() { return ((float)(word() >> 9) + 0.5f) * 0x1p-23f; }
+    __device__ float v32() { return ((float)(~word()) + 0.5f) * 0x1p-32f; }
+};
+
 // ---------------------------------------------------------------------------
 // FP64 exp / log written from the mathematics: Cody-Waite reduction by ln 2 and
 // a degree-13 Taylor polynomial; log via m = 2^-e x in [sqrt(1/2), sqrt 2) and

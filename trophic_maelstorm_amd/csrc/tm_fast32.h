@@ -67,8 +67,10 @@ __device__ __forceinline__ float stirlerr_f(float k)
     return r * (0.0833333333f - r2 * (0.00277777778f - r2 * 0.000793650794f));
 }
 
-// Poisson(z), z formed in FP64 by the caller.
-__device__ __forceinline__ int64_t poisson_f(Stream& s, double zd, bool compat)
+// Poisson(z), z formed in FP64 by the caller.  St: Stream, or Stream4 where at
+// most four words are drawn.
+template <class St>
+__device__ __forceinline__ int64_t poisson_f(St& s, double zd, bool compat)
 {
     if (!(zd > 0.0)) return 0;
     if (compat && zd > 100.0) {
@@ -137,8 +139,8 @@ __device__ __forceinline__ int64_t poisson_f(Stream& s, double zd, bool compat)
 // sampler does).  With fewer it returns false, and leaves s and g unspecified,
 // where the sampler would have drawn one more trial: the caller then starts the
 // transition again from its first word with the full sampler.
-template <int TRIALS>
-__device__ __forceinline__ bool gamma_f_try(Stream& s, float shape, float& g)
+template <int TRIALS, class St>
+__device__ __forceinline__ bool gamma_f_try(St& s, float shape, float& g)
 {
     if (!(shape > 0.0f)) { g = 0.0f; return true; }
     const float a = shape < 1.0f ? shape + 1.0f : shape;
@@ -172,7 +174,7 @@ __device__ __forceinline__ bool gamma_f_try(Stream& s, float shape, float& g)
 __device__ __forceinline__ float gamma_f(Stream& s, float shape)
 {
     float g;
-    gamma_f_try<64>(s, shape, g);
+    gamma_f_try<64, Stream>(s, shape, g);
     return g;
 }
 

@@ -191,8 +191,8 @@ __device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double 
 // returns false, leaving n and y unspecified, where the transition needs the
 // PTRS sampler (Poisson mean >= 12) or a second Gamma trial; otherwise it draws
 // the same words and computes the same value as ONE = false.
-template <bool ONE>
-__device__ __forceinline__ bool dornic_cell_f(Stream& s, double alpha_i, float cur, const DornicConst& k,
+template <bool ONE, class St>
+__device__ __forceinline__ bool dornic_cell_f(St& s, double alpha_i, float cur, const DornicConst& k,
                                               bool compat, int64_t& n, float& y)
 {
 #pragma clang fp contract(off)
@@ -205,7 +205,7 @@ __device__ __forceinline__ bool dornic_cell_f(Stream& s, double alpha_i, float c
     if (compat && gru > 100.0) {
         const float gf = (float)gru;
         g = norm_icdf_f(s.u32()) * __builtin_sqrtf(gf) + gf;
-    } else if (!gamma_f_try<ONE ? 1 : 64>(s, (float)gru, g)) {
+    } else if (!gamma_f_try<ONE ? 1 : 64, St>(s, (float)gru, g)) {
         return false;
     }
     // g / lambda with 1/lambda carried as hi + lo: a single FP32 rounding of 1/lambda
@@ -218,7 +218,7 @@ __device__ __forceinline__ float dornic_cell(Stream& s, double alpha_i, float cu
                                              bool compat, int64_t& n)
 {
     float y;
-    dornic_cell_f<false>(s, alpha_i, cur, k, compat, n, y);
+    dornic_cell_f<false, Stream>(s, alpha_i, cur, k, compat, n, y);
     return y;
 }
 
@@ -470,8 +470,10 @@ __device__ __forceinline__ bool cell_update_first(const DPConst& c, const DevIns
         y = cur;
         const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
         if (!(alpha_i == 0.0 && cur == 0.0f)) {
-            Stream st(di.seed, di.inst_id, cell, c.step);
-            if (!dornic_cell_f<true>(st, alpha_i, cur, c.k, c.compat, n, y)) return false;
+            // at most four words: the Poisson inversion's, one Gamma trial's two and
+            // the boost's (PTRS means and rejected trials return false instead)
+            Stream4 st(di.seed, di.inst_id, cell, c.step);
+            if (!dornic_cell_f<true, Stream4>(st, alpha_i, cur, c.k, c.compat, n, y)) return false;
         }
         y = dp_nonlinear<float>(c.scheme, (float)c.b, (float)c.dt, y);
         bad = y < 0.0f || !Arith<float>::finite(y);

@@ -103,6 +103,7 @@ struct Stream4 {
         b0 = b1; b1 = b2; b2 = b3;
         return w;
     }
+    __device__ double u64() { return ((double)word() + 0.5) * 0x1p-32; }
     __device__ float u32() { return ((float)(word() >> 9) + 0.5f) * 0x1p-23f; }
     __device__ float v32() { return ((float)(~word()) + 0.5f) * 0x1p-32f; }
 };
@@ -310,7 +311,8 @@ TM_HD double logfact(int64_t k)
 // transformed rejection (Hoermann 1993).  compat: the reference's Gaussian
 // shortcut trunc(N sqrt z + z) above 100 (rietkerk_bjork_FKE.cpp:4175-4180).
 // ---------------------------------------------------------------------------
-TM_HD int64_t poisson_d(Stream& s, double z, bool compat)
+template <class St>
+TM_HD int64_t poisson_d(St& s, double z, bool compat)
 {
 #pragma clang fp contract(off)
     if (!(z > 0.0)) return 0;
@@ -349,26 +351,39 @@ TM_HD int64_t poisson_d(Stream& s, double z, bool compat)
 }
 
 // Gamma(shape, 1), Marsaglia & Tsang (2000) with the U^(1/shape) boost below
-// shape 1; shape <= 0 gives 0 (the Dornic shape-0 case).  FP64 version.
-TM_HD double gamma_d(Stream& s, double shape)
+// shape 1; shape <= 0 gives 0 (the Dornic shape-0 case).  FP64 version, at most
+// TRIALS trials: with TRIALS = 64 it always returns true (64 rejections leave
+// g = d); with fewer it returns false, leaving s and g unspecified, where the
+// sampler would draw one more trial.
+template <int TRIALS, class St>
+TM_HD bool gamma_d_try(St& s, double shape, double& g)
 {
 #pragma clang fp contract(off)
-    if (!(shape > 0.0)) return 0.0;
+    if (!(shape > 0.0)) { g = 0.0; return true; }
     const double a = shape < 1.0 ? shape + 1.0 : shape;
     const double d = a - 0x1.5555555555555p-2;
     const double c = 1.0 / dsqrt(9.0 * d);
-    double g = d;
-    for (int it = 0; it < 64; ++it) {
+    g = d;
+    bool ok = false;
+    for (int it = 0; it < TRIALS; ++it) {
         const double x = norm_icdf_d(s.u64());
         double v = 1.0 + c * x;
         if (v <= 0.0) continue;
         v = v * v * v;
         const double u = s.u64();
         const double x2 = x * x;
-        if (u < 1.0 - 0.0331 * x2 * x2) { g = d * v; break; }
-        if (dlog(u) < 0.5 * x2 + d * (1.0 - v + dlog(v))) { g = d * v; break; }
+        if (u < 1.0 - 0.0331 * x2 * x2) { g = d * v; ok = true; break; }
+        if (dlog(u) < 0.5 * x2 + d * (1.0 - v + dlog(v))) { g = d * v; ok = true; break; }
     }
+    if (TRIALS < 64 && !ok) return false;
     if (shape < 1.0) g = g * dexp(dlog(s.u64()) / shape);
+    return true;
+}
+
+TM_HD double gamma_d(Stream& s, double shape)
+{
+    double g;
+    gamma_d_try<64, Stream>(s, shape, g);
     return g;
 }
 

@@ -166,12 +166,16 @@ template <> struct Arith<float> {
 };
 
 // Dornic Poisson-Gamma transition of one live cell (rietkerk_bjork_FKE.cpp:4168-4196).
-__device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double cur, const DornicConst& k,
-                                              bool compat, int64_t& n)
+// ONE = true: as the FP32 dornic_cell_f<true> -- false where the transition needs
+// PTRS (mean >= 12) or a second Gamma trial, otherwise the same words and value.
+template <bool ONE, class St>
+__device__ __forceinline__ bool dornic_cell_d(St& s, double alpha_i, double cur, const DornicConst& k,
+                                              bool compat, int64_t& n, double& y)
 {
 #pragma clang fp contract(off)
     const double mu = -1.0 + 2.0 * alpha_i * k.sigma2_1;
     const double ziggy = k.ll * cur;
+    if (ONE && ziggy >= 12.0) return false;
     double gru;
     if (ziggy == 0.0) {
         n = 0;
@@ -180,8 +184,22 @@ __device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double 
         n = poisson_d(s, ziggy, compat);
         gru = mu + 1.0 + (double)n;
     }
-    if (compat && gru > 100.0) return (norm_icdf_d(s.u64()) * dsqrt(gru) + gru) / k.lambda;
-    return gamma_d(s, gru) / k.lambda;
+    if (compat && gru > 100.0) {
+        y = (norm_icdf_d(s.u64()) * dsqrt(gru) + gru) / k.lambda;
+        return true;
+    }
+    double g;
+    if (!gamma_d_try<ONE ? 1 : 64, St>(s, gru, g)) return false;
+    y = g / k.lambda;
+    return true;
+}
+
+__device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double cur, const DornicConst& k,
+                                              bool compat, int64_t& n)
+{
+    double y;
+    dornic_cell_d<false, Stream>(s, alpha_i, cur, k, compat, n, y);
+    return y;
 }
 
 // FP32 storage: alpha_i, the Poisson mean and the Gamma shape in FP64 (the same
@@ -459,25 +477,30 @@ __device__ __forceinline__ T cell_update(const typename StepConst<MODE>::type& c
 // First pass of the deferring FP32 DP step: cell_update<float, 0> where the
 // one-trial transition succeeds (returns true, same y, n and bad); false where
 // the cell has to be computed again by cell_update.
-__device__ __forceinline__ bool cell_update_first(const DPConst& c, const DevInst& di, uint32_t cell, float cur,
-                                                  float up, float dn, float lf, float rt, float& y, int64_t& n,
-                                                  bool& bad)
+template <typename T>
+__device__ __forceinline__ bool cell_update_first(const DPConst& c, const DevInst& di, uint32_t cell, T cur, T up,
+                                                  T dn, T lf, T rt, T& y, int64_t& n, bool& bad)
 {
-    y = 0.0f;
+    y = (T)0;
     n = -1;
     bad = false;
-    if (!(cur == 0.0f && up == 0.0f && dn == 0.0f && lf == 0.0f && rt == 0.0f)) {
+    if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
         y = cur;
         const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
-        if (!(alpha_i == 0.0 && cur == 0.0f)) {
+        if (!(alpha_i == 0.0 && cur == (T)0)) {
             // at most four words: the Poisson inversion's, one Gamma trial's two and
             // the boost's (PTRS means and rejected trials return false instead)
             Stream4 st(di.seed, di.inst_id, cell, c.step);
-            if (!dornic_cell_f<true, Stream4>(st, alpha_i, cur, c.k, c.compat, n, y)) return false;
+            bool done;
+            if constexpr (sizeof(T) == 4)
+                done = dornic_cell_f<true, Stream4>(st, alpha_i, cur, c.k, c.compat, n, y);
+            else
+                done = dornic_cell_d<true, Stream4>(st, alpha_i, cur, c.k, c.compat, n, y);
+            if (!done) return false;
         }
-        y = dp_nonlinear<float>(c.scheme, (float)c.b, (float)c.dt, y);
-        bad = y < 0.0f || !Arith<float>::finite(y);
-        if (y < (float)c.eps && y >= (float)c.neg) y = 0.0f;
+        y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
+        bad = y < (T)0 || !Arith<T>::finite(y);
+        if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
     }
     return true;
 }
@@ -618,19 +641,19 @@ __device__ __forceinline__ void dp_tile(const StepArgs& a, int tile, const DevTi
 // thread and a workgroup prefix sum), so which thread finishes which cell, and
 // with it the rho(t) partials, is the same in every run.  It holds every cell
 // of the tile (16-bit tile offsets), so it cannot overflow.
-template <int TX, int TY, int NT>
+template <typename T, int TX, int TY, int NT>
 __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, const DevTile& tl, const DevInst& di,
-                                                 const DPConst& c, const float* s_rho, int64_t counter,
+                                                 const DPConst& c, const T* s_rho, int64_t counter,
                                                  int64_t slot, uint16_t* s_q, uint32_t* s_wn)
 {
     static_assert(TX * TY <= 65536, "16-bit tile offsets");
-    static_assert(DPShape<float, TX, TY, NT>::G * 4 <= 32, "one mask bit per cell of a thread");
-    typedef DPShape<float, TX, TY, NT> S;
+    static_assert(DPShape<T, TX, TY, NT>::G * 4 <= 32, "one mask bit per cell of a thread");
+    typedef DPShape<T, TX, TY, NT> S;
     constexpr int V = S::V, RWV = S::RWV, R4 = S::R4;
     const int tid = threadIdx.x;
     const int cg = tid % S::CG, ly0 = tid / S::CG;
     const int L = tl.L;
-    float* fout = (float*)a.out + tl.cell_off;
+    T* fout = (T*)a.out + tl.cell_off;
     const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
     double sum[1] = {0.0};
     uint32_t nlive = 0;
@@ -648,8 +671,8 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
             bits |= as_bits(s_rho[r + q]) | as_bits(s_rho[r - RWV + q]) | as_bits(s_rho[r + RWV + q]);
         bits |= as_bits(s_rho[r - 1]) | as_bits(s_rho[r + 4]);
         const int nvalid = min(4, L - gx0);
-        Quad<float> yq;
-        yq.v[0] = yq.v[1] = yq.v[2] = yq.v[3] = 0.0f;
+        Quad<T> yq;
+        yq.v[0] = yq.v[1] = yq.v[2] = yq.v[3] = (T)0;
         if (bits == 0 && full) {
             if (a.draws) {
 #pragma unroll
@@ -660,19 +683,19 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
             for (int q = 0; q < nvalid; ++q) {
                 const int rq = r + q;
                 const uint32_t cell = cell0 + q;
-                float y;
+                T y;
                 int64_t n;
                 bool bad;
-                const bool done = cell_update_first(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
+                const bool done = cell_update_first<T>(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
                                                     s_rho[rq - 1], s_rho[rq + 1], y, n, bad);
                 if (!done) {
                     dmask |= 1u << (4 * j + q);
-                    y = 0.0f;
+                    y = (T)0;
                     n = -1;
                 } else {
                     if (bad) record_fault(a.state, tl.inst, cell, counter);
                     sum[0] += (double)y;
-                    nlive += y > 0.0f ? 1u : 0u;
+                    nlive += y > (T)0 ? 1u : 0u;
                 }
                 if (a.draws) a.draws[tl.cell_off + cell] = n;
                 yq.v[0] = q == 0 ? y : yq.v[0];
@@ -718,12 +741,12 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
         const uint32_t cell = (uint32_t)(tl.y0 + ly) * (uint32_t)L + (uint32_t)(tl.x0 + lx);
         int64_t n;
         bool bad;
-        const float y = cell_update<float, 0>(c, di, cell, s_rho[r], s_rho[r - RWV], s_rho[r + RWV], s_rho[r - 1],
+        const T y = cell_update<T, 0>(c, di, cell, s_rho[r], s_rho[r - RWV], s_rho[r + RWV], s_rho[r - 1],
                                               s_rho[r + 1], n, bad);
         if (bad) record_fault(a.state, tl.inst, cell, counter);
         if (a.draws) a.draws[tl.cell_off + cell] = n;
         sum[0] += (double)y;
-        nlive += y > 0.0f ? 1u : 0u;
+        nlive += y > (T)0 ? 1u : 0u;
         fout[cell] = y;
     }
     double2* dst = a.partials + (slot * a.ntiles + tile) * S::NW + (tid >> 6);
@@ -748,7 +771,7 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
     typedef DPShape<T, TX, TY, NT> S;
     typedef typename StepConst<MODE>::type Const;
     constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
-    static_assert(Q == 0 || (sizeof(T) == 4 && MODE == 0), "deferring step: FP32 DP only");
+    static_assert(Q == 0 || MODE == 0, "deferring step: DP only");
     __shared__ __align__(16) uint32_t s_img[S::NBUF];
     __shared__ uint32_t s_flag[NT / 64];
     __shared__ uint16_t s_q[Q ? TX * TY : 1];         // deferred cells (tile offsets)
@@ -772,7 +795,7 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
         return;
     }
     if constexpr (Q != 0)
-        dp_tile_deferred<TX, TY, NT>(a, tile, tl, di, c, (const float*)s_img, counter, slot, s_q, s_wn);
+        dp_tile_deferred<T, TX, TY, NT>(a, tile, tl, di, c, (const T*)s_img, counter, slot, s_q, s_wn);
     else
         dp_tile<T, TX, TY, NT, MODE>(a, tile, tl, di, c, (const T*)s_img, counter, slot);
 }

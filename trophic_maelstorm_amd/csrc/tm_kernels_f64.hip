@@ -6,10 +6,10 @@
 
 namespace tmk {
 
-template <typename T, int TX, int TY, int NT, int MODE = 0, int MINW = 1>
+template <typename T, int TX, int TY, int NT, int MODE = 0, int MINW = 1, int Q = 0>
 static void launch_dp(const StepArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, MODE, MINW>), dim3(grid), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((dp_step_kernel<T, TX, TY, NT, MODE, MINW, Q>), dim3(grid), dim3(NT), 0, s, a);
 }
 
 template <typename T, int TX, int TY, int NT, int K, int MINW = 1>
@@ -47,6 +47,10 @@ static const Variant kDP[] = {
      &launch_dp<double, 64, 32, 256>, 20},
     {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x8a", &launch_dpk<double, 64, 32, 256, 8>, 0, 0, nullptr, 0, 8,
      &launch_dp<double, 64, 32, 256>, 20},
+    // _q: the single steps defer cells whose Gamma trial is rejected to a second
+    // pass over a queue in LDS (dp_tile_deferred)
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x8aq", &launch_dpk<double, 64, 32, 256, 8>, 0, 0, nullptr, 0, 8,
+     &launch_dp<double, 64, 32, 256, 0, 1, 1>, 20},
 };
 
 static const Variant kRK[] = {

@@ -250,7 +250,7 @@ def test_autotune_does_not_change_state(tm, oracle, tmp_path):
 
 
 @pytest.mark.parametrize("variant", ["dp_f64_64x32_t256_x4a", "dp_f64_64x32_t256_x4", "dp_f64_64x32_t256_x8aq",
-                                     "dp_f64_64x32_t256_x8a",
+                                     "dp_f64_64x32_t256_x8aq_w6", "dp_f64_64x32_t256_x8a",
                                      "dp_f64_64x32_t256_x8", "dp_f64_64x8_t128_x16"])
 def test_multistep_variant_through_extinction_bit_exact(tm, oracle, variant):
     """A DP batch that dies out (a = 0.3) over 300 steps in blocks of 10: the

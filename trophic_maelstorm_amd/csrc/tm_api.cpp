@@ -1507,7 +1507,7 @@ int default_variant(int model, int precision, int64_t total_cells)
         // adaptive eight-step launches: single steps while more than 2% of the
         // cells are occupied (the two-pass step that defers rejected Gamma
         // trials), eight steps per launch once the field is mostly absorbing
-        want = precision == TM_F64 ? "dp_f64_64x32_t256_x8aq" : "dp_f32_128x24_t256_x8aq";
+        want = precision == TM_F64 ? "dp_f64_64x32_t256_x8aq_w6" : "dp_f32_128x24_t256_x8aq";
     } else {
         // Rietkerk: small lattices are latency-bound and want the single fused
         // 16 x 16 launch; batches want the split step (each P* sampled once).

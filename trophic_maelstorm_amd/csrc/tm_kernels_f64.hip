@@ -51,6 +51,9 @@ static const Variant kDP[] = {
     // pass over a queue in LDS (dp_tile_deferred)
     {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x8aq", &launch_dpk<double, 64, 32, 256, 8>, 0, 0, nullptr, 0, 8,
      &launch_dp<double, 64, 32, 256, 0, 1, 1>, 20},
+    // _w6: its single step bounded to 80 VGPRs (six waves per SIMD)
+    {64, 32, 256, 0, 4, "dp_f64_64x32_t256_x8aq_w6", &launch_dpk<double, 64, 32, 256, 8>, 0, 0, nullptr, 0, 8,
+     &launch_dp<double, 64, 32, 256, 0, 6, 1>, 20},
 };
 
 static const Variant kRK[] = {

@@ -132,3 +132,70 @@ def test_dp_survival_matches_reference_integrator(oracle, a):
         se = math.sqrt(max(pool * (1 - pool), 1.0 / SURV_R) * 2 / SURV_R)
         z = abs(k_ref - k_orc) / SURV_R / se
         assert z < 4.5, (a, t, k_ref, k_orc, z)
+
+
+# ---------------------------------------------------------- steady state
+SS_L, SS_R = 32, 32
+SS_T = [30.0, 40.0, 50.0, 60.0, 70.0, 80.0]
+
+
+def _ss_reference(oracle, ref, a, rho0):
+    out = np.zeros((64, 8), np.float64)
+    tm = np.array(SS_T)
+    fd = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        n = ref.ref_ssp_percolation_dornic(SS_L, a, B, D, SIGMA, DT, 1.0, SS_R, rho0, tm, len(tm), SS_T[-1],
+                                           out, out.shape[0])
+    finally:
+        os.dup2(fd, 1)
+        os.close(fd)
+        os.close(devnull)
+    rows = {round(r[2], 6): r for r in out[:n]}
+    return np.array([rows[round(t, 6)] for t in SS_T])
+
+
+def _ss_oracle(oracle, a, rho0, sigma=SIGMA):
+    p = oracle.dp_params(a=a, b=B, D=D, sigma=sigma, dt=DT, scheme=2, sampler=0, flush_eps=0.0, flush_neg=0.0)
+    nsteps = int(round(SS_T[-1] / DT))
+    idx = [int(round(t / DT)) - 1 for t in SS_T]
+    rows = np.stack([oracle.dp_run(p, rho0, SS_L, 2468, k, 0, nsteps)["rows"][idx] for k in range(SS_R)])
+    return rows                                                     # [R, times, (count, mean)]
+
+
+def _ss_zscores(ref_rows, orc_rows):
+    """Steady-state density and occupied fraction: the time average over t = 30 ... 80
+    of the ensemble mean, reference against oracle.  The standard error comes from
+    the spread of the oracle replicates' own time averages (the times within one
+    replicate are correlated, the replicates are not), applied to both sides."""
+    assert np.all(orc_rows[:, :, 0] > 0) and np.all(ref_rows[:, 5] == SS_R), "a replicate went extinct"
+    rho_orc = orc_rows[:, :, 1].mean(1)
+    occ_orc = orc_rows[:, :, 0].mean(1) / SS_L ** 2
+    out = {}
+    for name, x, r in (("density", rho_orc, ref_rows[:, 3].mean()),
+                       ("occupied", occ_orc, ref_rows[:, 6].mean() / SS_L ** 2)):
+        se = max(x.std(ddof=1), 1.0 / SS_L ** 2) / math.sqrt(SS_R)
+        out[name] = abs(r - x.mean()) / (math.sqrt(2.0) * se)
+    return out
+
+
+@pytest.mark.parametrize("a", [1.2, 1.6])
+def test_dp_steady_state_density_matches_reference_integrator(oracle, a):
+    """Active phase (a = 1.2: rho ~ 0.20, a = 1.6: rho ~ 0.39 on 32^2): the steady-state
+    density and occupied fraction of the oracle's ensemble (32 replicates, the
+    exact nonlinear update as the reference) against percolationDornic_2D's
+    (stochasticSPDP.cpp:940-1143), each within 4.5 standard errors of the
+    difference.  At a = 1.2 a planted mistake in the oracle -- the noise amplitude
+    sigma 10% too large, which lowers the density to ~0.11 -- must fail the same
+    test."""
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("reference library not built: /root/reference sources absent")
+    rho0 = oracle.init_uniform(SS_L, [0.0], [1.0], 77, 0).reshape(-1).copy()
+    ref_rows = _ss_reference(oracle, ref, a, rho0)
+    z = _ss_zscores(ref_rows, _ss_oracle(oracle, a, rho0))
+    assert z["density"] < 4.5 and z["occupied"] < 4.5, (a, z)
+    if a == 1.2:
+        z_bad = _ss_zscores(ref_rows, _ss_oracle(oracle, a, rho0, sigma=1.1 * SIGMA))
+        assert z_bad["density"] > 10.0, z_bad

@@ -399,6 +399,7 @@ struct DornicConst {
     double sigma2_1;   // 1 / sigma^2
     double inv_lambda; // 1 / lambda
     float inv_lambda_hi, inv_lambda_lo;   // 1 / lambda as an FP32 pair (the FP32 path)
+    float shape_f;     // 2 (D/dx^2) / sigma^2 in FP32: mu + 1 = shape_f (up + down + left + right)
 };
 
 TM_HD DornicConst dornic_constants(double linear, double D, double sigma, double dt, double dx)
@@ -416,6 +417,7 @@ TM_HD DornicConst dornic_constants(double linear, double D, double sigma, double
     c.inv_lambda_lo = (float)(c.inv_lambda - (double)c.inv_lambda_hi);
     c.dc = D / (dx2);
     c.sigma2_1 = 1 / (sigma * sigma);
+    c.shape_f = (float)(2.0 * c.dc * c.sigma2_1);
     return c;
 }
 

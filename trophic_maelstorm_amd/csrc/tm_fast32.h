@@ -67,8 +67,65 @@ __device__ __forceinline__ float stirlerr_f(float k)
     return r * (0.0833333333f - r2 * (0.00277777778f - r2 * 0.000793650794f));
 }
 
+// 1/k! in FP32, k = 0 .. 32 (all normal numbers; 1/34! would be subnormal).
+static constexpr float kInvFact[33] = {
+    1.0f, 1.0f, 0.5f, 0.1666666716337204f, 0.0416666679084301f, 0.008333333767950535f, 0.0013888889225199819f,
+    0.00019841270113829523f, 2.4801587642286904e-05f, 2.7557318844628753e-06f, 2.755731998149713e-07f,
+    2.5052107943679403e-08f, 2.0876755879584152e-09f, 1.6059044372074283e-10f, 1.147074536050896e-11f,
+    7.647163609812713e-13f, 4.7794772561329454e-14f, 2.8114573589663704e-15f, 1.5619206814541513e-16f,
+    8.220635078476521e-18f, 4.110317590937049e-19f, 1.9572941524685808e-20f, 8.896790959566848e-22f,
+    3.868170297964731e-23f, 1.6117375912827668e-24f, 6.446950340479164e-26f, 2.4795962847996784e-27f,
+    9.183689809957378e-29f, 3.279889164105036e-30f, 1.13099624727078e-31f, 3.769987809265653e-33f,
+    1.2161250145930974e-34f, 3.800390670603429e-36f};
+
+constexpr int kPoisTerms = 32;   // terms of the counted inversion
+
+// Poisson(z), z < 12, by inversion on the upper tail S_k = P(N > k) against
+// v = 1 - u: N = #{k >= 0 : v < S_j for every j <= k}.  S_0 = -expm1(-z) stays
+// resolved as z -> 0 (P(N >= 1) = z near the absorbing state).  Every lane of a
+// wavefront evaluates term k at the same time, so 1/k! is a literal and a term
+// costs a multiply, a fused multiply-add, a compare and an add, with no branch
+// but a wave-wide exit test every four terms: S_k = S_{k-1} - (e^{-z} z^k) / k!.
+// Returns true when the count is final within kPoisTerms terms (every z < 12
+// but with probability below 1e-6); otherwise n = kPoisTerms + 1, and S and P
+// (= e^{-z} z^kPoisTerms) hold the state for poisson_tail.
+__device__ __forceinline__ bool poisson_count(float z, float v, int& n, float& S, float& P)
+{
+    P = exp2_hw(z * -1.44269504f);          // e^{-z} = P(N = 0)
+    S = neg_expm1_neg(z, P);                // P(N > 0)
+    bool alive = v < S;
+    n = alive ? 1 : 0;
+#pragma unroll
+    for (int k = 1; k <= kPoisTerms; ++k) {
+        if (k > 1 && ((k - 1) & 3) == 0 && __builtin_amdgcn_ballot_w64(alive) == 0) break;
+        P = P * z;
+        S = __builtin_fmaf(-P, kInvFact[k], S);
+        alive = alive && v < S;
+        n += alive ? 1 : 0;
+    }
+    return !alive;
+}
+
+// The inversion past kPoisTerms terms (the sequential search, one term at a
+// time).  It stops once the next term can no longer move S at FP32 resolution:
+// a v in the rounding gap left below S maps to that extreme quantile.
+__device__ __forceinline__ int poisson_tail(float z, float v, float S, float P)
+{
+    int k = kPoisTerms;
+    float p = P * kInvFact[kPoisTerms];      // P(N = k)
+    while (k < 200) {
+        const float p1 = p * z * rcpf((float)(k + 1));
+        if (p1 <= S * 0x1p-24f) return k + 1;
+        S = S - p1;
+        k += 1;
+        if (!(v < S)) return k;
+        p = p1;
+    }
+    return k;
+}
+
 // Poisson(z), z formed in FP64 by the caller.  St: Stream, or Stream4 where at
-// most four words are drawn.
+// most four words are drawn.  z < 12: the counted inversion; z >= 12: PTRS.
 template <class St>
 __device__ __forceinline__ int64_t poisson_f(St& s, double zd, bool compat)
 {
@@ -78,31 +135,12 @@ __device__ __forceinline__ int64_t poisson_f(St& s, double zd, bool compat)
         return (int64_t)((double)(norm_icdf_f(s.u32()) * __builtin_sqrtf(zf)) + zd);
     }
     if (zd < 12.0) {
-        // Search on the upper tail S_k = P(N > k) against v = 1 - u, for every
-        // mean: S_0 = -expm1(-z) stays resolved as z -> 0, and one loop (not one
-        // per range of z) runs in a wavefront whatever its lanes' means.  The
-        // search stops once the next term can no longer move S at FP32
-        // resolution; a v that falls in the rounding gap left below S maps to
-        // that extreme quantile instead of running on.
         const float z = (float)zd;
         const float v = s.v32();
-        float p = __expf(-z);                   // P(N = 0)
-        float S = neg_expm1_neg(z, p);          // P(N > 0)
-        // Two terms per trip with one exit test: the same terms and decisions as
-        // one term per trip, with half the loop's mask bookkeeping.  A trip that
-        // ends after its first term returns k + 1.
-        int k = 0;
-        while (v < S && k < 200) {
-            const float p1 = p * z * rcpf((float)(k + 1));
-            const float S1 = S - p1;
-            const bool go2 = !(p1 <= S * 0x1p-24f) && v < S1 && k + 1 < 200;
-            const float p2 = p1 * z * rcpf((float)(k + 2));
-            k += go2 ? 2 : 1;
-            if (!go2 || p2 <= S1 * 0x1p-24f) break;
-            p = p2;
-            S = S1 - p2;
-        }
-        return k;
+        int n;
+        float S, P;
+        if (poisson_count(z, v, n, S, P)) return n;
+        return poisson_tail(z, v, S, P);
     }
     const float z = (float)zd;
     const float slam = __builtin_sqrtf(z);

@@ -202,28 +202,35 @@ __device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double 
     return y;
 }
 
-// FP32 storage: alpha_i, the Poisson mean and the Gamma shape in FP64 (the same
-// values the FP64 path forms from the same fields); the Poisson and Gamma
-// deviates from the FP32 samplers of tm_fast32.h.
+// FP32 storage: the Poisson mean z = lambda e^{beta dt} rho in FP64 (it decides
+// the sampler and is resolved as rho -> 0), the Gamma shape mu + 1 + n in FP32
+// from shape0 = mu + 1 = 2 alpha_i / sigma^2 (formed by the caller in FP32), and
+// the Poisson and Gamma deviates from the FP32 samplers of tm_fast32.h.
 // ONE = true is the first pass of the deferring step (dp_tile_deferred): it
 // returns false, leaving n and y unspecified, where the transition needs the
-// PTRS sampler (Poisson mean >= 12) or a second Gamma trial; otherwise it draws
-// the same words and computes the same value as ONE = false.
+// PTRS sampler (Poisson mean >= 12), more inversion terms than the counted
+// search holds, or a second Gamma trial; otherwise it draws the same words and
+// computes the same value as ONE = false.
 template <bool ONE, class St>
-__device__ __forceinline__ bool dornic_cell_f(St& s, double alpha_i, float cur, const DornicConst& k,
-                                              bool compat, int64_t& n, float& y)
+__device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, const DornicConst& k, bool compat,
+                                              int64_t& n, float& y)
 {
-#pragma clang fp contract(off)
-    const double mu = -1.0 + 2.0 * alpha_i * k.sigma2_1;
-    const double ziggy = k.ll * (double)cur;
-    if (ONE && ziggy >= 12.0) return false;
-    n = poisson_f(s, ziggy, compat);
-    const double gru = mu + 1.0 + (double)n;
+    if (ONE) {
+        if (!(zd < 12.0)) return false;
+        int ni = 0;
+        if (zd > 0.0) {
+            float S, P;
+            if (!poisson_count((float)zd, s.v32(), ni, S, P)) return false;
+        }
+        n = ni;
+    } else {
+        n = poisson_f(s, zd, compat);
+    }
+    const float gru = shape0 + (float)n;
     float g;
-    if (compat && gru > 100.0) {
-        const float gf = (float)gru;
-        g = norm_icdf_f(s.u32()) * __builtin_sqrtf(gf) + gf;
-    } else if (!gamma_f_try<ONE ? 1 : 64, St>(s, (float)gru, g)) {
+    if (compat && gru > 100.0f) {
+        g = norm_icdf_f(s.u32()) * __builtin_sqrtf(gru) + gru;
+    } else if (!gamma_f_try<ONE ? 1 : 64, St>(s, gru, g)) {
         return false;
     }
     // g / lambda with 1/lambda carried as hi + lo: a single FP32 rounding of 1/lambda
@@ -232,13 +239,36 @@ __device__ __forceinline__ bool dornic_cell_f(St& s, double alpha_i, float cur, 
     return true;
 }
 
-__device__ __forceinline__ float dornic_cell(Stream& s, double alpha_i, float cur, const DornicConst& k,
-                                             bool compat, int64_t& n)
+__device__ __forceinline__ float dornic_cell_f32(Stream& s, float shape0, double zd, const DornicConst& k,
+                                                 bool compat, int64_t& n)
 {
     float y;
-    dornic_cell_f<false, Stream>(s, alpha_i, cur, k, compat, n, y);
+    dornic_cell_f<false, Stream>(s, shape0, zd, k, compat, n, y);
     return y;
 }
+
+// The Dornic transition of a cell that is not absorbing, from its value and its
+// four neighbours (old field), in the storage type's arithmetic: FP64 keeps
+// alpha_i and every Dornic quantity in FP64 in the reference's order; FP32 sums
+// the neighbours in FP32 (same order) and forms the Poisson mean in FP64.
+// Returns cur unchanged where alpha_i = 0 and cur = 0 (no draw, n = -1).
+template <typename T> struct DornicLive;
+template <> struct DornicLive<double> {
+    static __device__ __forceinline__ double step(uint32_t seed, uint32_t inst, uint32_t cell, int64_t step,
+                                                  double cur, double up, double dn, double lf, double rt,
+                                                  const DornicConst& k, bool compat, int64_t& n);
+};
+template <> struct DornicLive<float> {
+    static __device__ __forceinline__ float step(uint32_t seed, uint32_t inst, uint32_t cell, int64_t step, float cur,
+                                                 float up, float dn, float lf, float rt, const DornicConst& k,
+                                                 bool compat, int64_t& n)
+    {
+        const float nsum = ((up + dn) + lf) + rt;
+        if (nsum == 0.0f && cur == 0.0f) return cur;
+        Stream st(seed, inst, cell, step);
+        return dornic_cell_f32(st, k.shape_f * nsum, k.ll * (double)cur, k, compat, n);
+    }
+};
 
 // alpha_i = (D/dx^2)(up + down + left + right) in FP64, in the reference's order
 // (rietkerk_bjork_FKE.cpp:4158-4159).
@@ -247,6 +277,16 @@ __device__ __forceinline__ double nbr_alpha(double dc, T up, T dn, T lf, T rt)
 {
 #pragma clang fp contract(off)
     return dc * ((double)up + (double)dn + (double)lf + (double)rt);
+}
+
+__device__ __forceinline__ double DornicLive<double>::step(uint32_t seed, uint32_t inst, uint32_t cell, int64_t step,
+                                                           double cur, double up, double dn, double lf, double rt,
+                                                           const DornicConst& k, bool compat, int64_t& n)
+{
+    const double alpha_i = nbr_alpha(k.dc, up, dn, lf, rt);
+    if (alpha_i == 0.0 && cur == 0.0) return cur;
+    Stream st(seed, inst, cell, step);
+    return dornic_cell(st, alpha_i, cur, k, compat, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -326,6 +366,21 @@ __device__ __forceinline__ void wave_partials(double (&sum)[NF], double (&cnt)[N
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) dst[f] = make_double2(sum[f], cnt[f]);
+    }
+}
+
+// Wave partial (sum, count) of one step into its slot; an all-zero wave writes zeros.
+template <int NT>
+__device__ __forceinline__ void step_partial(const StepArgs& a, int tile, int64_t counter, double s,
+                                             uint32_t nlive)
+{
+    const int64_t slot = counter & (int64_t)(a.nslots - 1);
+    double2* dst = a.partials + (slot * a.ntiles + tile) * (NT / 64) + (threadIdx.x >> 6);
+    if (__ballot(nlive != 0 || s != 0.0) == 0) {
+        if ((threadIdx.x & 63) == 0) *dst = make_double2(0.0, 0.0);
+    } else {
+        double sum[1] = {s}, cnt[1] = {(double)nlive};
+        wave_partials<1>(sum, cnt, dst);
     }
 }
 
@@ -459,12 +514,7 @@ __device__ __forceinline__ T cell_update(const typename StepConst<MODE>::type& c
     n = -1;
     bad = false;
     if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
-        y = cur;
-        const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
-        if (!(alpha_i == 0.0 && cur == (T)0)) {
-            Stream st(di.seed, di.inst_id, cell, c.step);
-            y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
-        }
+        y = DornicLive<T>::step(di.seed, di.inst_id, cell, c.step, cur, up, dn, lf, rt, c.k, c.compat, n);
         if constexpr (MODE == 0) {
             y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
             bad = y < (T)0 || !Arith<T>::finite(y);
@@ -486,17 +536,22 @@ __device__ __forceinline__ bool cell_update_first(const DPConst& c, const DevIns
     bad = false;
     if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
         y = cur;
-        const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
-        if (!(alpha_i == 0.0 && cur == (T)0)) {
-            // at most four words: the Poisson inversion's, one Gamma trial's two and
-            // the boost's (PTRS means and rejected trials return false instead)
-            Stream4 st(di.seed, di.inst_id, cell, c.step);
-            bool done;
-            if constexpr (sizeof(T) == 4)
-                done = dornic_cell_f<true, Stream4>(st, alpha_i, cur, c.k, c.compat, n, y);
-            else
-                done = dornic_cell_d<true, Stream4>(st, alpha_i, cur, c.k, c.compat, n, y);
-            if (!done) return false;
+        // at most four words: the Poisson inversion's, one Gamma trial's two and
+        // the boost's (PTRS means, long inversions and rejected trials return
+        // false instead)
+        if constexpr (sizeof(T) == 4) {
+            const float nsum = ((up + dn) + lf) + rt;
+            if (!(nsum == 0.0f && cur == 0.0f)) {
+                Stream4 st(di.seed, di.inst_id, cell, c.step);
+                if (!dornic_cell_f<true, Stream4>(st, c.k.shape_f * nsum, c.k.ll * (double)cur, c.k, c.compat, n, y))
+                    return false;
+            }
+        } else {
+            const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
+            if (!(alpha_i == 0.0 && cur == (T)0)) {
+                Stream4 st(di.seed, di.inst_id, cell, c.step);
+                if (!dornic_cell_d<true, Stream4>(st, alpha_i, cur, c.k, c.compat, n, y)) return false;
+            }
         }
         y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
         bad = y < (T)0 || !Arith<T>::finite(y);
@@ -627,38 +682,39 @@ __device__ __forceinline__ void dp_tile(const StepArgs& a, int tile, const DevTi
     }
 }
 
-// The FP32 DP step of one tile in two passes.  A Gamma trial is rejected in a
-// few percent of the cells, but a wavefront runs the next trial, and the second
+// The DP step of one tile in two passes.  A Gamma trial is rejected in a few
+// percent of the cells, but a wavefront runs the next trial, and the second
 // Philox block it needs, whenever any of its 64 lanes needs it, which is nearly
 // always.  So the first pass stops each cell's transition after one trial: a
 // cell whose trial was rejected, or whose Poisson mean needs the PTRS sampler,
 // goes to a queue in LDS and gets a placeholder.  After a barrier the second
-// pass runs the unchanged full step (cell_update) on the queued cells, packed
-// densely into the lanes, and overwrites the placeholders.  Every cell is
-// computed with the same arithmetic from the same Philox words as in dp_tile,
-// so the field and the draws are identical; only the order of the rho(t)
-// partial sums differs.  The queue is filled in thread order (a bit mask per
-// thread and a workgroup prefix sum), so which thread finishes which cell, and
-// with it the rho(t) partials, is the same in every run.  It holds every cell
-// of the tile (16-bit tile offsets), so it cannot overflow.
+// pass runs the full step (cell_update) on the queued cells, packed densely
+// into the lanes, and overwrites the placeholders.  The first pass is the
+// first stage of the full step (same words, same arithmetic), so the field and
+// the draws are those of dp_tile.  The queue is filled in thread order (a bit
+// mask per thread and a workgroup prefix sum); it holds every cell of the tile,
+// so it cannot overflow.  A queue slot holds the cell's tile offset, then its
+// new value, which the cell's owner thread reads back after a second barrier:
+// each thread sums its own cells in dp_tile's order, so the rho(t) partials are
+// dp_tile's bit for bit.
 template <typename T, int TX, int TY, int NT>
 __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, const DevTile& tl, const DevInst& di,
                                                  const DPConst& c, const T* s_rho, int64_t counter,
-                                                 int64_t slot, uint16_t* s_q, uint32_t* s_wn)
+                                                 int64_t slot, T* s_q, uint32_t* s_wn)
 {
-    static_assert(TX * TY <= 65536, "16-bit tile offsets");
-    static_assert(DPShape<T, TX, TY, NT>::G * 4 <= 32, "one mask bit per cell of a thread");
     typedef DPShape<T, TX, TY, NT> S;
-    constexpr int V = S::V, RWV = S::RWV, R4 = S::R4;
+    static_assert(S::G * 4 <= 32, "one mask bit per cell of a thread");
+    constexpr int V = S::V, RWV = S::RWV, R4 = S::R4, G = S::G;
     const int tid = threadIdx.x;
     const int cg = tid % S::CG, ly0 = tid / S::CG;
     const int L = tl.L;
     T* fout = (T*)a.out + tl.cell_off;
     const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
-    double sum[1] = {0.0};
-    uint32_t nlive = 0;
     uint32_t dmask = 0;                                       // bit 4 j + q: deferred
-    for (int j = 0; j < S::G; ++j) {
+    Quad<T> own[G];                                           // first-pass values (0 where deferred)
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        own[j].v[0] = own[j].v[1] = own[j].v[2] = own[j].v[3] = (T)0;
         const int ly = ly0 + j * R4;
         const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
         if (gy >= L || gx0 >= L) continue;
@@ -692,10 +748,8 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
                     dmask |= 1u << (4 * j + q);
                     y = (T)0;
                     n = -1;
-                } else {
-                    if (bad) record_fault(a.state, tl.inst, cell, counter);
-                    sum[0] += (double)y;
-                    nlive += y > (T)0 ? 1u : 0u;
+                } else if (bad) {
+                    record_fault(a.state, tl.inst, cell, counter);
                 }
                 if (a.draws) a.draws[tl.cell_off + cell] = n;
                 yq.v[0] = q == 0 ? y : yq.v[0];
@@ -705,6 +759,7 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
             }
         }
         store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
+        own[j] = yq;
     }
     // Queue offsets: an inclusive scan of the per-thread counts within the wave,
     // then the totals of the waves before it.
@@ -727,35 +782,53 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
         base += w < (tid >> 6) ? s_wn[w] : 0u;
         nq += s_wn[w];
     }
+    uint32_t* const s_qo = (uint32_t*)s_q;                    // slot i: offset at word i * DW
+    constexpr int DW = (int)(sizeof(T) / 4);
+    {
+        uint32_t b = base;
 #pragma unroll 1
-    for (uint32_t m = dmask; m != 0; m &= m - 1) {
-        const int b = __builtin_ctz(m);
-        s_q[base++] = (uint16_t)((ly0 + (b >> 2) * R4) * TX + 4 * cg + (b & 3));
+        for (uint32_t m = dmask; m != 0; m &= m - 1) {
+            const int k = __builtin_ctz(m);
+            s_qo[(b++) * DW] = (uint32_t)((ly0 + (k >> 2) * R4) * TX + 4 * cg + (k & 3));
+        }
     }
     __syncthreads();
 #pragma unroll 1
     for (uint32_t i = tid; i < nq; i += NT) {
-        const uint32_t e = s_q[i];
+        const uint32_t e = s_qo[i * DW];
         const int ly = (int)(e / TX), lx = (int)(e % TX);
         const int r = (ly + 1) * RWV + V + lx;
         const uint32_t cell = (uint32_t)(tl.y0 + ly) * (uint32_t)L + (uint32_t)(tl.x0 + lx);
         int64_t n;
         bool bad;
         const T y = cell_update<T, 0>(c, di, cell, s_rho[r], s_rho[r - RWV], s_rho[r + RWV], s_rho[r - 1],
-                                              s_rho[r + 1], n, bad);
+                                      s_rho[r + 1], n, bad);
         if (bad) record_fault(a.state, tl.inst, cell, counter);
         if (a.draws) a.draws[tl.cell_off + cell] = n;
-        sum[0] += (double)y;
-        nlive += y > (T)0 ? 1u : 0u;
         fout[cell] = y;
+        s_q[i] = y;                                           // the slot's thread read its offset above
     }
-    double2* dst = a.partials + (slot * a.ntiles + tile) * S::NW + (tid >> 6);
-    if (__ballot(nlive != 0 || sum[0] != 0.0) == 0) {
-        if ((tid & 63) == 0) *dst = make_double2(0.0, 0.0);
-    } else {
-        double cnt[1] = {(double)nlive};
-        wave_partials<1>(sum, cnt, dst);
+    __syncthreads();
+    // rho(t) partials: each thread's own cells in dp_tile's order, the deferred
+    // ones read back from their queue slots (in bit order, as they were queued).
+    double sum = 0.0;
+    uint32_t nlive = 0;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const int ly = ly0 + j * R4;
+        const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
+        if (gy >= L || gx0 >= L) continue;
+        const int nvalid = min(4, L - gx0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (q >= nvalid) break;
+            T y = own[j].v[q];
+            if ((dmask >> (4 * j + q)) & 1u) y = s_q[base++];
+            sum += (double)y;
+            nlive += y > (T)0 ? 1u : 0u;
+        }
     }
+    step_partial<NT>(a, tile, counter, sum, nlive);
 }
 
 // MODE 0: the DP step.  MODE 1: the Dornic sub-step of the Rietkerk P field
@@ -774,7 +847,7 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
     static_assert(Q == 0 || MODE == 0, "deferring step: DP only");
     __shared__ __align__(16) uint32_t s_img[S::NBUF];
     __shared__ uint32_t s_flag[NT / 64];
-    __shared__ uint16_t s_q[Q ? TX * TY : 1];         // deferred cells (tile offsets)
+    __shared__ __align__(8) T s_q[Q ? TX * TY : 1];   // deferred cells: tile offsets, then values
     __shared__ uint32_t s_wn[NT / 64];                // ... and their count per wave
 
     const int64_t counter = a.state->counter + a.k;
@@ -827,18 +900,33 @@ struct DPKShape {
     static_assert(TX % 64 == 0 && NT % 64 == 0 && K >= 2 && K <= HC, "tile shape");
 };
 
-// Wave partial (sum, count) of one step into its slot; an all-zero wave writes zeros.
-template <int NT>
-__device__ __forceinline__ void step_partial(const StepArgs& a, int tile, int64_t counter, double s,
-                                             uint32_t nlive)
+// The (sum, count) of this thread's own cells in a frame of the tile's step
+// result (row width RW, tile column lx at HC + lx, tile row ly at row ly + R),
+// in the thread -> cell order of dp_tile: groups j = 0 .. G-1 at tile row
+// ly0 + j R4, cells q = 0 .. 3 of each, valid cells only.  Every kernel forms
+// its rho(t) partials in this order, so the single-step, two-pass and K-step
+// launches of one tile shape write the same partials bit for bit.
+template <typename T, int TX, int TY, int NT>
+__device__ __forceinline__ void own_partial(const T* fr, int RW, int HC, int R, const DevTile& tl, double& s,
+                                            uint32_t& nlive)
 {
-    const int64_t slot = counter & (int64_t)(a.nslots - 1);
-    double2* dst = a.partials + (slot * a.ntiles + tile) * (NT / 64) + (threadIdx.x >> 6);
-    if (__ballot(nlive != 0 || s != 0.0) == 0) {
-        if ((threadIdx.x & 63) == 0) *dst = make_double2(0.0, 0.0);
-    } else {
-        double sum[1] = {s}, cnt[1] = {(double)nlive};
-        wave_partials<1>(sum, cnt, dst);
+    typedef DPShape<T, TX, TY, NT> S;
+    const int cg = threadIdx.x % S::CG, ly0 = threadIdx.x / S::CG;
+    const int L = tl.L;
+    s = 0.0;
+    nlive = 0;
+    for (int j = 0; j < S::G; ++j) {
+        const int ly = ly0 + j * S::R4;
+        const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
+        if (gy >= L || gx0 >= L) continue;
+        const int nvalid = min(4, L - gx0);
+        const T* row = fr + (ly + R) * RW + HC + 4 * cg;
+#pragma unroll 1
+        for (int q = 0; q < nvalid; ++q) {
+            const T y = row[q];
+            s += (double)y;
+            nlive += y > (T)0 ? 1u : 0u;
+        }
     }
 }
 
@@ -880,8 +968,6 @@ __global__ __launch_bounds__(NT, MINW) void dp_stepk_kernel(StepArgs a)
         const int R = K - j;
         const DPConst c = ((const DPConst*)a.consts)[(int64_t)(a.k + j - 1) * a.n_inst + tl.inst];
         T* dstf = (j & 1) ? s_fr : (T*)s_img;                   // never the buffer `src` is in
-        double sum = 0.0;
-        uint32_t nlive = 0;
         const int ngroups = (TY + 2 * R) * CG;
         for (int g = tid; g < ngroups; g += NT) {
             const int ly = g / CG - R, cg = g % CG;
@@ -906,11 +992,7 @@ __global__ __launch_bounds__(NT, MINW) void dp_stepk_kernel(StepArgs a)
                     bool bad;
                     const T y = cell_update<T, 0>(c, di, cell, src[rq], src[rq - RW], src[rq + RW], src[rq - 1],
                                                   src[rq + 1], n, bad);
-                    if (own_row && tl.x0 + lx < L) {
-                        if (bad) record_fault(a.state, tl.inst, cell, counter + j - 1, j - 1);
-                        sum += (double)y;
-                        nlive += y > (T)0 ? 1u : 0u;
-                    }
+                    if (bad && own_row && tl.x0 + lx < L) record_fault(a.state, tl.inst, cell, counter + j - 1, j - 1);
                     yq.v[0] = q == 0 ? y : yq.v[0];
                     yq.v[1] = q == 1 ? y : yq.v[1];
                     yq.v[2] = q == 2 ? y : yq.v[2];
@@ -932,8 +1014,13 @@ __global__ __launch_bounds__(NT, MINW) void dp_stepk_kernel(StepArgs a)
             dstf[(ly + R) * RW + HC + lx] = cell_update<T, 0>(c, di, cell, src[rq], src[rq - RW], src[rq + RW],
                                                               src[rq - 1], src[rq + 1], n, bad);
         }
-        step_partial<NT>(a, tile, counter + j - 1, sum, nlive);
         __syncthreads();
+        {   // this step's partials from the finished frame, in the single-step order
+            double sum;
+            uint32_t nlive;
+            own_partial<T, TX, TY, NT>(dstf, RW, HC, R, tl, sum, nlive);
+            step_partial<NT>(a, tile, counter + j - 1, sum, nlive);
+        }
         src = dstf;
         srad = R;
     }
@@ -1149,15 +1236,10 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         // ---- stage 0: Dornic P* on the radius-3 frame (FKE:4154-4211) ----
         auto dornic_at = [&](int ly, int lx, int64_t* n_out) {
             const int r = i4(ly, lx);
-            const T cur = sP0[r];
-            const double alpha_i = nbr_alpha(c.k.dc, sP0[r - W4], sP0[r + W4], sP0[r - 1], sP0[r + 1]);
-            T y = cur;
+            const int gy = wrap(tl.y0 + ly, L), gx = wrap(tl.x0 + lx, L);
             int64_t n = -1;
-            if (!(alpha_i == 0.0 && cur == (T)0)) {
-                const int gy = wrap(tl.y0 + ly, L), gx = wrap(tl.x0 + lx, L);
-                Stream st(di.seed, di.inst_id, (uint32_t)((int64_t)gy * L + gx), c.step);
-                y = dornic_cell(st, alpha_i, cur, c.k, c.compat, n);
-            }
+            const T y = DornicLive<T>::step(di.seed, di.inst_id, (uint32_t)((int64_t)gy * L + gx), c.step, sP0[r],
+                                            sP0[r - W4], sP0[r + W4], sP0[r - 1], sP0[r + 1], c.k, c.compat != 0, n);
             *n_out = n;
             sPs[i3(ly, lx)] = y;
         };

@@ -4,17 +4,33 @@
 Headline (`value`): the batched FP32 DP quench over 256 mixed-size instances
 (128 x 128^2 + 64 x 256^2 + 48 x 512^2 + 16 x 1024^2 = 35,651,584 cells/step;
 a ramped 0.72 -> 0.70 over 10^4 steps; b = 2, D = 1, sigma = sqrt 2, dt = 0.1;
-rho(t) every step in FP64), timed over the full 10^4-step run with every
-rho(t) row copied back to the host.  Reported beside it, each also timed over
-its full run: the same sweep in FP64, the plain FP64 single-instance Rietkerk
-step (256^2, 2x10^4 steps), and the batched FP32 Rietkerk sweep (256 x 512^2,
-R = linspace(0.5, 1.5, 16) x 16 seeds, 5x10^3 steps).  Every FP32 run is
-compared with the FP64 run of the same seeds (rho(t) ensemble error).
+rho(t) every step in FP64), timed over its first --steps steps from the
+i.i.d. Uniform(0, 1) initial field, with every rho(t) row copied back to the
+host.  Those steps are the live start of the quench: every cell is occupied and
+takes a Poisson and a Gamma draw each step.
 
-Roofline: achieved HBM GB/s of the step kernel = algorithmic bytes per launch
-(8 B/cell FP32 DP, 24 B/cell FP32 Rietkerk, 16 / 48 in FP64: one read and one
-write of every field) / the kernel's average duration, measured with HIP
-events on the context stream over a replay of the same trajectory.
+Reported beside it:
+  * the whole 10^4-step C4 run in FP32 and FP64, timed per graph block, split
+    into its live phase (single-step launches) and its absorbing phase (eight
+    steps per launch), each with its own rate and HBM fraction;
+  * the plain FP64 single-instance Rietkerk step (C2: 256^2, 2x10^4 steps);
+  * the batched Rietkerk sweep (C3: 256 x 512^2, R = linspace(0.5, 1.5, 16) x 16
+    seeds, 5x10^3 steps) in FP32 and FP64;
+  * the 8192^2 Rietkerk lattice (C5, R = 0.75) for a bounded number of its 10^5
+    steps in FP32 and FP64, with the cost of one snapshot (it takes one every
+    1000 steps);
+  * FP32-vs-FP64 error of every batch over its full run (C5: its bounded run);
+  * the CPU baseline: the reference's own DP integrator (percolationDornic_2D,
+    built from its source) on C1 (128^2, 8 replicates, up to 10^4 steps) on 8
+    host threads and on 1, and the oracle port on C2 on 1 thread.
+
+Roofline: achieved HBM GB/s of the dominant kernel of the timed region =
+algorithmic bytes per launch (one read and one write of every field per
+cell-update: 8 B/cell FP32 DP, 24 B/cell FP32 Rietkerk, 16 / 48 in FP64) / its
+mean launch duration, from HIP events on the context stream.  A launch of K
+steps moves the field once: its algorithmic bytes are one read and one write of
+the field, and its cell-updates are K per cell.  `traffic` is the measured HBM
+bytes per launch from rocprofv3 counter runs of the same kernel and workload.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--c5-steps N] [--no-cpu]
 """
@@ -25,6 +41,7 @@ import json
 import math
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -35,28 +52,15 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s + % HBM peak: batched FP32 vs plain FP64, ρ(t) err ≤1e-3"
 HBM_PEAK_GBS = 8000.0
 SEED = 20251202
-# HBM bytes per launch of the FP32 DP step kernels on the C4 batch, per kernel
-# variant, from counter runs (profiles/dp_step_f32_c4_quench_hbm_traffic.md):
-# FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000 KiB, the factors a plain 256 MiB
-# copy needs to read its known size.
-FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-# Multi-step variants: (steps per launch, (FETCH_SIZE, WRITE_SIZE) of a K-step
-# launch, (FETCH_SIZE, WRITE_SIZE) of a single-step launch of the same tiles),
-# means over the absorbing phase of the first 4000 steps.
-C4_F32_BLOCKED_TRAFFIC_KIB = {
-    "dp_f32_128x32_t256_x4a": (4, (87921.6, 141440.2), (75136.4, 139848.7)),
-    "dp_f32_128x24_t256_x8a": (8, (175845.5, 145216.1), (114941.6, 140040.5)),
-    "dp_f32_128x24_t256_x8aq": (8, (175843.5, 145216.1), (115013.0, 149042.5)),
-}
-C4_F32_TRAFFIC_KIB = {   # single-step variant: (FETCH_SIZE, WRITE_SIZE) per dispatch
-    "dp_f32_128x32_t256_w6": (75198.8, 139824.1),
-    "dp_f32_64x32_t256": (80202.1, 140385.4),
-    "dp_f32_128x32_t256": (75199.2, 139824.1),
-    "dp_f32_64x64_t256": (85723.7, 139837.5),
-    "dp_f32_256x32_t256": (74841.1, 139556.2),
-    "dp_f32_64x64_t256_w6": (85720.6, 139837.4),
-    "dp_f32_128x16_t256_rm": (110913.2, 140372.5),
-    "dp_f32_256x32_t256_rm": (89995.4, 139556.0),
+
+# Measured HBM bytes per launch (read, write) of the kernels each workload runs,
+# from rocprofv3 counter runs of the same workload: reads from the request-size
+# counters (TCC_EA0_RDREQ_{32B,64B,128B} x 32/64/128 B), writes from WRITE_SIZE
+# (KiB), each checked on a plain copy of known size.  Keyed by (variant, launch
+# kind); "single" = one step per launch, "blocked" = K steps per launch.
+TRAFFIC = {   # bytes per launch (read, write); profiles/dp_step_f32_c4_quench_hbm_traffic.md
+    ("dp_f32_128x24_t256_x8aq", "single"): (234_266_000, 143_368_000),    # two-pass single step, live start
+    ("dp_f32_128x24_t256_x8aq", "blocked"): (359_357_000, 148_703_000),   # eight-step launch, absorbing field
 }
 
 
@@ -80,7 +84,7 @@ def c3_batch(tm, precision):
 def c2_batch(tm, precision):
     return dict(model=tm.MODEL_RIETKERK, precision=precision, params=[tm.RietkerkParams(R=1.0)],
                 inst=[tm.Instance(256, 0, SEED, 0)], lo=[0.0, 0.0, 0.0], hi=[90.0, 1.0, 1.0],
-                steps=20_000, nf=3)
+                steps=20_000, nf=3, chunk=5000)
 
 
 def c5_batch(tm, precision, steps=100_000):
@@ -135,77 +139,152 @@ def bytes_per_cell(b):
     return b["nf"] * 2 * (8 if b["precision"] == 0 else 4)
 
 
-def _kib_to_bytes(t):
-    return (t[0] * FETCH_SCALE + t[1] * WRITE_SCALE) * 1024
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_bytes(variant, single_steps=0, blocked_steps=0):
-    """Measured HBM bytes per step of the timed run: per dispatch for a
-    single-step variant; for a multi-step one, its two kinds of launch weighed by
-    the steps each advanced."""
-    if variant in C4_F32_BLOCKED_TRAFFIC_KIB:
-        k, tk, t1 = C4_F32_BLOCKED_TRAFFIC_KIB[variant]
-        n = single_steps + blocked_steps
-        if n == 0:
-            return None
-        return int(round((single_steps * _kib_to_bytes(t1) + blocked_steps / k * _kib_to_bytes(tk)) / n))
-    t = C4_F32_TRAFFIC_KIB.get(variant)
-    return None if t is None else int(round(_kib_to_bytes(t)))
+def _open(tm, b, variant_name=None, tune=True):
+    ctx = tm.Context(0)
+    ctx.configure(b["model"], b["precision"], b["params"], b["inst"])
+    variant = None
+    if variant_name:
+        v = tm.variant_names(b["model"], b["precision"]).index(variant_name)
+        ctx.set_launch(v, 100)
+        variant = (v, 100)
+    ctx.init_uniform(b["lo"], b["hi"])
+    if not variant_name and tune:   # tuned on the initial state of the run
+        variant = ctx.autotune(os.environ.get("TM_TUNE_CACHE"))
+    if variant is None:
+        variant = ctx.launch()
+    names = tm.variant_names(b["model"], b["precision"])
+    return ctx, names[variant[0]], variant[1]
 
 
-def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=None):
+def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=None, blocks=False,
+                 keep_open=False):
     """Warm up, then time `steps` steps of the real trajectory from t = 0: wall
     clock around the whole job, and the device time of every step launch (HIP
-    events on the context stream inside each run() call, summed over chunks).
-    The launch shape comes from the autotuner, or is the named variant."""
-    with tm.Context(0) as ctx:
-        ctx.configure(b["model"], b["precision"], b["params"], b["inst"])
-        variant = None
-        if variant_name:
-            v = tm.variant_names(b["model"], b["precision"]).index(variant_name)
-            ctx.set_launch(v, 100)
-            variant = (v, 100)
-        ctx.init_uniform(b["lo"], b["hi"])
-        if not variant_name and tune:   # tuned on the initial state of the run
-            variant = ctx.autotune(os.environ.get("TM_TUNE_CACHE"))
+    events on the context stream inside each run() call).  blocks: run one graph
+    block per call, and split the device time by launch kind (single-step
+    blocks, K-step blocks)."""
+    ctx, vname, gsteps = _open(tm, b, variant_name, tune)
+    try:
         if warmup:
             ctx.run(warmup, rows=want_rows)
         ctx.reset()
         ctx.init_uniform(b["lo"], b["hi"])
-        chunk = max(1, min(steps, b.get("chunk", steps)))
+        chunk = max(1, min(steps, gsteps if blocks and gsteps else b.get("chunk", steps)))
         parts = []
         dev_ms = 0.0
+        phase = {"single": [0, 0.0], "blocked": [0, 0.0]}
         m0 = ctx.mode_steps()
         t0 = time.perf_counter()
         done = 0
+        next_log = time.perf_counter() + 10.0
         while done < steps:
             n = min(chunk, steps - done)
+            a0 = ctx.mode_steps()
             parts.append(ctx.run(n, rows=want_rows))
-            dev_ms += ctx.last_run_ms()
+            ms = ctx.last_run_ms()
+            a1 = ctx.mode_steps()
+            dev_ms += ms
+            kind = "blocked" if a1[1] - a0[1] > a1[0] - a0[0] else "single"
+            phase[kind][0] += n
+            phase[kind][1] += ms
             done += n
-            if done < steps:
-                print(f"[bench]   ... {done}/{steps} steps", file=sys.stderr, flush=True)
+            if done < steps and time.perf_counter() > next_log:
+                log(f"[bench]   ... {done}/{steps} steps")
+                next_log = time.perf_counter() + 10.0
         t1 = time.perf_counter()
         m1 = ctx.mode_steps()
         rows = np.concatenate(parts, axis=1) if want_rows else None
         final = [ctx.download(k) for k in range(min(len(b["inst"]), 4))]
-        names = tm.variant_names(b["model"], b["precision"])
-        vname = names[variant[0]] if variant else None
+    finally:
+        if not keep_open:
+            ctx.close()
     cells = cells_of(b)
     wall = t1 - t0
     avg_step_s = dev_ms / 1e3 / steps
     achieved = cells * bytes_per_cell(b) / avg_step_s / 1e9
-    return dict(value=cells * steps / wall, wall_s=wall, ms_per_step=wall / steps * 1e3,
-                kernel_ms=avg_step_s * 1e3, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
-                cells=cells, steps=steps, variant=vname, graph_steps=variant[1] if variant else None,
-                single_steps=m1[0] - m0[0], blocked_steps=m1[1] - m0[1], rows=rows, final=final)
+    out = dict(value=cells * steps / wall, wall_s=wall, ms_per_step=wall / steps * 1e3,
+               kernel_ms=avg_step_s * 1e3, achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
+               cells=cells, steps=steps, variant=vname, graph_steps=gsteps,
+               single_steps=m1[0] - m0[0], blocked_steps=m1[1] - m0[1], rows=rows, final=final,
+               phase=phase)
+    if keep_open:
+        out["ctx"] = ctx
+    return out
+
+
+def phase_split(r, b, k_per_launch=8):
+    """Live (single-step launches) and absorbing (K-step launches) phases of a
+    run timed per graph block: steps, device ms per step, cell-updates/s and the
+    HBM fraction of each kind's algorithmic bytes per launch."""
+    cells, bpc = cells_of(b), bytes_per_cell(b)
+    out = {}
+    for kind, (n, ms) in r["phase"].items():
+        if n == 0:
+            continue
+        step_ms = ms / n
+        launch_ms = step_ms * (k_per_launch if kind == "blocked" else 1)
+        gbs = cells * bpc / (launch_ms / 1e3) / 1e9
+        out[kind] = dict(steps=n, device_ms_per_step=step_ms, cell_updates_per_s=cells / (step_ms / 1e3),
+                         steps_per_launch=k_per_launch if kind == "blocked" else 1,
+                         achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS)
+    return out
+
+
+def kernel_of(variant):
+    """The single-step kernel a DP variant (dp_f32_TXxTY_tNT[_suffixes]) runs while
+    the field is live: two-pass when a suffix ends in q, bounded to 80 VGPRs (six
+    waves per SIMD) for _w6 and for the adaptive (_xKa...) variants."""
+    parts = (variant or "").split("_")
+    if len(parts) < 4 or not parts[0] == "dp":
+        return variant
+    tx, ty = parts[2].split("x")
+    nt = parts[3][1:]
+    suf = parts[4:]
+    q = any(p.endswith("q") for p in suf)
+    w6 = any(p.startswith("w6") for p in suf) or any(p.startswith("x") and "a" in p for p in suf)
+    return (f"dp_step_kernel<{'float' if parts[1] == 'f32' else 'double'},{tx},{ty},{nt},0,{6 if w6 else 1},"
+            f"{1 if q else 0}> ({'two-pass ' if q else ''}single DP step)")
+
+
+def roofline(h, b, kind="single"):
+    """The headline's roofline: its dominant kernel is the single-step (two-pass)
+    DP kernel; algorithmic bytes per launch (one read and one write of the field)
+    over its mean launch duration (device time per step of the timed region,
+    HIP events on the context stream), against the measured HBM bytes per launch."""
+    step_s = h["kernel_ms"] / 1e3
+    achieved = cells_of(b) * bytes_per_cell(b) / step_s / 1e9
+    t = TRAFFIC.get((h["variant"], kind))
+    traffic = None if t is None else int(t[0] + t[1])
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "kernel": kernel_of(h["variant"]),
+           "bytes_per_cell": bytes_per_cell(b), "cells_per_launch": cells_of(b),
+           "algorithmic_bytes_per_launch": cells_of(b) * bytes_per_cell(b),
+           "step_device_ms": h["kernel_ms"], "single_steps": h["single_steps"],
+           "blocked_steps": h["blocked_steps"],
+           "timing": "HIP events on the context stream around every run() of the timed region; the per-step "
+                     "share also holds the per-block constants, rho(t) fold and counter bump launches "
+                     "(about 2% of it)"}
+    if traffic is not None:
+        out["traffic_gbs"] = traffic / step_s / 1e9
+        out["traffic_frac"] = out["traffic_gbs"] / HBM_PEAK_GBS
+        out["traffic_source"] = ("rocprofv3 counters of the same kernel on this workload: reads from "
+                                 "TCC_EA0_RDREQ_{32B,64B,128B}, writes from WRITE_SIZE (profiles/)")
+    return out
+
+
+def side_fields(r, keys=("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant", "steps")):
+    return {k: r[k] for k in keys}
 
 
 def sweep_leg(tm, steps=None):
     """The sweep pieces on the C3 batch (FP32 Rietkerk, 16 R x 16 replicates of
     512^2): device frames, a measured run over the reference's log-spaced times,
     the replicate statistics, and the output layer on one instance."""
-    import tempfile
     b = c3_batch(tm, tm.F32)
     dt = b["params"][0].dt
     t_meas = tm.sweep_times(25.0, dt)                       # 5000 steps of dt = 0.005
@@ -245,137 +324,108 @@ def sweep_leg(tm, steps=None):
     return out
 
 
-def _kernel_desc(variant):
-    """The kernels an adaptive multi-step variant runs, from its name (dp_f32_TXxTY_tNT_xKa)."""
-    v = variant or ""
-    if v in C4_F32_BLOCKED_TRAFFIC_KIB:
-        shape = v.split("_")[2]
-        tx, ty = shape.split("x")
-        k = C4_F32_BLOCKED_TRAFFIC_KIB[v][0]
-        one = "two-pass (deferred Gamma rejections) " if v.endswith("q") else ""
-        return (f"dp_stepk_kernel<float,{tx},{ty},256,{k}> ({k} steps per launch) once <= 2% of cells are "
-                f"occupied, {one}dp_step_kernel<float,{tx},{ty},256> single steps before")
-    return "dp_step_kernel<float>"
-
-
-def roofline(h, b):
-    """The headline's roofline: algorithmic bytes (one read and one write of the
-    field per cell-update) per step over the mean device time per step, and the
-    measured HBM bytes per step of the same run.  With temporal blocking (the
-    _xK kernels read and write the field once per K steps) the algorithmic rate
-    exceeds the HBM peak; traffic_frac is then the HBM utilisation."""
-    traffic = traffic_bytes(h["variant"], h["single_steps"], h["blocked_steps"])
-    step_s = h["kernel_ms"] / 1e3
-    out = {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": h["frac"], "traffic": traffic,
-           "kernel": _kernel_desc(h["variant"]),
-           "bytes_per_cell": bytes_per_cell(b), "cells_per_launch": cells_of(b),
-           "step_device_ms": h["kernel_ms"], "single_steps": h["single_steps"],
-           "blocked_steps": h["blocked_steps"],
-           "timing": "HIP events on the context stream around every run() of the timed region; "
-                     "the per-step share also holds the per-block constants, rho(t) fold and "
-                     "counter bump launches"}
-    if traffic is not None:
-        out["traffic_gbs"] = traffic / step_s / 1e9
-        out["traffic_frac"] = out["traffic_gbs"] / HBM_PEAK_GBS
-        out["traffic_note"] = ("traffic: measured HBM bytes per step (FETCH_SIZE x 2 + WRITE_SIZE, per launch "
-                               "kind, weighed by the steps each kind advanced in this run)")
-    return out
-
-
-def rho_error(r32, r64, nf):
-    """Errors of FP32 rho(t) against the FP64 run of the same seeds, per field.
-    ensemble: mean over instances of the spatial mean, per step -- relative error
-    max / mean over all steps with a nonzero FP64 ensemble, and over the steps
-    where the ensemble still holds >= 10% of its initial density (later, same-seed
-    trajectories decorrelate cluster by cluster and the difference is sampling
-    noise); bias_z_max: largest |paired mean difference| / standard error over
-    steps (a systematic FP32 bias shows up here); survival: max difference of the
-    fraction of instances with a live cell."""
-    out = {}
-    names = "rho" if nf == 1 else "PWO"
-    for f in range(nf):
-        c = 2 * f + 1
-        m32, m64 = r32[:, :, c].mean(axis=0), r64[:, :, c].mean(axis=0)
-        nz = np.abs(m64) > 0
-        rel = np.abs(m32[nz] - m64[nz]) / np.abs(m64[nz])
-        pop = m64 >= 0.1 * m64[0]
-        relp = np.abs(m32[pop] - m64[pop]) / np.abs(m64[pop])
-        d = r32[:, :, c] - r64[:, :, c]
-        se = d.std(axis=0, ddof=1) / np.sqrt(max(d.shape[0], 1))
-        zb = np.abs(d.mean(axis=0)) / np.where(se > 0, se, np.inf)
-        s32 = (r32[:, :, 2 * f] > 0).mean(axis=0)
-        s64 = (r64[:, :, 2 * f] > 0).mean(axis=0)
-        out[names[f]] = dict(
-            ensemble_rel_max=float(rel.max()) if rel.size else 0.0,
-            ensemble_rel_mean=float(rel.mean()) if rel.size else 0.0,
-            ensemble_rel_max_populated=float(relp.max()) if relp.size else 0.0,
-            populated_steps=int(pop.sum()),
-            bias_z_max=float(zb.max()) if zb.size else 0.0,
-            survival_diff_max=float(np.abs(s32 - s64).max()))
-    return out
-
-
-def field_error(f32, f64):
-    a = np.concatenate([x.ravel() for x in f32])
-    b = np.concatenate([x.ravel() for x in f64])
-    d = np.abs(a - b)
-    return dict(max_abs=float(d.max()), mean_abs=float(d.mean()))
-
-
-def cpu_baseline(max_threads=16):
-    """The reference's own DP-Dornic integrator (percolationDornic_2D, built from
-    its source into oracle/_ref) on the host cores: one independent call per
-    thread (the reference's thread-per-parameter OpenMP pattern), L = 128,
-    dt = 0.1, a = 0.72, b = 2, D = 1, sigma = sqrt 2, 2 replicates x 301 steps per
-    thread.  Falls back to the oracle port when the reference library is absent."""
-    from oracle import oracle
-    threads = max(1, min(os.cpu_count() or 1, max_threads, 8))
-    L, r, t_max, dt = 128, 2, 30.0, 0.1
-    steps = int(round(t_max / dt)) + 1
-    rho0 = oracle.init_uniform(L, [0.0], [1.0], SEED, 0).reshape(-1).copy()
-    ref = oracle.ref_lib()
-    if ref is not None:
-        tmeas = np.array([2.0, 5.0, 10.0, 20.0, 30.0])
-        fd = os.dup(1)
-        devnull = os.open(os.devnull, os.O_WRONLY)
-        os.dup2(devnull, 1)
-        try:
+def c5_leg(tm, steps, warmup):
+    """The 8192^2 Rietkerk lattice (R = 0.75, FP32) for `steps` of its 10^5 steps,
+    the FP64 run of the same seeds for its error, and the cost of one snapshot (the
+    reference writes a frame every 1000 steps, FKE:4090-4097), as a share of the
+    1000 steps it covers."""
+    from trophic_maelstorm_amd import compare
+    b = c5_batch(tm, tm.F32, steps)
+    r = run_workload(tm, b, steps, warmup, keep_open=True)
+    ctx = r.pop("ctx")
+    try:
+        with tempfile.TemporaryDirectory() as d:
             t0 = time.perf_counter()
-            ref.ref_ssp_parallel(threads, L, 0.72, 2.0, 1.0, math.sqrt(2.0), dt, 1.0, r, rho0, tmeas,
-                                 len(tmeas), t_max)
-            wall = time.perf_counter() - t0
-        finally:
-            os.dup2(fd, 1)
-            os.close(fd)
-            os.close(devnull)
-        work = threads * r * steps * L * L
-        return dict(value=work / wall, unit="cell-updates/s", cores=threads, kind="reference",
-                    sample=f"percolationDornic_2D (stochasticSPDP.cpp:940) x {threads} threads, "
-                           f"L={L}, {r} replicates x {steps} steps each, a=0.72 b=2 D=1 sigma=sqrt2 dt=0.1",
-                    wall_s=wall)
-    p = oracle.dp_params(scheme=0)
+            ctx.save_snapshot(0, os.path.join(d, "c5.tms"))
+            snap_s = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    bd = c5_batch(tm, tm.F64, steps)
+    rd = run_workload(tm, bd, steps, warmup)
+    out = side_fields(r)
+    out["fp64"] = side_fields(rd)
+    out["snapshot_s"] = snap_s
+    out["snapshot_gbs"] = cells_of(b) * 3 * 4 / snap_s / 1e9
+    out["snapshot_share_per_1000_steps"] = snap_s / (snap_s + 1000 * r["ms_per_step"] / 1e3)
+    out["full_run_estimate_s"] = 1e5 * r["ms_per_step"] / 1e3 + 100 * snap_s
+    err = dict(rho=compare.rows_report(r["rows"], rd["rows"], "PWO"),
+               fields=compare.field_error(r["final"], rd["final"]))
+    return out, err
+
+
+def cpu_baseline(threads=8):
+    """The reference's own DP-Dornic integrator (percolationDornic_2D,
+    stochasticSPDP.cpp:940, built from its source into oracle/_ref) on C1: L = 128,
+    a = 0.72, b = 2, D = 1, sigma = sqrt 2, dt = 0.1, 10^4 steps per replicate,
+    logged every time unit.  It ends a replicate at the first logged time with no
+    occupied cell (stochasticSPDP.cpp:1019-1020); the quench goes extinct after
+    about 700 steps, so the steps it ran are counted from its rows.  On `threads`
+    host threads (one replicate each, the reference's thread-per-parameter
+    pattern) and on one thread (all replicates in one call).  C2 (Rietkerk 256^2)
+    has no buildable reference here (exprtk is absent): the oracle port on one
+    thread, 400 steps."""
+    from oracle import oracle
+    out = {}
+    ref = oracle.ref_lib()
+    L, dt = 128, 0.1
+    rho0 = oracle.init_uniform(L, [0.0], [1.0], SEED, 0).reshape(-1).copy()
+    t_meas = np.arange(2.0, 1001.0)          # logged every time unit to t = 1000 (10^4 steps)
+    if ref is not None:
+        for nthr, reps in ((threads, 1), (1, threads)):
+            fd = os.dup(1)
+            devnull = os.open(os.devnull, os.O_WRONLY)
+            os.dup2(devnull, 1)     # the reference prints a line per replicate
+            try:
+                t0 = time.perf_counter()
+                steps = ref.ref_ssp_parallel_steps(nthr, L, 0.72, 2.0, 1.0, math.sqrt(2.0), dt, 1.0, reps, rho0,
+                                                   t_meas, len(t_meas), 1000.0)
+                wall = time.perf_counter() - t0
+            finally:
+                os.dup2(fd, 1)
+                os.close(fd)
+                os.close(devnull)
+            out[nthr] = dict(value=steps * L * L / wall, wall_s=wall, replicate_steps=steps, threads=nthr,
+                             replicates=threads)
+    c2 = oracle.rk_params(R=1.0, sigma=0.5)
+    f0 = oracle.init_uniform(256, [0.0, 0.0, 0.0], [90.0, 1.0, 1.0], SEED, 0)
+    n2 = 400
     t0 = time.perf_counter()
-    oracle.dp_run(p, rho0, L, SEED, 0, 0, steps, rows=True)
-    wall = time.perf_counter() - t0
-    return dict(value=L * L * steps / wall, unit="cell-updates/s", cores=1, kind="port",
-                sample=f"CPU restatement, L={L}, {steps} steps, 1 thread", wall_s=wall)
+    oracle.rk_run(c2, f0, 256, SEED, 0, 0, n2)
+    w2 = time.perf_counter() - t0
+    c2_out = dict(value=256 * 256 * n2 / w2, unit="cell-updates/s", threads=1, kind="port",
+                  sample=f"oracle port of the Rietkerk step (FKE:4154-4465), 256^2, {n2} of C2's 2x10^4 steps",
+                  wall_s=w2)
+    host = os.cpu_count()
+    if threads in out:
+        m = out[threads]
+        return dict(value=m["value"], unit="cell-updates/s", cores=threads, kind="reference",
+                    sample=f"C1: percolationDornic_2D (stochasticSPDP.cpp:940) built from source, L=128, a=0.72, "
+                           f"b=2, D=1, sigma=sqrt2, dt=0.1, {threads} replicates (one per thread) of up to 10^4 "
+                           f"steps; it stops each at extinction: {int(m['replicate_steps'])} replicate-steps run",
+                    wall_s=m["wall_s"], threads=threads, host_logical_cpus=host,
+                    one_thread=dict(value=out[1]["value"], wall_s=out[1]["wall_s"],
+                                    replicate_steps=out[1]["replicate_steps"]),
+                    c2_one_thread_port=c2_out)
+    c2_out["host_logical_cpus"] = host
+    c2_out["cores"] = 1
+    return c2_out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps of the headline sweep (default: full 10^4)")
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--c5-steps", type=int, default=0,
-                    help="also time the 8192^2 Rietkerk lattice for this many steps (its full run is 10^5)")
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps of the headline (the C4 quench from its initial field)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--c5-steps", type=int, default=200, help="steps of the 8192^2 lattice (its full run is 10^5)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-side", action="store_true", help="headline only")
+    ap.add_argument("--no-side", action="store_true", help="headline and the full C4 runs only")
     ap.add_argument("--variant", default=None,
                     help="kernel variant for the headline (or --only) run instead of the autotuner's choice")
-    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4-f64", "c5", "sweep", "dp2", "dp3", "movement"],
+    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4", "c4-f64", "c5", "sweep", "dp2", "dp3",
+                                       "movement"],
                     default=None,
-                    help="only this side workload, --steps steps (default: its full length), one JSON line")
+                    help="only this workload, --steps steps (default: its full length), one JSON line")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -396,9 +446,7 @@ def main():
         sys.exit(2)
 
     import trophic_maelstorm_amd as tm
-
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
+    from trophic_maelstorm_amd import compare
 
     if args.only == "movement":
         print(json.dumps({"workload": "fke_movement", **movement_leg(tm)}), flush=True)
@@ -408,22 +456,28 @@ def main():
         return
     if args.only:
         b = {"c2": lambda: c2_batch(tm, tm.F64), "c3": lambda: c3_batch(tm, tm.F32),
-             "c4-f64": lambda: c4_batch(tm, tm.F64),
+             "c4": lambda: c4_batch(tm, tm.F32), "c4-f64": lambda: c4_batch(tm, tm.F64),
              "c3-f64": lambda: c3_batch(tm, tm.F64), "c5": lambda: c5_batch(tm, tm.F32),
              "dp2": lambda: c6_batch(tm, tm.MODEL_DP2), "dp3": lambda: c6_batch(tm, tm.MODEL_DP3)}[args.only]()
-        n = args.steps or b["steps"]
+        n = args.steps if "--steps" in sys.argv else b["steps"]
         log(f"[bench] {args.only}: {cells_of(b)} cells/step x {n} steps")
-        r = run_workload(tm, b, n, args.warmup, variant_name=args.variant)
-        print(json.dumps({"workload": args.only, "value": r["value"], "unit": "cell-updates/s", "steps": n,
-                          "ms_per_step": r["ms_per_step"], "step_device_ms": r["kernel_ms"],
-                          "achieved_gbs": r["achieved_gbs"], "frac": r["frac"], "variant": r["variant"]}),
-              flush=True)
+        r = run_workload(tm, b, n, args.warmup, variant_name=args.variant, blocks=args.only in ("c4", "c4-f64"))
+        out = {"workload": args.only, "value": r["value"], "unit": "cell-updates/s", "steps": n,
+               "ms_per_step": r["ms_per_step"], "step_device_ms": r["kernel_ms"],
+               "achieved_gbs": r["achieved_gbs"], "frac": r["frac"], "variant": r["variant"]}
+        if args.only in ("c4", "c4-f64"):
+            out["phases"] = phase_split(r, b)
+        print(json.dumps(out), flush=True)
         return
 
+    # ---- headline: the live start of the C4 quench, FP32 ----
     c4 = c4_batch(tm, tm.F32)
-    K = args.steps or c4["steps"]
-    log(f"[bench] C4 FP32 batched DP quench: {cells_of(c4)} cells/step x {K} steps")
-    h = run_workload(tm, c4, K, args.warmup, variant_name=args.variant)
+    K = args.steps
+    log(f"[bench] C4 FP32 batched DP quench: {cells_of(c4)} cells/step x {K} steps from t = 0")
+    # the C4 legs run the library's default launch shape (an adaptive variant built
+    # for the whole quench); the autotuner, which times the initial live state only,
+    # is not asked
+    h = run_workload(tm, c4, K, args.warmup, variant_name=args.variant, tune=False)
     log(f"[bench]   {h['value']:.4g} cell-updates/s, kernel {h['kernel_ms']:.4f} ms, "
         f"{h['achieved_gbs']:.0f} GB/s ({100 * h['frac']:.1f}% of peak)")
     if args.headline_only:
@@ -432,51 +486,61 @@ def main():
                           "achieved_gbs": h["achieved_gbs"], "variant": h["variant"],
                           "roofline": roofline(h, c4)}), flush=True)
         return
+    side = {}
+    errors = {}
     c4d = c4_batch(tm, tm.F64)
-    log("[bench] C4 FP64 (same seeds)")
-    h64 = run_workload(tm, c4d, K, args.warmup)
-    log(f"[bench]   {h64['value']:.4g} cell-updates/s, {h64['achieved_gbs']:.0f} GB/s")
-    errors = {"c4": dict(rho=rho_error(h["rows"], h64["rows"], 1), fields=field_error(h["final"], h64["final"]))}
-    side = {"c4_fp64": {k: h64[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}}
+    log(f"[bench] C4 FP64, same seeds, {K} steps")
+    h64 = run_workload(tm, c4d, K, args.warmup, tune=False)
+    side["c4_fp64"] = side_fields(h64)
+
+    # ---- the whole C4 quench, per phase, and its FP32 error ----
+    full = {}
+    for name, b in (("fp32", c4_batch(tm, tm.F32)), ("fp64", c4_batch(tm, tm.F64))):
+        log(f"[bench] C4 {name} full run, {b['steps']} steps, timed per graph block")
+        r = run_workload(tm, b, b["steps"], 0, blocks=True, tune=False)
+        full[name] = r
+        side[f"c4_full_run_{name}"] = dict(side_fields(r, ("value", "ms_per_step", "kernel_ms", "variant", "steps")),
+                                           phases=phase_split(r, b))
+    errors["c4"] = dict(rho=compare.rows_report(full["fp32"]["rows"], full["fp64"]["rows"], "r"),
+                        fields=compare.field_error(full["fp32"]["final"], full["fp64"]["final"]),
+                        steps=c4["steps"])
+    del full
 
     if not args.no_side:
         c2 = c2_batch(tm, tm.F64)
         log("[bench] C2 plain FP64 Rietkerk 256^2")
-        r2 = run_workload(tm, c2, c2["steps"], args.warmup)
-        log(f"[bench]   {r2['value']:.4g} cell-updates/s, kernel {r2['kernel_ms']:.4f} ms")
-        side["c2_plain_fp64"] = {k: r2[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
+        r2 = run_workload(tm, c2, c2["steps"], args.warmup, want_rows=False)
+        side["c2_plain_fp64"] = side_fields(r2)
         c3 = c3_batch(tm, tm.F32)
         log("[bench] C3 FP32 batched Rietkerk 256 x 512^2")
         r3 = run_workload(tm, c3, c3["steps"], args.warmup)
-        side["c3_batched_fp32"] = {k: r3[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
+        side["c3_batched_fp32"] = side_fields(r3)
         log(f"[bench]   {r3['value']:.4g} cell-updates/s, {r3['achieved_gbs']:.0f} GB/s")
         c3d = c3_batch(tm, tm.F64)
         log("[bench] C3 FP64 (same seeds)")
         r3d = run_workload(tm, c3d, c3["steps"], args.warmup)
-        log(f"[bench]   {r3d['value']:.4g} cell-updates/s, {r3d['achieved_gbs']:.0f} GB/s")
-        side["c3_fp64"] = {k: r3d[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant")}
-        errors["c3"] = dict(rho=rho_error(r3["rows"], r3d["rows"], 3), fields=field_error(r3["final"], r3d["final"]))
+        side["c3_fp64"] = side_fields(r3d)
+        errors["c3"] = dict(rho=compare.rows_report(r3["rows"], r3d["rows"], "PWO"),
+                            fields=compare.field_error(r3["final"], r3d["final"]), steps=c3["steps"])
         side["ratio_c4_fp32_over_c2_plain_fp64"] = h["value"] / r2["value"]
         side["ratio_c3_fp32_over_c2_plain_fp64"] = r3["value"] / r2["value"]
+        del r3, r3d
+        if args.c5_steps > 0:
+            log(f"[bench] C5 Rietkerk 8192^2, {args.c5_steps} of its 10^5 steps, FP32 and FP64")
+            side["c5_fp32"], errors["c5"] = c5_leg(tm, args.c5_steps, min(args.warmup, 5))
+            errors["c5"]["steps"] = args.c5_steps
         log("[bench] sweep pieces on the C3 batch (frames, series, replicate stats, output)")
         side["c3_sweep"] = sweep_leg(tm)
-        log(f"[bench]   series {side['c3_sweep']['series_value']:.4g} cell-updates/s, "
-            f"stats {side['c3_sweep']['replicate_stats_ms']:.2f} ms")
         for name, model in (("c6_consumers_dp2_fp64", tm.MODEL_DP2), ("c6_consumers_dp3_fp64", tm.MODEL_DP3)):
             c6 = c6_batch(tm, model)
             log(f"[bench] {name}: 64 x 256^2, {c6['steps']} steps")
-            r6 = run_workload(tm, c6, c6["steps"], args.warmup, want_rows=True)
-            side[name] = {k: r6[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac", "variant",
-                                            "steps")}
-            log(f"[bench]   {r6['value']:.4g} cell-updates/s, kernel {r6['kernel_ms']:.4f} ms")
-        if args.c5_steps > 0:
-            c5 = c5_batch(tm, tm.F32, args.c5_steps)
-            log(f"[bench] C5 FP32 Rietkerk 8192^2, {args.c5_steps} of its 10^5 steps")
-            r5 = run_workload(tm, c5, c5["steps"], args.warmup)
-            side["c5_fp32"] = {k: r5[k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_gbs", "frac",
-                                                 "variant", "steps")}
+            r6 = run_workload(tm, c6, c6["steps"], args.warmup, want_rows=False)
+            side[name] = side_fields(r6)
 
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (C1 reference on 8 and 1 host threads, C2 oracle port)")
+        cpu = cpu_baseline()
     out = {
         "metric": METRIC,
         "value": h["value"],
@@ -490,8 +554,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: i.i.d. Uniform initial fields from Philox(seed, instance)",
-        "config": {"workload": "C4 batched FP32 DP quench: 128x128^2 + 64x256^2 + 48x512^2 + 16x1024^2, "
-                               "a 0.72->0.70 over 10^4 steps, rho(t) FP64 every step",
+        "config": {"workload": "C4 batched FP32 DP quench, live start: 128x128^2 + 64x256^2 + 48x512^2 + "
+                               "16x1024^2, a 0.72->0.70 over 10^4 steps, steps from t = 0, rho(t) FP64 every step",
                    "instances": len(c4["inst"]), "cells_per_step": cells_of(c4), "steps": K,
                    "kernel_variant": h["variant"], "graph_steps": h["graph_steps"]},
         "roofline": roofline(h, c4),

@@ -227,6 +227,8 @@ tm_status tm_reset(tm_ctx* ctx);
  * variant that advances several steps per launch.  tm_variant_count /
  * tm_variant_name describe the table. */
 tm_status tm_set_launch(tm_ctx* ctx, int variant, int graph_steps);
+/* The launch shape in use: variant index and graph block size (either nullable). */
+tm_status tm_get_launch(const tm_ctx* ctx, int* variant, int* graph_steps);
 int tm_variant_count(int model, int precision);
 const char* tm_variant_name(int model, int precision, int variant);
 

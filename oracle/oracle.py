@@ -151,6 +151,9 @@ def ref_lib():
         R.ref_ssp_parallel.restype = C.c_int
         R.ref_ssp_parallel.argtypes = [C.c_int, C.c_int] + [C.c_double] * 6 + [
             C.c_int, _f64p, _f64p, C.c_int, C.c_double]
+        R.ref_ssp_parallel_steps.restype = C.c_double
+        R.ref_ssp_parallel_steps.argtypes = [C.c_int, C.c_int] + [C.c_double] * 6 + [
+            C.c_int, _f64p, _f64p, C.c_int, C.c_double]
         R.ref_ssp_var_mean_surv.restype = None
         R.ref_ssp_var_mean_surv.argtypes = [_f64p, _f64p, C.c_int]
         R.ref_ssp_log10_time_bins.restype = C.c_int

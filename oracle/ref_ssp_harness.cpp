@@ -90,4 +90,42 @@ int ref_ssp_parallel(int nthreads, int g, double a, double b, double D, double s
     return 0;
 }
 
+// The time steps one call executed, summed over its replicates, from its rows:
+// a replicate stops at the first logged time with no occupied cell
+// (stochasticSPDP.cpp:1019-1020), so the survivors at each logged time times the
+// steps to the next logged time count the steps run (to within one logging
+// interval per replicate).
+static double steps_of_rows(const vector<vector<double>>& rows, double dt)
+{
+    double steps = 0.0;
+    for (size_t i = 0; i + 1 < rows.size(); ++i)
+        steps += rows[i][5] * (double)(long)((rows[i + 1][2] - rows[i][2]) / dt + 0.5);
+    if (!rows.empty()) steps += rows.back()[5];
+    return steps;
+}
+
+// As ref_ssp_parallel, returning the time steps executed by all calls together
+// (replicate-steps), for a rate in cell-updates per second.
+double ref_ssp_parallel_steps(int nthreads, int g, double a, double b, double D, double sigma, double dt,
+                              double dx, int r, const double* rho0, const double* t_meas, int n_meas,
+                              double t_max)
+{
+    vector<std::thread> th;
+    vector<double> steps(nthreads, 0.0);
+    for (int k = 0; k < nthreads; ++k)
+        th.emplace_back([=, &steps]() {
+            vector<vector<double>> rows;
+            vector<double> tm(t_meas, t_meas + n_meas);
+            vector<double> r0(rho0, rho0 + (size_t)g * g);
+            percolationDornic_2D(rows, tm, r0.data(), t_max, a, b, D, sigma, dt, dx, r, g);
+            steps[k] = steps_of_rows(rows, dt);
+        });
+    for (auto& t : th) t.join();
+    std::cout.flush();
+    fflush(stdout);
+    double total = 0.0;
+    for (double v : steps) total += v;
+    return total;
+}
+
 }  // extern "C"

@@ -163,19 +163,22 @@ def test_fp32_dp_variants_agree(tm):
         np.testing.assert_allclose(r, r0, rtol=1e-12, err_msg=names[v])
 
 
-@pytest.mark.parametrize("hi", [1.0, 3.0, 4.0])
-def test_fp32_deferring_step_is_identical(tm, hi):
-    """The two-pass single step (rejected Gamma trials and PTRS-sized Poisson means
-    deferred to a queue in LDS) computes every cell as the one-pass step does: the
-    same field and the same draws.  hi = 3 mixes PTRS cells in; hi = 4 defers
-    nearly every live cell, filling the queue to most of the tile."""
+@pytest.mark.parametrize("hi,sampler", [(1.0, 0), (3.0, 0), (4.0, 0), (30.0, 0), (30.0, 1), (3.0, 1)])
+def test_fp32_deferring_step_is_identical(tm, hi, sampler):
+    """The two-pass single step (cells whose first Gamma or PTRS trial is rejected,
+    or that need a fifth word, deferred to a queue in LDS) computes every cell as
+    the one-pass step does: the same field and the same draws.  hi = 3 and 4 mix
+    PTRS cells in; hi = 30 gives Poisson means up to 250 and Gamma shapes above
+    100, where the reference's Gaussian shortcuts (sampler 1, compat) take over:
+    the first pass defers compat means above 100 and draws the compat Gamma
+    shortcut itself."""
     names = tm.variant_names(tm.MODEL_DP, tm.F32)
     insts = [tm.Instance(L, 0, 5, k) for k, L in enumerate([96, 130, 64, 50, 257])]
     out = {}
     for want in ("dp_f32_128x32_t256_w6", "dp_f32_128x24_t256_w6q"):
         with tm.Context() as ctx:
             # the exact nonlinear sub-step keeps large densities positive
-            ctx.configure(tm.MODEL_DP, tm.F32, tm.DPParams(a=0.9, scheme=tm.DP_EXACT), insts)
+            ctx.configure(tm.MODEL_DP, tm.F32, tm.DPParams(a=0.9, scheme=tm.DP_EXACT, sampler=sampler), insts)
             ctx.set_launch(names.index(want), 4)
             ctx.init_uniform([0.0], [hi])
             ctx.record_draws(True)
@@ -195,8 +198,8 @@ def test_fp32_numeric_fault_is_reported(tm, variant):
     FP64 step.  With b = 0.3 the explicit Euler sub-step y - 0.1 b y^2 turns negative
     only above y = 33: the planted densities (about 72 after the transition) fault,
     their neighbours (about 8.5) do not.  A density of 100 has a Poisson mean far
-    above 12, so the two-pass step defers these cells and records the fault from
-    its second pass."""
+    above 12 (PTRS): the two-pass step records the fault from its first pass when
+    the first PTRS and Gamma trials accept, from its second pass otherwise."""
     L = 160
     rho = np.zeros((1, L, L))
     rho[0, 5, 7] = 100.0
@@ -212,3 +215,30 @@ def test_fp32_numeric_fault_is_reported(tm, variant):
                 ctx.run(1)
             info[prec] = ctx.fault_info()
     assert info[tm.F32] == info[tm.F64] == (0, 5 * L + 7, 0)
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+@pytest.mark.parametrize("want", ["rk_f32_split_32x16_t256_q", "rk_f32_split_32x16_t256_q24"])
+def test_fp32_rietkerk_deferring_dornic_is_identical(tm, want, sampler):
+    """The two-pass Dornic kernel of the split Rietkerk step (one PTRS trial and one
+    Gamma trial first, the rest of the cells from a queue in LDS) gives the same
+    P*, draws, fields and rho(t) rows as the one-pass kernel, on odd and even
+    lattices, under both samplers."""
+    names = tm.variant_names(tm.MODEL_RIETKERK, tm.F32)
+    insts = [tm.Instance(L, k % 2, 8, k) for k, L in enumerate([96, 130, 64, 50, 257])]
+    ps = [tm.RietkerkParams(R=0.9, sampler=sampler), tm.RietkerkParams(R=1.2, sampler=sampler)]
+    out = {}
+    for v in ("rk_f32_split_32x16_t256", want):
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_RIETKERK, tm.F32, ps, insts)
+            ctx.set_launch(names.index(v), 5)
+            ctx.init_uniform([0, 0, 0], [90, 1, 1])
+            ctx.record_draws(True)
+            rows = ctx.run(10)
+            out[v] = (np.concatenate([ctx.download(k).ravel() for k in range(len(insts))]),
+                      np.concatenate([ctx.draws(k).ravel() for k in range(len(insts))]), rows)
+    (f_a, d_a, r_a), (f_b, d_b, r_b) = out.values()
+    assert np.array_equal(d_a, d_b)
+    assert (d_a >= 0).sum() > 0.9 * d_a.size
+    assert np.array_equal(f_a, f_b)
+    assert np.array_equal(r_a, r_b)

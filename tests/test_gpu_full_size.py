@@ -115,3 +115,61 @@ def test_8192_lattice_fp32_deterministic(tm):
     assert np.array_equal(f1, f2)
     assert np.array_equal(r1, r2)
     check_rows_against_fields(f1, r1[0, -1], 3)
+
+
+def test_default_dp_rows_do_not_depend_on_mode_or_batch(tm):
+    """The shipped FP32 DP default switches from single-step to eight-step launches
+    when fewer than 2% of the cells are occupied, on the batch's own occupancy, at
+    a block boundary whose timing the host picks up asynchronously.  Every launch
+    kind forms its rho(t) partials in the same thread order, so the rows do not
+    depend on where the switch falls: two runs of the whole C4 batch past the
+    switch give identical rows, and instances run alone (which switch at other
+    steps) give their rows in the batch bit for bit."""
+    params, insts = dp_batch(tm)
+    steps, pick = 728, [3, 140, 230]
+    runs = []
+    for _ in range(2):
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_DP, tm.F32, params, insts)
+            ctx.init_uniform([0.0], [1.0])
+            runs.append(ctx.run(steps))
+            single, blocked = ctx.mode_steps()
+    assert blocked > 0 and single > 0, (single, blocked)   # the run crossed the switch
+    assert np.array_equal(runs[0], runs[1])
+    for k in pick:
+        d = insts[k]
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_DP, tm.F32, params, [tm.Instance(d.L, 0, d.seed, d.instance_id)])
+            ctx.init_uniform([0.0], [1.0])
+            alone = ctx.run(steps)
+        assert np.array_equal(alone[0], runs[0][k]), k
+
+
+def test_c4_fp32_quench_through_extinction_matches_fp64(tm):
+    """The whole C4 batch with the shipped defaults, FP32 against FP64 on the same
+    seeds, for 1,120 steps: through the live start, the switch to eight-step
+    launches and past extinction.  The acceptance of the FP32 path
+    (trophic_maelstorm_amd.compare): ensemble rho(t) within 1e-3 relative while
+    the FP64 ensemble holds >= 10% of its initial mean; afterwards within 1e-3 or
+    within 4 standard errors of two independent ensembles (same-seed trajectories
+    decorrelate once an integer draw differs); surviving fractions within the
+    two-proportion 95% band (plus one instance) at every step."""
+    from trophic_maelstorm_amd import compare
+    params, insts = dp_batch(tm)
+    steps = 1120
+    rows = {}
+    for prec in (tm.F32, tm.F64):
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_DP, prec, params, insts)
+            ctx.init_uniform([0.0], [1.0])
+            rows[prec] = ctx.run(steps)
+    rep = compare.field_report(rows[tm.F32], rows[tm.F64])
+    print(rep)
+    assert rep["populated_steps"] > 20
+    assert rep["ensemble_rel_max_populated"] < 1e-3, rep
+    assert rep["tail_ok"] and rep["survival_ok"], rep
+    assert rep["accepted"]
+    # the run reaches extinction: no FP64 instance is occupied at the end, and at most
+    # a straggler or two in FP32 (the survival band above bounds the difference)
+    assert not np.any(rows[tm.F64][:, -1, 0] > 0)
+    assert np.count_nonzero(rows[tm.F32][:, -1, 0] > 0) <= 2

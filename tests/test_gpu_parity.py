@@ -245,8 +245,20 @@ def test_autotune_does_not_change_state(tm, oracle, tmp_path):
         assert ctx.step(0) == 3
         # cached choice is reused
         assert ctx.autotune(str(tmp_path / "tune.txt")) == (v, g)
-        ms = ctx.time_steps(4)
-        assert ms > 0 and ctx.step(1) == 7
+        # the timing aid: a fixed-mode variant, timed twice with its kept graphs
+        names = tm.variant_names(tm.MODEL_DP, tm.F64)
+        for want, steps in (("dp_f64_64x32_t256", 4), ("dp_f64_64x32_t256_x4", 9)):
+            ctx.set_launch(names.index(want), 10)
+            n0 = ctx.step(1)
+            m0 = ctx.mode_steps()
+            assert ctx.time_steps(steps) > 0 and ctx.time_steps(steps) > 0
+            assert ctx.step(1) == n0 + 2 * steps
+            m1 = ctx.mode_steps()
+            assert (m1[0] - m0[0]) + (m1[1] - m0[1]) == 2 * steps
+        # an adaptive variant chooses its mode from the rows fold, which the timing aid skips
+        ctx.set_launch(names.index("dp_f64_64x32_t256_x8aq"), 10)
+        with pytest.raises(tm.TMError, match="TM_ERR_STATE"):
+            ctx.time_steps(4)
 
 
 @pytest.mark.parametrize("variant", ["dp_f64_64x32_t256_x4a", "dp_f64_64x32_t256_x4", "dp_f64_64x32_t256_x8aq",

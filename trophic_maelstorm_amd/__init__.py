@@ -346,6 +346,7 @@ def lib() -> C.CDLL:
         "tm_get_step": ([vp, C.c_int, C.POINTER(i64)], C.c_int),
         "tm_reset": ([vp], C.c_int),
         "tm_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
+        "tm_get_launch": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "tm_variant_count": ([C.c_int, C.c_int], C.c_int),
         "tm_variant_name": ([C.c_int, C.c_int, C.c_int], C.c_char_p),
         "tm_autotune": ([vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -690,6 +691,12 @@ class Context:
 
     def set_launch(self, variant: int = -1, graph_steps: int = 100):
         self._check(self._lib.tm_set_launch(self._h, variant, graph_steps))
+
+    def launch(self):
+        """(variant index, graph block steps) in use."""
+        v, g = C.c_int(), C.c_int()
+        self._check(self._lib.tm_get_launch(self._h, C.byref(v), C.byref(g)))
+        return v.value, g.value
 
     def autotune(self, cache_path: str | None = None):
         v, g = C.c_int(), C.c_int()

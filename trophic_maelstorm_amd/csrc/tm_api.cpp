@@ -74,6 +74,9 @@ struct tm_ctx {
     int64_t counter = 0;
 
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [blocked][parity]
+    hipGraphExec_t time_graph[2] = {nullptr, nullptr};   // tm_time_steps blocks (no rows fold), [parity]
+    int time_graph_steps = 0;                            // ... of this many steps
+    hipEvent_t time_ev[2] = {nullptr, nullptr};
 
     // Multi-step variants: blocked = launch K steps at a time, else single steps.
     // Adaptive ones choose per block from the occupied fraction of field 0 at
@@ -204,6 +207,11 @@ void drop_graphs(tm_ctx* c)
             if (g) hipGraphExecDestroy(g);
             g = nullptr;
         }
+    for (auto& g : c->time_graph) {
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    c->time_graph_steps = 0;
 }
 
 template <typename P>
@@ -440,6 +448,25 @@ tm_status capture(tm_ctx* c, int parity)
     return capture_block(c, parity, c->graph_steps, true, &g);
 }
 
+// Every graph the current variant can replay (both parities, and both modes of
+// an adaptive variant), captured and instantiated before a run's first launch:
+// once per context and launch shape, never in the middle of a run.
+tm_status capture_all(tm_ctx* c)
+{
+    if (c->graph_steps <= 0) return TM_OK;
+    const Variant& v = cur_variant(c);
+    const bool both = v.spl > 1 && v.adaptive;
+    const bool keep = c->blocked;
+    tm_status s = TM_OK;
+    for (int m = 0; m < 2 && s == TM_OK; ++m) {
+        if (!both && (m != 0) != keep) continue;
+        c->blocked = m != 0;
+        for (int par = 0; par < 2 && s == TM_OK; ++par) s = capture(c, par);
+    }
+    c->blocked = keep;
+    return s;
+}
+
 tm_status ensure_rows(tm_ctx* c, int64_t n_steps)
 {
     if (c->rows_cap >= n_steps) return TM_OK;
@@ -554,6 +581,7 @@ tm_status advance(tm_ctx* c, int64_t n_steps)
     if ((s = set_run_base(c)) != TM_OK) return s;
     const bool adaptive = cur_variant(c).spl > 1 && cur_variant(c).adaptive;
     if (adaptive && (s = ensure_live(c)) != TM_OK) return s;
+    if (n_steps >= c->graph_steps && (s = capture_all(c)) != TM_OK) return s;
     int64_t left = n_steps;
     const int B = c->graph_steps;
     volatile double* h_sum = (volatile double*)c->h_pinned + 2;   // [2], after the run-base staging word
@@ -780,6 +808,8 @@ void tm_destroy(tm_ctx* c)
     for (auto e : c->live_ev)
         if (e) hipEventDestroy(e);
     for (auto e : c->run_ev)
+        if (e) hipEventDestroy(e);
+    for (auto e : c->time_ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -1202,6 +1232,15 @@ tm_status tm_set_launch(tm_ctx* c, int variant, int graph_steps)
     return build_tiles(c);
 }
 
+tm_status tm_get_launch(const tm_ctx* c, int* variant, int* graph_steps)
+{
+    if (!c) return TM_ERR_ARG;
+    if (!c->model) return TM_ERR_STATE;
+    if (variant) *variant = c->variant;
+    if (graph_steps) *graph_steps = c->graph_steps;
+    return TM_OK;
+}
+
 void* tm_stream(tm_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 tm_status tm_time_steps(tm_ctx* c, int64_t n_steps, double* ms_total, int64_t* launches)
@@ -1209,24 +1248,33 @@ tm_status tm_time_steps(tm_ctx* c, int64_t n_steps, double* ms_total, int64_t* l
     tm_status s = check_ready(c);
     if (s != TM_OK) return s;
     if (n_steps <= 0 || !ms_total) return fail(c, TM_ERR_ARG, "n_steps must be positive");
-    // Step kernels only (no rows fold): graphs of up to 100 step launches, one
-    // per starting parity, replayed between two events on the context stream.
+    const Variant& v = cur_variant(c);
+    if (v.spl > 1 && v.adaptive)
+        return fail(c, TM_ERR_STATE,
+                    "tm_time_steps runs no rows fold, so an adaptive variant could not choose its launch mode "
+                    "between blocks: time it with tm_run and tm_last_run_time");
     if ((s = update_mode(c)) != TM_OK) return s;
+    // Step kernels only (no rows fold): graphs of up to 100 steps, one per
+    // starting parity, captured once per context and launch shape and kept with
+    // the run graphs, replayed between two events on the context stream.
     const int B = (int)std::min<int64_t>(std::min<int64_t>(n_steps, 100), c->nslots);
-    hipGraphExec_t ge[2] = {nullptr, nullptr};
     const int flip = launches_of(c, B) & 1;
-    for (int par = 0; par < 2; ++par) {
-        if (par == 1 && !flip) break;   // blocks with an even number of launches keep the parity
-        if ((s = capture_block(c, c->parity ^ par, B, false, &ge[par])) != TM_OK) return s;
+    if (c->time_graph_steps != B) {
+        for (auto& g : c->time_graph) {
+            if (g) hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+        c->time_graph_steps = 0;
+        for (int par = 0; par < 2; ++par)   // a block from either parity
+            if ((s = capture_block(c, par, B, false, &c->time_graph[par])) != TM_OK) return s;
+        c->time_graph_steps = B;
     }
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
-    const int p0 = c->parity;
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (auto& e : c->time_ev)
+        if (!e) HIPCHK(c, hipEventCreate(&e));
+    HIPCHK(c, hipEventRecord(c->time_ev[0], c->stream));
     int64_t left = n_steps;
     while (left >= B) {
-        HIPCHK(c, hipGraphLaunch(ge[(c->parity ^ p0) & 1], c->stream));
+        HIPCHK(c, hipGraphLaunch(c->time_graph[c->parity], c->stream));
         c->parity ^= flip;
         left -= B;
     }
@@ -1236,14 +1284,11 @@ tm_status tm_time_steps(tm_ctx* c, int64_t n_steps, double* ms_total, int64_t* l
         c->parity ^= (launches_of(c, left) & 1);
         nlaunch += launches_of(c, left);
     }
-    HIPCHK(c, hipEventRecord(e1, c->stream));
-    HIPCHK(c, hipEventSynchronize(e1));
+    HIPCHK(c, hipEventRecord(c->time_ev[1], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->time_ev[1]));
     float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-    for (auto g : ge)
-        if (g) hipGraphExecDestroy(g);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    HIPCHK(c, hipEventElapsedTime(&ms, c->time_ev[0], c->time_ev[1]));
+    count_steps(c, n_steps);
     c->counter += n_steps;
     *ms_total = ms;
     if (launches) *launches = nlaunch;

@@ -124,6 +124,49 @@ __device__ __forceinline__ int poisson_tail(float z, float v, float S, float P)
     return k;
 }
 
+// PTRS (Hoermann 1993) for z >= 12: the constants of a mean, and one trial
+// from its two uniforms (U - 1/2 and V, drawn in that order).  k is formed with
+// the large mean added in FP64; the log-pmf is -stirlerr(k) - bd0(k, z) -
+// log(2 pi k)/2 (Loader 2000).  Returns true with kd set when the trial accepts.
+struct PtrsF {
+    double zd;
+    float z, b, a, invalpha, vr;
+};
+
+__device__ __forceinline__ PtrsF ptrs_setup(double zd)
+{
+    PtrsF p;
+    p.zd = zd;
+    p.z = (float)zd;
+    const float slam = __builtin_sqrtf(p.z);
+    p.b = 0.931f + 2.53f * slam;
+    p.a = -0.059f + 0.02483f * p.b;
+    p.invalpha = 1.1239f + 1.1328f * rcpf(p.b - 3.4f);
+    p.vr = 0.9277f - 3.6224f * rcpf(p.b - 2.0f);
+    return p;
+}
+
+__device__ __forceinline__ bool ptrs_trial(const PtrsF& p, float U, float V, double& kd)
+{
+    const float us = 0.5f - __builtin_fabsf(U);
+    kd = __builtin_floor((double)((2.0f * p.a * rcpf(us) + p.b) * U) + p.zd + 0.43);
+    if (us >= 0.07f && V <= p.vr) return true;
+    if (kd < 0.0 || (us < 0.013f && V > us)) return false;
+    const float k = (float)kd, z = p.z;
+    // log p(k; z) = -stirlerr(k) - bd0(k, z) - log(2 pi k)/2, bd0 = k log(k/z) + z - k
+    const float dk = (float)(kd - p.zd);
+    float bd0;
+    if (__builtin_fabsf(dk) < 0.1f * (k + z)) {
+        const float w = dk * rcpf(k + z), w2 = w * w;
+        bd0 = dk * w + 2.0f * k * w * w2 * (0.333333333f + w2 * (0.2f + w2 * (0.142857143f + w2 * 0.111111111f)));
+    } else {
+        bd0 = k * logf_hw(k * rcpf(z)) + z - k;
+    }
+    const float logp = k > 0.0f ? -stirlerr_f(k) - bd0 - 0.5f * logf_hw(6.28318531f * k) : -z;
+    const float lhs = logf_hw(V * p.invalpha * rcpf(p.a * rcpf(us * us) + p.b));
+    return lhs <= logp;
+}
+
 // Poisson(z), z formed in FP64 by the caller.  St: Stream, or Stream4 where at
 // most four words are drawn.  z < 12: the counted inversion; z >= 12: PTRS.
 template <class St>
@@ -142,32 +185,12 @@ __device__ __forceinline__ int64_t poisson_f(St& s, double zd, bool compat)
         if (poisson_count(z, v, n, S, P)) return n;
         return poisson_tail(z, v, S, P);
     }
-    const float z = (float)zd;
-    const float slam = __builtin_sqrtf(z);
-    const float b = 0.931f + 2.53f * slam;
-    const float a = -0.059f + 0.02483f * b;
-    const float invalpha = 1.1239f + 1.1328f * rcpf(b - 3.4f);
-    const float vr = 0.9277f - 3.6224f * rcpf(b - 2.0f);
+    const PtrsF pt = ptrs_setup(zd);
     for (int it = 0; it < 64; ++it) {
         const float U = s.u32() - 0.5f;
         const float V = s.u32();
-        const float us = 0.5f - __builtin_fabsf(U);
-        const double kd = __builtin_floor((double)((2.0f * a * rcpf(us) + b) * U) + zd + 0.43);
-        if (us >= 0.07f && V <= vr) return (int64_t)kd;
-        if (kd < 0.0 || (us < 0.013f && V > us)) continue;
-        const float k = (float)kd;
-        // log p(k; z) = -stirlerr(k) - bd0(k, z) - log(2 pi k)/2, bd0 = k log(k/z) + z - k
-        const float dk = (float)(kd - zd);
-        float bd0;
-        if (__builtin_fabsf(dk) < 0.1f * (k + z)) {
-            const float w = dk * rcpf(k + z), w2 = w * w;
-            bd0 = dk * w + 2.0f * k * w * w2 * (0.333333333f + w2 * (0.2f + w2 * (0.142857143f + w2 * 0.111111111f)));
-        } else {
-            bd0 = k * logf_hw(k * rcpf(z)) + z - k;
-        }
-        const float logp = k > 0.0f ? -stirlerr_f(k) - bd0 - 0.5f * logf_hw(6.28318531f * k) : -z;
-        const float lhs = logf_hw(V * invalpha * rcpf(a * rcpf(us * us) + b));
-        if (lhs <= logp) return (int64_t)kd;
+        double kd;
+        if (ptrs_trial(pt, U, V, kd)) return (int64_t)kd;
     }
     return (int64_t)__builtin_floor(zd);
 }

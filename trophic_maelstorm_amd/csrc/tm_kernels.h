@@ -207,22 +207,35 @@ __device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double 
 // from shape0 = mu + 1 = 2 alpha_i / sigma^2 (formed by the caller in FP32), and
 // the Poisson and Gamma deviates from the FP32 samplers of tm_fast32.h.
 // ONE = true is the first pass of the deferring step (dp_tile_deferred): it
-// returns false, leaving n and y unspecified, where the transition needs the
-// PTRS sampler (Poisson mean >= 12), more inversion terms than the counted
-// search holds, or a second Gamma trial; otherwise it draws the same words and
-// computes the same value as ONE = false.
+// returns false, leaving n and y unspecified, where the transition needs more
+// inversion terms than the counted search holds, a second PTRS trial, a second
+// Gamma trial or a fifth word; otherwise it draws the same words and computes
+// the same value as ONE = false.
 template <bool ONE, class St>
 __device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, const DornicConst& k, bool compat,
                                               int64_t& n, float& y)
 {
+    bool big = false;   // the Poisson draw took a PTRS trial's two words
     if (ONE) {
-        if (!(zd < 12.0)) return false;
-        int ni = 0;
-        if (zd > 0.0) {
-            float S, P;
-            if (!poisson_count((float)zd, s.v32(), ni, S, P)) return false;
+        // at most four words: the inversion's one or one PTRS trial's two, then one
+        // Gamma trial's two, and the boost's where the inversion left room for it
+        if (compat && zd > 100.0) return false;
+        if (zd < 12.0) {
+            int ni = 0;
+            if (zd > 0.0) {
+                float S, P;
+                if (!poisson_count((float)zd, s.v32(), ni, S, P)) return false;
+            }
+            n = ni;
+        } else {
+            const PtrsF pt = ptrs_setup(zd);
+            const float U = s.u32() - 0.5f;
+            const float V = s.u32();
+            double kd;
+            if (!ptrs_trial(pt, U, V, kd)) return false;
+            n = (int64_t)kd;
+            big = true;
         }
-        n = ni;
     } else {
         n = poisson_f(s, zd, compat);
     }
@@ -230,8 +243,9 @@ __device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, co
     float g;
     if (compat && gru > 100.0f) {
         g = norm_icdf_f(s.u32()) * __builtin_sqrtf(gru) + gru;
-    } else if (!gamma_f_try<ONE ? 1 : 64, St>(s, gru, g)) {
-        return false;
+    } else {
+        if (ONE && big && gru < 1.0f) return false;
+        if (!gamma_f_try<ONE ? 1 : 64, St>(s, gru, g)) return false;
     }
     // g / lambda with 1/lambda carried as hi + lo: a single FP32 rounding of 1/lambda
     // would bias every cell by the same relative amount every step.
@@ -524,38 +538,40 @@ __device__ __forceinline__ T cell_update(const typename StepConst<MODE>::type& c
     return y;
 }
 
-// First pass of the deferring FP32 DP step: cell_update<float, 0> where the
-// one-trial transition succeeds (returns true, same y, n and bad); false where
-// the cell has to be computed again by cell_update.
-template <typename T>
-__device__ __forceinline__ bool cell_update_first(const DPConst& c, const DevInst& di, uint32_t cell, T cur, T up,
-                                                  T dn, T lf, T rt, T& y, int64_t& n, bool& bad)
+// First pass of the deferring step: cell_update<T, MODE> where the one-trial
+// transition succeeds (returns true, same y, n and bad); false where the cell
+// has to be computed again by cell_update.
+template <typename T, int MODE>
+__device__ __forceinline__ bool cell_update_first(const typename StepConst<MODE>::type& c, const DevInst& di,
+                                                  uint32_t cell, T cur, T up, T dn, T lf, T rt, T& y, int64_t& n,
+                                                  bool& bad)
 {
+    static_assert(MODE == 0 || sizeof(T) == 4, "the Rietkerk Dornic first pass is FP32");
     y = (T)0;
     n = -1;
     bad = false;
     if (!(cur == (T)0 && up == (T)0 && dn == (T)0 && lf == (T)0 && rt == (T)0)) {
         y = cur;
-        // at most four words: the Poisson inversion's, one Gamma trial's two and
-        // the boost's (PTRS means, long inversions and rejected trials return
-        // false instead)
         if constexpr (sizeof(T) == 4) {
             const float nsum = ((up + dn) + lf) + rt;
             if (!(nsum == 0.0f && cur == 0.0f)) {
                 Stream4 st(di.seed, di.inst_id, cell, c.step);
-                if (!dornic_cell_f<true, Stream4>(st, c.k.shape_f * nsum, c.k.ll * (double)cur, c.k, c.compat, n, y))
+                if (!dornic_cell_f<true, Stream4>(st, c.k.shape_f * nsum, c.k.ll * (double)cur, c.k, c.compat != 0,
+                                                  n, y))
                     return false;
             }
         } else {
             const double alpha_i = nbr_alpha(c.k.dc, up, dn, lf, rt);
             if (!(alpha_i == 0.0 && cur == (T)0)) {
                 Stream4 st(di.seed, di.inst_id, cell, c.step);
-                if (!dornic_cell_d<true, Stream4>(st, alpha_i, cur, c.k, c.compat, n, y)) return false;
+                if (!dornic_cell_d<true, Stream4>(st, alpha_i, cur, c.k, c.compat != 0, n, y)) return false;
             }
         }
-        y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
-        bad = y < (T)0 || !Arith<T>::finite(y);
-        if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
+        if constexpr (MODE == 0) {
+            y = dp_nonlinear<T>(c.scheme, (T)c.b, (T)c.dt, y);
+            bad = y < (T)0 || !Arith<T>::finite(y);
+            if (y < (T)c.eps && y >= (T)c.neg) y = (T)0;
+        }
     }
     return true;
 }
@@ -686,21 +702,22 @@ __device__ __forceinline__ void dp_tile(const StepArgs& a, int tile, const DevTi
 // percent of the cells, but a wavefront runs the next trial, and the second
 // Philox block it needs, whenever any of its 64 lanes needs it, which is nearly
 // always.  So the first pass stops each cell's transition after one trial: a
-// cell whose trial was rejected, or whose Poisson mean needs the PTRS sampler,
-// goes to a queue in LDS and gets a placeholder.  After a barrier the second
-// pass runs the full step (cell_update) on the queued cells, packed densely
-// into the lanes, and overwrites the placeholders.  The first pass is the
-// first stage of the full step (same words, same arithmetic), so the field and
-// the draws are those of dp_tile.  The queue is filled in thread order (a bit
-// mask per thread and a workgroup prefix sum); it holds every cell of the tile,
-// so it cannot overflow.  A queue slot holds the cell's tile offset, then its
-// new value, which the cell's owner thread reads back after a second barrier:
-// each thread sums its own cells in dp_tile's order, so the rho(t) partials are
-// dp_tile's bit for bit.
-template <typename T, int TX, int TY, int NT>
+// cell whose first PTRS or Gamma trial was rejected, or that needs more words
+// than the first Philox block, goes to a queue in LDS.  After a barrier the
+// second pass runs the full step (cell_update) on the queued cells, packed
+// densely into the lanes.  The first pass is the first stage of the full step
+// (same words, same arithmetic), so the field and the draws are those of
+// dp_tile.  The queue is filled in thread order (a bit mask per thread and a
+// workgroup prefix sum); it holds every cell of the tile, so it cannot
+// overflow.  A queue slot holds the cell's tile offset, then its new value,
+// which the cell's owner thread reads back after a second barrier.  Each
+// thread then stores its own 4-cell groups once (a patch store of single
+// deferred cells would fetch their whole cache lines again) and sums its own
+// cells in dp_tile's order, so the rho(t) partials are dp_tile's bit for bit.
+template <typename T, int TX, int TY, int NT, int MODE>
 __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, const DevTile& tl, const DevInst& di,
-                                                 const DPConst& c, const T* s_rho, int64_t counter,
-                                                 int64_t slot, T* s_q, uint32_t* s_wn)
+                                                 const typename StepConst<MODE>::type& c, const T* s_rho,
+                                                 int64_t counter, T* s_q, uint32_t* s_wn)
 {
     typedef DPShape<T, TX, TY, NT> S;
     static_assert(S::G * 4 <= 32, "one mask bit per cell of a thread");
@@ -708,7 +725,7 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
     const int tid = threadIdx.x;
     const int cg = tid % S::CG, ly0 = tid / S::CG;
     const int L = tl.L;
-    T* fout = (T*)a.out + tl.cell_off;
+    T* fout = (MODE == 1 ? (T*)a.aux : (T*)a.out) + tl.cell_off;
     const bool vec = (L % 4 == 0) && (tl.cell_off % 4 == 0);
     uint32_t dmask = 0;                                       // bit 4 j + q: deferred
     Quad<T> own[G];                                           // first-pass values (0 where deferred)
@@ -742,8 +759,9 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
                 T y;
                 int64_t n;
                 bool bad;
-                const bool done = cell_update_first<T>(c, di, cell, s_rho[rq], s_rho[rq - RWV], s_rho[rq + RWV],
-                                                    s_rho[rq - 1], s_rho[rq + 1], y, n, bad);
+                const bool done = cell_update_first<T, MODE>(c, di, cell, s_rho[rq], s_rho[rq - RWV],
+                                                             s_rho[rq + RWV], s_rho[rq - 1], s_rho[rq + 1], y, n,
+                                                             bad);
                 if (!done) {
                     dmask |= 1u << (4 * j + q);
                     y = (T)0;
@@ -758,7 +776,6 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
                 yq.v[3] = q == 3 ? y : yq.v[3];
             }
         }
-        store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
         own[j] = yq;
     }
     // Queue offsets: an inclusive scan of the per-thread counts within the wave,
@@ -772,9 +789,6 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
         if (lane >= d) incl += o;
     }
     if (lane == 63) s_wn[tid >> 6] = incl;
-    // The placeholders are written (and acknowledged) before any thread of the
-    // workgroup overwrites them.
-    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     uint32_t base = incl - ndef, nq = 0;
 #pragma unroll
@@ -801,16 +815,17 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
         const uint32_t cell = (uint32_t)(tl.y0 + ly) * (uint32_t)L + (uint32_t)(tl.x0 + lx);
         int64_t n;
         bool bad;
-        const T y = cell_update<T, 0>(c, di, cell, s_rho[r], s_rho[r - RWV], s_rho[r + RWV], s_rho[r - 1],
-                                      s_rho[r + 1], n, bad);
+        const T y = cell_update<T, MODE>(c, di, cell, s_rho[r], s_rho[r - RWV], s_rho[r + RWV], s_rho[r - 1],
+                                         s_rho[r + 1], n, bad);
         if (bad) record_fault(a.state, tl.inst, cell, counter);
         if (a.draws) a.draws[tl.cell_off + cell] = n;
-        fout[cell] = y;
         s_q[i] = y;                                           // the slot's thread read its offset above
     }
     __syncthreads();
-    // rho(t) partials: each thread's own cells in dp_tile's order, the deferred
-    // ones read back from their queue slots (in bit order, as they were queued).
+    // Each thread stores its own groups once, the deferred cells read back from
+    // their queue slots (in bit order, as they were queued): whole 16-byte
+    // stores, no second write of a cell.  Then (MODE 0) the rho(t) partials of
+    // the same values in dp_tile's order.
     double sum = 0.0;
     uint32_t nlive = 0;
 #pragma unroll
@@ -819,23 +834,27 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
         const int gy = tl.y0 + ly, gx0 = tl.x0 + 4 * cg;
         if (gy >= L || gx0 >= L) continue;
         const int nvalid = min(4, L - gx0);
+        Quad<T> yq = own[j];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (q >= nvalid) break;
-            T y = own[j].v[q];
-            if ((dmask >> (4 * j + q)) & 1u) y = s_q[base++];
-            sum += (double)y;
-            nlive += y > (T)0 ? 1u : 0u;
+            if ((dmask >> (4 * j + q)) & 1u) yq.v[q] = s_q[base++];
+            if constexpr (MODE == 0) {
+                sum += (double)yq.v[q];
+                nlive += yq.v[q] > (T)0 ? 1u : 0u;
+            }
         }
+        store_quad(fout + (int64_t)gy * L, gx0, nvalid, vec, yq);
     }
-    step_partial<NT>(a, tile, counter, sum, nlive);
+    if constexpr (MODE == 0) step_partial<NT>(a, tile, counter, sum, nlive);
 }
 
 // MODE 0: the DP step.  MODE 1: the Dornic sub-step of the Rietkerk P field
 // alone (rietkerk_bjork_FKE.cpp:4154-4211), writing P* to a.aux for the RK4
 // kernel -- same staging, absorbing-group skip and sampler, on its own tiles.
-// Q = 1 (FP32 DP step only): the two-pass tile with its queue of deferred
-// cells.  One tile per workgroup: its descriptor, instance and constants are
+// Q = 1: the two-pass tile (dp_tile_deferred) with its queue of deferred
+// cells -- the DP step in either precision, the Rietkerk Dornic sub-step in
+// FP32.  One tile per workgroup: its descriptor, instance and constants are
 // read before the kernel writes anything, so they are scalar loads issued
 // together, and the image copy follows at once.
 template <typename T, int TX, int TY, int NT, int MODE, int MINW = 1, int Q = 0>
@@ -844,7 +863,7 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
     typedef DPShape<T, TX, TY, NT> S;
     typedef typename StepConst<MODE>::type Const;
     constexpr int NFIN = MODE == 1 ? 3 : 1;           // fields per instance block of `in`
-    static_assert(Q == 0 || MODE == 0, "deferring step: DP only");
+    static_assert(Q == 0 || MODE == 0 || sizeof(T) == 4, "deferring Rietkerk Dornic step: FP32 only");
     __shared__ __align__(16) uint32_t s_img[S::NBUF];
     __shared__ uint32_t s_flag[NT / 64];
     __shared__ __align__(8) T s_q[Q ? TX * TY : 1];   // deferred cells: tile offsets, then values
@@ -868,7 +887,7 @@ __global__ __launch_bounds__(NT, MINW) void dp_step_kernel(StepArgs a)
         return;
     }
     if constexpr (Q != 0)
-        dp_tile_deferred<T, TX, TY, NT>(a, tile, tl, di, c, (const T*)s_img, counter, slot, s_q, s_wn);
+        dp_tile_deferred<T, TX, TY, NT, MODE>(a, tile, tl, di, c, (const T*)s_img, counter, s_q, s_wn);
     else
         dp_tile<T, TX, TY, NT, MODE>(a, tile, tl, di, c, (const T*)s_img, counter, slot);
 }
@@ -1172,10 +1191,51 @@ struct Ring {
     static constexpr int N = (TX + 2 * R) * (TY + 2 * R) - TX * TY;
 };
 
+// Fixed-order block reduction of NF (FP64 sum, integer count) pairs; thread 0
+// writes them as (sum, count) partials.  The sums are combined exactly as by
+// block_partials (same shuffle tree, waves in order); counts are exact either way.
+template <int NT, int NF>
+__device__ __forceinline__ void block_partials_sc(double (&sum)[NF], uint32_t (&cnt)[NF], double2* dst,
+                                                  double* red)
+{
+    constexpr int NW = NT / 64;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sum[f] += __shfl_xor(sum[f], o);
+            cnt[f] += __shfl_xor(cnt[f], o);
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            red[(2 * f) * NW + w] = sum[f];
+            red[(2 * f + 1) * NW + w] = (double)cnt[f];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            double s = 0.0, c = 0.0;
+            for (int i = 0; i < NW; ++i) {
+                s += red[(2 * f) * NW + i];
+                c += red[(2 * f + 1) * NW + i];
+            }
+            dst[f] = make_double2(s, c);
+        }
+    }
+}
+
 // SPLIT = false: the whole step in one kernel, the Dornic P* computed on the
 // radius-3 frame of every tile (ring cells are sampled by both neighbouring
 // tiles).  SPLIT = true: P* comes from the Dornic kernel (dp_step_kernel
 // MODE 1) through a.aux, and this kernel runs the RK4 part only.
+// Each thread owns NOWN tile cells and a few cells of each ring (radius 3, 2,
+// 1); their LDS offsets in the radius-4 and radius-3 frames are computed once
+// at entry and reused by every stage.
 template <typename T, int TX, int TY, int NT, bool SPLIT>
 __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
 {
@@ -1206,34 +1266,71 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
 
     auto i4 = [](int ly, int lx) { return (ly + 4) * W4 + (lx + 4); };
     auto i3 = [](int ly, int lx) { return (ly + 3) * W3 + (lx + 3); };
-    auto ps = [&](int ly, int lx) { return SPLIT ? sP0[i4(ly, lx)] : sPs[i3(ly, lx)]; };   // P*
     auto own = [&](int j, int& ly, int& lx) {
         const int k = tid + j * NT;
         ly = k / TX;
         lx = k % TX;
     };
 
-    {   // one tile per workgroup; its descriptor, instance and constants are
-        // read before the kernel writes anything (scalar loads), then the
-        // three frames are staged by DMA
-        if ((int)blockIdx.x >= a.ntiles) return;
-        const int tile = tile_of(blockIdx.x, a.ntiles, a.xcd);
-        const DevTile tl = a.tiles[tile];
-        const DevInst di = a.inst[tl.inst];
-        const RKConstD cd = consts[tl.inst];
-        const int L = tl.L;
-        const int64_t N = (int64_t)L * L;
-        const T* fin = in + 3 * tl.cell_off;
-        T* fout = out + 3 * tl.cell_off;
-        const T* p_src = SPLIT ? (const T*)a.aux + tl.cell_off : fin;
-        tile_to_lds<T, TX, H4, NT, 4>(p_src, L, tl.x0, tl.y0, s_fr[0]);
-        tile_to_lds<T, TX, H4, NT, 4>(fin + N, L, tl.x0, tl.y0, s_fr[1]);
-        tile_to_lds<T, TX, H4, NT, 4>(fin + 2 * N, L, tl.x0, tl.y0, s_fr[2]);
-        const RKConst<T> c = rk_cast<T>(cd);
-        __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
-        __syncthreads();                 // ... everyone's
+    if ((int)blockIdx.x >= a.ntiles) return;
+    // one tile per workgroup; its descriptor, instance and constants are read
+    // before the kernel writes anything (scalar loads), then the three frames
+    // are staged by DMA
+    const int tile = tile_of(blockIdx.x, a.ntiles, a.xcd);
+    const DevTile tl = a.tiles[tile];
+    const DevInst di = a.inst[tl.inst];
+    const RKConstD cd = consts[tl.inst];
+    const int L = tl.L;
+    const int64_t N = (int64_t)L * L;
+    const T* fin = in + 3 * tl.cell_off;
+    T* fout = out + 3 * tl.cell_off;
+    const T* p_src = SPLIT ? (const T*)a.aux + tl.cell_off : fin;
+    tile_to_lds<T, TX, H4, NT, 4>(p_src, L, tl.x0, tl.y0, s_fr[0]);
+    tile_to_lds<T, TX, H4, NT, 4>(fin + N, L, tl.x0, tl.y0, s_fr[1]);
+    tile_to_lds<T, TX, H4, NT, 4>(fin + 2 * N, L, tl.x0, tl.y0, s_fr[2]);
+    const RKConst<T> c = rk_cast<T>(cd);
 
-        // ---- stage 0: Dornic P* on the radius-3 frame (FKE:4154-4211) ----
+    // LDS offsets (radius-4 frame, radius-3 frame) of this thread's cells; ring
+    // entries past the ring's end hold -1
+    int o4[NOWN], o3[NOWN], q4[NR3], q3[NR3], p4[NR2], p3[NR2], m4[NR1], m3[NR1];
+#pragma unroll
+    for (int j = 0; j < NOWN; ++j) {
+        int ly, lx;
+        own(j, ly, lx);
+        o4[j] = i4(ly, lx);
+        o3[j] = i3(ly, lx);
+    }
+#pragma unroll
+    for (int j = 0; j < NR3; ++j) {
+        const int k = tid + j * NT;
+        int ly = 0, lx = 0;
+        ring_cell<TX, TY, 3>(k < Ring<TX, TY, 3>::N ? k : 0, ly, lx);
+        q4[j] = k < Ring<TX, TY, 3>::N ? i4(ly, lx) : -1;
+        q3[j] = i3(ly, lx);
+    }
+#pragma unroll
+    for (int j = 0; j < NR2; ++j) {
+        const int k = tid + j * NT;
+        int ly = 0, lx = 0;
+        ring_cell<TX, TY, 2>(k < Ring<TX, TY, 2>::N ? k : 0, ly, lx);
+        p4[j] = k < Ring<TX, TY, 2>::N ? i4(ly, lx) : -1;
+        p3[j] = i3(ly, lx);
+    }
+#pragma unroll
+    for (int j = 0; j < NR1; ++j) {
+        const int k = tid + j * NT;
+        int ly = 0, lx = 0;
+        ring_cell<TX, TY, 1>(k < Ring<TX, TY, 1>::N ? k : 0, ly, lx);
+        m4[j] = k < Ring<TX, TY, 1>::N ? i4(ly, lx) : -1;
+        m3[j] = i3(ly, lx);
+    }
+    auto ps = [&](int r4, int r3) { return SPLIT ? sP0[r4] : sPs[r3]; };   // P*
+
+    __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
+    __syncthreads();                 // ... everyone's
+
+    // ---- stage 0 (fused step): Dornic P* on the radius-3 frame (FKE:4154-4211) ----
+    if constexpr (!SPLIT) {
         auto dornic_at = [&](int ly, int lx, int64_t* n_out) {
             const int r = i4(ly, lx);
             const int gy = wrap(tl.y0 + ly, L), gx = wrap(tl.x0 + lx, L);
@@ -1243,207 +1340,165 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
             *n_out = n;
             sPs[i3(ly, lx)] = y;
         };
-        if constexpr (!SPLIT) {
-            for (int j = 0; j < NOWN; ++j) {
-                int ly, lx;
-                own(j, ly, lx);
-                int64_t n;
-                dornic_at(ly, lx, &n);
-                const int gy = tl.y0 + ly, gx = tl.x0 + lx;
-                if (a.draws && gy < L && gx < L) a.draws[di.cell_off + (int64_t)gy * L + gx] = n;
-            }
-            for (int j = 0; j < NR3; ++j) {
-                const int k = tid + j * NT;
-                if (k < Ring<TX, TY, 3>::N) {
-                    int ly, lx;
-                    ring_cell<TX, TY, 3>(k, ly, lx);
-                    int64_t n;
-                    dornic_at(ly, lx, &n);
-                }
-            }
-            __syncthreads();
-        }
-
-        // ---- stage 1: K1 from (P*, W, O) on the radius-3 frame ----
-        T accP[NOWN], accW[NOWN], accO[NOWN];
-        auto stage1 = [&](int ly, int lx, T& kP, T& kW, T& kO) {
-            const int r4 = i4(ly, lx);
-            const T P = ps(ly, lx), W = sW0[r4], O = sO0[r4];
-            const T nW = sW0[r4 - W4] + sW0[r4 + W4] + sW0[r4 - 1] + sW0[r4 + 1];
-            const T nO = sO0[r4 - W4] + sO0[r4 + W4] + sO0[r4 - 1] + sO0[r4 + 1];
-            rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
-            const int r3 = i3(ly, lx);
-            sS[0][r3] = P + c.dt2 * kP;
-            sS[1][r3] = W + c.dt2 * kW;
-            sS[2][r3] = O + c.dt2 * kO;
-        };
-#pragma unroll
         for (int j = 0; j < NOWN; ++j) {
             int ly, lx;
             own(j, ly, lx);
-            stage1(ly, lx, accP[j], accW[j], accO[j]);
+            int64_t n;
+            dornic_at(ly, lx, &n);
+            const int gy = tl.y0 + ly, gx = tl.x0 + lx;
+            if (a.draws && gy < L && gx < L) a.draws[di.cell_off + (int64_t)gy * L + gx] = n;
         }
-#pragma unroll
         for (int j = 0; j < NR3; ++j) {
             const int k = tid + j * NT;
             if (k < Ring<TX, TY, 3>::N) {
                 int ly, lx;
                 ring_cell<TX, TY, 3>(k, ly, lx);
-                T kP, kW, kO;
-                stage1(ly, lx, kP, kW, kO);
+                int64_t n;
+                dornic_at(ly, lx, &n);
             }
         }
-        __syncthreads();
-
-        // ---- stages 2 and 3: K from the stage state in sS, new state held in
-        //      registers until every thread has read the old one ----
-        auto kstage = [&](int ly, int lx, T& kP, T& kW, T& kO) {
-            const int r = i3(ly, lx);
-            const T P = sS[0][r], W = sS[1][r], O = sS[2][r];
-            const T nW = sS[1][r - W3] + sS[1][r + W3] + sS[1][r - 1] + sS[1][r + 1];
-            const T nO = sS[2][r - W3] + sS[2][r + W3] + sS[2][r - 1] + sS[2][r + 1];
-            rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
-        };
-        auto ybase = [&](int ly, int lx, T& P, T& W, T& O) {
-            const int r4 = i4(ly, lx);
-            P = ps(ly, lx);
-            W = sW0[r4];
-            O = sO0[r4];
-        };
-        {   // stage 2: K2 on radius 2, ys3 = y + dt/2 K2
-            T nP[NOWN], nW[NOWN], nO[NOWN];
-            T rP[NR2], rW[NR2], rO[NR2];
-#pragma unroll
-            for (int j = 0; j < NOWN; ++j) {
-                int ly, lx;
-                own(j, ly, lx);
-                T kP, kW, kO, P, W, O;
-                kstage(ly, lx, kP, kW, kO);
-                ybase(ly, lx, P, W, O);
-                accP[j] = accP[j] + (T)2.0 * kP;
-                accW[j] = accW[j] + (T)2.0 * kW;
-                accO[j] = accO[j] + (T)2.0 * kO;
-                nP[j] = P + c.dt2 * kP;
-                nW[j] = W + c.dt2 * kW;
-                nO[j] = O + c.dt2 * kO;
-            }
-#pragma unroll
-            for (int j = 0; j < NR2; ++j) {
-                const int k = tid + j * NT;
-                if (k < Ring<TX, TY, 2>::N) {
-                    int ly, lx;
-                    ring_cell<TX, TY, 2>(k, ly, lx);
-                    T kP, kW, kO, P, W, O;
-                    kstage(ly, lx, kP, kW, kO);
-                    ybase(ly, lx, P, W, O);
-                    rP[j] = P + c.dt2 * kP;
-                    rW[j] = W + c.dt2 * kW;
-                    rO[j] = O + c.dt2 * kO;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < NOWN; ++j) {
-                int ly, lx;
-                own(j, ly, lx);
-                const int r = i3(ly, lx);
-                sS[0][r] = nP[j]; sS[1][r] = nW[j]; sS[2][r] = nO[j];
-            }
-#pragma unroll
-            for (int j = 0; j < NR2; ++j) {
-                const int k = tid + j * NT;
-                if (k < Ring<TX, TY, 2>::N) {
-                    int ly, lx;
-                    ring_cell<TX, TY, 2>(k, ly, lx);
-                    const int r = i3(ly, lx);
-                    sS[0][r] = rP[j]; sS[1][r] = rW[j]; sS[2][r] = rO[j];
-                }
-            }
-            __syncthreads();
-        }
-        {   // stage 3: K3 on radius 1, ys4 = y + dt K3
-            T nP[NOWN], nW[NOWN], nO[NOWN];
-            T rP[NR1], rW[NR1], rO[NR1];
-#pragma unroll
-            for (int j = 0; j < NOWN; ++j) {
-                int ly, lx;
-                own(j, ly, lx);
-                T kP, kW, kO, P, W, O;
-                kstage(ly, lx, kP, kW, kO);
-                ybase(ly, lx, P, W, O);
-                accP[j] = accP[j] + (T)2.0 * kP;
-                accW[j] = accW[j] + (T)2.0 * kW;
-                accO[j] = accO[j] + (T)2.0 * kO;
-                nP[j] = P + c.dt * kP;
-                nW[j] = W + c.dt * kW;
-                nO[j] = O + c.dt * kO;
-            }
-#pragma unroll
-            for (int j = 0; j < NR1; ++j) {
-                const int k = tid + j * NT;
-                if (k < Ring<TX, TY, 1>::N) {
-                    int ly, lx;
-                    ring_cell<TX, TY, 1>(k, ly, lx);
-                    T kP, kW, kO, P, W, O;
-                    kstage(ly, lx, kP, kW, kO);
-                    ybase(ly, lx, P, W, O);
-                    rP[j] = P + c.dt * kP;
-                    rW[j] = W + c.dt * kW;
-                    rO[j] = O + c.dt * kO;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < NOWN; ++j) {
-                int ly, lx;
-                own(j, ly, lx);
-                const int r = i3(ly, lx);
-                sS[0][r] = nP[j]; sS[1][r] = nW[j]; sS[2][r] = nO[j];
-            }
-#pragma unroll
-            for (int j = 0; j < NR1; ++j) {
-                const int k = tid + j * NT;
-                if (k < Ring<TX, TY, 1>::N) {
-                    int ly, lx;
-                    ring_cell<TX, TY, 1>(k, ly, lx);
-                    const int r = i3(ly, lx);
-                    sS[0][r] = rP[j]; sS[1][r] = rW[j]; sS[2][r] = rO[j];
-                }
-            }
-            __syncthreads();
-        }
-
-        // ---- stage 4: K4 on the tile, combine, checks, flush, write ----
-        double sum[3] = {0.0, 0.0, 0.0}, cnt[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int j = 0; j < NOWN; ++j) {
-            int ly, lx;
-            own(j, ly, lx);
-            T kP, kW, kO, P, W, O;
-            kstage(ly, lx, kP, kW, kO);
-            ybase(ly, lx, P, W, O);
-            T y[3];
-            y[0] = P + c.dt6 * (accP[j] + kP);
-            y[1] = W + c.dt6 * (accW[j] + kW);
-            y[2] = O + c.dt6 * (accO[j] + kO);
-            const int gy = tl.y0 + ly, gx = tl.x0 + lx;
-            if (gy < L && gx < L) {
-                const int64_t cell = (int64_t)gy * L + gx;
-#pragma unroll
-                for (int f = 0; f < 3; ++f) {
-                    T v = y[f];
-                    if (v < (T)0 || !Arith<T>::finite(v)) record_fault(a.state, tl.inst, f * N + cell, counter);
-                    if (v < c.eps && v >= (T)0) v = (T)0;
-                    fout[f * N + cell] = v;
-                    sum[f] += (double)v;
-                    cnt[f] += v > (T)0 ? 1.0 : 0.0;
-                }
-            }
-        }
-        double2* dst = a.partials + (slot * a.ntiles + tile) * 3;
-        block_partials<NT, 3>(sum, cnt, dst, s_red);
         __syncthreads();
     }
+
+    // ---- stage 1: K1 from (P*, W, O) on the radius-3 frame ----
+    T accP[NOWN], accW[NOWN], accO[NOWN];
+    auto stage1 = [&](int r4, int r3, T& kP, T& kW, T& kO) {
+        const T P = ps(r4, r3), W = sW0[r4], O = sO0[r4];
+        const T nW = sW0[r4 - W4] + sW0[r4 + W4] + sW0[r4 - 1] + sW0[r4 + 1];
+        const T nO = sO0[r4 - W4] + sO0[r4 + W4] + sO0[r4 - 1] + sO0[r4 + 1];
+        rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
+        sS[0][r3] = P + c.dt2 * kP;
+        sS[1][r3] = W + c.dt2 * kW;
+        sS[2][r3] = O + c.dt2 * kO;
+    };
+#pragma unroll
+    for (int j = 0; j < NOWN; ++j) stage1(o4[j], o3[j], accP[j], accW[j], accO[j]);
+#pragma unroll
+    for (int j = 0; j < NR3; ++j) {
+        if (q4[j] >= 0) {
+            T kP, kW, kO;
+            stage1(q4[j], q3[j], kP, kW, kO);
+        }
+    }
+    __syncthreads();
+
+    // ---- stages 2 and 3: K from the stage state in sS, new state held in
+    //      registers until every thread has read the old one ----
+    auto kstage = [&](int r, T& kP, T& kW, T& kO) {
+        const T P = sS[0][r], W = sS[1][r], O = sS[2][r];
+        const T nW = sS[1][r - W3] + sS[1][r + W3] + sS[1][r - 1] + sS[1][r + 1];
+        const T nO = sS[2][r - W3] + sS[2][r + W3] + sS[2][r - 1] + sS[2][r + 1];
+        rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
+    };
+    {   // stage 2: K2 on radius 2, ys3 = y + dt/2 K2
+        T nP[NOWN], nW[NOWN], nO[NOWN];
+        T rP[NR2], rW[NR2], rO[NR2];
+#pragma unroll
+        for (int j = 0; j < NOWN; ++j) {
+            T kP, kW, kO;
+            kstage(o3[j], kP, kW, kO);
+            const T P = ps(o4[j], o3[j]), W = sW0[o4[j]], O = sO0[o4[j]];
+            accP[j] = accP[j] + (T)2.0 * kP;
+            accW[j] = accW[j] + (T)2.0 * kW;
+            accO[j] = accO[j] + (T)2.0 * kO;
+            nP[j] = P + c.dt2 * kP;
+            nW[j] = W + c.dt2 * kW;
+            nO[j] = O + c.dt2 * kO;
+        }
+#pragma unroll
+        for (int j = 0; j < NR2; ++j) {
+            if (p4[j] >= 0) {
+                T kP, kW, kO;
+                kstage(p3[j], kP, kW, kO);
+                const T P = ps(p4[j], p3[j]), W = sW0[p4[j]], O = sO0[p4[j]];
+                rP[j] = P + c.dt2 * kP;
+                rW[j] = W + c.dt2 * kW;
+                rO[j] = O + c.dt2 * kO;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NOWN; ++j) {
+            sS[0][o3[j]] = nP[j]; sS[1][o3[j]] = nW[j]; sS[2][o3[j]] = nO[j];
+        }
+#pragma unroll
+        for (int j = 0; j < NR2; ++j) {
+            if (p4[j] >= 0) {
+                sS[0][p3[j]] = rP[j]; sS[1][p3[j]] = rW[j]; sS[2][p3[j]] = rO[j];
+            }
+        }
+        __syncthreads();
+    }
+    {   // stage 3: K3 on radius 1, ys4 = y + dt K3
+        T nP[NOWN], nW[NOWN], nO[NOWN];
+        T rP[NR1], rW[NR1], rO[NR1];
+#pragma unroll
+        for (int j = 0; j < NOWN; ++j) {
+            T kP, kW, kO;
+            kstage(o3[j], kP, kW, kO);
+            const T P = ps(o4[j], o3[j]), W = sW0[o4[j]], O = sO0[o4[j]];
+            accP[j] = accP[j] + (T)2.0 * kP;
+            accW[j] = accW[j] + (T)2.0 * kW;
+            accO[j] = accO[j] + (T)2.0 * kO;
+            nP[j] = P + c.dt * kP;
+            nW[j] = W + c.dt * kW;
+            nO[j] = O + c.dt * kO;
+        }
+#pragma unroll
+        for (int j = 0; j < NR1; ++j) {
+            if (m4[j] >= 0) {
+                T kP, kW, kO;
+                kstage(m3[j], kP, kW, kO);
+                const T P = ps(m4[j], m3[j]), W = sW0[m4[j]], O = sO0[m4[j]];
+                rP[j] = P + c.dt * kP;
+                rW[j] = W + c.dt * kW;
+                rO[j] = O + c.dt * kO;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NOWN; ++j) {
+            sS[0][o3[j]] = nP[j]; sS[1][o3[j]] = nW[j]; sS[2][o3[j]] = nO[j];
+        }
+#pragma unroll
+        for (int j = 0; j < NR1; ++j) {
+            if (m4[j] >= 0) {
+                sS[0][m3[j]] = rP[j]; sS[1][m3[j]] = rW[j]; sS[2][m3[j]] = rO[j];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- stage 4: K4 on the tile, combine, checks, flush, write ----
+    double sum[3] = {0.0, 0.0, 0.0};
+    uint32_t cnt[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int j = 0; j < NOWN; ++j) {
+        int ly, lx;
+        own(j, ly, lx);
+        T kP, kW, kO;
+        kstage(o3[j], kP, kW, kO);
+        const T P = ps(o4[j], o3[j]), W = sW0[o4[j]], O = sO0[o4[j]];
+        T y[3];
+        y[0] = P + c.dt6 * (accP[j] + kP);
+        y[1] = W + c.dt6 * (accW[j] + kW);
+        y[2] = O + c.dt6 * (accO[j] + kO);
+        const int gy = tl.y0 + ly, gx = tl.x0 + lx;
+        if (gy < L && gx < L) {
+            const int64_t cell = (int64_t)gy * L + gx;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                T v = y[f];
+                if (v < (T)0 || !Arith<T>::finite(v)) record_fault(a.state, tl.inst, f * N + cell, counter);
+                if (v < c.eps && v >= (T)0) v = (T)0;
+                fout[f * N + cell] = v;
+                sum[f] += (double)v;
+                cnt[f] += v > (T)0 ? 1u : 0u;
+            }
+        }
+    }
+    double2* dst = a.partials + (slot * a.ntiles + tile) * 3;
+    block_partials_sc<NT, 3>(sum, cnt, dst, s_red);
 }
 
 // ---------------------------------------------------------------------------

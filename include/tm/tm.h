@@ -35,6 +35,7 @@ typedef enum tm_status {
     TM_ERR_NOMEM = -3,     /* device or pinned allocation failed                */
     TM_ERR_STATE = -4,     /* call out of order (e.g. tm_run before configure)  */
     TM_ERR_NODEV = -5,     /* no usable gfx950 device                           */
+    TM_ERR_TOO_LARGE = -6, /* beyond an allocation-free entry point's fixed scratch */
     TM_FAULT_NUMERIC = 1   /* a negative or non-finite value was produced; see  *
                             * tm_fault_info.  Replaces the reference's error-   *
                             * frame dump + exit(3) (rietkerk_bjork_FKE.cpp:4489-4560) */
@@ -417,6 +418,16 @@ int tm_fke_rk4(int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t
 
 /* The same step on host arrays, on `device`. */
 int tm_fke_rk4_host(int device, int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t* n_bad);
+
+/* tm_fke_rk4 without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels and result as
+ * tm_fke_rk4; the stage buffers and the negative-value counter live in a fixed
+ * library array on each GPU (3 x 5 x 256^2 doubles, 7.5 MiB, held until the
+ * process ends), so L <= 256 (ns <= 2), else TM_ERR_TOO_LARGE.  Runs on the
+ * current device, whose stream `stream` must be; waits for it before returning.
+ * Calls that overlap on one GPU share the array and race. */
+int tm_fke_rk4_noalloc(int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t* n_bad, void* stream);
 
 /* The Rietkerk consumer model's vegetation Dornic sub-step (rietkerk_bjork_FKE.cpp:
  * 4150-4211): Pstar [L*L] from the old vegetation P [L*L], periodic lattice, with

@@ -34,7 +34,7 @@ STATS_INTDIV_VARIANCE, STATS_STOP_AT_EXTINCTION, STATS_REFERENCE = 1, 2, 3
 
 TM_OK, TM_FAULT_NUMERIC = 0, 1
 _STATUS = {0: "TM_OK", -1: "TM_ERR_ARG", -2: "TM_ERR_HIP", -3: "TM_ERR_NOMEM", -4: "TM_ERR_STATE",
-           -5: "TM_ERR_NODEV", 1: "TM_FAULT_NUMERIC"}
+           -5: "TM_ERR_NODEV", -6: "TM_ERR_TOO_LARGE", 1: "TM_FAULT_NUMERIC"}
 
 
 class TMError(RuntimeError):
@@ -380,6 +380,8 @@ def lib() -> C.CDLL:
                                     dp, dp], C.c_int),
         "tm_fke_rk4_host": ([C.c_int, C.c_int, C.c_int, C.c_void_p, dp, C.POINTER(C.c_int64)], C.c_int),
         "tm_fke_rk4": ([C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p], C.c_int),
+        "tm_fke_rk4_noalloc": ([C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
+                               C.c_int),
         "tm_fke_veg_dornic_host": ([C.c_int, C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, i64, dp, dp], C.c_int),
         "tm_fke_veg_dornic": ([C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, i64, C.c_void_p, C.c_void_p, C.c_void_p],
                               C.c_int),

@@ -19,6 +19,7 @@
 // in their order.
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "tm/tm.h"
@@ -615,6 +616,64 @@ int tm_fke_rk4(int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t
     if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
     // the scratch is freed on return: wait for the stages that use it
     if (hipMemcpyAsync(&bad, d_bad.p, sizeof(bad), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return TM_ERR_HIP;
+    if (n_bad) *n_bad = (int64_t)bad;
+    return TM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tm_fke_rk4_noalloc: tm_fke_rk4 with its stage buffers and its negative-value
+// counter in `scratch`, a file-scope device array, instead of per-call device
+// allocations.  Each GPU the library runs on holds its own copy until the
+// process ends.  In host code the name is only a handle: the array's device
+// address on a device comes from hipGetSymbolAddress with that device current,
+// taken on the first call on the device and kept per device.  Every call on a
+// GPU shares that GPU's array, so calls that overlap on one GPU race; the
+// counter is cleared by a kernel at the start of each call.
+// ---------------------------------------------------------------------------
+constexpr int kRk4NoAllocL = 256;            // largest lattice side (ns <= 2 consumers)
+constexpr size_t kRk4ScratchDoubles = 3 * 5 * (size_t)kRk4NoAllocL * kRk4NoAllocL;   // 3 (ns + 3) L^2
+static __device__ __align__(256) double scratch[kRk4ScratchDoubles + 1];             // + the counter
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (nullptr on failure).
+static double* rk4_scratch(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+__global__ void clear_counter_kernel(unsigned long long* n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *n = 0ull;
+}
+
+int tm_fke_rk4_noalloc(int ns, int L, const tm_fke_rk_params* p, double* fields, int64_t* n_bad, void* stream)
+{
+    FkeRk c;
+    if (!fields || !fke_rk_args(ns, L, p, &c)) return TM_ERR_ARG;
+    if (L > kRk4NoAllocL) return TM_ERR_TOO_LARGE;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return TM_ERR_NODEV;
+    double* s = rk4_scratch(dev);
+    if (!s) return TM_ERR_HIP;
+    unsigned long long* d_bad = (unsigned long long*)(s + kRk4ScratchDoubles);
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(clear_counter_kernel, dim3(1), dim3(64), 0, st, d_bad);
+    fke_rk_launch(c, fields, s, d_bad, st);
+    if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
+    unsigned long long bad = 0;
+    if (hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return TM_ERR_HIP;
     if (n_bad) *n_bad = (int64_t)bad;

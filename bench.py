@@ -58,7 +58,7 @@ SEED = 20251202
 # counters (TCC_EA0_RDREQ_{32B,64B,128B} x 32/64/128 B), writes from WRITE_SIZE
 # (KiB), each checked on a plain copy of known size.  Keyed by (variant, launch
 # kind); "single" = one step per launch, "blocked" = K steps per launch.
-TRAFFIC = {   # bytes per launch (read, write); profiles/dp_step_f32_c4_quench_hbm_traffic.md
+TRAFFIC = {   # bytes per launch (read, write); profiles/dp_step_f32_c4_live_window.md
     ("dp_f32_128x24_t256_x8aq", "single"): (234_266_000, 143_368_000),    # two-pass single step, live start
     ("dp_f32_128x24_t256_x8aq", "blocked"): (359_357_000, 148_703_000),   # eight-step launch, absorbing field
 }
@@ -428,6 +428,8 @@ def main():
                     help="only this workload, --steps steps (default: its full length), one JSON line")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP32 headline run (no FP64 comparison, side workloads or CPU baseline)")
+    ap.add_argument("--tune", action="store_true",
+                    help="with --only: run the autotuner's choice instead of the library's default launch shape")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -461,12 +463,17 @@ def main():
              "dp2": lambda: c6_batch(tm, tm.MODEL_DP2), "dp3": lambda: c6_batch(tm, tm.MODEL_DP3)}[args.only]()
         n = args.steps if "--steps" in sys.argv else b["steps"]
         log(f"[bench] {args.only}: {cells_of(b)} cells/step x {n} steps")
-        r = run_workload(tm, b, n, args.warmup, variant_name=args.variant, blocks=args.only in ("c4", "c4-f64"))
+        r = run_workload(tm, b, n, args.warmup, variant_name=args.variant, tune=args.tune,
+                         blocks=args.only in ("c4", "c4-f64"))
         out = {"workload": args.only, "value": r["value"], "unit": "cell-updates/s", "steps": n,
-               "ms_per_step": r["ms_per_step"], "step_device_ms": r["kernel_ms"],
-               "achieved_gbs": r["achieved_gbs"], "frac": r["frac"], "variant": r["variant"]}
-        if args.only in ("c4", "c4-f64"):
+               "ms_per_step": r["ms_per_step"], "step_device_ms": r["kernel_ms"], "variant": r["variant"]}
+        if r["blocked_steps"]:
+            # K-step launches advance several steps per field read: an HBM fraction
+            # per step would count bytes never moved, so only the per-phase figures,
+            # each over its own launches, are stated
             out["phases"] = phase_split(r, b)
+        else:
+            out.update(achieved_gbs=r["achieved_gbs"], frac=r["frac"])
         print(json.dumps(out), flush=True)
         return
 

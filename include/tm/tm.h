@@ -46,8 +46,12 @@ typedef enum tm_model {
     TM_MODEL_DP2 = 2,       /* vegetation + grazer: fields rho_0, rho_1, gamma_1
                              * (multiSPDP.cpp built with SPB = 2, MultiSPDP_constants_2Sp.h:5) */
     TM_MODEL_RIETKERK = 3,  /* three fields P, W, O (rietkerk_bjork_constants_1Sp.h:5, Sp = 3) */
-    TM_MODEL_DP3 = 4        /* vegetation + grazer + predator: fields rho_0, rho_1, rho_2,
+    TM_MODEL_DP3 = 4,       /* vegetation + grazer + predator: fields rho_0, rho_1, rho_2,
                              * gamma_1, gamma_2 (SPB = 3, MultiSPDP_constants_3Sp.h:5) */
+    TM_MODEL_FKE2 = 5,      /* Rietkerk vegetation + grazer: fields P, G, W, O, gamma_1
+                             * (rietkerk_bjork_FKE.cpp built with SPB = 2) */
+    TM_MODEL_FKE3 = 6       /* Rietkerk vegetation + grazer + predator: fields P, G, Pr, W, O,
+                             * gamma_1, gamma_2 (SPB = 3) */
 } tm_model;
 
 typedef enum tm_precision {
@@ -122,6 +126,44 @@ typedef struct tm_rietkerk_params {
     int32_t sampler;    /* tm_sampler */
     int32_t reserved;
 } tm_rietkerk_params;
+
+/* Rietkerk vegetation + consumer parameters (TM_MODEL_FKE2 / TM_MODEL_FKE3):
+ * rietkerk_DorFPE_2D_MultiSp(..., a, c, gmax, alpha, rW, W0, D[Sp], v[SpB], K[3], sigma[], ...,
+ * A, H, E, M, pR[], ..., dt, dx, ...) (Rietkerk_FPE/rietkerk_bjork_FKE.cpp:3697-3699) with
+ * set_global_FKE_params(D, v, dtAdv, Dt_Analytic, dx, L, CI) (:195-250).  Species 0 is the
+ * vegetation P, 1 the grazer G, 2 the predator Pr (TM_MODEL_FKE3 only).  One step is the
+ * loop body :4100-4462: perception gamma (and, with the predator, velocities) from the old
+ * densities, the vegetation's Dornic step, the RK4 of the trophic and water terms, the
+ * consumers' Fokker-Planck movement, checks and flush.  The reference's D[Sp] is split
+ * into D_P (vegetation, its Dornic step), D_mov[s] (consumer s, its movement) and
+ * D_W / D_O. */
+typedef struct tm_fke_params {
+    double a;                     /* rainfall (R)                                               */
+    double c, gmax, alpha, rW, W0;
+    double K[3];                  /* k0 (unused), k1, k2                                        */
+    double A[3][3], H[3][3];      /* attack rates and handling times: [0][1] grazer on P,
+                                   * [1][2] predator on G                                       */
+    double E[3], M[3];            /* efficiencies (E[0] unused) and mortalities; M[0] is the
+                                   * vegetation's d in its Dornic step (FKE:3847)               */
+    double D_P, D_W, D_O;
+    double D_mov[3];              /* consumers' movement diffusion; [0] unused                  */
+    double v[3];                  /* speeds: movement and, with the predator, the velocity
+                                   * draws of every species                                     */
+    double sigma[3];              /* sigma[0]: vegetation noise; sigma[1], sigma[2]: width of
+                                   * the consumers' movement stencils (FKE:3764-3772)           */
+    double pR[3];                 /* perception radii, length units; pR[0] = 0 as the
+                                   * reference's                                                */
+    double dt;                    /* time step: the Dornic step and the RK4's third stage       */
+    double dt_analytic;           /* Dt_Analytic: RK4 half and sixth steps (FKE:206-210) and the
+                                   * movement kernels                                          */
+    double dt_adv[3];             /* per consumer, with mv_invariance = 1                       */
+    double dx;
+    double flush_eps;             /* 1e-10 (rietkerk_bjork_basic.h:119)                         */
+    int32_t mv_invariance;        /* 0: beta_vel = 3 / dt_analytic; 1: 3 / dt_adv[s]            */
+    int32_t CI;                   /* stencil half-widths in sigmas, 4 in the reference          */
+    int32_t sampler;              /* tm_sampler, of the vegetation's Dornic step                */
+    int32_t reserved;
+} tm_fke_params;
 
 /* One lattice instance (a replicate at one parameter point).  Instances of
  * different L share one launch.  The Philox key is (seed, instance_id); the
@@ -203,6 +245,17 @@ tm_status tm_run(tm_ctx* ctx, int64_t n_steps, double* rho_rows);
 
 /* rho(t) row of the current state: [n_inst][2*nfields]. */
 tm_status tm_measure(tm_ctx* ctx, double* rows);
+
+/* TM_MODEL_FKE2 / TM_MODEL_FKE3: the movement statistics the reference logs at each
+ * measurement time into Rho_Mov (rietkerk_bjork_FKE.cpp:4034-4060), of the current state:
+ * out [n_inst][4 SpB] (SpB = 2 or 3 species); for each consumer s = 1 .. SpB-1, columns
+ * 4s .. 4s+3 hold the spatial mean and variance of gamma_s and the spatial means of its
+ * velocity components vx, vy in the last step (columns 0-3 are zero, as there).  The
+ * velocities are those the last step drew (zero for TM_MODEL_FKE2, which draws none, and
+ * before the first step).  The variance is the population variance,
+ * sum (gamma - mean)^2 / L^2 (variance_of_vector, :1248-1257), in FP64 with the
+ * consumer models' fixed-order sums; the reference accumulates that sum in float. */
+tm_status tm_mov_stats(tm_ctx* ctx, double* out);
 
 /* Debug aid for parity tests: when enabled, tm_run records the Poisson integer
  * of every cell of the last step (-1 where the absorbing-state skip applied,

@@ -9,6 +9,7 @@ function is pinned.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -126,6 +127,11 @@ def lib():
         L.tmo_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tmo_fke_rk4.restype = C.c_int64
         L.tmo_fke_rk4.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p]
+        L.tmo_fke_veg_dornic.argtypes = [C.POINTER(RKParams), C.c_int, C.c_uint32, C.c_uint32, C.c_int64, _f64p,
+                                         _f64p]
+        L.tmo_fke_movement_gather.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
+                                              C.c_double, _f64p, _f64p, _f64p]
+        L.tmo_fke_mov_stats.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p]
         L.tmo_fixed_sum.restype = C.c_double
         L.tmo_fixed_sum.argtypes = [_f64p, C.c_int64]
         L.tmo_dpc_rk4.argtypes = [C.POINTER(DPCParams), C.c_int, _f64p]
@@ -469,3 +475,70 @@ def fke_rk4(p: FkeRkParams, y):
     y = np.ascontiguousarray(y, np.float64).copy()
     n = lib().tmo_fke_rk4(C.cast(C.byref(p), C.c_void_p), y.shape[0] - 3, y.shape[-1], y.reshape(-1))
     return y, int(n)
+
+
+# ------------------------------------------- the Rietkerk consumer model as one step
+def fke_veg_dornic(p: RKParams, P, seed, inst, step):
+    """tmo_fke_veg_dornic: P* [L, L] from P (p.d = the vegetation's M[0])."""
+    P = np.ascontiguousarray(P, np.float64)
+    L = P.shape[-1]
+    out = np.empty(L * L)
+    lib().tmo_fke_veg_dornic(C.byref(p), L, seed, inst, step, P.reshape(-1), out)
+    return out.reshape(L, L)
+
+
+def fke_movement_gather(rho, gamma, vx, vy, rD, rV, W, pref, SD, SV):
+    """tmo_fke_movement_gather: one species' movement, summed per output cell."""
+    r = np.ascontiguousarray(rho, np.float64)
+    L = r.shape[-1]
+    out = np.empty(L * L)
+    f = lambda x: np.ascontiguousarray(x, np.float64).reshape(-1)
+    lib().tmo_fke_movement_gather(L, f(r), f(gamma), f(vx), f(vy), rD, rV, W, pref, f(SD), f(SV), out)
+    return out.reshape(L, L)
+
+
+def fke_mov_stats(gamma, vx, vy):
+    """tmo_fke_mov_stats: (mean, population variance) of gamma, means of vx and vy."""
+    g = np.ascontiguousarray(gamma, np.float64)
+    out = np.zeros(4)
+    f = lambda x: np.ascontiguousarray(x, np.float64).reshape(-1)
+    lib().tmo_fke_mov_stats(g.shape[-1], f(g), f(vx), f(vy), out)
+    return out
+
+
+def fke_window(rD, rV, pref, vmax):
+    """The gather window of a species: at least its largest reach."""
+    return max(rD, rV + int(math.floor(vmax * abs(pref) + 0.5)) + 1)
+
+
+def fke_step(p, SP, fields, seed, inst, step):
+    """One step of the Rietkerk consumer model (FKE:4100-4462) composed from the
+    operators above: perception from the old densities, the vegetation's Dornic
+    step, the RK4 of the trophic and water terms, each consumer's movement (gather
+    order), checks and flush.  p: an object with tm_fke_params' fields; SP = 2 or 3
+    species; fields [2 SP + 1, L, L] = P, G, (Pr,) W, O, gamma_1 (, gamma_2).
+    Returns (fields, fault, v_eff [SP, L, L, 2])."""
+    f = np.ascontiguousarray(fields, np.float64).copy()
+    L, ND = f.shape[-1], SP + 2
+    g, ve = fke_perception(f[:SP], f[ND:], np.zeros((SP, L, L, 2)), list(p.pR)[:SP], p.dx, list(p.v)[:SP],
+                           seed, inst, step)
+    veg = rk_params(c=p.c, gmax=p.gmax, alpha=p.alpha, rW=p.rW, W0=p.W0, d=p.M[0], D=(p.D_P, p.D_W, p.D_O),
+                    K=p.K, sigma=p.sigma[0], dt=p.dt, dx=p.dx, flush_eps=p.flush_eps, sampler=p.sampler)
+    y = f[:ND].copy()
+    y[0] = fke_veg_dornic(veg, f[0], seed, inst, step)
+    rk = fke_rk_params(a=p.a, c=p.c, gmax=p.gmax, alpha=p.alpha, rW=p.rW, W0=p.W0, K=p.K, A=p.A, H=p.H, E=p.E,
+                       M=p.M, D_W=p.D_W, D_O=p.D_O, dt=p.dt, dt_rk=p.dt_analytic, dx=p.dx)
+    y, _ = fke_rk4(rk, y)
+    for s in range(1, SP):
+        (rD, rV, pref), SD, SV = fke_move_constants(p.D_mov[s], p.v[s], p.sigma[s], p.dt_analytic, p.dx,
+                                                    p.dt_adv[s], p.mv_invariance, p.CI)
+        drawn = SP == 3
+        vx = ve[s, ..., 0] if drawn else np.zeros((L, L))
+        vy = ve[s, ..., 1] if drawn else np.zeros((L, L))
+        W = fke_window(rD, rV, pref, abs(p.v[s]) if drawn else 0.0)
+        y[s] = fke_movement_gather(y[s], g[s - 1], vx, vy, rD, rV, W, pref, SD, SV)
+    fault = bool(np.any((y < 0) | ~np.isfinite(y)))
+    y = np.where((y < p.flush_eps) & (y >= 0.0), 0.0, y)
+    if SP == 2:
+        ve = np.zeros_like(ve)
+    return np.concatenate([y, g]), fault, ve

@@ -1209,3 +1209,103 @@ int64_t tmo_fke_rk4(const tmo_fke_rk_params* p, int ns, int L, double* y)
     free(T);
     return bad;
 }
+
+/* ------------------------------------------------------------------------ */
+/* The Rietkerk consumer model as one step: its vegetation Dornic step, the */
+/* movement in gather order and the movement statistics.                    */
+/* ------------------------------------------------------------------------ */
+
+/* FKE:4150-4211: alpha_i from the old P (nR2 neighbours), the skip of empty
+ * cells with empty neighbours, the transition of tmo_rk_run's P step. */
+void tmo_fke_veg_dornic(const tmo_rk_params* p, int L, uint32_t seed, uint32_t inst, int64_t step,
+                        const double* P, double* Pstar)
+{
+    const double dx2 = p->dx * p->dx;
+    double cst[3];
+    tmo_dornic_constants(-p->d, p->D[0], p->sigma, p->dt, p->dx, cst);   /* FKE:3847-3849 */
+    const double lambda = cst[2], ll = cst[2] * cst[1];
+    const double Dp = p->D[0] / (dx2);
+    const double sigma2_1 = 1 / (p->sigma * p->sigma);
+    for (int pp = 0; pp < L; ++pp)
+        for (int q = 0; q < L; ++q) {
+            const int64_t i = (int64_t)pp * L + q;
+            const double alpha_i = Dp * (nsum(P, L, pp, q));
+            Pstar[i] = P[i];
+            if (!(alpha_i == 0 && P[i] == 0)) {
+                stream_t s;
+                st_init(&s, seed, inst, (uint32_t)i, step);
+                int64_t draw = -1;
+                Pstar[i] = dornic_cell(&s, alpha_i, P[i], ll, lambda, sigma2_1, p->sampler == 1, &draw);
+            }
+        }
+}
+
+/* x - m in (-L/2, L/2] */
+static int gap_of(int x, int m, int L)
+{
+    int e = x - m;
+    if (2 * e > L) e -= L;
+    if (2 * e <= -L) e += L;
+    return e;
+}
+
+void tmo_fke_movement_gather(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
+                             int rD, int rV, int W, double mu_pref, const double* SD, const double* SV,
+                             double* out)
+{
+    const int64_t L2 = (int64_t)L * L;
+    double* cx = (double*)malloc(sizeof(double) * L2 * 3);
+    int* mx = (int*)malloc(sizeof(int) * L2 * 2);
+    for (int64_t i = 0; i < L2; ++i) {           /* each source's terms and advected mean (FKE:3626-3637) */
+        if (rho[i] < 1.0e-10) {
+            mx[2 * i] = -1;
+            continue;
+        }
+        const int c_i = (int)(i / L), c_j = (int)(i % L);
+        const double g = gamma[i], g2 = g * g, r = rho[i];
+        mx[2 * i] = (int)(round(c_i + vx[i] * (mu_pref)) + L) % L;
+        mx[2 * i + 1] = (int)(round(c_j + vy[i] * (mu_pref)) + L) % L;
+        cx[3 * i] = (g2) * r;
+        cx[3 * i + 1] = (g - g2) * r;
+        cx[3 * i + 2] = (1 - g) * (1 - g) * r;
+    }
+    for (int x = 0; x < L; ++x)
+        for (int y = 0; y < L; ++y) {
+            double acc = 0.0;
+            for (int dr = -W; dr <= W; ++dr) {
+                const int ci = wrapi(x - dr, L), adr = abs(dr);
+                const int rowD = adr <= rD;
+                for (int dc = -W; dc <= W; ++dc) {
+                    const int cj = wrapi(y - dc, L);
+                    const int64_t i = (int64_t)ci * L + cj;
+                    if (mx[2 * i] < 0) continue;
+                    const int aex = abs(gap_of(x, mx[2 * i], L)), aey = abs(gap_of(y, mx[2 * i + 1], L));
+                    const int adc = abs(dc);
+                    const int rowV = aex <= rV, colD = adc <= rD, colV = aey <= rV;
+                    if (!((rowD || rowV) && (colD || colV))) continue;
+                    double t = 0.0;
+                    if (rowD && colD) t += cx[3 * i] * (SD[adr] * SD[adc]);
+                    if (rowD && colV) t += cx[3 * i + 1] * (SD[adr] * SV[aey]);
+                    if (rowV && colD) t += cx[3 * i + 1] * (SV[aex] * SD[adc]);
+                    if (rowV && colV) t += cx[3 * i + 2] * (SV[aex] * SV[aey]);
+                    acc += t;
+                }
+            }
+            out[(int64_t)x * L + y] = acc;
+        }
+    free(cx);
+    free(mx);
+}
+
+void tmo_fke_mov_stats(int L, const double* gamma, const double* vx, const double* vy, double out[4])
+{
+    const int64_t N = (int64_t)L * L;
+    const double mean = tmo_fixed_sum(gamma, N) / (double)N;
+    double* d = (double*)malloc(sizeof(double) * N);
+    for (int64_t i = 0; i < N; ++i) d[i] = (gamma[i] - mean) * (gamma[i] - mean);
+    out[0] = mean;
+    out[1] = tmo_fixed_sum(d, N) / (double)N;
+    out[2] = tmo_fixed_sum(vx, N) / (double)N;
+    out[3] = tmo_fixed_sum(vy, N) / (double)N;
+    free(d);
+}

@@ -33,7 +33,10 @@
  * source, and against numpy means and unbiased variances; the consumer RK4
  * (tmo_fke_rk4) by the single-species RK4 bit for bit with no grazers, the
  * two-species form bit for bit with no predators, and the ODE (solve_ivp) on
- * uniform fields.
+ * uniform fields; the vegetation Dornic step (tmo_fke_veg_dornic) by the
+ * single-species step with no plant growth; the gather-form movement by the
+ * scatter restatement (summation order apart), mass conservation and its
+ * independence of W; the movement statistics by numpy's mean and variance.
  */
 #ifndef TM_ORACLE_H
 #define TM_ORACLE_H
@@ -214,6 +217,25 @@ typedef struct tmo_fke_rk_params {
     double dt, dt_rk, dx;
 } tmo_fke_rk_params;
 int64_t tmo_fke_rk4(const tmo_fke_rk_params* p, int ns, int L, double* y);
+
+/* The Rietkerk consumer model's vegetation Dornic step (FKE:4150-4211): Pstar
+ * from the old vegetation P, with the single-species step's constants and
+ * transition (p->d is the vegetation's mortality M[0]).  No flush. */
+void tmo_fke_veg_dornic(const tmo_rk_params* p, int L, uint32_t seed, uint32_t inst, int64_t step,
+                        const double* P, double* Pstar);
+
+/* One species' movement step in gather form: every output cell sums, over the
+ * sources within W rows and columns (rows then columns, each from -W to W), the
+ * terms of each source that reach it (D x D, D x V, V x D, V x V).  The same
+ * terms as tmo_fke_movement, summed per output cell in this fixed order. */
+void tmo_fke_movement_gather(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
+                             int rD, int rV, int W, double mu_pref, const double* SD, const double* SV,
+                             double* out);
+
+/* Movement statistics of one consumer (Rho_Mov, FKE:4034-4060): out[0] mean and
+ * out[1] population variance of gamma, out[2], out[3] means of vx and vy, each
+ * sum by tmo_fixed_sum and divided by L^2; the variance sums (g - mean)^2. */
+void tmo_fke_mov_stats(int L, const double* gamma, const double* vx, const double* vy, double out[4]);
 
 #ifdef __cplusplus
 }

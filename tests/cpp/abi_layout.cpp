@@ -16,6 +16,9 @@ static_assert(offsetof(tm_dpc_params, A) == 136 && offsetof(tm_dpc_params, sampl
                   sizeof(tm_dpc_params) == 360,
               "tm_dpc_params");
 static_assert(sizeof(tm_fke_move_species) == 16, "tm_fke_move_species");
+static_assert(offsetof(tm_fke_params, D_P) == 264 && offsetof(tm_fke_params, mv_invariance) == 440 &&
+                  sizeof(tm_fke_params) == 456,
+              "tm_fke_params");
 
 #define F(S, m) std::printf("%s %s %zu %zu\n", #S, #m, offsetof(S, m), sizeof(((S*)0)->m))
 #define SZ(S) std::printf("%s sizeof %zu %zu\n", #S, sizeof(S), sizeof(S))
@@ -47,5 +50,13 @@ int main()
     F(tm_fke_rk_params, rW); F(tm_fke_rk_params, W0); F(tm_fke_rk_params, K); F(tm_fke_rk_params, A);
     F(tm_fke_rk_params, H); F(tm_fke_rk_params, E); F(tm_fke_rk_params, M); F(tm_fke_rk_params, D_W);
     F(tm_fke_rk_params, D_O); F(tm_fke_rk_params, dt); F(tm_fke_rk_params, dt_rk); F(tm_fke_rk_params, dx);
+    SZ(tm_fke_params);
+    F(tm_fke_params, a); F(tm_fke_params, c); F(tm_fke_params, gmax); F(tm_fke_params, alpha); F(tm_fke_params, rW);
+    F(tm_fke_params, W0); F(tm_fke_params, K); F(tm_fke_params, A); F(tm_fke_params, H); F(tm_fke_params, E);
+    F(tm_fke_params, M); F(tm_fke_params, D_P); F(tm_fke_params, D_W); F(tm_fke_params, D_O);
+    F(tm_fke_params, D_mov); F(tm_fke_params, v); F(tm_fke_params, sigma); F(tm_fke_params, pR);
+    F(tm_fke_params, dt); F(tm_fke_params, dt_analytic); F(tm_fke_params, dt_adv); F(tm_fke_params, dx);
+    F(tm_fke_params, flush_eps); F(tm_fke_params, mv_invariance); F(tm_fke_params, CI); F(tm_fke_params, sampler);
+    F(tm_fke_params, reserved);
     return 0;
 }

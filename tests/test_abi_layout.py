@@ -30,7 +30,8 @@ def layout(tmp_path_factory):
 
 
 MIRRORS = {"tm_dp_params": "_DP", "tm_rietkerk_params": "_RK", "tm_dpc_params": "_DPC",
-           "tm_instance_desc": "_Inst", "tm_fke_move_species": "_MoveSpecies", "tm_fke_rk_params": "_FKERK"}
+           "tm_instance_desc": "_Inst", "tm_fke_move_species": "_MoveSpecies", "tm_fke_rk_params": "_FKERK",
+           "tm_fke_params": "_FKE"}
 
 
 @pytest.mark.parametrize("cname", sorted(MIRRORS))

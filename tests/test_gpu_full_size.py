@@ -79,7 +79,7 @@ def test_full_batch_fp32_instances_independent_of_batch(tm, model):
     else:
         params, insts = rk_batch(tm)
         mid, lo, hi, steps, pick = tm.MODEL_RIETKERK, [0.0, 0.0, 0.0], [90.0, 1.0, 1.0], 6, [0, 77, 255]
-        vname = "rk_f32_split_32x16_t256"
+        vname = "rk_f32_split_32x16_t256_q24"   # the FP32 batch default
     variant = tm.variant_names(mid, tm.F32).index(vname)   # the same kernels in both runs
     nf = 1 if model == "dp" else 3
     with tm.Context() as ctx:

@@ -21,10 +21,10 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libtm.so")
 
-MODEL_DP, MODEL_DP2, MODEL_RIETKERK, MODEL_DP3 = 1, 2, 3, 4
+MODEL_DP, MODEL_DP2, MODEL_RIETKERK, MODEL_DP3, MODEL_FKE2, MODEL_FKE3 = 1, 2, 3, 4, 5, 6
 # fields per instance; the consumer models hold the densities, then the perception fields
-NFIELDS = {MODEL_DP: 1, MODEL_DP2: 3, MODEL_RIETKERK: 3, MODEL_DP3: 5}
-NSPECIES = {MODEL_DP: 1, MODEL_DP2: 2, MODEL_RIETKERK: 3, MODEL_DP3: 3}
+NFIELDS = {MODEL_DP: 1, MODEL_DP2: 3, MODEL_RIETKERK: 3, MODEL_DP3: 5, MODEL_FKE2: 5, MODEL_FKE3: 7}
+NSPECIES = {MODEL_DP: 1, MODEL_DP2: 2, MODEL_RIETKERK: 3, MODEL_DP3: 3, MODEL_FKE2: 4, MODEL_FKE3: 5}
 DPC_MAX_RADIUS = 12
 F64, F32 = 0, 1
 SAMPLER_EXACT, SAMPLER_COMPAT = 0, 1
@@ -70,6 +70,16 @@ class _DPC(C.Structure):
                 ("dt", C.c_double), ("dx", C.c_double), ("a_end", C.c_double), ("ramp_steps", C.c_int64),
                 ("flush_eps", C.c_double), ("flush_neg", C.c_double), ("sampler", C.c_int32),
                 ("reserved", C.c_int32)]
+
+
+class _FKE(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("a", "c", "gmax", "alpha", "rW", "W0")] + [
+        ("K", C.c_double * 3), ("A", (C.c_double * 3) * 3), ("H", (C.c_double * 3) * 3), ("E", C.c_double * 3),
+        ("M", C.c_double * 3), ("D_P", C.c_double), ("D_W", C.c_double), ("D_O", C.c_double),
+        ("D_mov", C.c_double * 3), ("v", C.c_double * 3), ("sigma", C.c_double * 3), ("pR", C.c_double * 3),
+        ("dt", C.c_double), ("dt_analytic", C.c_double), ("dt_adv", C.c_double * 3), ("dx", C.c_double),
+        ("flush_eps", C.c_double), ("mv_invariance", C.c_int32), ("CI", C.c_int32), ("sampler", C.c_int32),
+        ("reserved", C.c_int32)]
 
 
 class _Inst(C.Structure):
@@ -280,6 +290,55 @@ class FKERKParams:
                       self.dt if self.dt_rk is None else self.dt_rk, self.dx)
 
 
+@dataclass
+class FKEParams:
+    """The Rietkerk model with consumers (MODEL_FKE2: vegetation and grazer; MODEL_FKE3:
+    and predator) as one step: rietkerk_DorFPE_2D_MultiSp's arguments
+    (Rietkerk_FPE/rietkerk_bjork_FKE.cpp:3697-3699) with set_global_FKE_params'
+    (:195-250).  The reference's D[Sp] is split into D_P (vegetation), D_mov
+    (consumers' movement) and D_W, D_O.  Defaults: the values of
+    Rietkerk_FPE/order_3stoc_test_rietkerk.cpp:20-89 (km, hr, kg), with
+    Dt_Analytic the smallest advection time (:47-48, FKE:197-203) and dt = 0.1 hr."""
+    a: float = 1.0 / 24.0
+    c: float = 10000.0
+    gmax: float = 0.05e-3 / 24.0
+    alpha: float = 0.2 / 24.0
+    rW: float = 0.2 / 24.0
+    W0: float = 0.2
+    K: tuple = (0.0, 5.0, 5000.0)
+    A: tuple | None = None
+    H: tuple = ((0.0, 1.0, 0.0), (1.0, 0.0, 1.0), (0.0, 1.0, 0.0))
+    E: tuple = (1.0, 0.45, 0.85)
+    M: tuple = (0.25 / 24.0, 0.061609 * 20.0 ** -0.25 / 8760.0, 0.061609 * 100.0 ** -0.25 / 8760.0)
+    D_P: float = 0.00025 / 24.0
+    D_W: float = 0.00025 / 24.0
+    D_O: float = 0.025 / 24.0
+    D_mov: tuple = (0.0, 0.01028, 0.215965)
+    v: tuple = (0.0, 0.45427, 0.40587)
+    sigma: tuple = ((0.00025 / 24.0) ** 0.5 / 0.1, 1.0, 10.0)
+    pR: tuple = (0.0, 1.04285 / 2.0, 1.25536 / 2.0)
+    dt: float = 0.1
+    dt_analytic: float = 0.11817455
+    dt_adv: tuple = (0.0, 0.11817455, 0.765270868)
+    dx: float = 0.1
+    flush_eps: float = 1e-10
+    mv_invariance: int = 0
+    CI: int = 4
+    sampler: int = SAMPLER_EXACT
+
+    def __post_init__(self):
+        if self.A is None:
+            aij = 3.6 * 10.0 ** -6.08 * 20.0 ** -0.37
+            ajm = 3.6 * 10.0 ** -6.08 * 100.0 ** -0.37
+            self.A = ((0.0, aij, 0.0), (aij, 0.0, ajm), (0.0, ajm, 0.0))
+
+    def _c(self) -> _FKE:
+        return _FKE(self.a, self.c, self.gmax, self.alpha, self.rW, self.W0, _vec3(self.K), _mat3(self.A),
+                    _mat3(self.H), _vec3(self.E), _vec3(self.M), self.D_P, self.D_W, self.D_O, _vec3(self.D_mov),
+                    _vec3(self.v), _vec3(self.sigma), _vec3(self.pR), self.dt, self.dt_analytic,
+                    _vec3(self.dt_adv), self.dx, self.flush_eps, self.mv_invariance, self.CI, self.sampler, 0)
+
+
 def fke_rk4(fields, params: FKERKParams, device: int = 0):
     """One RK4 step of the Rietkerk consumer model on the GPU (tm_fke_rk4_host):
     fields [ns + 3, L, L] in the order P, G, (Pr,) W, O.  Returns (fields, n_bad):
@@ -340,6 +399,7 @@ def lib() -> C.CDLL:
         "tm_init_uniform": ([vp, dp, dp], C.c_int),
         "tm_run": ([vp, i64, vp], C.c_int),
         "tm_measure": ([vp, dp], C.c_int),
+        "tm_mov_stats": ([vp, dp], C.c_int),
         "tm_set_record_draws": ([vp, C.c_int], C.c_int),
         "tm_download_draws": ([vp, C.c_int, C.POINTER(i64)], C.c_int),
         "tm_fault_info": ([vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)], C.c_int),
@@ -581,7 +641,7 @@ class Context:
 
     def configure(self, model: int, precision: int, params, instances):
         plist = params if isinstance(params, (list, tuple)) else [params]
-        ctype = _DP if model == MODEL_DP else (_DPC if model in (MODEL_DP2, MODEL_DP3) else _RK)
+        ctype = {MODEL_DP: _DP, MODEL_DP2: _DPC, MODEL_DP3: _DPC, MODEL_FKE2: _FKE, MODEL_FKE3: _FKE}.get(model, _RK)
         parr = (ctype * len(plist))(*[p._c() for p in plist])
         iarr = (_Inst * len(instances))(*[_Inst(i.L, i.params_index, i.seed, i.instance_id, i.step0)
                                           for i in instances])
@@ -666,6 +726,14 @@ class Context:
     def measure(self) -> np.ndarray:
         out = np.empty((len(self.instances), 2 * self.nfields), np.float64)
         self._check(self._lib.tm_measure(self._h, _dptr(out)))
+        return out
+
+    def mov_stats(self) -> np.ndarray:
+        """MODEL_FKE2 / MODEL_FKE3: the Rho_Mov columns of the current state,
+        [n_inst, 4 SpB] (tm_mov_stats)."""
+        sp = 3 if self.model == MODEL_FKE3 else 2
+        out = np.empty((len(self.instances), 4 * sp), np.float64)
+        self._check(self._lib.tm_mov_stats(self._h, _dptr(out)))
         return out
 
     def record_draws(self, enable: bool = True):

@@ -113,6 +113,8 @@ struct tm_ctx {
     void* d_dpc = nullptr;
     int dpc_nchunks = 0;
     int dpc_halo = 1;
+    int fke_nblocks = 0;           // Rietkerk consumer models: 256-cell blocks (FkeWork in d_dpc)
+    double* d_mov = nullptr;       // tm_mov_stats output [n_inst][4 SpB]
 };
 
 namespace {
@@ -147,11 +149,14 @@ const Variant* variant_table(int model, int prec, int* n)
     if (model == TM_MODEL_RIETKERK) return prec == TM_F64 ? rk_variants_f64(n) : rk_variants_f32(n);
     if (model == TM_MODEL_DP2 && prec == TM_F64) return dpc2_variants_f64(n);
     if (model == TM_MODEL_DP3 && prec == TM_F64) return dpc3_variants_f64(n);
+    if (model == TM_MODEL_FKE2 && prec == TM_F64) return fke2_variants_f64(n);
+    if (model == TM_MODEL_FKE3 && prec == TM_F64) return fke3_variants_f64(n);
     *n = 0;
     return nullptr;
 }
 
 bool is_consumer_model(int model) { return model == TM_MODEL_DP2 || model == TM_MODEL_DP3; }
+bool is_fke_model(int model) { return model == TM_MODEL_FKE2 || model == TM_MODEL_FKE3; }
 
 // Fields per instance: the densities, then the consumers' perception fields.
 int nfields_of(int model)
@@ -160,6 +165,8 @@ int nfields_of(int model)
     case TM_MODEL_DP: return 1;
     case TM_MODEL_DP2: return 3;
     case TM_MODEL_DP3: return 5;
+    case TM_MODEL_FKE2: return 5;   // P, G, W, O, gamma_1
+    case TM_MODEL_FKE3: return 7;   // P, G, Pr, W, O, gamma_1, gamma_2
     default: return 3;
     }
 }
@@ -170,6 +177,8 @@ int species_of(int model)
     switch (model) {
     case TM_MODEL_DP2: return 2;
     case TM_MODEL_DP3: return 3;
+    case TM_MODEL_FKE2: return 4;   // every density the reference's frames and Rho_M hold
+    case TM_MODEL_FKE3: return 5;
     default: return nfields_of(model);
     }
 }
@@ -243,8 +252,10 @@ void release_config(tm_ctx* c)
     free_dev(c->d_frame_tab);
     free_dev(c->d_live);
     free_dev(c->d_dpc);
+    free_dev(c->d_mov);
     c->dpc_nchunks = 0;
     c->dpc_halo = 1;
+    c->fke_nblocks = 0;
     invalidate_live(c);
     c->series_cap = c->groups_cap = c->stats_cap = c->frame_tab_cap = 0;
     c->series_steps_cap = c->n_series = 0;
@@ -343,6 +354,7 @@ StepArgs step_args(tm_ctx* c, int parity, int k)
     a.work = c->d_dpc;
     a.nchunks = c->dpc_nchunks;
     a.halo = c->dpc_halo;
+    a.nblocks = c->fke_nblocks;
     return a;
 }
 
@@ -821,10 +833,10 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
                        const tm_instance_desc* instances, int n_inst)
 {
     if (!c) return TM_ERR_ARG;
-    if (model != TM_MODEL_DP && model != TM_MODEL_RIETKERK && !is_consumer_model(model))
+    if (model != TM_MODEL_DP && model != TM_MODEL_RIETKERK && !is_consumer_model(model) && !is_fke_model(model))
         return fail(c, TM_ERR_ARG, "unknown model");
     if (precision != TM_F64 && precision != TM_F32) return fail(c, TM_ERR_ARG, "unknown precision");
-    if (is_consumer_model(model) && precision != TM_F64)
+    if ((is_consumer_model(model) || is_fke_model(model)) && precision != TM_F64)
         return fail(c, TM_ERR_ARG, "the consumer models run in TM_F64 only");
     if (!params || n_params <= 0 || !instances || n_inst <= 0) return fail(c, TM_ERR_ARG, "empty batch");
     if (is_consumer_model(model)) {
@@ -860,8 +872,10 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
     c->prec = precision;
     c->nf = nfields_of(model);
     c->esize = precision == TM_F64 ? sizeof(double) : sizeof(float);
-    c->param_size = model == TM_MODEL_DP ? sizeof(tm_dp_params)
-                    : is_consumer_model(model) ? sizeof(tm_dpc_params) : sizeof(tm_rietkerk_params);
+    c->param_size = model == TM_MODEL_DP         ? sizeof(tm_dp_params)
+                    : is_consumer_model(model) ? sizeof(tm_dpc_params)
+                    : is_fke_model(model)      ? sizeof(tm_fke_params)
+                                               : sizeof(tm_rietkerk_params);
     c->params.assign((const unsigned char*)params, (const unsigned char*)params + c->param_size * n_params);
     c->n_params = n_params;
     c->inst.assign(instances, instances + n_inst);
@@ -891,6 +905,17 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
     tm_status s = build_tiles(c);
     if (s != TM_OK) return s;
     if (is_consumer_model(model) && (s = setup_dpc(c)) != TM_OK) return s;
+    if (is_fke_model(model)) {
+        const char* err = "";
+        const int rc = fke_setup(model, (const tm_fke_params*)c->params.data(), n_params, c->inst.data(), n_inst,
+                                 c->cell_off.data(), c->total_cells, &c->d_dpc, &c->dpc_nchunks, &c->fke_nblocks,
+                                 &err);
+        if (rc != TM_OK) {
+            const std::string msg = err;
+            release_config(c);
+            return fail(c, (tm_status)rc, msg);
+        }
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return TM_OK;
 }
@@ -976,7 +1001,8 @@ tm_status tm_init_uniform(tm_ctx* c, const double* lo, const double* hi)
             v.b[f] = hi[f];
         }
     }
-    const double sd_none[kMaxFields] = {1.0, 1.0, 1.0, 1.0, 1.0};
+    double sd_none[kMaxFields];
+    std::fill(sd_none, sd_none + kMaxFields, 1.0);
     return enqueue_frames(c, 2, 0, sd_none, vals);
 }
 
@@ -1061,7 +1087,8 @@ tm_status tm_init_speckle(tm_ctx* c, const double* perc, const double* high, con
             v.b[f] = low[(size_t)g * c->nf + f];
         }
     }
-    const double sd_none[kMaxFields] = {1.0, 1.0, 1.0, 1.0, 1.0};
+    double sd_none[kMaxFields];
+    std::fill(sd_none, sd_none + kMaxFields, 1.0);
     return enqueue_frames(c, 0, 0, sd_none, vals);
 }
 
@@ -1072,7 +1099,8 @@ tm_status tm_init_gauss(tm_ctx* c, int n_gauss, const double* sd_div, int n_sd, 
     if (!amp || n_gauss < 0 || n_gauss > c->nf) return fail(c, TM_ERR_ARG, "tm_init_gauss: bad n_gauss or amp");
     if (n_gauss > 0 && (!sd_div || n_sd < 1)) return fail(c, TM_ERR_ARG, "tm_init_gauss: need sd_div[n_sd >= 1]");
     // Missing widths repeat the last one given (rietkerk_bjork_FKE.cpp:1118-1127).
-    double sd[kMaxFields] = {1.0, 1.0, 1.0, 1.0, 1.0};
+    double sd[kMaxFields];
+    std::fill(sd, sd + kMaxFields, 1.0);
     for (int f = 0; f < n_gauss; ++f) {
         sd[f] = sd_div[std::min(f, n_sd - 1)];
         if (!(sd[f] > 0.0)) return fail(c, TM_ERR_ARG, "tm_init_gauss: sd_div must be > 0");
@@ -1155,6 +1183,23 @@ tm_status tm_measure(tm_ctx* c, double* rows)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(d_out);
     if (e != hipSuccess) return fail(c, TM_ERR_HIP, std::string("tm_measure: ") + hipGetErrorString(e));
+    return TM_OK;
+}
+
+tm_status tm_mov_stats(tm_ctx* c, double* out)
+{
+    tm_status s = check_ready(c);
+    if (s != TM_OK) return s;
+    if (!out) return fail(c, TM_ERR_ARG, "null out");
+    if (!is_fke_model(c->model)) return fail(c, TM_ERR_STATE, "tm_mov_stats: not a Rietkerk consumer model");
+    const int sp = c->model == TM_MODEL_FKE3 ? 3 : 2;
+    const size_t bytes = sizeof(double) * 4 * sp * c->inst.size();
+    if (!c->d_mov) HIPCHK(c, hipMalloc((void**)&c->d_mov, bytes));
+    launch_fke_mov_stats(c->model, step_args(c, c->parity, 0), c->d_mov, c->stream);
+    HIPCHK(c, hipGetLastError());
+    hipError_t e = hipMemcpyAsync(out, c->d_mov, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, TM_ERR_HIP, std::string("tm_mov_stats: ") + hipGetErrorString(e));
     return TM_OK;
 }
 
@@ -1505,7 +1550,8 @@ tm_status tm_replicate_stats(tm_ctx* c, int flags, double* stats)
     if (flags & ~TM_STATS_REFERENCE) return fail(c, TM_ERR_ARG, "tm_replicate_stats: unknown flags");
     if (c->n_series <= 0) return fail(c, TM_ERR_STATE, "tm_run_series has not been called");
     const int n_inst = (int)c->inst.size(), ng = c->n_params, n_meas = c->n_series;
-    const int nsp = is_consumer_model(c->model) ? species_of(c->model) : 0;   // multi-species rows
+    const int nsp = is_consumer_model(c->model) || is_fke_model(c->model) ? species_of(c->model)
+                                                                           : 0;   // multi-species rows
     const int ncol = nsp > 0 ? 4 * nsp : c->nf + 3;
     // Instances grouped by parameter point, in instance (= replicate) order; then
     // the first extinct measurement per (instance, species).
@@ -1544,7 +1590,9 @@ namespace tmk {
 int default_variant(int model, int precision, int64_t total_cells)
 {
     const char* want;
-    if (model == TM_MODEL_DP2) {
+    if (model == TM_MODEL_FKE2 || model == TM_MODEL_FKE3) {
+        return 0;                          // one launch shape
+    } else if (model == TM_MODEL_DP2) {
         want = "dpc2_f64_32x16_t256_w5";   // measured fastest on the consumer sweep batch
     } else if (model == TM_MODEL_DP3) {
         want = "dpc3_f64_32x16_t256_w4";
@@ -1555,9 +1603,11 @@ int default_variant(int model, int precision, int64_t total_cells)
         want = precision == TM_F64 ? "dp_f64_64x32_t256_x8aq_w6" : "dp_f32_128x24_t256_x8aq";
     } else {
         // Rietkerk: small lattices are latency-bound and want the single fused
-        // 16 x 16 launch; batches want the split step (each P* sampled once).
+        // 16 x 16 launch; batches want the split step (each P* sampled once),
+        // in FP32 with the two-pass Dornic kernel on 128 x 24 tiles (C3 batch:
+        // 4.6e10 against 4.3e10 cell-updates/s for the one-pass 128 x 32 kernel)
         if (total_cells < (1 << 20)) return 0;
-        want = precision == TM_F64 ? "rk_f64_split_32x16_t256" : "rk_f32_split_32x16_t256";
+        want = precision == TM_F64 ? "rk_f64_split_32x16_t256" : "rk_f32_split_32x16_t256_q24";
     }
     const int n = tm_variant_count(model, precision);
     for (int v = 0; v < n; ++v)

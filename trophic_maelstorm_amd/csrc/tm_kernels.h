@@ -26,7 +26,7 @@
 
 namespace tmk {
 
-constexpr int kMaxFields = 5;   // fields per instance: 1 (DP), 3 (DP2, Rietkerk), 5 (DP3)
+constexpr int kMaxFields = 8;   // fields per instance: 1 (DP), 3 (DP2, Rietkerk), 5 (DP3, FKE2), 7 (FKE3)
 
 struct DevInst {
     int32_t L;
@@ -77,9 +77,11 @@ struct StepArgs {
     int32_t n_inst;
     int32_t xcd;         // XCD-aware tile order
     int32_t ntiles_a;
-    const void* work;    // consumer models: DPCWork (device) of the batch
+    const void* work;    // consumer models: DPCWork / FkeWork (device) of the batch
     int32_t nchunks;     // consumer models: statistics chunks of the batch
     int32_t halo;        // consumer models: LDS halo (largest perception radius, >= 1)
+    int32_t nblocks;     // Rietkerk consumer models: 256-cell blocks of the batch
+    int32_t pad;
 };
 
 // Periodic index: v mod L in [0, L).
@@ -1534,6 +1536,33 @@ __device__ __forceinline__ DPCConst dpc_constants(const tm_dpc_params& p, int64_
     return c;
 }
 
+// The vegetation's Dornic step of the Rietkerk consumer models as a Rietkerk
+// parameter set: d = M[0], D_P, sigma[0], dt, dx and the sampler (FKE:3847-3849,
+// :4150-4211); the RK4 fields are not used by that step.
+__host__ __device__ inline tm_rietkerk_params fke_veg_params(const tm_fke_params& p)
+{
+    tm_rietkerk_params r{};
+    r.R = p.a;
+    r.c = p.c;
+    r.gmax = p.gmax;
+    r.alpha = p.alpha;
+    r.rW = p.rW;
+    r.W0 = p.W0;
+    r.d = p.M[0];
+    r.D[0] = p.D_P;
+    r.D[1] = p.D_W;
+    r.D[2] = p.D_O;
+    r.K[0] = p.K[0];
+    r.K[1] = p.K[1];
+    r.K[2] = p.K[2];
+    r.sigma = p.sigma[0];
+    r.dt = p.dt;
+    r.dx = p.dx;
+    r.flush_eps = p.flush_eps;
+    r.sampler = p.sampler;
+    return r;
+}
+
 template <int MODEL>
 __global__ __launch_bounds__(256) void prep_consts_kernel(const DevInst* inst, const void* params,
                                                           const DevState* state, int n_inst, int nsteps,
@@ -1548,6 +1577,8 @@ __global__ __launch_bounds__(256) void prep_consts_kernel(const DevInst* inst, c
         ((DPConst*)table)[idx] = dp_constants(((const tm_dp_params*)params)[di.pidx], step);
     else if (MODEL == TM_MODEL_DP2)   // both consumer models
         ((DPCConst*)table)[idx] = dpc_constants(((const tm_dpc_params*)params)[di.pidx], step);
+    else if (MODEL == TM_MODEL_FKE2)  // both Rietkerk consumer models: the vegetation's Dornic step
+        ((RKConstD*)table)[idx] = rk_constants(fke_veg_params(((const tm_fke_params*)params)[di.pidx]), step);
     else
         ((RKConstD*)table)[idx] = rk_constants(((const tm_rietkerk_params*)params)[di.pidx], step);
 }
@@ -1619,7 +1650,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
 __global__ __launch_bounds__(256) void combine_rows_kernel(CombineArgs a)
 {
 #pragma clang fp contract(off)
-    __shared__ double s_s[kMaxFields][256], s_e[kMaxFields][256], s_n[kMaxFields][256];
+    __shared__ double s_s[256], s_e[256], s_n[256];
     const int i = blockIdx.x, j = blockIdx.y;
     const DevInst di = a.inst[i];
     int64_t slot = 0, row = 0;
@@ -1631,6 +1662,9 @@ __global__ __launch_bounds__(256) void combine_rows_kernel(CombineArgs a)
     const double2* p = a.partials + ((slot * a.ntiles + di.tile0) * a.pw) * a.nf;
     const int nsub = di.ntiles * a.pw;
     const int tid = threadIdx.x;
+    const double cells = (double)di.L * (double)di.L;
+    double* out = a.rows + ((int64_t)i * a.rows_cap + row) * (2 * a.nf);
+    // one field at a time: the fold of each field is independent of the others
     for (int f = 0; f < a.nf; ++f) {
         double s = 0.0, e = 0.0, n = 0.0;
         for (int t = tid; t < nsub; t += 256) {
@@ -1641,30 +1675,25 @@ __global__ __launch_bounds__(256) void combine_rows_kernel(CombineArgs a)
             s = tt;
             n += v.y;
         }
-        s_s[f][tid] = s;
-        s_e[f][tid] = -e;
-        s_n[f][tid] = n;
-    }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) {
-            for (int f = 0; f < a.nf; ++f) {
-                double s, e;
-                two_sum(s_s[f][tid], s_s[f][tid + w], s, e);
-                s_s[f][tid] = s;
-                s_e[f][tid] = s_e[f][tid] + s_e[f][tid + w] + e;
-                s_n[f][tid] += s_n[f][tid + w];
-            }
-        }
+        s_s[tid] = s;
+        s_e[tid] = -e;
+        s_n[tid] = n;
         __syncthreads();
-    }
-    if (tid == 0) {
-        const double cells = (double)di.L * (double)di.L;
-        double* out = a.rows + ((int64_t)i * a.rows_cap + row) * (2 * a.nf);
-        for (int f = 0; f < a.nf; ++f) {
-            out[2 * f] = s_n[f][0];
-            out[2 * f + 1] = (s_s[f][0] + s_e[f][0]) / cells;
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) {
+                double ss, ee;
+                two_sum(s_s[tid], s_s[tid + w], ss, ee);
+                s_s[tid] = ss;
+                s_e[tid] = s_e[tid] + s_e[tid + w] + ee;
+                s_n[tid] += s_n[tid + w];
+            }
+            __syncthreads();
         }
+        if (tid == 0) {
+            out[2 * f] = s_n[0];
+            out[2 * f + 1] = (s_s[0] + s_e[0]) / cells;
+        }
+        __syncthreads();   // the next field reuses the arrays
     }
 }
 #endif

@@ -81,8 +81,11 @@ void launch_measure_f64(int nf, const DevInst* inst, const DevTile* tiles, int n
     else if (nf == 3)
         hipLaunchKernelGGL((measure_kernel<double, 3>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
                            ty, (const double*)fields, partials);
-    else
+    else if (nf == 5)
         hipLaunchKernelGGL((measure_kernel<double, 5>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
+                           ty, (const double*)fields, partials);
+    else
+        hipLaunchKernelGGL((measure_kernel<double, 7>), dim3(ntiles), dim3(256), 0, s, inst, tiles, ntiles, tx,
                            ty, (const double*)fields, partials);
 }
 
@@ -101,6 +104,9 @@ void launch_prep(int model, const DevInst* inst, const void* params, const DevSt
                            n_inst, nsteps, table);
     else if (model == TM_MODEL_DP2 || model == TM_MODEL_DP3)
         hipLaunchKernelGGL(prep_consts_kernel<TM_MODEL_DP2>, dim3(grid), dim3(256), 0, s, inst, params, state,
+                           n_inst, nsteps, table);
+    else if (model == TM_MODEL_FKE2 || model == TM_MODEL_FKE3)
+        hipLaunchKernelGGL(prep_consts_kernel<TM_MODEL_FKE2>, dim3(grid), dim3(256), 0, s, inst, params, state,
                            n_inst, nsteps, table);
     else
         hipLaunchKernelGGL(prep_consts_kernel<TM_MODEL_RIETKERK>, dim3(grid), dim3(256), 0, s, inst, params,

@@ -68,6 +68,19 @@ inline size_t dpc_lds_bytes(int ns, int tx, int ty, int halo)
            sizeof(int) * (size_t)(2 * halo + 1) * (2 * halo + 1);
 }
 
+// Rietkerk consumer models (tm_movement.hip), FP64 only.  fke_setup builds the
+// batch's device tables and scratch behind one allocation (*work, released by
+// the caller with hipFree): per-point constants, statistics chunks, perception
+// offsets, the 256-cell block table and the RK4 stage and movement buffers.
+// Returns TM_OK, or a tm_status with *err naming the problem.
+const Variant* fke2_variants_f64(int* n);
+const Variant* fke3_variants_f64(int* n);
+int fke_setup(int model, const tm_fke_params* params, int n_params, const tm_instance_desc* inst, int n_inst,
+              const int64_t* cell_off, int64_t total_cells, void** work, int* nchunks, int* nblocks,
+              const char** err);
+// Movement statistics of the current fields into out [n_inst][4 SpB] (device).
+void launch_fke_mov_stats(int model, const StepArgs& a, double* out, hipStream_t s);
+
 // Default variant index per (model, precision, batch size in cells).
 int default_variant(int model, int precision, int64_t total_cells);
 

@@ -18,13 +18,16 @@
 // deterministic.  The sums differ from the reference's sequential scatter only
 // in their order.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
 #include "tm/tm.h"
 #include "tm_device.h"
 #include "tm_kernels.h"
+#include "tm_launch.h"
 
 namespace {
 
@@ -58,6 +61,22 @@ __device__ __forceinline__ int wrapi(int v, int L)
 // Per source cell: its three term coefficients (gamma^2 rho, (gamma - gamma^2) rho,
 // (1 - gamma)^2 rho, formed as the reference forms them) and its advected mean
 // cell, once per call; mu.x = -1 marks a source below the 1e-10 floor.
+__device__ __forceinline__ void move_prep_at(double pref, int L, int64_t i, double r, double g, double2 v,
+                                             double4& coef, int2& mu)
+{
+#pragma clang fp contract(off)
+    if (r < kMoveEps) {
+        mu = make_int2(-1, -1);
+        return;
+    }
+    const int ci = (int)(i / L), cj = (int)(i % L);
+    const int mux = (int)(__builtin_round(ci + v.x * (pref)) + L) % L;
+    const int muy = (int)(__builtin_round(cj + v.y * (pref)) + L) % L;
+    const double g2 = g * g;
+    coef = make_double4((g2) * r, (g - g2) * r, (1 - g) * (1 - g) * r, 0.0);
+    mu = make_int2(mux, muy);
+}
+
 __global__ __launch_bounds__(256) void fke_move_prep_kernel(MoveArgs a, const double* __restrict__ rho,
                                                             const double* __restrict__ gamma,
                                                             const double2* __restrict__ veff,
@@ -69,19 +88,8 @@ __global__ __launch_bounds__(256) void fke_move_prep_kernel(MoveArgs a, const do
     const int64_t L2 = (int64_t)L * L;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= L2) return;
-    const double r = rho[s * L2 + i];
-    if (r < kMoveEps) {
-        mu[s * L2 + i] = make_int2(-1, -1);
-        return;
-    }
-    const double pref = a.sp[s].pref;
-    const int ci = (int)(i / L), cj = (int)(i % L);
-    const double2 v = veff[s * L2 + i];
-    const int mux = (int)(__builtin_round(ci + v.x * (pref)) + L) % L;
-    const int muy = (int)(__builtin_round(cj + v.y * (pref)) + L) % L;
-    const double g = gamma[s * L2 + i], g2 = g * g;
-    coef[s * L2 + i] = make_double4((g2) * r, (g - g2) * r, (1 - g) * (1 - g) * r, 0.0);
-    mu[s * L2 + i] = make_int2(mux, muy);
+    move_prep_at(a.sp[s].pref, L, i, rho[s * L2 + i], gamma[s * L2 + i], veff[s * L2 + i], coef[s * L2 + i],
+                 mu[s * L2 + i]);
 }
 
 // v in [-L, 2L) -> v mod L
@@ -91,23 +99,16 @@ __device__ __forceinline__ int wrap1(int v, int L)
     return v >= L ? v - L : v;
 }
 
-__global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double4* __restrict__ coef,
-                                                       const int2* __restrict__ mu,
-                                                       const double* __restrict__ SDall,
-                                                       const double* __restrict__ SVall, double* __restrict__ out)
+// One output cell (x, y) of one species: the sum over the sources within the
+// window W of each source's terms that reach the cell, in a fixed order (rows
+// dr = -W .. W, then columns dc = -W .. W; within a source D x D, D x V, V x D,
+// V x V).  Sources farther than their reach add nothing, so any W at least the
+// largest reach gives the same sum.
+__device__ __forceinline__ double move_gather_at(const MoveSpecies& sp, const double4* __restrict__ cs,
+                                                 const int2* __restrict__ ms, const double* __restrict__ SD,
+                                                 const double* __restrict__ SV, int L, int x, int y)
 {
 #pragma clang fp contract(off)
-    const int s = blockIdx.y;
-    const int L = a.L;
-    const int64_t L2 = (int64_t)L * L;
-    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (o >= L2) return;
-    const MoveSpecies sp = a.sp[s];
-    const double* SD = SDall + sp.offD;
-    const double* SV = SVall + sp.offV;
-    const double4* cs = coef + s * L2;
-    const int2* ms = mu + s * L2;
-    const int x = (int)(o / L), y = (int)(o % L);
     double acc = 0.0;
     for (int dr = -sp.W; dr <= sp.W; ++dr) {
         const int ci = wrap1(x - dr, L);
@@ -132,8 +133,25 @@ __global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double4
             acc += t;
         }
     }
-    out[s * L2 + o] = acc;
+    return acc;
 }
+
+__global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double4* __restrict__ coef,
+                                                       const int2* __restrict__ mu,
+                                                       const double* __restrict__ SDall,
+                                                       const double* __restrict__ SVall, double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int s = blockIdx.y;
+    const int L = a.L;
+    const int64_t L2 = (int64_t)L * L;
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= L2) return;
+    const MoveSpecies sp = a.sp[s];
+    out[s * L2 + o] = move_gather_at(sp, coef + s * L2, mu + s * L2, SDall + sp.offD, SVall + sp.offV, L,
+                                     (int)(o / L), (int)(o % L));
+}
+
 
 // The largest |vx|, |vy| of each species (one workgroup per species); a NaN or
 // infinite velocity gives +inf, which the host refuses.
@@ -200,43 +218,63 @@ struct PerceptArgs {
     int64_t step;
 };
 
+// Velocity of species s at a cell in the three-species perception step: v (cos,
+// sin)(2 pi u), u = word 0 of Philox(seed, inst; cell, step, block (s << 24) | 0x800000).
+__device__ __forceinline__ double2 fke_velocity(uint32_t seed, uint32_t inst, uint32_t cell, int64_t step, int s,
+                                                double v)
+{
+#pragma clang fp contract(off)
+    tmk::Stream st(seed, inst, cell, step);
+    st.blk = ((uint32_t)s << 24) | 0x800000u;
+    double sn, cs;
+    tmk::dsincos2pi(st.u64(), sn, cs);
+    return make_double2(v * cs, v * sn);
+}
+
+// Perception gamma of consumer s (1 .. ns-1) at cell (p, q) from its previous
+// value g: the old densities rho [ns][N] summed over the first eff[s] perception
+// offsets (weighed down by the predators for the grazer of the three-species
+// form), scaled by the prey's inverse mean and clamped to [0, 1]; 1 when the
+// consumer is extinct, 0 when its prey is; unchanged when it perceives nothing.
+__device__ __forceinline__ double fke_gamma_at(int ns, int s, const double* mean, const double* inv, const double* fr,
+                                               const int* eff, const int2* __restrict__ disk,
+                                               const double* __restrict__ rho, int64_t N, int L, int p, int q,
+                                               double g)
+{
+#pragma clang fp contract(off)
+    if (fr[s] == 0.0) return g;
+    if (mean[s] < 1.0e-12) return 1.0;
+    if (mean[s - 1] < 1.0e-12) return 0.0;
+    const double* src = rho + (s - 1) * N;
+    const bool weigh = ns == 3 && s == 1 && !(mean[2] < 1.0e-12);
+    double acc = g;
+    for (int k = 0; k < eff[s]; ++k) {
+        const int2 o = disk[k];
+        const int64_t j = (int64_t)wrapi(p + o.x, L) * L + wrapi(q + o.y, L);
+        if (weigh) acc += src[j] * (1 - rho[2 * N + j] * inv[2]);
+        else acc += src[j];
+    }
+    if (ns == 2 || s == 1) acc = (acc * inv[s - 1]) / (double)eff[s];
+    else acc = (acc / (double)eff[s]) * inv[s - 1];
+    if (acc > 1) acc = 1.0;
+    else if (acc < 0) acc = 0.0;
+    return acc;
+}
+
 __global__ __launch_bounds__(256) void fke_perception_kernel(PerceptArgs a, const int2* __restrict__ disk,
                                                              const double* __restrict__ rho,
                                                              double* __restrict__ gamma, double2* __restrict__ veff)
 {
-#pragma clang fp contract(off)
     const int L = a.L;
     const int64_t N = (int64_t)L * L;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
     const int p = (int)(i / L), q = (int)(i % L);
     if (a.ns == 3)
-        for (int s = 0; s < 3; ++s) {
-            tmk::Stream st(a.seed, a.inst, (uint32_t)i, a.step);
-            st.blk = ((uint32_t)s << 24) | 0x800000u;
-            double sn, cs;
-            tmk::dsincos2pi(st.u64(), sn, cs);
-            veff[s * N + i] = make_double2(a.v[s] * cs, a.v[s] * sn);
-        }
+        for (int s = 0; s < 3; ++s) veff[s * N + i] = fke_velocity(a.seed, a.inst, (uint32_t)i, a.step, s, a.v[s]);
     for (int s = 1; s < a.ns; ++s) {
         double* g = gamma + (s - 1) * N + i;
-        if (a.fr[s] == 0.0) continue;
-        if (a.mean[s] < 1.0e-12) { *g = 1.0; continue; }
-        if (a.mean[s - 1] < 1.0e-12) { *g = 0.0; continue; }
-        const double* src = rho + (s - 1) * N;
-        const bool weigh = a.ns == 3 && s == 1 && !(a.mean[2] < 1.0e-12);
-        double acc = *g;
-        for (int k = 0; k < a.eff[s]; ++k) {
-            const int2 o = disk[k];
-            const int64_t j = (int64_t)wrapi(p + o.x, L) * L + wrapi(q + o.y, L);
-            if (weigh) acc += src[j] * (1 - rho[2 * N + j] * a.inv[2]);
-            else acc += src[j];
-        }
-        if (a.ns == 2 || s == 1) acc = (acc * a.inv[s - 1]) / (double)a.eff[s];
-        else acc = (acc / (double)a.eff[s]) * a.inv[s - 1];
-        if (acc > 1) acc = 1.0;
-        else if (acc < 0) acc = 0.0;
-        *g = acc;
+        *g = fke_gamma_at(a.ns, s, a.mean, a.inv, a.fr, a.eff, disk, rho, N, L, p, q, *g);
     }
 }
 
@@ -326,20 +364,17 @@ __device__ __forceinline__ void fke_rhs(const FkeRk& c, const double* y, int64_t
 // rho + dt6 (acc + K4) over rho and counts the cells that turned negative or
 // non-finite (the reference logs them, FKE:2511-2529, and carries on).
 template <int NS>
-__global__ __launch_bounds__(256) void fke_rk_stage_kernel(FkeRk c, int stage, double* rho, const double* src,
-                                                           double* nxt, double* acc, unsigned long long* n_bad)
+__device__ __forceinline__ uint32_t fke_stage_at(const FkeRk& c, int stage, double* rho, const double* src,
+                                                 double* nxt, double* acc, int64_t L2, int i)
 {
 #pragma clang fp contract(off)
     constexpr int NF = NS + 3;
     const int L = c.L;
-    const int64_t L2 = (int64_t)L * L;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L2) return;
-    const int y = (int)(i / L), x = (int)(i % L);
+    const int y = i / L, x = i % L;
     const int up = (y == 0 ? L - 1 : y - 1) * L + x, dn = (y == L - 1 ? 0 : y + 1) * L + x;
     const int lf = y * L + (x == 0 ? L - 1 : x - 1), rt = y * L + (x == L - 1 ? 0 : x + 1);
     double f[NF];
-    fke_rhs<NS>(c, src, L2, (int)i, up, dn, lf, rt, f);
+    fke_rhs<NS>(c, src, L2, i, up, dn, lf, rt, f);
     uint32_t bad = 0;
 #pragma unroll
     for (int s = 0; s < NF; ++s) {
@@ -356,6 +391,17 @@ __global__ __launch_bounds__(256) void fke_rk_stage_kernel(FkeRk c, int stage, d
             bad += (r < 0 || !__builtin_isfinite(r)) ? 1u : 0u;
         }
     }
+    return bad;
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void fke_rk_stage_kernel(FkeRk c, int stage, double* rho, const double* src,
+                                                           double* nxt, double* acc, unsigned long long* n_bad)
+{
+    const int64_t L2 = (int64_t)c.L * c.L;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L2) return;
+    const uint32_t bad = fke_stage_at<NS>(c, stage, rho, src, nxt, acc, L2, (int)i);
     if (stage == 4 && bad) atomicAdd(n_bad, (unsigned long long)bad);
 }
 
@@ -442,6 +488,596 @@ bool veg_dornic_ok(int L, const tm_rietkerk_params* p)
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// The Rietkerk consumer model as one step of a batch (TM_MODEL_FKE2: vegetation
+// P and grazer G; TM_MODEL_FKE3: and predator Pr), FP64, the loop body of
+// rietkerk_bjork_FKE.cpp:4100-4462.  Fields per instance, species-major
+// [NF][L*L] at NF cell_off: P, G, (Pr,) W, O, then gamma_1 (, gamma_2);
+// NF = 2 SP + 1 for SP = 2 or 3 biota species.  One step, every launch over the
+// whole batch:
+//   fke_chunk / fke_state  spatial means of the old densities and the cover
+//                          factor (fixed-order sums, FKE:4100-4112), per instance
+//   fke_percept            gamma of each consumer from the old densities
+//                          (FKE:4114-4118); velocities are drawn where used
+//   fke_veg                the vegetation's Dornic step P* (FKE:4150-4211), the
+//                          other densities carried into the output buffer
+//   fke_stage x 4          RK4 of the trophic and water terms (FKE:4233-4238),
+//                          one grid-wide pass per stage (W and O diffuse)
+//   fke_mprep / fke_mgather  the consumers' Fokker-Planck movement (FKE:4276-4309)
+//   fke_final              checks and flush of the densities, rho(t) partials of
+//                          every field (FKE:4352-4462)
+// Every operation is the one the per-operator entry points (tm_fke_perception,
+// tm_fke_veg_dornic, tm_fke_rk4, tm_fke_movement) perform, in their order.
+// ---------------------------------------------------------------------------
+namespace tmk {
+namespace {
+
+struct FkeModelConst {     // per parameter point
+    FkeRk rk;              // L is set per instance
+    MoveSpecies ms[2];     // consumers 1 and 2
+    double v[3];
+    double eps;
+};
+
+struct FkeWork {
+    const FkeModelConst* mc;   // [n_params]
+    const DPCInstInfo* info;   // [n_inst]: chunks, perception table, fr
+    DPCStepState* st;          // [n_inst]: means of the step
+    double* chunk;             // [nchunks][8]
+    const int2* chunks;        // [nchunks]: (instance, chunk within it)
+    const int2* disk;          // perception offsets of every table
+    const int2* blocks;        // [nblocks]: (instance, first cell) of 256 cells
+    const double* SD;          // movement stencils, offsets in MoveSpecies
+    const double* SV;
+    double* sa;                // RK4 stage values, (SP + 2) fields at (SP + 2) cell_off
+    double* sb;
+    double* acc;
+    double4* coef;             // movement terms, (SP - 1) fields at (SP - 1) cell_off
+    int2* mu;
+    double* mstat;             // [n_inst][8] means of the movement statistics
+};
+
+constexpr int kFkeChunkCells = kDpcChunkCells;
+
+// Sums over the chunks of NV per-cell values in the consumer models' fixed order
+// (lanes of 256 strided over the chunk, then a pairwise fold).
+template <int NV>
+__device__ __forceinline__ void chunk_fold(double (&v)[NV], double (&red)[NV][256], double* dst)
+{
+#pragma clang fp contract(off)
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int f = 0; f < NV; ++f) red[f][t] = v[f];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+#pragma unroll
+            for (int f = 0; f < NV; ++f) red[f][t] += red[f][t + o];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int f = 0; f < NV; ++f) dst[f] = red[f][0];
+    }
+}
+
+template <int SP>
+__global__ __launch_bounds__(256) void fke_chunk_kernel(StepArgs a)
+{
+#pragma clang fp contract(off)
+    constexpr int NF = 2 * SP + 1;
+    __shared__ double red[SP + 1][256];
+    const FkeWork* w = (const FkeWork*)a.work;
+    const int2 ch = w->chunks[blockIdx.x];
+    const DevInst di = a.inst[ch.x];
+    const int64_t N = (int64_t)di.L * di.L;
+    const double* f0 = (const double*)a.in + (int64_t)NF * di.cell_off;
+    const int64_t base = (int64_t)ch.y * kFkeChunkCells;
+    double s[SP + 1];
+#pragma unroll
+    for (int f = 0; f <= SP; ++f) s[f] = 0.0;
+    for (int64_t i = base + threadIdx.x; i < base + kFkeChunkCells && i < N; i += 256) {
+#pragma unroll
+        for (int f = 0; f < SP; ++f) s[f] += f0[f * N + i];
+        s[SP] += f0[i] > 0 ? 1.0 : 0.0;
+    }
+    chunk_fold<SP + 1>(s, red, w->chunk + (int64_t)blockIdx.x * 8);
+}
+
+// The per-instance numbers of the perception step, as tm_fke_perception_host
+// forms them (FKE:4100-4112, :2534-2606, :3371-3482).
+template <int SP>
+__global__ __launch_bounds__(64) void fke_state_kernel(StepArgs a)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= a.n_inst) return;
+    const FkeWork* w = (const FkeWork*)a.work;
+    const DPCInstInfo in = w->info[i];
+    const DevInst di = a.inst[i];
+    const double N = (double)((int64_t)di.L * di.L);
+    double tot[SP], occ = 0.0;
+#pragma unroll
+    for (int f = 0; f < SP; ++f) tot[f] = 0.0;
+    for (int c = 0; c < in.nchunks; ++c) {
+        const double* p = w->chunk + (int64_t)(in.chunk0 + c) * 8;
+#pragma unroll
+        for (int f = 0; f < SP; ++f) tot[f] += p[f];
+        occ += p[SP];
+    }
+    DPCStepState st{};
+    double nfac = 1 - occ / N;                       // FKE:4109-4112
+    if (nfac < 0.35) nfac = 0.35;
+    int new_len = (int)(nfac * nfac * (double)in.disk_len);
+    if (SP == 3 && new_len < 1) new_len = 1;         // FKE:3386
+    for (int f = 0; f < SP; ++f) {
+        st.mean[f] = tot[f] / N;
+        st.inv[f] = st.mean[f] >= 1.0e-12 ? 1 / st.mean[f] : 1 / 1.0e-12;
+        int e = (int)(in.fr[f] * in.fr[f] * (double)new_len);
+        if (SP == 3 && e < 1 && in.fr[f] > 0) e = 1;     // FKE:3415-3416
+        st.eff[f] = e;
+    }
+    w->st[i] = st;
+}
+
+// The cell of the thread in the block table; false past its instance's end.
+__device__ __forceinline__ bool fke_cell(const StepArgs& a, const FkeWork* w, int& inst, int& cell, DevInst& di)
+{
+    const int2 b = w->blocks[blockIdx.x];
+    inst = b.x;
+    cell = b.y + (int)threadIdx.x;
+    di = a.inst[inst];
+    return (int64_t)cell < (int64_t)di.L * di.L;
+}
+
+template <int SP>
+__global__ __launch_bounds__(256) void fke_percept_kernel(StepArgs a)
+{
+    constexpr int NF = 2 * SP + 1;
+    const FkeWork* w = (const FkeWork*)a.work;
+    int inst, cell;
+    DevInst di;
+    if (!fke_cell(a, w, inst, cell, di)) return;
+    const int L = di.L;
+    const int64_t N = (int64_t)L * L;
+    const double* fin = (const double*)a.in + (int64_t)NF * di.cell_off;
+    double* fout = (double*)a.out + (int64_t)NF * di.cell_off;
+    const DPCStepState st = w->st[inst];
+    const DPCInstInfo info = w->info[inst];
+    const int p = cell / L, q = cell % L;
+#pragma unroll
+    for (int s = 1; s < SP; ++s) {
+        const int64_t j = (int64_t)(SP + 1 + s) * N + cell;
+        fout[j] = fke_gamma_at(SP, s, st.mean, st.inv, info.fr, st.eff, w->disk + info.disk_off, fin, N, L, p, q,
+                               fin[j]);
+    }
+}
+
+template <int SP>
+__global__ __launch_bounds__(256) void fke_veg_kernel(StepArgs a)
+{
+    constexpr int NF = 2 * SP + 1;
+    const FkeWork* w = (const FkeWork*)a.work;
+    int inst, cell;
+    DevInst di;
+    if (!fke_cell(a, w, inst, cell, di)) return;
+    const int L = di.L;
+    const int64_t N = (int64_t)L * L;
+    const double* fin = (const double*)a.in + (int64_t)NF * di.cell_off;
+    double* fout = (double*)a.out + (int64_t)NF * di.cell_off;
+    const RKConstD c = ((const RKConstD*)a.consts)[(int64_t)a.k * a.n_inst + inst];
+    const int y = cell / L, x = cell % L;
+    const int up = (y == 0 ? L - 1 : y - 1) * L + x, dn = (y == L - 1 ? 0 : y + 1) * L + x;
+    const int lf = y * L + (x == 0 ? L - 1 : x - 1), rt = y * L + (x == L - 1 ? 0 : x + 1);
+    int64_t n;
+    bool bad;
+    fout[cell] = cell_update<double, 1>(c, di, (uint32_t)cell, fin[cell], fin[up], fin[dn], fin[lf], fin[rt], n, bad);
+#pragma unroll
+    for (int f = 1; f < SP + 2; ++f) fout[f * N + cell] = fin[f * N + cell];
+}
+
+template <int SP>
+__global__ __launch_bounds__(256) void fke_stage_kernel(StepArgs a, int stage)
+{
+    constexpr int NF = 2 * SP + 1, ND = SP + 2;
+    const FkeWork* w = (const FkeWork*)a.work;
+    int inst, cell;
+    DevInst di;
+    if (!fke_cell(a, w, inst, cell, di)) return;
+    FkeRk c = w->mc[di.pidx].rk;
+    c.L = di.L;
+    const int64_t N = (int64_t)di.L * di.L;
+    double* rho = (double*)a.out + (int64_t)NF * di.cell_off;
+    double* sa = w->sa + (int64_t)ND * di.cell_off;
+    double* sb = w->sb + (int64_t)ND * di.cell_off;
+    double* acc = w->acc + (int64_t)ND * di.cell_off;
+    const double* src = stage == 1 ? rho : (stage == 3 ? sb : sa);
+    double* nxt = stage == 2 ? sb : sa;   // unused by stage 4
+    (void)fke_stage_at<SP - 1>(c, stage, rho, src, nxt, acc, N, cell);
+}
+
+template <int SP>
+__global__ __launch_bounds__(256) void fke_mprep_kernel(StepArgs a)
+{
+    constexpr int NF = 2 * SP + 1;
+    const FkeWork* w = (const FkeWork*)a.work;
+    int inst, cell;
+    DevInst di;
+    if (!fke_cell(a, w, inst, cell, di)) return;
+    const int s = blockIdx.y + 1;
+    const int L = di.L;
+    const int64_t N = (int64_t)L * L;
+    const double* f = (const double*)a.out + (int64_t)NF * di.cell_off;
+    const FkeModelConst& mc = w->mc[di.pidx];
+    double2 v = make_double2(0.0, 0.0);   // the two-species step draws no velocity
+    if (SP == 3) {
+        const int64_t step = ((const RKConstD*)a.consts)[(int64_t)a.k * a.n_inst + inst].step;
+        v = fke_velocity(di.seed, di.inst_id, (uint32_t)cell, step, s, mc.v[s]);
+    }
+    const int64_t j = (int64_t)(SP - 1) * di.cell_off + (s - 1) * N + cell;
+    move_prep_at(mc.ms[s - 1].pref, L, cell, f[s * N + cell], f[(SP + 1 + s) * N + cell], v, w->coef[j], w->mu[j]);
+}
+
+template <int SP>
+__global__ __launch_bounds__(256) void fke_mgather_kernel(StepArgs a)
+{
+    constexpr int NF = 2 * SP + 1;
+    const FkeWork* w = (const FkeWork*)a.work;
+    int inst, cell;
+    DevInst di;
+    if (!fke_cell(a, w, inst, cell, di)) return;
+    const int s = blockIdx.y + 1;
+    const int L = di.L;
+    const int64_t N = (int64_t)L * L;
+    const MoveSpecies sp = w->mc[di.pidx].ms[s - 1];
+    const int64_t base = (int64_t)(SP - 1) * di.cell_off + (s - 1) * N;
+    double* f = (double*)a.out + (int64_t)NF * di.cell_off;
+    f[s * N + cell] = move_gather_at(sp, w->coef + base, w->mu + base, w->SD + sp.offD, w->SV + sp.offV, L,
+                                     cell / L, cell % L);
+}
+
+// Checks and flush of the densities (the perception fields are left as they
+// are), then the (sum, occupied count) partials of every field of the tile.
+template <int SP, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void fke_final_kernel(StepArgs a)
+{
+#pragma clang fp contract(off)
+    constexpr int NF = 2 * SP + 1, ND = SP + 2;
+    __shared__ double s_red[2 * NF * (NT / 64)];
+    const FkeWork* w = (const FkeWork*)a.work;
+    const int tile = blockIdx.x;
+    if (tile >= a.ntiles) return;
+    const int64_t counter = a.state->counter + a.k;
+    const int64_t slot = counter & (int64_t)(a.nslots - 1);
+    const DevTile tl = a.tiles[tile];
+    const DevInst di = a.inst[tl.inst];
+    const double eps = w->mc[di.pidx].eps;
+    const int L = tl.L;
+    const int64_t N = (int64_t)L * L;
+    double* f = (double*)a.out + (int64_t)NF * tl.cell_off;
+    double sum[NF], cnt[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) { sum[k] = 0.0; cnt[k] = 0.0; }
+    for (int e = threadIdx.x; e < TX * TY; e += NT) {
+        const int gy = tl.y0 + e / TX, gx = tl.x0 + e % TX;
+        if (gy >= L || gx >= L) continue;
+        const int64_t cell = (int64_t)gy * L + gx;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            double v = f[k * N + cell];
+            if (k < ND) {
+                if (v < 0 || !__builtin_isfinite(v)) record_fault(a.state, tl.inst, k * N + cell, counter);
+                if (v < eps && v >= 0.0) {   // FKE:4459-4461
+                    v = 0.0;
+                    f[k * N + cell] = v;
+                }
+            }
+            sum[k] += v;
+            cnt[k] += v > 0.0 ? 1.0 : 0.0;
+        }
+    }
+    block_partials<NT, NF>(sum, cnt, a.partials + (slot * a.ntiles + tile) * NF, s_red);
+}
+
+template <int SP, int TX, int TY, int NT>
+void launch_fke(const StepArgs& a, int grid, hipStream_t s)
+{
+    const dim3 blocks((unsigned)a.nblocks), moves((unsigned)a.nblocks, SP - 1);
+    hipLaunchKernelGGL(fke_chunk_kernel<SP>, dim3(a.nchunks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(fke_state_kernel<SP>, dim3((a.n_inst + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(fke_percept_kernel<SP>, blocks, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(fke_veg_kernel<SP>, blocks, dim3(256), 0, s, a);
+    for (int st = 1; st <= 4; ++st) hipLaunchKernelGGL(fke_stage_kernel<SP>, blocks, dim3(256), 0, s, a, st);
+    hipLaunchKernelGGL(fke_mprep_kernel<SP>, moves, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(fke_mgather_kernel<SP>, moves, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((fke_final_kernel<SP, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+// ---- movement statistics (Rho_Mov, FKE:4034-4060) ---------------------------
+// pass 0: chunk sums of gamma, vx, vy of each consumer; pass 1: of (gamma - mean)^2.
+template <int SP>
+__global__ __launch_bounds__(256) void fke_mstat_chunk_kernel(StepArgs a, int pass)
+{
+#pragma clang fp contract(off)
+    constexpr int NF = 2 * SP + 1, NV = 3 * (SP - 1);
+    __shared__ double red[NV][256];
+    const FkeWork* w = (const FkeWork*)a.work;
+    const int2 ch = w->chunks[blockIdx.x];
+    const DevInst di = a.inst[ch.x];
+    const FkeModelConst& mc = w->mc[di.pidx];
+    const int64_t N = (int64_t)di.L * di.L;
+    const double* f0 = (const double*)a.in + (int64_t)NF * di.cell_off;
+    const int64_t base = (int64_t)ch.y * kFkeChunkCells;
+    // the velocities of the last step taken (none before the first)
+    const int64_t last = di.step0 + a.state->counter - 1;
+    const bool drawn = SP == 3 && last >= 0;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    for (int64_t i = base + threadIdx.x; i < base + kFkeChunkCells && i < N; i += 256) {
+#pragma unroll
+        for (int s = 1; s < SP; ++s) {
+            const double g = f0[(SP + 1 + s) * N + i];
+            if (pass == 0) {
+                v[3 * (s - 1)] += g;
+                if (drawn) {
+                    const double2 u = fke_velocity(di.seed, di.inst_id, (uint32_t)i, last, s, mc.v[s]);
+                    v[3 * (s - 1) + 1] += u.x;
+                    v[3 * (s - 1) + 2] += u.y;
+                }
+            } else {
+                const double m = w->mstat[(int64_t)ch.x * 8 + 3 * (s - 1)];
+                v[3 * (s - 1)] += (g - m) * (g - m);
+            }
+        }
+    }
+    chunk_fold<NV>(v, red, w->chunk + (int64_t)blockIdx.x * 8);
+}
+
+template <int SP>
+__global__ __launch_bounds__(64) void fke_mstat_fold_kernel(StepArgs a, int pass, double* out)
+{
+#pragma clang fp contract(off)
+    constexpr int NV = 3 * (SP - 1);
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= a.n_inst) return;
+    const FkeWork* w = (const FkeWork*)a.work;
+    const DPCInstInfo in = w->info[i];
+    const DevInst di = a.inst[i];
+    const double N = (double)((int64_t)di.L * di.L);
+    double tot[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = 0.0;
+    for (int c = 0; c < in.nchunks; ++c) {
+        const double* p = w->chunk + (int64_t)(in.chunk0 + c) * 8;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) tot[k] += p[k];
+    }
+    double* o = out + (int64_t)i * 4 * SP;
+    if (pass == 0) {
+        for (int k = 0; k < 4; ++k) o[k] = 0.0;
+        for (int s = 1; s < SP; ++s) {
+            const double m = tot[3 * (s - 1)] / N;
+            w->mstat[(int64_t)i * 8 + 3 * (s - 1)] = m;
+            o[4 * s] = m;
+            o[4 * s + 2] = tot[3 * (s - 1) + 1] / N;
+            o[4 * s + 3] = tot[3 * (s - 1) + 2] / N;
+        }
+    } else {
+        for (int s = 1; s < SP; ++s) o[4 * s + 1] = tot[3 * (s - 1)] / N;
+    }
+}
+
+template <int SP>
+void launch_mstat(const StepArgs& a, double* out, hipStream_t s)
+{
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(fke_mstat_chunk_kernel<SP>, dim3(a.nchunks), dim3(256), 0, s, a, pass);
+        hipLaunchKernelGGL(fke_mstat_fold_kernel<SP>, dim3((a.n_inst + 63) / 64), dim3(64), 0, s, a, pass, out);
+    }
+}
+
+// 64 x 4 tiles (one cell per thread) for the final kernel's rho(t) partials.
+const Variant kFKE2[] = {
+    {64, 4, 256, 0, 1, "fke2_f64_t256", &launch_fke<2, 64, 4, 256>, 0, 0, nullptr, 1},
+};
+const Variant kFKE3[] = {
+    {64, 4, 256, 0, 1, "fke3_f64_t256", &launch_fke<3, 64, 4, 256>, 0, 0, nullptr, 1},
+};
+
+// The reference's movement constants of consumer s of a point (FKE:195-250,
+// :3764-3772), its gather window W and its stencils appended to SD / SV.
+int fke_move_of(const tm_fke_params& p, int s, bool drawn, std::vector<double>& SD, std::vector<double>& SV,
+                MoveSpecies* out)
+{
+    tm_fke_move_species ms{};
+    double sd[256], sv[256];
+    if (tm_fke_move_constants(p.D_mov[s], p.v[s], p.sigma[s], p.dt_analytic, p.dt_adv[s], p.mv_invariance, p.dx,
+                              p.CI, &ms, sd, 256, sv, 256) != 0)
+        return TM_ERR_ARG;
+    out->rD = ms.rD;
+    out->rV = ms.rV;
+    out->pref = ms.mu_prefactor;
+    // the largest advection shift: |v cos|, |v sin| <= |v| where velocities are
+    // drawn, none otherwise (as in move_args)
+    const double shift = (drawn ? std::fabs(p.v[s]) : 0.0) * std::fabs(ms.mu_prefactor);
+    if (!(shift < 1e9)) return TM_ERR_ARG;
+    const int S = (int)std::floor(shift + 0.5) + 1;
+    out->W = std::max(ms.rD, ms.rV + S);
+    out->offD = (int)SD.size();
+    out->offV = (int)SV.size();
+    SD.insert(SD.end(), sd, sd + ms.rD + 1);
+    SV.insert(SV.end(), sv, sv + ms.rV + 1);
+    return TM_OK;
+}
+
+}  // namespace
+
+const Variant* fke2_variants_f64(int* n) { *n = (int)(sizeof(kFKE2) / sizeof(kFKE2[0])); return kFKE2; }
+const Variant* fke3_variants_f64(int* n) { *n = (int)(sizeof(kFKE3) / sizeof(kFKE3[0])); return kFKE3; }
+
+void launch_fke_mov_stats(int model, const StepArgs& a, double* out, hipStream_t s)
+{
+    if (model == TM_MODEL_FKE3) launch_mstat<3>(a, out, s);
+    else launch_mstat<2>(a, out, s);
+}
+
+int fke_setup(int model, const tm_fke_params* P, int n_params, const tm_instance_desc* inst, int n_inst,
+              const int64_t* cell_off, int64_t total_cells, void** work, int* nchunks, int* nblocks,
+              const char** err)
+{
+    const int SP = model == TM_MODEL_FKE3 ? 3 : 2, ns = SP - 1;
+    *work = nullptr;
+    std::vector<FkeModelConst> mc((size_t)n_params);
+    std::vector<double> SD, SV;
+    std::vector<int2> disk;
+    std::vector<int> p_off((size_t)n_params), p_len((size_t)n_params);
+    for (int g = 0; g < n_params; ++g) {
+        const tm_fke_params& p = P[g];
+        FkeModelConst& m = mc[(size_t)g];
+        m = FkeModelConst{};
+        tm_fke_rk_params rk{};
+        rk.a = p.a;
+        rk.c = p.c;
+        rk.gmax = p.gmax;
+        rk.alpha = p.alpha;
+        rk.rW = p.rW;
+        rk.W0 = p.W0;
+        for (int k = 0; k < 3; ++k) {
+            rk.K[k] = p.K[k];
+            rk.E[k] = p.E[k];
+            rk.M[k] = p.M[k];
+            for (int l = 0; l < 3; ++l) {
+                rk.A[k][l] = p.A[k][l];
+                rk.H[k][l] = p.H[k][l];
+            }
+        }
+        rk.D_W = p.D_W;
+        rk.D_O = p.D_O;
+        rk.dt = p.dt;
+        rk.dt_rk = p.dt_analytic;
+        rk.dx = p.dx;
+        if (!fke_rk_args(ns, 1, &rk, &m.rk)) {
+            *err = "tm_fke_params: dx and dt_analytic must be > 0 and dt finite";
+            return TM_ERR_ARG;
+        }
+        for (int s = 1; s < SP; ++s)
+            if (fke_move_of(p, s, SP == 3, SD, SV, &m.ms[s - 1]) != TM_OK) {
+                *err = "tm_fke_params: bad movement constants (need D_mov, sigma, dt_analytic, dx > 0; CI >= 1)";
+                return TM_ERR_ARG;
+            }
+        for (int s = 0; s < 3; ++s) m.v[s] = p.v[s];
+        m.eps = p.flush_eps;
+        // the perception table: radius (int)(max pR / dx) (FKE:3858-3864)
+        double rmax = p.pR[0];
+        for (int s = 1; s < SP; ++s) rmax = p.pR[s] > rmax ? p.pR[s] : rmax;
+        rmax = rmax / p.dx;
+        if (!(rmax > 0.0) || !(rmax < TM_DPC_MAX_RADIUS + 1.0)) {
+            *err = "tm_fke_params: perception radius max pR / dx must be in (0, TM_DPC_MAX_RADIUS]";
+            return TM_ERR_ARG;
+        }
+        const int R = (int)rmax;
+        std::vector<int32_t> off(2 * (size_t)(2 * R + 1) * (2 * R + 1));
+        const int len = tm_perception_offsets(R, off.data(), (int)off.size() / 2);
+        p_off[(size_t)g] = (int)disk.size();
+        p_len[(size_t)g] = len;
+        for (int k = 0; k < len; ++k) disk.push_back(make_int2(off[2 * k], off[2 * k + 1]));
+        if (!(p.sigma[0] > 0.0) || !(p.dt > 0.0) || (p.sampler != TM_SAMPLER_EXACT && p.sampler != TM_SAMPLER_COMPAT)) {
+            *err = "tm_fke_params: sigma[0] and dt must be > 0 and sampler a tm_sampler";
+            return TM_ERR_ARG;
+        }
+    }
+    std::vector<DPCInstInfo> info((size_t)n_inst);
+    std::vector<int2> chunks, blocks;
+    for (int i = 0; i < n_inst; ++i) {
+        const tm_instance_desc& d = inst[i];
+        const tm_fke_params& p = P[d.params_index];
+        const FkeModelConst& m = mc[(size_t)d.params_index];
+        for (int s = 0; s < ns; ++s)
+            if (2 * m.ms[s].W + 1 > d.L) {
+                *err = "tm_fke_params: a consumer's movement reaches past its lattice (2 W + 1 > L)";
+                return TM_ERR_ARG;
+            }
+        double rmax = p.pR[0];
+        for (int s = 1; s < SP; ++s) rmax = p.pR[s] > rmax ? p.pR[s] : rmax;
+        rmax = rmax / p.dx;
+        DPCInstInfo& in = info[(size_t)i];
+        in = DPCInstInfo{};
+        in.R = (int)rmax;
+        in.disk_off = p_off[(size_t)d.params_index];
+        in.disk_len = p_len[(size_t)d.params_index];
+        for (int s = 0; s < SP; ++s) in.fr[s] = p.pR[s] / (p.dx * rmax);
+        const int64_t cells = (int64_t)d.L * d.L;
+        in.chunk0 = (int)chunks.size();
+        in.nchunks = (int)((cells + kFkeChunkCells - 1) / kFkeChunkCells);
+        for (int k = 0; k < in.nchunks; ++k) chunks.push_back(make_int2(i, k));
+        for (int64_t c0 = 0; c0 < cells; c0 += 256) blocks.push_back(make_int2(i, (int)c0));
+    }
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t nd = (size_t)(SP + 2) * total_cells, nm = (size_t)ns * total_cells;
+    size_t o = up(sizeof(FkeWork));
+    const size_t o_mc = o;     o += up(sizeof(FkeModelConst) * mc.size());
+    const size_t o_info = o;   o += up(sizeof(DPCInstInfo) * info.size());
+    const size_t o_st = o;     o += up(sizeof(DPCStepState) * info.size());
+    const size_t o_chunk = o;  o += up(sizeof(double) * 8 * chunks.size());
+    const size_t o_chunks = o; o += up(sizeof(int2) * chunks.size());
+    const size_t o_disk = o;   o += up(sizeof(int2) * disk.size());
+    const size_t o_blocks = o; o += up(sizeof(int2) * blocks.size());
+    const size_t o_sd = o;     o += up(sizeof(double) * SD.size());
+    const size_t o_sv = o;     o += up(sizeof(double) * SV.size());
+    const size_t o_mstat = o;  o += up(sizeof(double) * 8 * info.size());
+    const size_t o_tables = o;   // everything above is uploaded; the scratch below is not
+    const size_t o_sa = o;     o += up(sizeof(double) * nd);
+    const size_t o_sb = o;     o += up(sizeof(double) * nd);
+    const size_t o_acc = o;    o += up(sizeof(double) * nd);
+    const size_t o_coef = o;   o += up(sizeof(double4) * nm);
+    const size_t o_mu = o;     o += up(sizeof(int2) * nm);
+    void* dev = nullptr;
+    if (hipMalloc(&dev, o) != hipSuccess) {
+        *err = "Rietkerk consumer tables and scratch: allocation failed";
+        return TM_ERR_NOMEM;
+    }
+    char* base = (char*)dev;
+    FkeWork w{};
+    w.mc = (const FkeModelConst*)(base + o_mc);
+    w.info = (const DPCInstInfo*)(base + o_info);
+    w.st = (DPCStepState*)(base + o_st);
+    w.chunk = (double*)(base + o_chunk);
+    w.chunks = (const int2*)(base + o_chunks);
+    w.disk = (const int2*)(base + o_disk);
+    w.blocks = (const int2*)(base + o_blocks);
+    w.SD = (const double*)(base + o_sd);
+    w.SV = (const double*)(base + o_sv);
+    w.mstat = (double*)(base + o_mstat);
+    w.sa = (double*)(base + o_sa);
+    w.sb = (double*)(base + o_sb);
+    w.acc = (double*)(base + o_acc);
+    w.coef = (double4*)(base + o_coef);
+    w.mu = (int2*)(base + o_mu);
+    std::vector<char> host(o_tables, 0);
+    auto put = [&](size_t off, const void* src, size_t n) { if (n) std::memcpy(host.data() + off, src, n); };
+    put(0, &w, sizeof(w));
+    put(o_mc, mc.data(), sizeof(FkeModelConst) * mc.size());
+    put(o_info, info.data(), sizeof(DPCInstInfo) * info.size());
+    put(o_chunks, chunks.data(), sizeof(int2) * chunks.size());
+    put(o_disk, disk.data(), sizeof(int2) * disk.size());
+    put(o_blocks, blocks.data(), sizeof(int2) * blocks.size());
+    put(o_sd, SD.data(), sizeof(double) * SD.size());
+    put(o_sv, SV.data(), sizeof(double) * SV.size());
+    if (hipMemcpy(dev, host.data(), o_tables, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dev);
+        *err = "Rietkerk consumer tables: upload failed";
+        return TM_ERR_HIP;
+    }
+    *work = dev;
+    *nchunks = (int)chunks.size();
+    *nblocks = (int)blocks.size();
+    return TM_OK;
+}
+
+}  // namespace tmk
 
 extern "C" {
 

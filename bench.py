@@ -108,6 +108,22 @@ def c6_batch(tm, model):
                 nf=tm.NFIELDS[model], chunk=500)
 
 
+def c7_batch(tm, model):
+    """Rietkerk vegetation + consumers as one step (FP64): 4 rainfall values x 4
+    replicates of 256^2 with the constants of the reference's 3-species driver
+    (order_3stoc_test_rietkerk.cpp: km, hr, kg; dt = 0.1 hr, Dt_Analytic = 0.118 hr),
+    P ~ Uniform(0, 60), grazers Uniform(0, 3), predators Uniform(0, 0.5),
+    W, O ~ Uniform(0, 2), perception fields 0."""
+    ps = [tm.FKEParams(a=float(a) / 24.0) for a in np.linspace(0.8, 1.6, 4)]
+    inst = [tm.Instance(256, k // 4, SEED, k) for k in range(16)]
+    if model == tm.MODEL_FKE2:
+        lo, hi = [0.0] * 5, [60.0, 3.0, 2.0, 2.0, 0.0]
+    else:
+        lo, hi = [0.0] * 7, [60.0, 3.0, 0.5, 2.0, 2.0, 0.0, 0.0]
+    return dict(model=model, precision=tm.F64, params=ps, inst=inst, lo=lo, hi=hi, steps=200,
+                nf=tm.NFIELDS[model], chunk=100)
+
+
 def movement_leg(tm, L=1024, reps=20):
     """The Rietkerk consumers' movement step (two species, 1024^2, the grazer and
     predator values of the reference's 3-species driver: D = 0.0298 and 0.05221
@@ -422,7 +438,7 @@ def main():
     ap.add_argument("--no-side", action="store_true", help="headline and the full C4 runs only")
     ap.add_argument("--variant", default=None,
                     help="kernel variant for the headline (or --only) run instead of the autotuner's choice")
-    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4", "c4-f64", "c5", "sweep", "dp2", "dp3",
+    ap.add_argument("--only", choices=["c2", "c3", "c3-f64", "c4", "c4-f64", "c5", "sweep", "dp2", "dp3", "fke2", "fke3",
                                        "movement"],
                     default=None,
                     help="only this workload, --steps steps (default: its full length), one JSON line")
@@ -460,7 +476,8 @@ def main():
         b = {"c2": lambda: c2_batch(tm, tm.F64), "c3": lambda: c3_batch(tm, tm.F32),
              "c4": lambda: c4_batch(tm, tm.F32), "c4-f64": lambda: c4_batch(tm, tm.F64),
              "c3-f64": lambda: c3_batch(tm, tm.F64), "c5": lambda: c5_batch(tm, tm.F32),
-             "dp2": lambda: c6_batch(tm, tm.MODEL_DP2), "dp3": lambda: c6_batch(tm, tm.MODEL_DP3)}[args.only]()
+             "dp2": lambda: c6_batch(tm, tm.MODEL_DP2), "dp3": lambda: c6_batch(tm, tm.MODEL_DP3),
+             "fke2": lambda: c7_batch(tm, tm.MODEL_FKE2), "fke3": lambda: c7_batch(tm, tm.MODEL_FKE3)}[args.only]()
         n = args.steps if "--steps" in sys.argv else b["steps"]
         log(f"[bench] {args.only}: {cells_of(b)} cells/step x {n} steps")
         r = run_workload(tm, b, n, args.warmup, variant_name=args.variant, tune=args.tune,
@@ -543,6 +560,12 @@ def main():
             log(f"[bench] {name}: 64 x 256^2, {c6['steps']} steps")
             r6 = run_workload(tm, c6, c6["steps"], args.warmup, want_rows=False)
             side[name] = side_fields(r6)
+        for name, model in (("c7_rietkerk_consumers_fke2_fp64", tm.MODEL_FKE2),
+                            ("c7_rietkerk_consumers_fke3_fp64", tm.MODEL_FKE3)):
+            c7 = c7_batch(tm, model)
+            log(f"[bench] {name}: 16 x 256^2, {c7['steps']} steps")
+            r7 = run_workload(tm, c7, c7["steps"], args.warmup, want_rows=False)
+            side[name] = side_fields(r7, ("value", "ms_per_step", "kernel_ms", "variant", "steps"))
 
     cpu = None
     if not args.no_cpu:

@@ -161,10 +161,12 @@ template <typename T> struct Arith;
 template <> struct Arith<double> {
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
     static __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
+    static __device__ __forceinline__ double largest() { return 1.7976931348623157e308; }
 };
 template <> struct Arith<float> {
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ bool finite(float x) { return __builtin_isfinite(x); }
+    static __device__ __forceinline__ float largest() { return 3.40282347e38f; }
 };
 
 // Dornic Poisson-Gamma transition of one live cell (rietkerk_bjork_FKE.cpp:4168-4196).
@@ -463,6 +465,44 @@ __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0,
             }
             __builtin_amdgcn_global_load_lds(src0 + off, lds + j * NT + wave_base, 4, 0, 0);
         }
+    }
+#endif
+}
+
+// As tile_to_lds for NFR fields of one tile at once (same L, same tile): each
+// chunk's address is formed once and serves every field's copy.
+template <typename T, int TX, int RH, int NT, int H, int NFR>
+__device__ __forceinline__ void tiles_to_lds(const T* const (&fin)[NFR], int L, int x0, int y0,
+                                             uint32_t* const (&lds)[NFR])
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr int V = LdsFrame<T, H>::V;
+    constexpr int HC = LdsFrame<T, H>::HC;
+    constexpr int RWV = TX + 2 * HC;
+    if (L % V != 0 || L < RWV || L < RH) {
+#pragma unroll
+        for (int f = 0; f < NFR; ++f) tile_to_lds<T, TX, RH, NT, H>(fin[f], L, x0, y0, lds[f]);
+        return;
+    }
+    const int tid = threadIdx.x;
+    const int wave_base = tid & ~63;
+    constexpr int CPR = RWV / V;                  // 16-byte chunks per row
+    constexpr int NC = RH * CPR;
+    constexpr int NLD = (NC + NT - 1) / NT;
+#pragma unroll 1
+    for (int j = 0; j < NLD; ++j) {
+        const int q = j * NT + tid;
+        uint32_t off = 0;
+        if (q < NC) {
+            const int ry = q / CPR, cx = q - (q / CPR) * CPR;
+            int gy = y0 + ry - H, gx = x0 - HC + cx * V;
+            gy = gy < 0 ? gy + L : (gy >= L ? gy - L : gy);   // one wrap: L >= RH, L >= RWV
+            gx = gx < 0 ? gx + L : (gx >= L ? gx - L : gx);
+            off = (uint32_t)gy * (uint32_t)L + (uint32_t)gx;
+        }
+#pragma unroll
+        for (int f = 0; f < NFR; ++f)
+            __builtin_amdgcn_global_load_lds(fin[f] + off, lds[f] + 4 * (j * NT + wave_base), 16, 0, 0);
     }
 #endif
 }
@@ -1168,6 +1208,22 @@ __device__ __forceinline__ void rk_rhs(const RKConst<T>& c, T P, T W, T O, T nW,
     fO = c.R - A + c.dO * (nO - (T)4 * O);
 }
 
+// FP32: the same terms with the hardware reciprocal shared by the two W/(W + k1)
+// quotients and explicit fused multiply-adds (no rounding order to keep: the
+// FP32 path is checked against FP64 statistically, and every FP32 kernel uses
+// this one form).
+template <>
+__device__ __forceinline__ void rk_rhs<float>(const RKConst<float>& c, float P, float W, float O, float nW,
+                                              float nO, float& fP, float& fW, float& fO)
+{
+    const float rw = __builtin_amdgcn_rcpf(W + c.K1);
+    const float wp = W * P;
+    fP = (c.cgmax * wp) * rw;
+    const float A = (c.alpha * (P + c.K2W0)) * __builtin_amdgcn_rcpf(P + c.K2) * O;
+    fW = __builtin_fmaf(c.dw, __builtin_fmaf(-4.0f, W, nW), __builtin_fmaf(-c.rW, W, A - (c.gmax * wp) * rw));
+    fO = __builtin_fmaf(c.dO, __builtin_fmaf(-4.0f, O, nO), c.R - A);
+}
+
 // Ring of radius r around a TX x TY tile: index k -> tile-relative (ly, lx).
 template <int TX, int TY, int R>
 __device__ __forceinline__ void ring_cell(int k, int& ly, int& lx)
@@ -1287,9 +1343,11 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
     const T* fin = in + 3 * tl.cell_off;
     T* fout = out + 3 * tl.cell_off;
     const T* p_src = SPLIT ? (const T*)a.aux + tl.cell_off : fin;
-    tile_to_lds<T, TX, H4, NT, 4>(p_src, L, tl.x0, tl.y0, s_fr[0]);
-    tile_to_lds<T, TX, H4, NT, 4>(fin + N, L, tl.x0, tl.y0, s_fr[1]);
-    tile_to_lds<T, TX, H4, NT, 4>(fin + 2 * N, L, tl.x0, tl.y0, s_fr[2]);
+    {
+        const T* const src[3] = {p_src, fin + N, fin + 2 * N};
+        uint32_t* const dst[3] = {s_fr[0], s_fr[1], s_fr[2]};
+        tiles_to_lds<T, TX, H4, NT, 4, 3>(src, L, tl.x0, tl.y0, dst);
+    }
     const RKConst<T> c = rk_cast<T>(cd);
 
     // LDS offsets (radius-4 frame, radius-3 frame) of this thread's cells; ring
@@ -1491,7 +1549,8 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 T v = y[f];
-                if (v < (T)0 || !Arith<T>::finite(v)) record_fault(a.state, tl.inst, f * N + cell, counter);
+                // negative, NaN or infinite (finite >= 0 values pass both comparisons)
+                if (!(v >= (T)0 && v <= Arith<T>::largest())) record_fault(a.state, tl.inst, f * N + cell, counter);
                 if (v < c.eps && v >= (T)0) v = (T)0;
                 fout[f * N + cell] = v;
                 sum[f] += (double)v;

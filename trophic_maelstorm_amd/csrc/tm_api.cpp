@@ -886,6 +886,10 @@ tm_status tm_configure(tm_ctx* c, int model, int precision, const void* params, 
         c->total_cells += (int64_t)instances[i].L * instances[i].L;
     }
     c->variant = default_variant(model, precision, c->total_cells);
+    {   // whole multi-step launches per graph block, as tm_set_launch makes them
+        const int spl = cur_variant(c).spl;
+        if (spl > 1 && c->graph_steps > 0) c->graph_steps = (c->graph_steps + spl - 1) / spl * spl;
+    }
     HIPCHK(c, hipMalloc(&c->d_params, c->param_size * n_params));
     HIPCHK(c, hipMemcpy(c->d_params, c->params.data(), c->param_size * n_params, hipMemcpyHostToDevice));
     const size_t fbytes = c->esize * (size_t)c->nf * c->total_cells;

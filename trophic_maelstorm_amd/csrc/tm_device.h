@@ -248,30 +248,49 @@ TM_HD double norm_icdf_d(double p)
     return num / den;
 }
 
-// Same approximation in FP32 for the FP32 Gamma sampler, on the hardware
-// log2, square root and reciprocal (about 1 ulp each; the IEEE forms of the
-// divide and square root cost ten instructions more apiece).  Both rational
-// forms are evaluated and one is selected: the tail form is needed somewhere in
-// almost every wavefront, and a branch would add the mask bookkeeping to both.
-// p comes from Stream::u32(), so p and 1 - p are at least 2^-24: the tail form's
-// logarithm is finite for every p.
+// The standard normal quantile in FP32, for the FP32 samplers: with t = 2p - 1 and
+// w = -ln(4p(1 - p)) (= -ln(1 - t^2), formed without cancellation),
+//   Phi^{-1}(p) = t * A(w - 5/2)         for w < 5 (|t| < 0.99663),
+//               = t * B(sqrt(w) - 3)     above,
+// the functional form of M. Giles, "Approximating the erfinv function" (GPU Computing
+// Gems Jade Edition, 2011), scaled by sqrt(2).  A and B are degree-9 polynomials whose
+// coefficients are this library's own weighted least-squares fit to Phi^{-1}
+// (trophic_maelstorm_amd/csrc/fit_normal_quantile.py): 6e-9 relative in FP64, 2.4e-7
+// (median 3e-8) evaluated in FP32 over every p that Stream::u32() yields.  One
+// hardware log2 and nine FMAs per deviate; the square root and the tail polynomial
+// run only in the waves that hold a tail lane.  p comes from Stream::u32(), so
+// 4p(1 - p) >= 2^-22 and w is finite.
 __device__ __forceinline__ float norm_icdf_f(float p)
 {
-    const float plow = 0.02425f;
-    const bool lo = p < plow, tail = lo || p > 1.0f - plow;
-    const float qt = __builtin_amdgcn_sqrtf(-1.38629436f * __builtin_amdgcn_logf(lo ? p : 1.0f - p));  // sqrt(-2 ln)
-    const float nt = ((((-7.784894002430293e-03f * qt - 3.223964580411365e-01f) * qt - 2.400758277161838e+00f) * qt
-                       - 2.549732539343734e+00f) * qt + 4.374664141464968e+00f) * qt + 2.938163982698783e+00f;
-    const float dt = (((7.784695709041462e-03f * qt + 3.224671290700398e-01f) * qt + 2.445134137142996e+00f) * qt
-                      + 3.754408661907416e+00f) * qt + 1.0f;
-    const float qc = p - 0.5f;
-    const float r = qc * qc;
-    const float nc = (((((-3.969683028665376e+01f * r + 2.209460984245205e+02f) * r - 2.759285104469687e+02f) * r
-                        + 1.383577518672690e+02f) * r - 3.066479806614716e+01f) * r + 2.506628277459239e+00f) * qc;
-    const float dc = ((((-5.447609879822406e+01f * r + 1.615858368580409e+02f) * r - 1.556989798598866e+02f) * r
-                       + 6.680131188771972e+01f) * r - 1.328068155288572e+01f) * r + 1.0f;
-    const float num = tail ? (lo ? nt : -nt) : nc;
-    return num * __builtin_amdgcn_rcpf(tail ? dt : dc);
+    const float t = 2.0f * p - 1.0f;
+    const float w = -0.693147181f * __builtin_amdgcn_logf((2.0f * p) * (2.0f * (1.0f - p)));
+    float r;
+    if (w < 5.0f) {
+        const float z = w - 2.5f;
+        r = -1.243243997e-08f;
+        r = __builtin_fmaf(r, z, 3.285592243e-08f);
+        r = __builtin_fmaf(r, z, 6.592649697e-07f);
+        r = __builtin_fmaf(r, z, -4.897004146e-06f);
+        r = __builtin_fmaf(r, z, -7.019234545e-06f);
+        r = __builtin_fmaf(r, z, 3.087856749e-04f);
+        r = __builtin_fmaf(r, z, -1.771644573e-03f);
+        r = __builtin_fmaf(r, z, -5.907717161e-03f);
+        r = __builtin_fmaf(r, z, 3.488020301e-01f);
+        r = __builtin_fmaf(r, z, 2.123313427e+00f);
+    } else {
+        const float z = __builtin_amdgcn_sqrtf(w) - 3.0f;
+        r = 1.839189063e-04f;
+        r = __builtin_fmaf(r, z, -4.515237524e-04f);
+        r = __builtin_fmaf(r, z, -1.106743875e-04f);
+        r = __builtin_fmaf(r, z, 2.116327872e-03f);
+        r = __builtin_fmaf(r, z, -5.071566440e-03f);
+        r = __builtin_fmaf(r, z, 8.036936633e-03f);
+        r = __builtin_fmaf(r, z, -1.080399193e-02f);
+        r = __builtin_fmaf(r, z, 1.335824095e-02f);
+        r = __builtin_fmaf(r, z, 1.416582465e+00f);
+        r = __builtin_fmaf(r, z, 4.006434441e+00f);
+    }
+    return r * t;
 }
 
 // log(k!): exact table below 16, Stirling series (4 terms) above.

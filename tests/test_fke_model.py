@@ -12,6 +12,8 @@ Rietkerk_FPE/rietkerk_bjork_FKE.cpp:4100-4462), built from its operators:
     constants: Dornic, RK4, flush), and the extinct grazer perceives gamma 1;
   * the composed step equals the same composition with the scatter movement.
 No GPU needed."""
+import math
+
 import numpy as np
 import pytest
 
@@ -24,12 +26,12 @@ class P:
     def __init__(self, **kw):
         aij = 3.6 * 10.0 ** -6.08 * 20.0 ** -0.37
         ajm = 3.6 * 10.0 ** -6.08 * 100.0 ** -0.37
-        d = dict(a=1.0 / 24.0, c=10000.0, gmax=0.05e-3 / 24.0, alpha=0.2 / 24.0, rW=0.2 / 24.0, W0=0.2,
+        d = dict(a=1.0 / 24.0, c=10000.0, gmax=0.05 * 10.0 ** -3.0 / 24.0, alpha=0.2 / 24.0, rW=0.2 / 24.0, W0=0.2,
                  K=(0.0, 5.0, 5000.0), A=((0.0, aij, 0.0), (aij, 0.0, ajm), (0.0, ajm, 0.0)),
                  H=((0.0, 1.0, 0.0), (1.0, 0.0, 1.0), (0.0, 1.0, 0.0)), E=(1.0, 0.45, 0.85),
                  M=(0.25 / 24.0, 0.061609 * 20.0 ** -0.25 / 8760.0, 0.061609 * 100.0 ** -0.25 / 8760.0),
                  D_P=0.00025 / 24.0, D_W=0.00025 / 24.0, D_O=0.025 / 24.0, D_mov=(0.0, 0.01028, 0.215965),
-                 v=(0.0, 0.45427, 0.40587), sigma=((0.00025 / 24.0) ** 0.5 / 0.1, 1.0, 10.0),
+                 v=(0.0, 0.45427, 0.40587), sigma=(math.sqrt((0.00025 / 24.0) / (0.1 * 0.1)), 1.0, 10.0),
                  pR=(0.0, 1.04285 / 2.0, 1.25536 / 2.0), dt=0.1, dt_analytic=0.11817455,
                  dt_adv=(0.0, 0.11817455, 0.765270868), dx=0.1, flush_eps=1e-10, mv_invariance=0, CI=4,
                  sampler=0)

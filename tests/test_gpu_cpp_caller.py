@@ -62,3 +62,45 @@ def test_cpp_caller_matches_ctypes(tm, oracle, tmp_path):
     assert np.array_equal(rho, rho_cpp)
     assert np.array_equal(fields, fields_cpp)
     assert np.all(rho[:, -1, 0] > 0)          # the vegetation is alive: the run did work
+
+
+def test_cpp_fke_caller_matches_ctypes(tm, tmp_path):
+    """tests/cpp/fke_loop.cpp: the Rietkerk model with grazer and predator
+    (TM_MODEL_FKE3) from C++, with the reference driver's constants spelled out as
+    INTEGRATION.md maps them, stepped and read back with tm_mov_stats; bit for bit the
+    run of tm.FKEParams() (the same constants) through the ctypes binding."""
+    from tests.test_fke_model import model_fields
+    cxx = shutil.which("g++") or shutil.which("c++")
+    assert cxx, "a host C++ compiler is needed"
+    pkg = os.path.join(ROOT, "trophic_maelstorm_amd")
+    exe = str(tmp_path / "fke_loop")
+    subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "fke_loop.cpp"), "-o", exe, "-L", pkg, "-ltm",
+                    f"-Wl,-rpath,{pkg}"], check=True)
+    g, r, seed, n = 40, 2, 17, 6
+    frames = np.stack([model_fields(3, g, 70 + j) for j in range(r)])
+    fin = tmp_path / "fields.bin"
+    frames.astype(np.float64).tofile(fin)
+    fout = tmp_path / "out.bin"
+    print(f"cpp fke caller: g={g} r={r} seed={seed} n={n}", flush=True)
+    subprocess.run([exe, str(fin), str(fout), str(g), str(r), str(seed), str(n)], check=True, timeout=120)
+    buf = np.fromfile(fout, np.float64)
+    o = r * n * 14
+    rows_cpp = buf[:o].reshape(r, n, 14)
+    mov_cpp = buf[o:o + r * 12].reshape(r, 12)
+    o += r * 12
+    fault_cpp = tuple(int(x) for x in buf[o:o + 3])
+    fields_cpp = buf[o + 3:].reshape(r, 7, g, g)
+    with tm.Context() as ctx:
+        ctx.configure(tm.MODEL_FKE3, tm.F64, tm.FKEParams(), [tm.Instance(g, 0, seed, j) for j in range(r)])
+        for j in range(r):
+            ctx.upload(j, frames[j])
+        rows = ctx.run(n)
+        mov = ctx.mov_stats()
+        fault = ctx.fault_info()
+        fields = np.stack([ctx.download(j) for j in range(r)])
+    assert fault == fault_cpp == (-1, -1, -1)
+    assert np.array_equal(rows, rows_cpp)
+    assert np.array_equal(mov, mov_cpp)
+    assert np.array_equal(fields, fields_cpp)
+    assert np.any(fields[:, 1] != frames[:, 1])         # the grazers moved

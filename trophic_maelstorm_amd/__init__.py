@@ -13,6 +13,7 @@ Reference call sites (relative to the reference's simulations/ directory):
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass, field
 
@@ -301,7 +302,7 @@ class FKEParams:
     Dt_Analytic the smallest advection time (:47-48, FKE:197-203) and dt = 0.1 hr."""
     a: float = 1.0 / 24.0
     c: float = 10000.0
-    gmax: float = 0.05e-3 / 24.0
+    gmax: float = 0.05 * 10.0 ** -3.0 / 24.0
     alpha: float = 0.2 / 24.0
     rW: float = 0.2 / 24.0
     W0: float = 0.2
@@ -315,7 +316,7 @@ class FKEParams:
     D_O: float = 0.025 / 24.0
     D_mov: tuple = (0.0, 0.01028, 0.215965)
     v: tuple = (0.0, 0.45427, 0.40587)
-    sigma: tuple = ((0.00025 / 24.0) ** 0.5 / 0.1, 1.0, 10.0)
+    sigma: tuple = (math.sqrt((0.00025 / 24.0) / (0.1 * 0.1)), 1.0, 10.0)
     pR: tuple = (0.0, 1.04285 / 2.0, 1.25536 / 2.0)
     dt: float = 0.1
     dt_analytic: float = 0.11817455

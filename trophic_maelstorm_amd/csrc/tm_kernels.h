@@ -1249,41 +1249,76 @@ struct Ring {
     static constexpr int N = (TX + 2 * R) * (TY + 2 * R) - TX * TY;
 };
 
-// Fixed-order block reduction of NF (FP64 sum, integer count) pairs; thread 0
-// writes them as (sum, count) partials.  The sums are combined exactly as by
-// block_partials (same shuffle tree, waves in order); counts are exact either way.
-template <int NT, int NF>
-__device__ __forceinline__ void block_partials_sc(double (&sum)[NF], uint32_t (&cnt)[NF], double2* dst,
-                                                  double* red)
+// Cross-lane moves of an FP64 value on the VALU (no LDS traffic, unlike
+// ds_bpermute): gfx950's v_permlane32_swap / v_permlane16_swap exchange the
+// upper half (odd rows) of x with the lower half (even rows) of y, and DPP
+// row shifts under bank masks give lane l the value of lane l ^ 8 or l ^ 4.
+__device__ __forceinline__ void lanes_swap32(double& x, double& y)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
+    x = __hiloint2double(hi[0], lo[0]);
+    y = __hiloint2double(hi[1], lo[1]);
+}
+
+__device__ __forceinline__ void lanes_swap16(double& x, double& y)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+    x = __hiloint2double(hi[0], lo[0]);
+    y = __hiloint2double(hi[1], lo[1]);
+}
+
+// DPP move of both halves: lanes in BANK (groups of 4 lanes of each 16-lane
+// row) take src from the lane CTRL names, the others keep old.
+template <int CTRL, int BANK>
+__device__ __forceinline__ double dpp_f64(double old, double src)
+{
+    return __hiloint2double(__builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(src), CTRL, 0xF, BANK, false),
+                            __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(src), CTRL, 0xF, BANK, false));
+}
+
+// Wave totals of four FP64 values in one pass: on return every lane of row r
+// (lanes 16 r .. 16 r + 15) holds the sum of v[r] over the 64 lanes.  Fixed
+// order: v[0] and v[2] (v[1] and v[3]) fold lane l with l + 32 while the two
+// halves trade values, the pair folds rows 2 k with 2 k + 1 the same way, and
+// each row then folds lanes l ^ 8, l ^ 4, l ^ 2, l ^ 1.
+__device__ __forceinline__ double wave_sum4(double v0, double v1, double v2, double v3)
+{
+    lanes_swap32(v0, v2);                 // v0 = (v0 lower, v2 lower), v2 = (v0 upper, v2 upper)
+    lanes_swap32(v1, v3);
+    double a = v0 + v2, b = v1 + v3;      // lanes < 32: v0, v1 folded; lanes >= 32: v2, v3
+    lanes_swap16(a, b);
+    double t = a + b;                     // row r: v[r] folded over lanes l, l + 16, l + 32, l + 48
+    t += dpp_f64<0x118, 0xC>(dpp_f64<0x108, 0x3>(t, t), t);   // row_shl:8 | row_shr:8 = l ^ 8
+    t += dpp_f64<0x114, 0xA>(dpp_f64<0x104, 0x5>(t, t), t);   // row_shl:4 | row_shr:4 = l ^ 4
+    t += dpp_f64<0x4E, 0xF>(t, t);                            // quad_perm [2,3,0,1] = l ^ 2
+    t += dpp_f64<0xB1, 0xF>(t, t);                            // quad_perm [1,0,3,2] = l ^ 1
+    return t;
+}
+
+// Fixed-order block reduction of three (FP64 sum, count) pairs, each wave's
+// sums through wave_sum4 and the counts already wave totals (ballots); threads
+// 0..5 add the NT / 64 wave values in wave order and write (sum, count)
+// partials.  Every lane of the block must be active.
+template <int NT>
+__device__ __forceinline__ void block_partials3(const double (&sum)[3], const uint32_t (&wcnt)[3], double2* dst,
+                                                double* red)
 {
     constexpr int NW = NT / 64;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            sum[f] += __shfl_xor(sum[f], o);
-            cnt[f] += __shfl_xor(cnt[f], o);
-        }
-    }
+    const double t = wave_sum4(sum[0], sum[1], sum[2], 0.0);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if ((lane & 15) == 0 && lane < 48) red[(2 * (lane >> 4)) * NW + w] = t;
     if (lane == 0) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            red[(2 * f) * NW + w] = sum[f];
-            red[(2 * f + 1) * NW + w] = (double)cnt[f];
-        }
+        for (int f = 0; f < 3; ++f) red[(2 * f + 1) * NW + w] = (double)wcnt[f];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 6) {
+        double s = 0.0;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            double s = 0.0, c = 0.0;
-            for (int i = 0; i < NW; ++i) {
-                s += red[(2 * f) * NW + i];
-                c += red[(2 * f + 1) * NW + i];
-            }
-            dst[f] = make_double2(s, c);
-        }
+        for (int i = 0; i < NW; ++i) s += red[threadIdx.x * NW + i];
+        ((double*)dst)[threadIdx.x] = s;
     }
 }
 
@@ -1531,7 +1566,7 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
 
     // ---- stage 4: K4 on the tile, combine, checks, flush, write ----
     double sum[3] = {0.0, 0.0, 0.0};
-    uint32_t cnt[3] = {0u, 0u, 0u};
+    uint32_t wcnt[3] = {0u, 0u, 0u};           // wave totals: ballots of the occupied cells
 #pragma unroll
     for (int j = 0; j < NOWN; ++j) {
         int ly, lx;
@@ -1544,22 +1579,23 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         y[1] = W + c.dt6 * (accW[j] + kW);
         y[2] = O + c.dt6 * (accO[j] + kO);
         const int gy = tl.y0 + ly, gx = tl.x0 + lx;
-        if (gy < L && gx < L) {
-            const int64_t cell = (int64_t)gy * L + gx;
+        const bool in = gy < L && gx < L;
+        const int64_t cell = (int64_t)gy * L + gx;
 #pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                T v = y[f];
+        for (int f = 0; f < 3; ++f) {
+            T v = y[f];
+            if (in) {
                 // negative, NaN or infinite (finite >= 0 values pass both comparisons)
                 if (!(v >= (T)0 && v <= Arith<T>::largest())) record_fault(a.state, tl.inst, f * N + cell, counter);
                 if (v < c.eps && v >= (T)0) v = (T)0;
                 fout[f * N + cell] = v;
                 sum[f] += (double)v;
-                cnt[f] += v > (T)0 ? 1u : 0u;
             }
+            wcnt[f] += (uint32_t)__popcll(__ballot(in && v > (T)0));
         }
     }
     double2* dst = a.partials + (slot * a.ntiles + tile) * 3;
-    block_partials_sc<NT, 3>(sum, cnt, dst, s_red);
+    block_partials3<NT>(sum, wcnt, dst, s_red);
 }
 
 // ---------------------------------------------------------------------------

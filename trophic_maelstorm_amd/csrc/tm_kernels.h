@@ -117,39 +117,98 @@ __device__ __forceinline__ void record_fault(DevState* st, int inst, int64_t cel
                                                        ((unsigned long long)inst << 40) | (unsigned long long)cell);
 }
 
-// Fixed-order block reduction of NF (sum, count) pairs; thread 0 writes them.
+// Cross-lane moves of an FP64 value on the VALU (no LDS traffic, unlike
+// ds_bpermute): gfx950's v_permlane32_swap / v_permlane16_swap exchange the
+// upper half (odd rows) of x with the lower half (even rows) of y, and DPP
+// row shifts under bank masks give lane l the value of lane l ^ 8 or l ^ 4.
+__device__ __forceinline__ void lanes_swap32(double& x, double& y)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
+    x = __hiloint2double(hi[0], lo[0]);
+    y = __hiloint2double(hi[1], lo[1]);
+}
+
+__device__ __forceinline__ void lanes_swap16(double& x, double& y)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+    x = __hiloint2double(hi[0], lo[0]);
+    y = __hiloint2double(hi[1], lo[1]);
+}
+
+// DPP move of both halves: lanes in BANK (groups of 4 lanes of each 16-lane
+// row) take src from the lane CTRL names, the others keep old.
+template <int CTRL, int BANK>
+__device__ __forceinline__ double dpp_f64(double old, double src)
+{
+    return __hiloint2double(__builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(src), CTRL, 0xF, BANK, false),
+                            __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(src), CTRL, 0xF, BANK, false));
+}
+
+// Wave totals of four FP64 values in one pass: on return every lane of row r
+// (lanes 16 r .. 16 r + 15) holds the sum of v[r] over the 64 lanes.  Fixed
+// order: v[0] and v[2] (v[1] and v[3]) fold lane l with l + 32 while the two
+// halves trade values, the pair folds rows 2 k with 2 k + 1 the same way, and
+// each row then folds lanes l ^ 8, l ^ 4, l ^ 2, l ^ 1.
+__device__ __forceinline__ double wave_sum4(double v0, double v1, double v2, double v3)
+{
+    lanes_swap32(v0, v2);                 // v0 = (v0 lower, v2 lower), v2 = (v0 upper, v2 upper)
+    lanes_swap32(v1, v3);
+    double a = v0 + v2, b = v1 + v3;      // lanes < 32: v0, v1 folded; lanes >= 32: v2, v3
+    lanes_swap16(a, b);
+    double t = a + b;                     // row r: v[r] folded over lanes l, l + 16, l + 32, l + 48
+    t += dpp_f64<0x118, 0xC>(dpp_f64<0x108, 0x3>(t, t), t);   // row_shl:8 | row_shr:8 = l ^ 8
+    t += dpp_f64<0x114, 0xA>(dpp_f64<0x104, 0x5>(t, t), t);   // row_shl:4 | row_shr:4 = l ^ 4
+    t += dpp_f64<0x4E, 0xF>(t, t);                            // quad_perm [2,3,0,1] = l ^ 2
+    t += dpp_f64<0xB1, 0xF>(t, t);                            // quad_perm [1,0,3,2] = l ^ 1
+    return t;
+}
+
+// Wave totals of two FP64 values: on return lanes 0..31 hold the sum of v0 over
+// the 64 lanes and lanes 32..63 that of v1.  The same folds as wave_sum4: lane
+// l with l + 32, then rows 2 k with 2 k + 1, then l ^ 8, l ^ 4, l ^ 2, l ^ 1.
+__device__ __forceinline__ double wave_sum2(double v0, double v1)
+{
+    lanes_swap32(v0, v1);
+    double a = v0 + v1, b = a;            // lanes < 32: v0 folded; lanes >= 32: v1
+    lanes_swap16(a, b);
+    double t = a + b;
+    t += dpp_f64<0x118, 0xC>(dpp_f64<0x108, 0x3>(t, t), t);
+    t += dpp_f64<0x114, 0xA>(dpp_f64<0x104, 0x5>(t, t), t);
+    t += dpp_f64<0x4E, 0xF>(t, t);
+    t += dpp_f64<0xB1, 0xF>(t, t);
+    return t;
+}
+
+// Fixed-order block reduction of NF (sum, count) pairs: the 2 NF values of each
+// wave through wave_sum4, four at a time, then threads 0 .. 2 NF - 1 add the
+// NT / 64 wave values in wave order and write (sum, count) partials.  Every
+// lane of the block must be active.
 template <int NT, int NF>
-__device__ __forceinline__ void block_partials(double (&sum)[NF], double (&cnt)[NF], double2* dst,
+__device__ __forceinline__ void block_partials(const double (&sum)[NF], const double (&cnt)[NF], double2* dst,
                                                double* red)
 {
-    constexpr int NW = NT / 64;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            sum[f] += __shfl_xor(sum[f], o);
-            cnt[f] += __shfl_xor(cnt[f], o);
-        }
-    }
+    constexpr int NW = NT / 64, NV = 2 * NF;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            red[(2 * f) * NW + w] = sum[f];
-            red[(2 * f + 1) * NW + w] = cnt[f];
+    for (int g = 0; g < NV; g += 4) {
+        double v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = g + r;
+            v[r] = i < NV ? ((i & 1) ? cnt[i >> 1] : sum[i >> 1]) : 0.0;
         }
+        const double t = wave_sum4(v[0], v[1], v[2], v[3]);
+        const int i = g + (lane >> 4);
+        if ((lane & 15) == 0 && i < NV) red[i * NW + w] = t;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < NV) {
+        double s = 0.0;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            double s = 0.0, c = 0.0;
-            for (int i = 0; i < NW; ++i) {
-                s += red[(2 * f) * NW + i];
-                c += red[(2 * f + 1) * NW + i];
-            }
-            dst[f] = make_double2(s, c);
-        }
+        for (int i = 0; i < NW; ++i) s += red[threadIdx.x * NW + i];
+        ((double*)dst)[threadIdx.x] = s;
     }
 }
 
@@ -369,22 +428,13 @@ __device__ __forceinline__ uint32_t as_bits(double x)
     return (uint32_t)b | (uint32_t)(b >> 32);
 }
 
-// Wave-level (sum, count) reduction; lane 0 writes the wave's partial.
-template <int NF>
-__device__ __forceinline__ void wave_partials(double (&sum)[NF], double (&cnt)[NF], double2* dst)
+// Wave-level (sum, count) reduction through wave_sum2; lanes 0 and 32 write the
+// wave's (sum, count) partial.  Every lane of the wave must be active.
+__device__ __forceinline__ void wave_partial(double sum, double cnt, double2* dst)
 {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            sum[f] += __shfl_xor(sum[f], o);
-            cnt[f] += __shfl_xor(cnt[f], o);
-        }
-    }
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int f = 0; f < NF; ++f) dst[f] = make_double2(sum[f], cnt[f]);
-    }
+    const double t = wave_sum2(sum, cnt);
+    const int lane = threadIdx.x & 63;
+    if ((lane & 31) == 0) ((double*)dst)[lane >> 5] = t;
 }
 
 // Wave partial (sum, count) of one step into its slot; an all-zero wave writes zeros.
@@ -397,8 +447,7 @@ __device__ __forceinline__ void step_partial(const StepArgs& a, int tile, int64_
     if (__ballot(nlive != 0 || s != 0.0) == 0) {
         if ((threadIdx.x & 63) == 0) *dst = make_double2(0.0, 0.0);
     } else {
-        double sum[1] = {s}, cnt[1] = {(double)nlive};
-        wave_partials<1>(sum, cnt, dst);
+        wave_partial(s, (double)nlive, dst);
     }
 }
 
@@ -734,8 +783,7 @@ __device__ __forceinline__ void dp_tile(const StepArgs& a, int tile, const DevTi
         if (__ballot(nlive != 0 || sum[0] != 0.0) == 0) {   // an all-zero wave: no reduction
             if ((tid & 63) == 0) *dst = make_double2(0.0, 0.0);
         } else {
-            double cnt[1] = {(double)nlive};
-            wave_partials<1>(sum, cnt, dst);
+            wave_partial(sum[0], (double)nlive, dst);
         }
     }
 }
@@ -1248,54 +1296,6 @@ template <int TX, int TY, int R>
 struct Ring {
     static constexpr int N = (TX + 2 * R) * (TY + 2 * R) - TX * TY;
 };
-
-// Cross-lane moves of an FP64 value on the VALU (no LDS traffic, unlike
-// ds_bpermute): gfx950's v_permlane32_swap / v_permlane16_swap exchange the
-// upper half (odd rows) of x with the lower half (even rows) of y, and DPP
-// row shifts under bank masks give lane l the value of lane l ^ 8 or l ^ 4.
-__device__ __forceinline__ void lanes_swap32(double& x, double& y)
-{
-    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
-    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
-    x = __hiloint2double(hi[0], lo[0]);
-    y = __hiloint2double(hi[1], lo[1]);
-}
-
-__device__ __forceinline__ void lanes_swap16(double& x, double& y)
-{
-    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
-    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
-    x = __hiloint2double(hi[0], lo[0]);
-    y = __hiloint2double(hi[1], lo[1]);
-}
-
-// DPP move of both halves: lanes in BANK (groups of 4 lanes of each 16-lane
-// row) take src from the lane CTRL names, the others keep old.
-template <int CTRL, int BANK>
-__device__ __forceinline__ double dpp_f64(double old, double src)
-{
-    return __hiloint2double(__builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(src), CTRL, 0xF, BANK, false),
-                            __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(src), CTRL, 0xF, BANK, false));
-}
-
-// Wave totals of four FP64 values in one pass: on return every lane of row r
-// (lanes 16 r .. 16 r + 15) holds the sum of v[r] over the 64 lanes.  Fixed
-// order: v[0] and v[2] (v[1] and v[3]) fold lane l with l + 32 while the two
-// halves trade values, the pair folds rows 2 k with 2 k + 1 the same way, and
-// each row then folds lanes l ^ 8, l ^ 4, l ^ 2, l ^ 1.
-__device__ __forceinline__ double wave_sum4(double v0, double v1, double v2, double v3)
-{
-    lanes_swap32(v0, v2);                 // v0 = (v0 lower, v2 lower), v2 = (v0 upper, v2 upper)
-    lanes_swap32(v1, v3);
-    double a = v0 + v2, b = v1 + v3;      // lanes < 32: v0, v1 folded; lanes >= 32: v2, v3
-    lanes_swap16(a, b);
-    double t = a + b;                     // row r: v[r] folded over lanes l, l + 16, l + 32, l + 48
-    t += dpp_f64<0x118, 0xC>(dpp_f64<0x108, 0x3>(t, t), t);   // row_shl:8 | row_shr:8 = l ^ 8
-    t += dpp_f64<0x114, 0xA>(dpp_f64<0x104, 0x5>(t, t), t);   // row_shl:4 | row_shr:4 = l ^ 4
-    t += dpp_f64<0x4E, 0xF>(t, t);                            // quad_perm [2,3,0,1] = l ^ 2
-    t += dpp_f64<0xB1, 0xF>(t, t);                            // quad_perm [1,0,3,2] = l ^ 1
-    return t;
-}
 
 // Fixed-order block reduction of three (FP64 sum, count) pairs, each wave's
 // sums through wave_sum4 and the counts already wave totals (ballots); threads

@@ -165,6 +165,21 @@ __device__ __forceinline__ double wave_sum4(double v0, double v1, double v2, dou
     return t;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave on the VALU: shifts by 1, 2,
+// 4 and 8 lanes within each 16-lane row, then each row adds the totals of the
+// rows before it (row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and
+// 3).  Integer sums: exact in any order.  Every lane must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);   // row_bcast:31
+    return x;
+}
+
 // Wave totals of two FP64 values: on return lanes 0..31 hold the sum of v0 over
 // the 64 lanes and lanes 32..63 that of v1.  The same folds as wave_sum4: lane
 // l with l + 32, then rows 2 k with 2 k + 1, then l ^ 8, l ^ 4, l ^ 2, l ^ 1.
@@ -872,12 +887,7 @@ __device__ __forceinline__ void dp_tile_deferred(const StepArgs& a, int tile, co
     // then the totals of the waves before it.
     const uint32_t ndef = (uint32_t)__builtin_popcount(dmask);
     const int lane = tid & 63;
-    uint32_t incl = ndef;
-#pragma unroll
-    for (int d = 1; d < 64; d *= 2) {
-        const uint32_t o = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += o;
-    }
+    const uint32_t incl = wave_incl_scan(ndef);
     if (lane == 63) s_wn[tid >> 6] = incl;
     __syncthreads();
     uint32_t base = incl - ndef, nq = 0;

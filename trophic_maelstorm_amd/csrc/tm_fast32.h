@@ -207,6 +207,7 @@ __device__ __forceinline__ bool gamma_f_try(St& s, float shape, float& g)
     const float a = shape < 1.0f ? shape + 1.0f : shape;
     const float d = a - 0.333333343f;
     const float c = __builtin_amdgcn_rsqf(9.0f * d);
+    const float k = c * c * 0.0833333358f;   // 1 / (108 d)
     g = d;
     bool ok = false;
     for (int it = 0; it < TRIALS; ++it) {
@@ -214,10 +215,20 @@ __device__ __forceinline__ bool gamma_f_try(St& s, float shape, float& g)
         const float y = c * x;
         float v = 1.0f + y;
         if (v <= 0.0f) continue;
+        const float m = __builtin_fminf(v, 1.0f);
         v = v * v * v;
         const float u = s.u32();
         const float x2 = x * x;
-        if (u < 1.0f - 0.0331f * x2 * x2) { g = d * v; ok = true; break; }
+        // Squeeze: u < 1 - x^4 / (108 d m), m = 1 + min(y, 0).  The test below is
+        // log u < Q = x^2/2 + d(1 - v + log v) = 3d R(y), R(y) = log(1 + y) - y +
+        // y^2/2 - y^3/3, and R(y) >= -y^4 / (4m) (the Taylor remainder of
+        // log(1 + y) after y^4 is >= 0 for y >= 0; for y < 0, R = -sum_{k>=4}
+        // |y|^k / k >= -y^4 / (4(1 + y))).  With y = c x, c^2 = 1/(9d):
+        // Q >= -x^4/(108 d m), and e^Q >= 1 + Q, so every accepted u passes the
+        // test too.  It accepts more than Marsaglia-Tsang's 1 - 0.0331 x^4 at
+        // every d >= 2/3 (misses 6% of trials at d = 2/3, 1% at d = 4, under
+        // 0.2% above d = 25), so a wave skips the logarithms more often.
+        if (u * m < m - k * x2 * x2) { g = d * v; ok = true; break; }
         // x^2/2 + d(1 - v + log v); below |y| = 0.1 its series in y, which does not
         // cancel: d y^4 (-3/4 + 3/5 y - 1/2 y^2 + 3/7 y^3 - 3/8 y^4).  Both are
         // formed and one selected (v > 0 here, so the logarithm is finite).

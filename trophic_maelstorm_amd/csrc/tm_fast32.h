@@ -167,33 +167,36 @@ __device__ __forceinline__ bool ptrs_trial(const PtrsF& p, float U, float V, dou
     return lhs <= logp;
 }
 
-// Poisson(z), z formed in FP64 by the caller.  St: Stream, or Stream4 where at
-// most four words are drawn.  z < 12: the counted inversion; z >= 12: PTRS.
+// Poisson(z), z formed in FP64 by the caller, returned as an integer-valued
+// double (the callers want it as int64 and as float, each one conversion from
+// FP64).  St: Stream, or Stream4 where at most four words are drawn.  z < 12:
+// the counted inversion; z >= 12: PTRS.
 template <class St>
-__device__ __forceinline__ int64_t poisson_f(St& s, double zd, bool compat)
+__device__ __forceinline__ double poisson_fd(St& s, double zd, bool compat)
 {
-    if (!(zd > 0.0)) return 0;
+    if (!(zd > 0.0)) return 0.0;
     if (compat && zd > 100.0) {
         const float zf = (float)zd;
-        return (int64_t)((double)(norm_icdf_f(s.u32()) * __builtin_sqrtf(zf)) + zd);
+        return __builtin_trunc((double)(norm_icdf_f(s.u32()) * __builtin_sqrtf(zf)) + zd);
     }
     if (zd < 12.0) {
         const float z = (float)zd;
         const float v = s.v32();
         int n;
         float S, P;
-        if (poisson_count(z, v, n, S, P)) return n;
-        return poisson_tail(z, v, S, P);
+        if (poisson_count(z, v, n, S, P)) return (double)n;
+        return (double)poisson_tail(z, v, S, P);
     }
     const PtrsF pt = ptrs_setup(zd);
     for (int it = 0; it < 64; ++it) {
         const float U = s.u32() - 0.5f;
         const float V = s.u32();
         double kd;
-        if (ptrs_trial(pt, U, V, kd)) return (int64_t)kd;
+        if (ptrs_trial(pt, U, V, kd)) return kd;
     }
-    return (int64_t)__builtin_floor(zd);
+    return __builtin_floor(zd);
 }
+
 
 // Gamma(shape, 1), Marsaglia-Tsang in FP32, at most TRIALS trials.  With
 // TRIALS = 64 it always returns true (64 rejections leave g = d, as the FP64

@@ -294,6 +294,8 @@ __device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, co
                                               int64_t& n, float& y)
 {
     bool big = false;   // the Poisson draw took a PTRS trial's two words
+    float nf;           // the count in FP32, formed from its int or FP64 source: an
+                        // int64 -> float conversion costs a dozen instructions
     if (ONE) {
         // at most four words: the inversion's one or one PTRS trial's two, then one
         // Gamma trial's two, and the boost's where the inversion left room for it
@@ -305,6 +307,7 @@ __device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, co
                 if (!poisson_count((float)zd, s.v32(), ni, S, P)) return false;
             }
             n = ni;
+            nf = (float)ni;
         } else {
             const PtrsF pt = ptrs_setup(zd);
             const float U = s.u32() - 0.5f;
@@ -312,12 +315,15 @@ __device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, co
             double kd;
             if (!ptrs_trial(pt, U, V, kd)) return false;
             n = (int64_t)kd;
+            nf = (float)kd;
             big = true;
         }
     } else {
-        n = poisson_f(s, zd, compat);
+        const double nd = poisson_fd(s, zd, compat);
+        n = (int64_t)nd;
+        nf = (float)nd;
     }
-    const float gru = shape0 + (float)n;
+    const float gru = shape0 + nf;
     float g;
     if (compat && gru > 100.0f) {
         g = norm_icdf_f(s.u32()) * __builtin_sqrtf(gru) + gru;

@@ -129,8 +129,8 @@ def lib():
         L.tmo_fke_rk4.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p]
         L.tmo_fke_veg_dornic.argtypes = [C.POINTER(RKParams), C.c_int, C.c_uint32, C.c_uint32, C.c_int64, _f64p,
                                          _f64p]
-        L.tmo_fke_movement_gather.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
-                                              C.c_double, _f64p, _f64p, _f64p]
+        L.tmo_fke_movement_sep.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
+                                           C.c_double, _f64p, _f64p, _f64p]
         L.tmo_fke_mov_stats.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p]
         L.tmo_fixed_sum.restype = C.c_double
         L.tmo_fixed_sum.argtypes = [_f64p, C.c_int64]
@@ -487,13 +487,13 @@ def fke_veg_dornic(p: RKParams, P, seed, inst, step):
     return out.reshape(L, L)
 
 
-def fke_movement_gather(rho, gamma, vx, vy, rD, rV, W, pref, SD, SV):
-    """tmo_fke_movement_gather: one species' movement, summed per output cell."""
+def fke_movement_sep(rho, gamma, vx, vy, rD, rV, S, pref, SD, SV):
+    """tmo_fke_movement_sep: one species' movement, factored by rows and columns."""
     r = np.ascontiguousarray(rho, np.float64)
     L = r.shape[-1]
     out = np.empty(L * L)
     f = lambda x: np.ascontiguousarray(x, np.float64).reshape(-1)
-    lib().tmo_fke_movement_gather(L, f(r), f(gamma), f(vx), f(vy), rD, rV, W, pref, f(SD), f(SV), out)
+    lib().tmo_fke_movement_sep(L, f(r), f(gamma), f(vx), f(vy), rD, rV, S, pref, f(SD), f(SV), out)
     return out.reshape(L, L)
 
 
@@ -506,15 +506,20 @@ def fke_mov_stats(gamma, vx, vy):
     return out
 
 
+def fke_shift_reach(pref, vmax):
+    """The largest shift |m - c| of a source's advected mean, plus one."""
+    return int(math.floor(vmax * abs(pref) + 0.5)) + 1
+
+
 def fke_window(rD, rV, pref, vmax):
-    """The gather window of a species: at least its largest reach."""
-    return max(rD, rV + int(math.floor(vmax * abs(pref) + 0.5)) + 1)
+    """The largest reach of a species' movement: 2 W + 1 must fit the lattice."""
+    return max(rD, rV + fke_shift_reach(pref, vmax))
 
 
 def fke_step(p, SP, fields, seed, inst, step):
     """One step of the Rietkerk consumer model (FKE:4100-4462) composed from the
     operators above: perception from the old densities, the vegetation's Dornic
-    step, the RK4 of the trophic and water terms, each consumer's movement (gather
+    step, the RK4 of the trophic and water terms, each consumer's movement (separable
     order), checks and flush.  p: an object with tm_fke_params' fields; SP = 2 or 3
     species; fields [2 SP + 1, L, L] = P, G, (Pr,) W, O, gamma_1 (, gamma_2).
     Returns (fields, fault, v_eff [SP, L, L, 2])."""
@@ -535,8 +540,8 @@ def fke_step(p, SP, fields, seed, inst, step):
         drawn = SP == 3
         vx = ve[s, ..., 0] if drawn else np.zeros((L, L))
         vy = ve[s, ..., 1] if drawn else np.zeros((L, L))
-        W = fke_window(rD, rV, pref, abs(p.v[s]) if drawn else 0.0)
-        y[s] = fke_movement_gather(y[s], g[s - 1], vx, vy, rD, rV, W, pref, SD, SV)
+        S = fke_shift_reach(pref, abs(p.v[s]) if drawn else 0.0)
+        y[s] = fke_movement_sep(y[s], g[s - 1], vx, vy, rD, rV, S, pref, SD, SV)
     fault = bool(np.any((y < 0) | ~np.isfinite(y)))
     y = np.where((y < p.flush_eps) & (y >= 0.0), 0.0, y)
     if SP == 2:

@@ -1212,7 +1212,7 @@ int64_t tmo_fke_rk4(const tmo_fke_rk_params* p, int ns, int L, double* y)
 
 /* ------------------------------------------------------------------------ */
 /* The Rietkerk consumer model as one step: its vegetation Dornic step, the */
-/* movement in gather order and the movement statistics.                    */
+/* movement in separable form and the movement statistics.               */
 /* ------------------------------------------------------------------------ */
 
 /* FKE:4150-4211: alpha_i from the old P (nR2 neighbours), the skip of empty
@@ -1240,22 +1240,14 @@ void tmo_fke_veg_dornic(const tmo_rk_params* p, int L, uint32_t seed, uint32_t i
         }
 }
 
-/* x - m in (-L/2, L/2] */
-static int gap_of(int x, int m, int L)
-{
-    int e = x - m;
-    if (2 * e > L) e -= L;
-    if (2 * e <= -L) e += L;
-    return e;
-}
-
-void tmo_fke_movement_gather(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
-                             int rD, int rV, int W, double mu_pref, const double* SD, const double* SV,
-                             double* out)
+void tmo_fke_movement_sep(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
+                          int rD, int rV, int S, double mu_pref, const double* SD, const double* SV, double* out)
 {
     const int64_t L2 = (int64_t)L * L;
     double* cx = (double*)malloc(sizeof(double) * L2 * 3);
     int* mx = (int*)malloc(sizeof(int) * L2 * 2);
+    double* y4 = (double*)malloc(sizeof(double) * L2 * 4);   /* cx, Y1, Y2, Z */
+    double* pq = (double*)malloc(sizeof(double) * L2 * 2);   /* P, Q */
     for (int64_t i = 0; i < L2; ++i) {           /* each source's terms and advected mean (FKE:3626-3637) */
         if (rho[i] < 1.0e-10) {
             mx[2 * i] = -1;
@@ -1269,32 +1261,60 @@ void tmo_fke_movement_gather(int L, const double* rho, const double* gamma, cons
         cx[3 * i + 1] = (g - g2) * r;
         cx[3 * i + 2] = (1 - g) * (1 - g) * r;
     }
+    /* displacement sums over the (2S + 1)^2 sources around (p, q), rows then
+     * columns ascending: Y1 over row p (sources whose my = q), Y2 over column q
+     * (mx = p), Z over all (mx = p and my = q) */
+    for (int p = 0; p < L; ++p)
+        for (int q = 0; q < L; ++q) {
+            const int64_t o = (int64_t)p * L + q;
+            double c0 = 0.0, a1 = 0.0, a2 = 0.0, z = 0.0;
+            if (mx[2 * o] >= 0) c0 = cx[3 * o];
+            for (int di = -S; di <= S; ++di) {
+                const int ci = wrapi(p + di, L);
+                for (int dj = -S; dj <= S; ++dj) {
+                    const int cj = wrapi(q + dj, L);
+                    const int64_t i = (int64_t)ci * L + cj;
+                    if (mx[2 * i] < 0) continue;
+                    const int hx = mx[2 * i] == p, hy = mx[2 * i + 1] == q;
+                    if (di == 0 && hy) a1 += cx[3 * i + 1];
+                    if (dj == 0 && hx) a2 += cx[3 * i + 1];
+                    if (hx && hy) z += cx[3 * i + 2];
+                }
+            }
+            y4[4 * o] = c0;
+            y4[4 * o + 1] = a1;
+            y4[4 * o + 2] = a2;
+            y4[4 * o + 3] = z;
+        }
+    /* along each row: P = SD * cx + SV * Y1, Q = SD * Y2 + SV * Z (taps ascending) */
+    for (int p = 0; p < L; ++p)
+        for (int q = 0; q < L; ++q) {
+            double P = 0.0, Q = 0.0;
+            for (int b = -rD; b <= rD; ++b) {
+                const int64_t i = (int64_t)p * L + wrapi(q - b, L);
+                P += SD[abs(b)] * y4[4 * i];
+                Q += SD[abs(b)] * y4[4 * i + 2];
+            }
+            for (int b = -rV; b <= rV; ++b) {
+                const int64_t i = (int64_t)p * L + wrapi(q - b, L);
+                P += SV[abs(b)] * y4[4 * i + 1];
+                Q += SV[abs(b)] * y4[4 * i + 3];
+            }
+            pq[2 * ((int64_t)p * L + q)] = P;
+            pq[2 * ((int64_t)p * L + q) + 1] = Q;
+        }
+    /* along each column: out = SD * P + SV * Q (taps ascending) */
     for (int x = 0; x < L; ++x)
         for (int y = 0; y < L; ++y) {
             double acc = 0.0;
-            for (int dr = -W; dr <= W; ++dr) {
-                const int ci = wrapi(x - dr, L), adr = abs(dr);
-                const int rowD = adr <= rD;
-                for (int dc = -W; dc <= W; ++dc) {
-                    const int cj = wrapi(y - dc, L);
-                    const int64_t i = (int64_t)ci * L + cj;
-                    if (mx[2 * i] < 0) continue;
-                    const int aex = abs(gap_of(x, mx[2 * i], L)), aey = abs(gap_of(y, mx[2 * i + 1], L));
-                    const int adc = abs(dc);
-                    const int rowV = aex <= rV, colD = adc <= rD, colV = aey <= rV;
-                    if (!((rowD || rowV) && (colD || colV))) continue;
-                    double t = 0.0;
-                    if (rowD && colD) t += cx[3 * i] * (SD[adr] * SD[adc]);
-                    if (rowD && colV) t += cx[3 * i + 1] * (SD[adr] * SV[aey]);
-                    if (rowV && colD) t += cx[3 * i + 1] * (SV[aex] * SD[adc]);
-                    if (rowV && colV) t += cx[3 * i + 2] * (SV[aex] * SV[aey]);
-                    acc += t;
-                }
-            }
+            for (int a = -rD; a <= rD; ++a) acc += SD[abs(a)] * pq[2 * ((int64_t)wrapi(x - a, L) * L + y)];
+            for (int a = -rV; a <= rV; ++a) acc += SV[abs(a)] * pq[2 * ((int64_t)wrapi(x - a, L) * L + y) + 1];
             out[(int64_t)x * L + y] = acc;
         }
     free(cx);
     free(mx);
+    free(y4);
+    free(pq);
 }
 
 void tmo_fke_mov_stats(int L, const double* gamma, const double* vx, const double* vy, double out[4])

@@ -34,9 +34,9 @@
  * (tmo_fke_rk4) by the single-species RK4 bit for bit with no grazers, the
  * two-species form bit for bit with no predators, and the ODE (solve_ivp) on
  * uniform fields; the vegetation Dornic step (tmo_fke_veg_dornic) by the
- * single-species step with no plant growth; the gather-form movement by the
+ * single-species step with no plant growth; the separable movement by the
  * scatter restatement (summation order apart), mass conservation and its
- * independence of W; the movement statistics by numpy's mean and variance.
+ * independence of S; the movement statistics by numpy's mean and variance.
  */
 #ifndef TM_ORACLE_H
 #define TM_ORACLE_H
@@ -224,13 +224,23 @@ int64_t tmo_fke_rk4(const tmo_fke_rk_params* p, int ns, int L, double* y);
 void tmo_fke_veg_dornic(const tmo_rk_params* p, int L, uint32_t seed, uint32_t inst, int64_t step,
                         const double* P, double* Pstar);
 
-/* One species' movement step in gather form: every output cell sums, over the
- * sources within W rows and columns (rows then columns, each from -W to W), the
- * terms of each source that reach it (D x D, D x V, V x D, V x V).  The same
- * terms as tmo_fke_movement, summed per output cell in this fixed order. */
-void tmo_fke_movement_gather(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
-                             int rD, int rV, int W, double mu_pref, const double* SD, const double* SV,
-                             double* out);
+/* One species' movement step in separable form.  With each source's terms
+ * (c.x, c.y, c.z) = (gamma^2 rho, (gamma - gamma^2) rho, (1 - gamma)^2 rho) and its
+ * advected mean (mx, my), as tmo_fke_movement forms them:
+ *   cx(p, q) = c.x of source (p, q) (0 where rho < 1e-10);
+ *   Y1(p, q) = sum of c.y over the sources of row p with my = q;
+ *   Y2(p, q) = sum of c.y over the sources of column q with mx = p;
+ *   Z(p, q)  = sum of c.z over the sources with (mx, my) = (p, q);
+ * each over the (2S + 1)^2 sources around (p, q), rows then columns ascending
+ * (S >= every source's shift |m - c|).  Then along each row
+ *   P = sum_b SD|b| cx(p, q - b) + sum_b SV|b| Y1(p, q - b),
+ *   Q = sum_b SD|b| Y2(p, q - b) + sum_b SV|b| Z(p, q - b),
+ * and along each column out(x, y) = sum_a SD|a| P(x - a, y) + sum_a SV|a| Q(x - a, y),
+ * taps from -r to r, indices mod L.  The same four terms as tmo_fke_movement (D x D,
+ * D x V, V x D, V x V), factored by rows and columns and summed in this fixed
+ * order.  Needs 2 max(rD, rV + S) + 1 <= L. */
+void tmo_fke_movement_sep(int L, const double* rho, const double* gamma, const double* vx, const double* vy,
+                          int rD, int rV, int S, double mu_pref, const double* SD, const double* SV, double* out);
 
 /* Movement statistics of one consumer (Rho_Mov, FKE:4034-4060): out[0] mean and
  * out[1] population variance of gamma, out[2], out[3] means of vx and vy, each

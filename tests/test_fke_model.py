@@ -2,9 +2,9 @@
 Rietkerk_FPE/rietkerk_bjork_FKE.cpp:4100-4462), built from its operators:
   * the vegetation's Dornic step (tmo_fke_veg_dornic) equals the single-species
     Rietkerk step (tmo_rk_run) with no plant growth, after that step's flush;
-  * the gather-form movement (tmo_fke_movement_gather) equals the scatter
+  * the separable movement (tmo_fke_movement_sep) equals the scatter
     restatement of advdiff_FKE_MultiSp (tmo_fke_movement) up to summation order,
-    conserves the moved mass, and does not depend on its window W;
+    conserves the moved mass, and does not depend on its shift reach S;
   * the movement statistics (tmo_fke_mov_stats) equal numpy's mean and
     population variance;
   * with no grazers, one step of the two-species model is the single-species
@@ -70,23 +70,23 @@ def test_vegetation_dornic_is_the_single_species_step_without_growth(oracle, sam
 
 @pytest.mark.parametrize("D,v,sigma,dt,dx,vscale", [(0.5, 0.9, 0.8, 0.5, 0.25, 1.0), (0.215965, 0.40587, 10.0,
                                                                                       0.11817455, 0.1, 3.0)])
-def test_gather_movement_is_the_scatter_in_another_order(oracle, D, v, sigma, dt, dx, vscale):
-    L = 40
+def test_separable_movement_is_the_scatter_in_another_order(oracle, D, v, sigma, dt, dx, vscale):
+    L = 48
     (rD, rV, pref), SD, SV = oracle.fke_move_constants(D, v, sigma, dt, dx)
     rho, gam, vx, vy = _case(7, L)
     vx, vy = vx * vscale, vy * vscale
     vmax = max(np.abs(vx).max(), np.abs(vy).max())
-    W = oracle.fke_window(rD, rV, pref, vmax)
-    assert 2 * W + 1 <= L
-    got = oracle.fke_movement_gather(rho, gam, vx, vy, rD, rV, W, pref, SD, SV)
+    S = oracle.fke_shift_reach(pref, vmax)
+    assert 2 * oracle.fke_window(rD, rV, pref, vmax) + 1 <= L and S >= 2
+    got = oracle.fke_movement_sep(rho, gam, vx, vy, rD, rV, S, pref, SD, SV)
     want = oracle.fke_movement(rho, gam, vx, vy, rD, rV, pref, SD, SV)
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
     assert got.sum() == pytest.approx(rho[rho >= 1e-10].sum(), rel=1e-12)
-    # any window at least the largest reach gives the same sums, bit for bit
-    wider = oracle.fke_movement_gather(rho, gam, vx, vy, rD, rV, W + 3, pref, SD, SV)
+    # any reach at least the largest shift gives the same sums, bit for bit
+    wider = oracle.fke_movement_sep(rho, gam, vx, vy, rD, rV, S + 2, pref, SD, SV)
     assert np.array_equal(got, wider)
-    # a window short of the reach loses mass (the check above is not vacuous)
-    short = oracle.fke_movement_gather(rho, gam, vx, vy, rD, rV, max(0, min(rD, rV) - 1), pref, SD, SV)
+    # a reach short of the shifts loses mass (the check above is not vacuous)
+    short = oracle.fke_movement_sep(rho, gam, vx, vy, rD, rV, S - 2, pref, SD, SV)
     assert short.sum() < got.sum() * (1 - 1e-6)
 
 

@@ -10,13 +10,14 @@
 // (SD, SV: the diffusion and advection half-stencils, mu_x(i) the rounded
 // advected mean) and B_i the same in columns.  Expanded, that is the
 // reference's four scatter terms (gamma^2, twice gamma - gamma^2, (1 - gamma)^2).
-// The reference scatters with atomicAdd; this kernel gathers instead: a first
-// pass forms each source's term coefficients and advected mean once, then every
-// output cell loops over the sources within reach -- those with |x - c_i| <=
-// W and |y - c_j| <= W, W = max(rD, rV + largest shift) -- and adds their
-// terms in FP64, in a fixed order.  No atomics, and the result is
-// deterministic.  The sums differ from the reference's sequential scatter only
-// in their order.
+// The reference scatters with atomicAdd.  Here the four terms are factored by
+// rows and columns instead: a first pass forms each source's term coefficients
+// and advected mean, a second sums, per cell, the coefficients of the sources
+// whose advected mean lands on it (within the largest shift), and two 1-D
+// passes apply the row and column stencils (move_disp_at, move_row_at,
+// move_col_at).  No atomics, every sum in FP64 in a fixed order, and the
+// result is deterministic; the sums differ from the reference's sequential
+// scatter only in their order.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
@@ -34,23 +35,13 @@ namespace {
 constexpr double kMoveEps = 1.0e-10;   // epsilon, rietkerk_bjork_basic.h:119
 
 struct MoveSpecies {
-    int rD, rV, W, offD, offV;
+    int rD, rV, W, S, offD, offV;   // S: the largest shift of an advected mean, plus one
     double pref;
 };
 struct MoveArgs {
     MoveSpecies sp[3];
     int L;
 };
-
-// x - m (both in [0, L)) as its representative in (-L/2, L/2]: the one offset that
-// matters while every reach stays below L/2.
-__device__ __forceinline__ int signed_gap(int x, int m, int L)
-{
-    int e = x - m;
-    if (2 * e > L) e -= L;
-    if (2 * e <= -L) e += L;
-    return e;
-}
 
 __device__ __forceinline__ int wrapi(int v, int L)
 {
@@ -99,59 +90,113 @@ __device__ __forceinline__ int wrap1(int v, int L)
     return v >= L ? v - L : v;
 }
 
-// One output cell (x, y) of one species: the sum over the sources within the
-// window W of each source's terms that reach the cell, in a fixed order (rows
-// dr = -W .. W, then columns dc = -W .. W; within a source D x D, D x V, V x D,
-// V x V).  Sources farther than their reach add nothing, so any W at least the
-// largest reach gives the same sum.
-__device__ __forceinline__ double move_gather_at(const MoveSpecies& sp, const double4* __restrict__ cs,
-                                                 const int2* __restrict__ ms, const double* __restrict__ SD,
-                                                 const double* __restrict__ SV, int L, int x, int y)
+// The separable form, in three passes over one species' lattice.  With each
+// source's terms (c.x, c.y, c.z) and advected mean (mx, my) from move_prep_at:
+//   1. move_disp_at: cx(p, q) = c.x of source (p, q) (0 if empty), and over the
+//      (2S + 1)^2 sources around (p, q), rows then columns ascending,
+//      Y1 = sum of c.y of the sources of row p with my = q,
+//      Y2 = sum of c.y of the sources of column q with mx = p,
+//      Z  = sum of c.z of the sources with (mx, my) = (p, q);
+//   2. move_row_at, along the row: P = sum_b SD|b| cx(p, q - b) + sum_b SV|b| Y1(p, q - b),
+//      Q = sum_b SD|b| Y2(p, q - b) + sum_b SV|b| Z(p, q - b);
+//   3. move_col_at, along the column: out(x, y) = sum_a SD|a| P(x - a, y) + sum_a SV|a| Q(x - a, y);
+// taps from -r to r, indices mod L.  Summed in FP64 in exactly this order, which
+// is the oracle's tmo_fke_movement_sep.  The four terms of the scatter (D x D,
+// D x V, V x D, V x V) factor by rows and columns: per output cell 2(2 rD + 1) +
+// 2(2 rV + 1) taps and (2S + 1)^2 shift tests instead of a (2W + 1)^2 window.
+__device__ __forceinline__ double4 move_disp_at(const double4* __restrict__ cs, const int2* __restrict__ ms, int L,
+                                                int S, int p, int q)
 {
 #pragma clang fp contract(off)
-    double acc = 0.0;
-    for (int dr = -sp.W; dr <= sp.W; ++dr) {
-        const int ci = wrap1(x - dr, L);
-        const int adr = dr < 0 ? -dr : dr;
-        const bool rowD = adr <= sp.rD;
-        for (int dc = -sp.W; dc <= sp.W; ++dc) {
-            const int cj = wrap1(y - dc, L);
-            const int64_t i = (int64_t)ci * L + cj;
+    const int64_t o = (int64_t)p * L + q;
+    double c0 = 0.0, a1 = 0.0, a2 = 0.0, z = 0.0;
+    if (ms[o].x >= 0) c0 = cs[o].x;
+    for (int di = -S; di <= S; ++di) {
+        const int ci = wrap1(p + di, L);
+        for (int dj = -S; dj <= S; ++dj) {
+            const int64_t i = (int64_t)ci * L + wrap1(q + dj, L);
             const int2 m = ms[i];
-            if (m.x < 0) continue;
-            const int ex = signed_gap(x, m.x, L), ey = signed_gap(y, m.y, L);
-            const int adc = dc < 0 ? -dc : dc;
-            const int aex = ex < 0 ? -ex : ex, aey = ey < 0 ? -ey : ey;
-            const bool rowV = aex <= sp.rV, colD = adc <= sp.rD, colV = aey <= sp.rV;
-            if (!((rowD || rowV) && (colD || colV))) continue;
-            const double4 c = cs[i];
-            double t = 0.0;
-            if (rowD && colD) t += c.x * (SD[adr] * SD[adc]);
-            if (rowD && colV) t += c.y * (SD[adr] * SV[aey]);
-            if (rowV && colD) t += c.y * (SV[aex] * SD[adc]);
-            if (rowV && colV) t += c.z * (SV[aex] * SV[aey]);
-            acc += t;
+            const bool hx = m.x == p, hy = m.y == q;   // an empty source has m.x = -1
+            if (!(hx || hy)) continue;
+            if (di == 0 && hy) a1 += cs[i].y;
+            if (dj == 0 && hx) a2 += cs[i].y;
+            if (hx && hy) z += cs[i].z;
         }
     }
+    return make_double4(c0, a1, a2, z);
+}
+
+__device__ __forceinline__ double2 move_row_at(const double4* __restrict__ mv, const double* __restrict__ SD,
+                                               const double* __restrict__ SV, int rD, int rV, int L, int p, int q)
+{
+#pragma clang fp contract(off)
+    const double* f = (const double*)(mv + (int64_t)p * L);
+    double P = 0.0, Q = 0.0;
+    for (int b = -rD; b <= rD; ++b) {
+        const int j = wrap1(q - b, L);
+        const double w = SD[b < 0 ? -b : b];
+        P += w * f[4 * j];
+        Q += w * f[4 * j + 2];
+    }
+    for (int b = -rV; b <= rV; ++b) {
+        const int j = wrap1(q - b, L);
+        const double w = SV[b < 0 ? -b : b];
+        P += w * f[4 * j + 1];
+        Q += w * f[4 * j + 3];
+    }
+    return make_double2(P, Q);
+}
+
+__device__ __forceinline__ double move_col_at(const double2* __restrict__ pq, const double* __restrict__ SD,
+                                              const double* __restrict__ SV, int rD, int rV, int L, int x, int y)
+{
+#pragma clang fp contract(off)
+    const double* f = (const double*)pq;
+    double acc = 0.0;
+    for (int a = -rD; a <= rD; ++a) acc += SD[a < 0 ? -a : a] * f[2 * ((int64_t)wrap1(x - a, L) * L + y)];
+    for (int a = -rV; a <= rV; ++a) acc += SV[a < 0 ? -a : a] * f[2 * ((int64_t)wrap1(x - a, L) * L + y) + 1];
     return acc;
 }
 
-__global__ __launch_bounds__(256) void fke_move_kernel(MoveArgs a, const double4* __restrict__ coef,
-                                                       const int2* __restrict__ mu,
-                                                       const double* __restrict__ SDall,
-                                                       const double* __restrict__ SVall, double* __restrict__ out)
+// The three passes of the host-array operator: one thread per (cell, species).
+__global__ __launch_bounds__(256) void fke_move_disp_kernel(MoveArgs a, const double4* __restrict__ coef,
+                                                            const int2* __restrict__ mu, double4* __restrict__ mv)
 {
-#pragma clang fp contract(off)
+    const int s = blockIdx.y;
+    const int L = a.L;
+    const int64_t L2 = (int64_t)L * L;
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= L2) return;
+    mv[s * L2 + o] = move_disp_at(coef + s * L2, mu + s * L2, L, a.sp[s].S, (int)(o / L), (int)(o % L));
+}
+
+__global__ __launch_bounds__(256) void fke_move_row_kernel(MoveArgs a, const double4* __restrict__ mv,
+                                                           const double* __restrict__ SDall,
+                                                           const double* __restrict__ SVall, double2* __restrict__ pq)
+{
     const int s = blockIdx.y;
     const int L = a.L;
     const int64_t L2 = (int64_t)L * L;
     const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (o >= L2) return;
     const MoveSpecies sp = a.sp[s];
-    out[s * L2 + o] = move_gather_at(sp, coef + s * L2, mu + s * L2, SDall + sp.offD, SVall + sp.offV, L,
-                                     (int)(o / L), (int)(o % L));
+    pq[s * L2 + o] = move_row_at(mv + s * L2, SDall + sp.offD, SVall + sp.offV, sp.rD, sp.rV, L, (int)(o / L),
+                                 (int)(o % L));
 }
 
+__global__ __launch_bounds__(256) void fke_move_col_kernel(MoveArgs a, const double2* __restrict__ pq,
+                                                           const double* __restrict__ SDall,
+                                                           const double* __restrict__ SVall, double* __restrict__ out)
+{
+    const int s = blockIdx.y;
+    const int L = a.L;
+    const int64_t L2 = (int64_t)L * L;
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= L2) return;
+    const MoveSpecies sp = a.sp[s];
+    out[s * L2 + o] = move_col_at(pq + s * L2, SDall + sp.offD, SVall + sp.offV, sp.rD, sp.rV, L, (int)(o / L),
+                                  (int)(o % L));
+}
 
 // The largest |vx|, |vy| of each species (one workgroup per species); a NaN or
 // infinite velocity gives +inf, which the host refuses.
@@ -195,6 +240,7 @@ int move_args(int ns, int L, const tm_fke_move_species* sp, const double* vmax, 
         m.rV = sp[s].rV;
         m.pref = sp[s].mu_prefactor;
         m.W = std::max(m.rD, m.rV + S);
+        m.S = S;
         m.offD = offD;
         m.offV = offV;
         if (2 * m.W + 1 > L) return TM_ERR_ARG;   // reaches must not wrap onto themselves
@@ -504,7 +550,8 @@ bool veg_dornic_ok(int L, const tm_rietkerk_params* p)
 //                          other densities carried into the output buffer
 //   fke_stage x 4          RK4 of the trophic and water terms (FKE:4233-4238),
 //                          one grid-wide pass per stage (W and O diffuse)
-//   fke_mprep / fke_mgather  the consumers' Fokker-Planck movement (FKE:4276-4309)
+//   fke_mprep, fke_move_pass x 3  the consumers' Fokker-Planck movement
+//                          (FKE:4276-4309), in separable form
 //   fke_final              checks and flush of the densities, rho(t) partials of
 //                          every field (FKE:4352-4462)
 // Every operation is the one the per-operator entry points (tm_fke_perception,
@@ -535,6 +582,8 @@ struct FkeWork {
     double* acc;
     double4* coef;             // movement terms, (SP - 1) fields at (SP - 1) cell_off
     int2* mu;
+    double4* mv;               // separable movement: (cx, Y1, Y2, Z), then (P, Q), same layout
+    double2* pq;
     double* mstat;             // [n_inst][8] means of the movement statistics
 };
 
@@ -720,8 +769,10 @@ __global__ __launch_bounds__(256) void fke_mprep_kernel(StepArgs a)
     move_prep_at(mc.ms[s - 1].pref, L, cell, f[s * N + cell], f[(SP + 1 + s) * N + cell], v, w->coef[j], w->mu[j]);
 }
 
-template <int SP>
-__global__ __launch_bounds__(256) void fke_mgather_kernel(StepArgs a)
+// The separable movement's passes (move_disp_at, move_row_at, move_col_at) on
+// each consumer's lattice; the last writes the moved density.
+template <int SP, int PASS>
+__global__ __launch_bounds__(256) void fke_move_pass_kernel(StepArgs a)
 {
     constexpr int NF = 2 * SP + 1;
     const FkeWork* w = (const FkeWork*)a.work;
@@ -733,9 +784,15 @@ __global__ __launch_bounds__(256) void fke_mgather_kernel(StepArgs a)
     const int64_t N = (int64_t)L * L;
     const MoveSpecies sp = w->mc[di.pidx].ms[s - 1];
     const int64_t base = (int64_t)(SP - 1) * di.cell_off + (s - 1) * N;
-    double* f = (double*)a.out + (int64_t)NF * di.cell_off;
-    f[s * N + cell] = move_gather_at(sp, w->coef + base, w->mu + base, w->SD + sp.offD, w->SV + sp.offV, L,
-                                     cell / L, cell % L);
+    const int p = cell / L, q = cell % L;
+    if (PASS == 0) {
+        w->mv[base + cell] = move_disp_at(w->coef + base, w->mu + base, L, sp.S, p, q);
+    } else if (PASS == 1) {
+        w->pq[base + cell] = move_row_at(w->mv + base, w->SD + sp.offD, w->SV + sp.offV, sp.rD, sp.rV, L, p, q);
+    } else {
+        double* f = (double*)a.out + (int64_t)NF * di.cell_off;
+        f[s * N + cell] = move_col_at(w->pq + base, w->SD + sp.offD, w->SV + sp.offV, sp.rD, sp.rV, L, p, q);
+    }
 }
 
 // Checks and flush of the densities (the perception fields are left as they
@@ -791,7 +848,9 @@ void launch_fke(const StepArgs& a, int grid, hipStream_t s)
     hipLaunchKernelGGL(fke_veg_kernel<SP>, blocks, dim3(256), 0, s, a);
     for (int st = 1; st <= 4; ++st) hipLaunchKernelGGL(fke_stage_kernel<SP>, blocks, dim3(256), 0, s, a, st);
     hipLaunchKernelGGL(fke_mprep_kernel<SP>, moves, dim3(256), 0, s, a);
-    hipLaunchKernelGGL(fke_mgather_kernel<SP>, moves, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((fke_move_pass_kernel<SP, 0>), moves, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((fke_move_pass_kernel<SP, 1>), moves, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((fke_move_pass_kernel<SP, 2>), moves, dim3(256), 0, s, a);
     hipLaunchKernelGGL((fke_final_kernel<SP, TX, TY, NT>), dim3(grid), dim3(NT), 0, s, a);
 }
 
@@ -888,7 +947,8 @@ const Variant kFKE3[] = {
 };
 
 // The reference's movement constants of consumer s of a point (FKE:195-250,
-// :3764-3772), its gather window W and its stencils appended to SD / SV.
+// :3764-3772), its reach W (2 W + 1 must fit the lattice), the largest shift
+// reach S and its stencils appended to SD / SV.
 int fke_move_of(const tm_fke_params& p, int s, bool drawn, std::vector<double>& SD, std::vector<double>& SV,
                 MoveSpecies* out)
 {
@@ -906,6 +966,7 @@ int fke_move_of(const tm_fke_params& p, int s, bool drawn, std::vector<double>& 
     if (!(shift < 1e9)) return TM_ERR_ARG;
     const int S = (int)std::floor(shift + 0.5) + 1;
     out->W = std::max(ms.rD, ms.rV + S);
+    out->S = S;
     out->offD = (int)SD.size();
     out->offV = (int)SV.size();
     SD.insert(SD.end(), sd, sd + ms.rD + 1);
@@ -1034,6 +1095,8 @@ int fke_setup(int model, const tm_fke_params* P, int n_params, const tm_instance
     const size_t o_acc = o;    o += up(sizeof(double) * nd);
     const size_t o_coef = o;   o += up(sizeof(double4) * nm);
     const size_t o_mu = o;     o += up(sizeof(int2) * nm);
+    const size_t o_mv = o;     o += up(sizeof(double4) * nm);
+    const size_t o_pq = o;     o += up(sizeof(double2) * nm);
     void* dev = nullptr;
     if (hipMalloc(&dev, o) != hipSuccess) {
         *err = "Rietkerk consumer tables and scratch: allocation failed";
@@ -1056,6 +1119,8 @@ int fke_setup(int model, const tm_fke_params* P, int n_params, const tm_instance
     w.acc = (double*)(base + o_acc);
     w.coef = (double4*)(base + o_coef);
     w.mu = (int2*)(base + o_mu);
+    w.mv = (double4*)(base + o_mv);
+    w.pq = (double2*)(base + o_pq);
     std::vector<char> host(o_tables, 0);
     auto put = [&](size_t off, const void* src, size_t n) { if (n) std::memcpy(host.data() + off, src, n); };
     put(0, &w, sizeof(w));
@@ -1124,15 +1189,21 @@ int tm_fke_movement(int ns, int L, const tm_fke_move_species* sp, const double* 
     int nD, nV;
     const int rc = move_args(ns, L, sp, vmax, &a, &nD, &nV);
     if (rc != TM_OK) return rc;
-    DevBuf d_coef, d_mu;
+    DevBuf d_coef, d_mu, d_mv, d_pq;
     if (hipMalloc(&d_coef.p, sizeof(double4) * (size_t)ns * L2) != hipSuccess ||
-        hipMalloc(&d_mu.p, sizeof(int2) * (size_t)ns * L2) != hipSuccess)
+        hipMalloc(&d_mu.p, sizeof(int2) * (size_t)ns * L2) != hipSuccess ||
+        hipMalloc(&d_mv.p, sizeof(double4) * (size_t)ns * L2) != hipSuccess ||
+        hipMalloc(&d_pq.p, sizeof(double2) * (size_t)ns * L2) != hipSuccess)
         return TM_ERR_NOMEM;
     const dim3 grid((unsigned)((L2 + 255) / 256), (unsigned)ns);
     hipLaunchKernelGGL(fke_move_prep_kernel, grid, dim3(256), 0, st, a, rho, gamma, (const double2*)v_eff,
                        (double4*)d_coef.p, (int2*)d_mu.p);
-    hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, st, a, (const double4*)d_coef.p, (const int2*)d_mu.p,
-                       stencil_D, stencil_V, out);
+    hipLaunchKernelGGL(fke_move_disp_kernel, grid, dim3(256), 0, st, a, (const double4*)d_coef.p,
+                       (const int2*)d_mu.p, (double4*)d_mv.p);
+    hipLaunchKernelGGL(fke_move_row_kernel, grid, dim3(256), 0, st, a, (const double4*)d_mv.p, stencil_D, stencil_V,
+                       (double2*)d_pq.p);
+    hipLaunchKernelGGL(fke_move_col_kernel, grid, dim3(256), 0, st, a, (const double2*)d_pq.p, stencil_D, stencil_V,
+                       out);
     if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
     // the scratch is released when this returns: wait for the kernels that use it
     return hipStreamSynchronize(st) == hipSuccess ? TM_OK : TM_ERR_HIP;
@@ -1158,12 +1229,13 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
     int rc = move_args(ns, L, sp, vmax, &a, &nD, &nV);
     if (rc != TM_OK) return rc;
     if (hipSetDevice(device) != hipSuccess) return TM_ERR_NODEV;
-    DevBuf d_rho, d_gam, d_v, d_out, d_sd, d_sv, d_coef, d_mu;
+    DevBuf d_rho, d_gam, d_v, d_out, d_sd, d_sv, d_coef, d_mu, d_mv, d_pq;
     const size_t fb = sizeof(double) * (size_t)ns * L2;
     if (hipMalloc(&d_rho.p, fb) != hipSuccess || hipMalloc(&d_gam.p, fb) != hipSuccess ||
         hipMalloc(&d_v.p, 2 * fb) != hipSuccess || hipMalloc(&d_out.p, fb) != hipSuccess ||
         hipMalloc(&d_sd.p, sizeof(double) * nD) != hipSuccess || hipMalloc(&d_sv.p, sizeof(double) * nV) != hipSuccess ||
-        hipMalloc(&d_coef.p, 4 * fb) != hipSuccess || hipMalloc(&d_mu.p, sizeof(int2) * (size_t)ns * L2) != hipSuccess)
+        hipMalloc(&d_coef.p, 4 * fb) != hipSuccess || hipMalloc(&d_mu.p, sizeof(int2) * (size_t)ns * L2) != hipSuccess ||
+        hipMalloc(&d_mv.p, 4 * fb) != hipSuccess || hipMalloc(&d_pq.p, 2 * fb) != hipSuccess)
         return TM_ERR_NOMEM;
     if (hipMemcpy(d_rho.p, rho, fb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_gam.p, gamma, fb, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1174,8 +1246,12 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
     const dim3 grid((unsigned)((L2 + 255) / 256), (unsigned)ns);
     hipLaunchKernelGGL(fke_move_prep_kernel, grid, dim3(256), 0, nullptr, a, (const double*)d_rho.p,
                        (const double*)d_gam.p, (const double2*)d_v.p, (double4*)d_coef.p, (int2*)d_mu.p);
-    hipLaunchKernelGGL(fke_move_kernel, grid, dim3(256), 0, nullptr, a, (const double4*)d_coef.p,
-                       (const int2*)d_mu.p, (const double*)d_sd.p, (const double*)d_sv.p, (double*)d_out.p);
+    hipLaunchKernelGGL(fke_move_disp_kernel, grid, dim3(256), 0, nullptr, a, (const double4*)d_coef.p,
+                       (const int2*)d_mu.p, (double4*)d_mv.p);
+    hipLaunchKernelGGL(fke_move_row_kernel, grid, dim3(256), 0, nullptr, a, (const double4*)d_mv.p,
+                       (const double*)d_sd.p, (const double*)d_sv.p, (double2*)d_pq.p);
+    hipLaunchKernelGGL(fke_move_col_kernel, grid, dim3(256), 0, nullptr, a, (const double2*)d_pq.p,
+                       (const double*)d_sd.p, (const double*)d_sv.p, (double*)d_out.p);
     if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
     if (hipMemcpy(out, d_out.p, fb, hipMemcpyDeviceToHost) != hipSuccess) return TM_ERR_HIP;
     return TM_OK;

@@ -410,10 +410,10 @@ int tm_fke_move_constants(double D, double v, double sigma, double dt_analytic, 
  * out [ns][L*L] receives the moved fields (it may alias rho).  Replaces the
  * reference's host copies, cudaMemset, launchAdvDiffKernel_MultiSp and copy back
  * (FKE:4276-4301, rietkerk_bjork_basic.h:461-467) and the CPU path (FKE:4303-4309).
- * The kernel gathers (each cell pulls from the cells whose stencils cover it), so
- * it needs no atomics; stencils and advection shifts must fit the lattice
- * (2 max(rD, rV + max |v| mu_prefactor + 1) + 1 <= L).  Returns 0 or a negative
- * tm_status. */
+ * The kernels factor the four terms by rows and columns and every cell pulls its
+ * own sums, so no atomics are needed; stencils and advection shifts must fit the
+ * lattice (2 max(rD, rV + max |v| mu_prefactor + 1) + 1 <= L).  Returns 0 or a
+ * negative tm_status. */
 int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* sp, const double* stencil_D,
                          const double* stencil_V, const double* rho, const double* gamma, const double* v_eff,
                          double* out);
@@ -424,7 +424,7 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
  * reference's __constant__ copies (copyToDeviceConstantMemory_AdvTerms, :20-86).
  * out is overwritten, so the reference's cudaMemset of Rho_tsar is not needed;
  * out must not alias rho.  A first small kernel finds the largest speed, which
- * the host reads back (one stream synchronisation) to size the gather window.
+ * the host reads back (one stream synchronisation) to size the shift reach.
  * Returns 0 or a negative tm_status. */
 int tm_fke_movement(int ns, int L, const tm_fke_move_species* sp, const double* stencil_D, const double* stencil_V,
                     const double* rho, const double* gamma, const double* v_eff, double* out, void* stream);

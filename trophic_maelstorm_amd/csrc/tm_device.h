@@ -30,15 +30,17 @@ struct Philox4 {
     uint32_t v0, v1, v2, v3;
 };
 
-TM_HD Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
-                            uint32_t k1)
+// Each round's two three-way xors are one v_bitop3_b32 each (truth table 0x96 =
+// a ^ b ^ c, new in gfx950); the compiler does not form them from ^ chains.
+__device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                                 uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -56,13 +58,13 @@ struct Stream {
     uint32_t b0, b1, b2, b3;
     int left;
 
-    TM_HD Stream(uint32_t seed, uint32_t inst, uint32_t cell, int64_t step)
+    __device__ Stream(uint32_t seed, uint32_t inst, uint32_t cell, int64_t step)
         : k0(seed), k1(inst), c0(cell), c1((uint32_t)(uint64_t)step),
           c2((uint32_t)((uint64_t)step >> 32)), blk(0), b0(0), b1(0), b2(0), b3(0), left(0)
     {
     }
 
-    TM_HD uint32_t word()
+    __device__ uint32_t word()
     {
         if (left == 0) {
             const Philox4 r = philox4x32_10(c0, c1, c2, blk, k0, k1);
@@ -76,7 +78,7 @@ struct Stream {
         return w;
     }
     // (w + 1/2) 2^-32: exact in double, never 0 or 1.
-    TM_HD double u64() { return ((double)word() + 0.5) * 0x1p-32; }
+    __device__ double u64() { return ((double)word() + 0.5) * 0x1p-32; }
     // FP32 uniform ((w >> 9) + 1/2) 2^-23: exact, never 0 or 1.
     __device__ float u32() { return ((float)(word() >> 9) + 0.5f) * 0x1p-23f; }
     // 1 - u from the same kind of word with full FP32 resolution near 0:

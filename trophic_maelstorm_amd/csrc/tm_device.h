@@ -51,6 +51,15 @@ __device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint3
     return Philox4{c0, c1, c2, c3};
 }
 
+// FP32 uniform ((w >> 9) + 1/2) 2^-23 of a word: exact, never 0 or 1.  Formed
+// from bits in two instructions: v_alignbit_b32 puts w >> 9 under the exponent
+// of 1.0 (the float 1 + (w >> 9) 2^-23), and subtracting 1 - 2^-24 leaves
+// (w >> 9) 2^-23 + 2^-24 exactly (Sterbenz: both operands lie in [1/2, 2]).
+__device__ __forceinline__ float u32_of(uint32_t w)
+{
+    return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, w, 9u)) - 0x1.fffffep-1f;
+}
+
 // A per-(cell, step) stream of 32-bit words, four per Philox block.  Words are
 // handed out by rotation (no dynamic register indexing).
 struct Stream {
@@ -80,10 +89,11 @@ struct Stream {
     // (w + 1/2) 2^-32: exact in double, never 0 or 1.
     __device__ double u64() { return ((double)word() + 0.5) * 0x1p-32; }
     // FP32 uniform ((w >> 9) + 1/2) 2^-23: exact, never 0 or 1.
-    __device__ float u32() { return ((float)(word() >> 9) + 0.5f) * 0x1p-23f; }
+    __device__ float u32() { return u32_of(word()); }
     // 1 - u from the same kind of word with full FP32 resolution near 0:
-    // ((2^32 - 1 - w) + 1/2) 2^-32, in (0, 1] (1 only through rounding at the top).
-    __device__ float v32() { return ((float)(~word()) + 0.5f) * 0x1p-32f; }
+    // ((2^32 - 1 - w) + 1/2) 2^-32, in (0, 1] (1 only through rounding at the top),
+    // rounded once: the fused multiply-add rounds the exact (x + 1/2) 2^-32.
+    __device__ float v32() { return __builtin_fmaf((float)(~word()), 0x1p-32f, 0x1p-33f); }
 };
 
 // The first four words of a cell's stream, computed at once: block 0 of the
@@ -106,8 +116,8 @@ struct Stream4 {
         return w;
     }
     __device__ double u64() { return ((double)word() + 0.5) * 0x1p-32; }
-    __device__ float u32() { return ((float)(word() >> 9) + 0.5f) * 0x1p-23f; }
-    __device__ float v32() { return ((float)(~word()) + 0.5f) * 0x1p-32f; }
+    __device__ float u32() { return u32_of(word()); }
+    __device__ float v32() { return __builtin_fmaf((float)(~word()), 0x1p-32f, 0x1p-33f); }
 };
 
 // ---------------------------------------------------------------------------

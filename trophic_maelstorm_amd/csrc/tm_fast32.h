@@ -201,8 +201,9 @@ __device__ __forceinline__ double poisson_fd(St& s, double zd, bool compat)
 // Gamma(shape, 1), Marsaglia-Tsang in FP32, at most TRIALS trials.  With
 // TRIALS = 64 it always returns true (64 rejections leave g = d, as the FP64
 // sampler does).  With fewer it returns false, and leaves s and g unspecified,
-// where the sampler would have drawn one more trial: the caller then starts the
-// transition again from its first word with the full sampler.
+// where the sampler would have drawn one more trial, or (TRIALS = 1) where the
+// squeeze does not settle the trial: the caller then starts the transition
+// again from its first word with the full sampler.
 template <int TRIALS, class St>
 __device__ __forceinline__ bool gamma_f_try(St& s, float shape, float& g)
 {
@@ -232,6 +233,10 @@ __device__ __forceinline__ bool gamma_f_try(St& s, float shape, float& g)
         // every d >= 2/3 (misses 6% of trials at d = 2/3, 1% at d = 4, under
         // 0.2% above d = 25), so a wave skips the logarithms more often.
         if (u * m < m - k * x2 * x2) { g = d * v; ok = true; break; }
+        // One trial (the first pass of the deferring step): a trial the squeeze
+        // leaves open is finished by the full sampler in the second pass, so
+        // no lane of the first pass waits on the logarithms.
+        if (TRIALS == 1) break;
         // x^2/2 + d(1 - v + log v); below |y| = 0.1 its series in y, which does not
         // cancel: d y^4 (-3/4 + 3/5 y - 1/2 y^2 + 3/7 y^3 - 3/8 y^4).  Both are
         // formed and one selected (v > 0 here, so the logarithm is finite).

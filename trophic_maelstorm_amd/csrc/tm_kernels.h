@@ -286,9 +286,10 @@ __device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double 
 // the Poisson and Gamma deviates from the FP32 samplers of tm_fast32.h.
 // ONE = true is the first pass of the deferring step (dp_tile_deferred): it
 // returns false, leaving n and y unspecified, where the transition needs more
-// inversion terms than the counted search holds, a second PTRS trial, a second
-// Gamma trial or a fifth word; otherwise it draws the same words and computes
-// the same value as ONE = false.
+// inversion terms than the counted search holds, a second PTRS trial, the
+// Gamma trial's logarithmic test (its squeeze failed) or a second Gamma trial,
+// or a fifth word; otherwise it draws the same words and computes the same
+// value as ONE = false.
 template <bool ONE, class St>
 __device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, const DornicConst& k, bool compat,
                                               int64_t& n, float& y)

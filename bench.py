@@ -432,7 +432,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20,
                     help="timed steps of the headline (the C4 quench from its initial field)")
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=40,
+                    help="untimed steps first (the run then restarts from its initial field); the GPU "
+                         "reaches its working clock within them")
     ap.add_argument("--c5-steps", type=int, default=200, help="steps of the 8192^2 lattice (its full run is 10^5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-side", action="store_true", help="headline and the full C4 runs only")

@@ -65,6 +65,7 @@ struct tm_ctx {
     int nslots = 0;
     DevState* d_state = nullptr;
     double* d_rows = nullptr;
+    double* d_rows_pack = nullptr;   // rows of a run packed contiguously for one host copy
     int64_t rows_cap = 0;
     int64_t* d_draws = nullptr;
     bool record_draws = false;
@@ -244,6 +245,7 @@ void release_config(tm_ctx* c)
     free_dev(c->d_partials);
     free_dev(c->d_consts);
     free_dev(c->d_rows);
+    free_dev(c->d_rows_pack);
     free_dev(c->d_draws);
     free_dev(c->d_series);
     free_dev(c->d_series_steps);
@@ -484,8 +486,12 @@ tm_status ensure_rows(tm_ctx* c, int64_t n_steps)
     if (c->rows_cap >= n_steps) return TM_OK;
     drop_graphs(c);
     free_dev(c->d_rows);
+    free_dev(c->d_rows_pack);
+    c->rows_cap = 0;
     const int64_t cap = std::max<int64_t>(n_steps, 256);
-    HIPCHK(c, hipMalloc((void**)&c->d_rows, sizeof(double) * (size_t)cap * c->inst.size() * 2 * c->nf));
+    const size_t bytes = sizeof(double) * (size_t)cap * c->inst.size() * 2 * c->nf;
+    HIPCHK(c, hipMalloc((void**)&c->d_rows, bytes));
+    HIPCHK(c, hipMalloc((void**)&c->d_rows_pack, bytes));
     c->rows_cap = cap;
     return TM_OK;
 }
@@ -1146,9 +1152,17 @@ tm_status tm_run(tm_ctx* c, int64_t n_steps, double* rows)
     if ((s = advance(c, n_steps)) != TM_OK) return s;
     HIPCHK(c, hipEventRecord(c->run_ev[1], c->stream));
     if (rows) {
+        // The run's rows are strided on the device (rows_cap steps per instance):
+        // packed there first, then one contiguous copy.  A strided copy into
+        // pageable host memory goes one instance row at a time (about 7 us each).
         const size_t w = sizeof(double) * 2 * c->nf;
-        HIPCHK(c, hipMemcpy2DAsync(rows, w * n_steps, c->d_rows, w * c->rows_cap, w * n_steps, c->inst.size(),
-                                   hipMemcpyDeviceToHost, c->stream));
+        const double* src = c->d_rows;
+        if (n_steps < c->rows_cap && c->inst.size() > 1) {
+            HIPCHK(c, hipMemcpy2DAsync(c->d_rows_pack, w * n_steps, c->d_rows, w * c->rows_cap, w * n_steps,
+                                       c->inst.size(), hipMemcpyDeviceToDevice, c->stream));
+            src = c->d_rows_pack;
+        }
+        HIPCHK(c, hipMemcpyAsync(rows, src, w * n_steps * c->inst.size(), hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;

@@ -385,7 +385,9 @@ TM_HD int64_t poisson_d(St& s, double z, bool compat)
 // shape 1; shape <= 0 gives 0 (the Dornic shape-0 case).  FP64 version, at most
 // TRIALS trials: with TRIALS = 64 it always returns true (64 rejections leave
 // g = d); with fewer it returns false, leaving s and g unspecified, where the
-// sampler would draw one more trial.
+// sampler would draw one more trial, or (TRIALS = 1) where the squeeze does not
+// settle the trial: the two software logarithms of the test are left to the
+// full sampler, which the caller runs from the transition's first word.
 template <int TRIALS, class St>
 TM_HD bool gamma_d_try(St& s, double shape, double& g)
 {
@@ -404,6 +406,7 @@ TM_HD bool gamma_d_try(St& s, double shape, double& g)
         const double u = s.u64();
         const double x2 = x * x;
         if (u < 1.0 - 0.0331 * x2 * x2) { g = d * v; ok = true; break; }
+        if (TRIALS == 1) break;
         if (dlog(u) < 0.5 * x2 + d * (1.0 - v + dlog(v))) { g = d * v; ok = true; break; }
     }
     if (TRIALS < 64 && !ok) return false;

@@ -245,7 +245,8 @@ template <> struct Arith<float> {
 
 // Dornic Poisson-Gamma transition of one live cell (rietkerk_bjork_FKE.cpp:4168-4196).
 // ONE = true: as the FP32 dornic_cell_f<true> -- false where the transition needs
-// PTRS (mean >= 12) or a second Gamma trial, otherwise the same words and value.
+// PTRS (mean >= 12), the Gamma trial's logarithmic test (its squeeze failed) or
+// a second Gamma trial, otherwise the same words and value.
 template <bool ONE, class St>
 __device__ __forceinline__ bool dornic_cell_d(St& s, double alpha_i, double cur, const DornicConst& k,
                                               bool compat, int64_t& n, double& y)

@@ -126,53 +126,66 @@ __device__ __forceinline__ double4 move_disp_at(const double4* __restrict__ cs, 
     return make_double4(c0, a1, a2, z);
 }
 
-__device__ __forceinline__ double2 move_row_at(const double4* __restrict__ mv, const double* __restrict__ SD,
-                                               const double* __restrict__ SV, int rD, int rV, int L, int p, int q)
+// mv: the four planes cx, Y1, Y2, Z of one lattice, N = L^2 apart, so that
+// neighbouring threads read neighbouring words.
+__device__ __forceinline__ double2 move_row_at(const double* __restrict__ mv, int64_t N,
+                                               const double* __restrict__ SD, const double* __restrict__ SV,
+                                               int rD, int rV, int L, int p, int q)
 {
 #pragma clang fp contract(off)
-    const double* f = (const double*)(mv + (int64_t)p * L);
+    const double* row = mv + (int64_t)p * L;
     double P = 0.0, Q = 0.0;
     for (int b = -rD; b <= rD; ++b) {
         const int j = wrap1(q - b, L);
         const double w = SD[b < 0 ? -b : b];
-        P += w * f[4 * j];
-        Q += w * f[4 * j + 2];
+        P += w * row[j];
+        Q += w * row[2 * N + j];
     }
     for (int b = -rV; b <= rV; ++b) {
         const int j = wrap1(q - b, L);
         const double w = SV[b < 0 ? -b : b];
-        P += w * f[4 * j + 1];
-        Q += w * f[4 * j + 3];
+        P += w * row[N + j];
+        Q += w * row[3 * N + j];
     }
     return make_double2(P, Q);
 }
 
-__device__ __forceinline__ double move_col_at(const double2* __restrict__ pq, const double* __restrict__ SD,
-                                              const double* __restrict__ SV, int rD, int rV, int L, int x, int y)
+// pq: the planes P and Q of one lattice, N apart.
+__device__ __forceinline__ double move_col_at(const double* __restrict__ pq, int64_t N,
+                                              const double* __restrict__ SD, const double* __restrict__ SV,
+                                              int rD, int rV, int L, int x, int y)
 {
 #pragma clang fp contract(off)
-    const double* f = (const double*)pq;
     double acc = 0.0;
-    for (int a = -rD; a <= rD; ++a) acc += SD[a < 0 ? -a : a] * f[2 * ((int64_t)wrap1(x - a, L) * L + y)];
-    for (int a = -rV; a <= rV; ++a) acc += SV[a < 0 ? -a : a] * f[2 * ((int64_t)wrap1(x - a, L) * L + y) + 1];
+    for (int a = -rD; a <= rD; ++a) acc += SD[a < 0 ? -a : a] * pq[(int64_t)wrap1(x - a, L) * L + y];
+    for (int a = -rV; a <= rV; ++a) acc += SV[a < 0 ? -a : a] * pq[N + (int64_t)wrap1(x - a, L) * L + y];
     return acc;
 }
 
+__device__ __forceinline__ void put4(double* __restrict__ planes, int64_t N, int64_t o, double4 v)
+{
+    planes[o] = v.x;
+    planes[N + o] = v.y;
+    planes[2 * N + o] = v.z;
+    planes[3 * N + o] = v.w;
+}
+
 // The three passes of the host-array operator: one thread per (cell, species).
+// mv: 4 planes per species, pq: 2 planes per species.
 __global__ __launch_bounds__(256) void fke_move_disp_kernel(MoveArgs a, const double4* __restrict__ coef,
-                                                            const int2* __restrict__ mu, double4* __restrict__ mv)
+                                                            const int2* __restrict__ mu, double* __restrict__ mv)
 {
     const int s = blockIdx.y;
     const int L = a.L;
     const int64_t L2 = (int64_t)L * L;
     const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (o >= L2) return;
-    mv[s * L2 + o] = move_disp_at(coef + s * L2, mu + s * L2, L, a.sp[s].S, (int)(o / L), (int)(o % L));
+    put4(mv + 4 * s * L2, L2, o, move_disp_at(coef + s * L2, mu + s * L2, L, a.sp[s].S, (int)(o / L), (int)(o % L)));
 }
 
-__global__ __launch_bounds__(256) void fke_move_row_kernel(MoveArgs a, const double4* __restrict__ mv,
+__global__ __launch_bounds__(256) void fke_move_row_kernel(MoveArgs a, const double* __restrict__ mv,
                                                            const double* __restrict__ SDall,
-                                                           const double* __restrict__ SVall, double2* __restrict__ pq)
+                                                           const double* __restrict__ SVall, double* __restrict__ pq)
 {
     const int s = blockIdx.y;
     const int L = a.L;
@@ -180,11 +193,13 @@ __global__ __launch_bounds__(256) void fke_move_row_kernel(MoveArgs a, const dou
     const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (o >= L2) return;
     const MoveSpecies sp = a.sp[s];
-    pq[s * L2 + o] = move_row_at(mv + s * L2, SDall + sp.offD, SVall + sp.offV, sp.rD, sp.rV, L, (int)(o / L),
-                                 (int)(o % L));
+    const double2 r = move_row_at(mv + 4 * s * L2, L2, SDall + sp.offD, SVall + sp.offV, sp.rD, sp.rV, L,
+                                  (int)(o / L), (int)(o % L));
+    pq[2 * s * L2 + o] = r.x;
+    pq[2 * s * L2 + L2 + o] = r.y;
 }
 
-__global__ __launch_bounds__(256) void fke_move_col_kernel(MoveArgs a, const double2* __restrict__ pq,
+__global__ __launch_bounds__(256) void fke_move_col_kernel(MoveArgs a, const double* __restrict__ pq,
                                                            const double* __restrict__ SDall,
                                                            const double* __restrict__ SVall, double* __restrict__ out)
 {
@@ -194,8 +209,8 @@ __global__ __launch_bounds__(256) void fke_move_col_kernel(MoveArgs a, const dou
     const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (o >= L2) return;
     const MoveSpecies sp = a.sp[s];
-    out[s * L2 + o] = move_col_at(pq + s * L2, SDall + sp.offD, SVall + sp.offV, sp.rD, sp.rV, L, (int)(o / L),
-                                  (int)(o % L));
+    out[s * L2 + o] = move_col_at(pq + 2 * s * L2, L2, SDall + sp.offD, SVall + sp.offV, sp.rD, sp.rV, L,
+                                  (int)(o / L), (int)(o % L));
 }
 
 // The largest |vx|, |vy| of each species (one workgroup per species); a NaN or
@@ -582,8 +597,8 @@ struct FkeWork {
     double* acc;
     double4* coef;             // movement terms, (SP - 1) fields at (SP - 1) cell_off
     int2* mu;
-    double4* mv;               // separable movement: (cx, Y1, Y2, Z), then (P, Q), same layout
-    double2* pq;
+    double* mv;                // separable movement: planes cx, Y1, Y2, Z of each lattice (4 per
+    double* pq;                // coef entry, at 4 x its offset), then planes P, Q (2, at 2 x)
     double* mstat;             // [n_inst][8] means of the movement statistics
 };
 
@@ -786,12 +801,14 @@ __global__ __launch_bounds__(256) void fke_move_pass_kernel(StepArgs a)
     const int64_t base = (int64_t)(SP - 1) * di.cell_off + (s - 1) * N;
     const int p = cell / L, q = cell % L;
     if (PASS == 0) {
-        w->mv[base + cell] = move_disp_at(w->coef + base, w->mu + base, L, sp.S, p, q);
+        put4(w->mv + 4 * base, N, cell, move_disp_at(w->coef + base, w->mu + base, L, sp.S, p, q));
     } else if (PASS == 1) {
-        w->pq[base + cell] = move_row_at(w->mv + base, w->SD + sp.offD, w->SV + sp.offV, sp.rD, sp.rV, L, p, q);
+        const double2 r = move_row_at(w->mv + 4 * base, N, w->SD + sp.offD, w->SV + sp.offV, sp.rD, sp.rV, L, p, q);
+        w->pq[2 * base + cell] = r.x;
+        w->pq[2 * base + N + cell] = r.y;
     } else {
         double* f = (double*)a.out + (int64_t)NF * di.cell_off;
-        f[s * N + cell] = move_col_at(w->pq + base, w->SD + sp.offD, w->SV + sp.offV, sp.rD, sp.rV, L, p, q);
+        f[s * N + cell] = move_col_at(w->pq + 2 * base, N, w->SD + sp.offD, w->SV + sp.offV, sp.rD, sp.rV, L, p, q);
     }
 }
 
@@ -1095,8 +1112,8 @@ int fke_setup(int model, const tm_fke_params* P, int n_params, const tm_instance
     const size_t o_acc = o;    o += up(sizeof(double) * nd);
     const size_t o_coef = o;   o += up(sizeof(double4) * nm);
     const size_t o_mu = o;     o += up(sizeof(int2) * nm);
-    const size_t o_mv = o;     o += up(sizeof(double4) * nm);
-    const size_t o_pq = o;     o += up(sizeof(double2) * nm);
+    const size_t o_mv = o;     o += up(sizeof(double) * 4 * nm);
+    const size_t o_pq = o;     o += up(sizeof(double) * 2 * nm);
     void* dev = nullptr;
     if (hipMalloc(&dev, o) != hipSuccess) {
         *err = "Rietkerk consumer tables and scratch: allocation failed";
@@ -1119,8 +1136,8 @@ int fke_setup(int model, const tm_fke_params* P, int n_params, const tm_instance
     w.acc = (double*)(base + o_acc);
     w.coef = (double4*)(base + o_coef);
     w.mu = (int2*)(base + o_mu);
-    w.mv = (double4*)(base + o_mv);
-    w.pq = (double2*)(base + o_pq);
+    w.mv = (double*)(base + o_mv);
+    w.pq = (double*)(base + o_pq);
     std::vector<char> host(o_tables, 0);
     auto put = [&](size_t off, const void* src, size_t n) { if (n) std::memcpy(host.data() + off, src, n); };
     put(0, &w, sizeof(w));
@@ -1192,17 +1209,17 @@ int tm_fke_movement(int ns, int L, const tm_fke_move_species* sp, const double* 
     DevBuf d_coef, d_mu, d_mv, d_pq;
     if (hipMalloc(&d_coef.p, sizeof(double4) * (size_t)ns * L2) != hipSuccess ||
         hipMalloc(&d_mu.p, sizeof(int2) * (size_t)ns * L2) != hipSuccess ||
-        hipMalloc(&d_mv.p, sizeof(double4) * (size_t)ns * L2) != hipSuccess ||
-        hipMalloc(&d_pq.p, sizeof(double2) * (size_t)ns * L2) != hipSuccess)
+        hipMalloc(&d_mv.p, sizeof(double) * 4 * (size_t)ns * L2) != hipSuccess ||
+        hipMalloc(&d_pq.p, sizeof(double) * 2 * (size_t)ns * L2) != hipSuccess)
         return TM_ERR_NOMEM;
     const dim3 grid((unsigned)((L2 + 255) / 256), (unsigned)ns);
     hipLaunchKernelGGL(fke_move_prep_kernel, grid, dim3(256), 0, st, a, rho, gamma, (const double2*)v_eff,
                        (double4*)d_coef.p, (int2*)d_mu.p);
     hipLaunchKernelGGL(fke_move_disp_kernel, grid, dim3(256), 0, st, a, (const double4*)d_coef.p,
-                       (const int2*)d_mu.p, (double4*)d_mv.p);
-    hipLaunchKernelGGL(fke_move_row_kernel, grid, dim3(256), 0, st, a, (const double4*)d_mv.p, stencil_D, stencil_V,
-                       (double2*)d_pq.p);
-    hipLaunchKernelGGL(fke_move_col_kernel, grid, dim3(256), 0, st, a, (const double2*)d_pq.p, stencil_D, stencil_V,
+                       (const int2*)d_mu.p, (double*)d_mv.p);
+    hipLaunchKernelGGL(fke_move_row_kernel, grid, dim3(256), 0, st, a, (const double*)d_mv.p, stencil_D, stencil_V,
+                       (double*)d_pq.p);
+    hipLaunchKernelGGL(fke_move_col_kernel, grid, dim3(256), 0, st, a, (const double*)d_pq.p, stencil_D, stencil_V,
                        out);
     if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
     // the scratch is released when this returns: wait for the kernels that use it
@@ -1247,10 +1264,10 @@ int tm_fke_movement_host(int device, int ns, int L, const tm_fke_move_species* s
     hipLaunchKernelGGL(fke_move_prep_kernel, grid, dim3(256), 0, nullptr, a, (const double*)d_rho.p,
                        (const double*)d_gam.p, (const double2*)d_v.p, (double4*)d_coef.p, (int2*)d_mu.p);
     hipLaunchKernelGGL(fke_move_disp_kernel, grid, dim3(256), 0, nullptr, a, (const double4*)d_coef.p,
-                       (const int2*)d_mu.p, (double4*)d_mv.p);
-    hipLaunchKernelGGL(fke_move_row_kernel, grid, dim3(256), 0, nullptr, a, (const double4*)d_mv.p,
-                       (const double*)d_sd.p, (const double*)d_sv.p, (double2*)d_pq.p);
-    hipLaunchKernelGGL(fke_move_col_kernel, grid, dim3(256), 0, nullptr, a, (const double2*)d_pq.p,
+                       (const int2*)d_mu.p, (double*)d_mv.p);
+    hipLaunchKernelGGL(fke_move_row_kernel, grid, dim3(256), 0, nullptr, a, (const double*)d_mv.p,
+                       (const double*)d_sd.p, (const double*)d_sv.p, (double*)d_pq.p);
+    hipLaunchKernelGGL(fke_move_col_kernel, grid, dim3(256), 0, nullptr, a, (const double*)d_pq.p,
                        (const double*)d_sd.p, (const double*)d_sv.p, (double*)d_out.p);
     if (hipGetLastError() != hipSuccess) return TM_ERR_HIP;
     if (hipMemcpy(out, d_out.p, fb, hipMemcpyDeviceToHost) != hipSuccess) return TM_ERR_HIP;

@@ -1,5 +1,6 @@
-"""The Rietkerk consumers' movement step on the GPU (gather form) against the
-CPU restatement of the reference's scatter: equal to summation-order rounding."""
+"""The Rietkerk consumers' movement step on the GPU (separable form) against the
+CPU restatement of the reference's scatter, equal to summation-order rounding,
+and bit for bit against the oracle's separable form (tmo_fke_movement_sep)."""
 import numpy as np
 import pytest
 
@@ -29,9 +30,14 @@ def test_movement_matches_restatement_two_species(tm, oracle):
                           [c[1] for c in consts], [c[2] for c in consts])
     for s in range(2):
         (rD, rV, pref), SD, SV = consts[s]
-        want = oracle.fke_movement(rho[s], gam[s], v[s][..., 0], v[s][..., 1], rD, rV, pref, SD, SV)
+        vx, vy = v[s][..., 0], v[s][..., 1]
+        want = oracle.fke_movement(rho[s], gam[s], vx, vy, rD, rV, pref, SD, SV)
         assert np.allclose(got[s], want, rtol=1e-12, atol=1e-15), np.max(np.abs(got[s] - want))
         assert got[s].sum() == pytest.approx(rho[s][rho[s] >= 1e-10].sum(), rel=1e-12)
+        S = oracle.fke_shift_reach(pref, max(np.abs(vx).max(), np.abs(vy).max()))
+        assert S >= 2 if s == 1 else S >= 1
+        sep = oracle.fke_movement_sep(rho[s], gam[s], vx, vy, rD, rV, S, pref, SD, SV)
+        assert np.array_equal(got[s], sep), np.max(np.abs(got[s] - sep))
 
 
 def test_movement_refuses_reaches_beyond_the_lattice(tm, oracle):

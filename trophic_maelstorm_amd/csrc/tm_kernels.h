@@ -1214,12 +1214,15 @@ struct RKConst {
     int64_t step;
 };
 
-// FP64 constants as tabled by prep_consts; rk_cast gives the stepping type's copy.
+// FP64 constants as tabled by prep_consts, with their FP32 roundings beside them
+// (the scalar unit has no FP64 -> FP32 conversion: formed once per (step,
+// instance) here rather than by every wave); rk_cast gives the stepping type's copy.
 struct RKConstD {
     DornicConst k;
     double R, cgmax, gmax, alpha, rW, K1, K2, K2W0, dw, dO, dt, dt2, dt6, eps;
     int compat, pad;
     int64_t step;
+    float fR, fcgmax, fgmax, falpha, frW, fK1, fK2, fK2W0, fdw, fdO, fdt, fdt2, fdt6, feps;
 };
 
 __device__ __forceinline__ RKConstD rk_constants(const tm_rietkerk_params& p, int64_t step)
@@ -1245,6 +1248,10 @@ __device__ __forceinline__ RKConstD rk_constants(const tm_rietkerk_params& p, in
     c.compat = p.sampler == TM_SAMPLER_COMPAT;
     c.pad = 0;
     c.step = step;
+    c.fR = (float)c.R; c.fcgmax = (float)c.cgmax; c.fgmax = (float)c.gmax; c.falpha = (float)c.alpha;
+    c.frW = (float)c.rW; c.fK1 = (float)c.K1; c.fK2 = (float)c.K2; c.fK2W0 = (float)c.K2W0;
+    c.fdw = (float)c.dw; c.fdO = (float)c.dO; c.fdt = (float)c.dt; c.fdt2 = (float)c.dt2;
+    c.fdt6 = (float)c.dt6; c.feps = (float)c.eps;
     return c;
 }
 
@@ -1253,9 +1260,22 @@ __device__ __forceinline__ RKConst<T> rk_cast(const RKConstD& d)
 {
     RKConst<T> c;
     c.k = d.k;
-    c.R = (T)d.R; c.cgmax = (T)d.cgmax; c.gmax = (T)d.gmax; c.alpha = (T)d.alpha; c.rW = (T)d.rW;
-    c.K1 = (T)d.K1; c.K2 = (T)d.K2; c.K2W0 = (T)d.K2W0; c.dw = (T)d.dw; c.dO = (T)d.dO;
-    c.dt = (T)d.dt; c.dt2 = (T)d.dt2; c.dt6 = (T)d.dt6; c.eps = (T)d.eps;
+    c.R = d.R; c.cgmax = d.cgmax; c.gmax = d.gmax; c.alpha = d.alpha; c.rW = d.rW;
+    c.K1 = d.K1; c.K2 = d.K2; c.K2W0 = d.K2W0; c.dw = d.dw; c.dO = d.dO;
+    c.dt = d.dt; c.dt2 = d.dt2; c.dt6 = d.dt6; c.eps = d.eps;
+    c.compat = d.compat;
+    c.step = d.step;
+    return c;
+}
+
+template <>
+__device__ __forceinline__ RKConst<float> rk_cast<float>(const RKConstD& d)
+{
+    RKConst<float> c;
+    c.k = d.k;
+    c.R = d.fR; c.cgmax = d.fcgmax; c.gmax = d.fgmax; c.alpha = d.falpha; c.rW = d.frW;
+    c.K1 = d.fK1; c.K2 = d.fK2; c.K2W0 = d.fK2W0; c.dw = d.fdw; c.dO = d.fdO;
+    c.dt = d.fdt; c.dt2 = d.fdt2; c.dt6 = d.fdt6; c.eps = d.feps;
     c.compat = d.compat;
     c.step = d.step;
     return c;
@@ -1291,8 +1311,9 @@ __device__ __forceinline__ void rk_rhs<float>(const RKConst<float>& c, float P, 
 }
 
 // Ring of radius r around a TX x TY tile: index k -> tile-relative (ly, lx).
+// constexpr: also evaluated at compile time for the ring offset tables below.
 template <int TX, int TY, int R>
-__device__ __forceinline__ void ring_cell(int k, int& ly, int& lx)
+constexpr void ring_cell(int k, int& ly, int& lx)
 {
     constexpr int W = TX + 2 * R;
     if (k < R * W) {
@@ -1310,10 +1331,68 @@ __device__ __forceinline__ void ring_cell(int k, int& ly, int& lx)
     }
 }
 
+// (W, O) of one cell, stored together so that one 2-element LDS read fetches both.
+template <typename T>
+struct alignas(2 * sizeof(T)) WOPair {
+    T w, o;
+};
+
 template <int TX, int TY, int R>
 struct Ring {
     static constexpr int N = (TX + 2 * R) * (TY + 2 * R) - TX * TY;
 };
+
+// The ring cells each thread of an NT-thread workgroup of rk_step_kernel owns,
+// as LDS offsets formed at compile time.  Entry e of thread t's row is
+// (i4 << 16) | i3 for its e-th ring cell -- the radius-3 ring's entries t + j NT
+// (j < NR3), then the radius-2 ring's and the radius-1 ring's -- with i4 / i3
+// the cell's index in the radius-4 / radius-3 frame, and 0xFFFF0000 past a
+// ring's end (i4 reads as -1 after an arithmetic shift).  A thread reads its row
+// with one 16-byte load instead of forming the offsets from divisions and
+// selects (about a hundred instructions per thread).
+template <int TX, int TY, int NT>
+struct RingOffsets {
+    static constexpr int NR3 = (Ring<TX, TY, 3>::N + NT - 1) / NT;
+    static constexpr int NR2 = (Ring<TX, TY, 2>::N + NT - 1) / NT;
+    static constexpr int NR1 = (Ring<TX, TY, 1>::N + NT - 1) / NT;
+    static constexpr int NE = NR3 + NR2 + NR1;
+    static constexpr int NE4 = (NE + 3) / 4 * 4;
+    static_assert((TY + 8) * (TX + 8) < 0x8000, "frame indices fit 15 bits");
+    struct alignas(16) Row { uint32_t v[NE4]; };
+    Row row[NT];
+};
+
+template <int TX, int TY, int R, int NT, int NR>
+constexpr void ring_offsets_fill(RingOffsets<TX, TY, NT>& t, int e0)
+{
+    for (int tid = 0; tid < NT; ++tid) {
+        for (int j = 0; j < NR; ++j) {
+            const int k = tid + j * NT;
+            uint32_t v = 0xFFFF0000u;
+            if (k < Ring<TX, TY, R>::N) {
+                int ly = 0, lx = 0;
+                ring_cell<TX, TY, R>(k, ly, lx);
+                const int i4 = (ly + 4) * (TX + 8) + (lx + 4), i3 = (ly + 3) * (TX + 6) + (lx + 3);
+                v = ((uint32_t)i4 << 16) | (uint32_t)i3;
+            }
+            t.row[tid].v[e0 + j] = v;
+        }
+    }
+}
+
+template <int TX, int TY, int NT>
+constexpr RingOffsets<TX, TY, NT> make_ring_offsets()
+{
+    RingOffsets<TX, TY, NT> t{};
+    using RO = RingOffsets<TX, TY, NT>;
+    ring_offsets_fill<TX, TY, 3, NT, RO::NR3>(t, 0);
+    ring_offsets_fill<TX, TY, 2, NT, RO::NR2>(t, RO::NR3);
+    ring_offsets_fill<TX, TY, 1, NT, RO::NR1>(t, RO::NR3 + RO::NR2);
+    return t;
+}
+
+template <int TX, int TY, int NT>
+__device__ const RingOffsets<TX, TY, NT> kRingOffsets = make_ring_offsets<TX, TY, NT>();
 
 // Fixed-order block reduction of three (FP64 sum, count) pairs, each wave's
 // sums through wave_sum4 and the counts already wave totals (ballots); threads
@@ -1359,14 +1438,21 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
     constexpr int NR1 = (Ring<TX, TY, 1>::N + NT - 1) / NT;
     static_assert(LdsFrame<T, 4>::HC == 4, "radius-4 frames are W4 = TX + 8 wide");
     constexpr int NBUF4 = ((H4 * W4 * (int)sizeof(T) / 4 + 4 * NT - 1) / (4 * NT)) * 4 * NT;
-    // radius-4 frames of P (P* for the split step), W and O, staged by DMA
-    __shared__ uint32_t s_fr[3][NBUF4];
-    __shared__ T sPs[SPLIT ? 1 : H3 * W3];   // fused step: P* on the radius-3 frame
-    __shared__ T sS[3][H3 * W3];
-    __shared__ double s_red[6 * (NT / 64)];
-    T* const sP0 = (T*)s_fr[0];
-    T* const sW0 = (T*)s_fr[1];
-    T* const sO0 = (T*)s_fr[2];
+    // One LDS object, so that its members sit at fixed addresses: the (W, O)
+    // stage states first, at address 0, where a cell and its four neighbours are
+    // read at constant offsets (all under 256 pairs) from one base -- two paired
+    // reads and one single in FP32, no address arithmetic per read.
+    struct Lds {
+        WOPair<T> wo[H3 * W3];              // stage states W, O on the radius-3 frame
+        T p[H3 * W3];                       // stage state P
+        T ps[SPLIT ? 1 : H3 * W3];          // fused step: P* on the radius-3 frame
+        alignas(16) uint32_t fr[3][NBUF4];  // radius-4 frames of P (P* for the split step), W, O (DMA)
+        double red[6 * (NT / 64)];
+    };
+    __shared__ Lds sm;
+    T* const sP0 = (T*)sm.fr[0];
+    T* const sW0 = (T*)sm.fr[1];
+    T* const sO0 = (T*)sm.fr[2];
 
     const int64_t counter = a.state->counter + a.k;
     const int64_t slot = counter & (int64_t)(a.nslots - 1);   // nslots: a power of two
@@ -1398,13 +1484,13 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
     const T* p_src = SPLIT ? (const T*)a.aux + tl.cell_off : fin;
     {
         const T* const src[3] = {p_src, fin + N, fin + 2 * N};
-        uint32_t* const dst[3] = {s_fr[0], s_fr[1], s_fr[2]};
+        uint32_t* const dst[3] = {sm.fr[0], sm.fr[1], sm.fr[2]};
         tiles_to_lds<T, TX, H4, NT, 4, 3>(src, L, tl.x0, tl.y0, dst);
     }
     const RKConst<T> c = rk_cast<T>(cd);
 
     // LDS offsets (radius-4 frame, radius-3 frame) of this thread's cells; ring
-    // entries past the ring's end hold -1
+    // entries past the ring's end hold -1 (from the compile-time table)
     int o4[NOWN], o3[NOWN], q4[NR3], q3[NR3], p4[NR2], p3[NR2], m4[NR1], m3[NR1];
 #pragma unroll
     for (int j = 0; j < NOWN; ++j) {
@@ -1413,31 +1499,23 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         o4[j] = i4(ly, lx);
         o3[j] = i3(ly, lx);
     }
+    {
+        using RO = RingOffsets<TX, TY, NT>;
+        static_assert(RO::NR3 == NR3 && RO::NR2 == NR2 && RO::NR1 == NR1, "ring table shape");
+        const typename RO::Row row = kRingOffsets<TX, TY, NT>.row[tid];
+        auto hi = [](uint32_t v) { return (int)v >> 16; };
+        auto lo = [](uint32_t v) { return (int)(v & 0xFFFFu); };
 #pragma unroll
-    for (int j = 0; j < NR3; ++j) {
-        const int k = tid + j * NT;
-        int ly = 0, lx = 0;
-        ring_cell<TX, TY, 3>(k < Ring<TX, TY, 3>::N ? k : 0, ly, lx);
-        q4[j] = k < Ring<TX, TY, 3>::N ? i4(ly, lx) : -1;
-        q3[j] = i3(ly, lx);
-    }
+        for (int j = 0; j < NR3; ++j) { q4[j] = hi(row.v[j]); q3[j] = lo(row.v[j]); }
 #pragma unroll
-    for (int j = 0; j < NR2; ++j) {
-        const int k = tid + j * NT;
-        int ly = 0, lx = 0;
-        ring_cell<TX, TY, 2>(k < Ring<TX, TY, 2>::N ? k : 0, ly, lx);
-        p4[j] = k < Ring<TX, TY, 2>::N ? i4(ly, lx) : -1;
-        p3[j] = i3(ly, lx);
-    }
+        for (int j = 0; j < NR2; ++j) { p4[j] = hi(row.v[NR3 + j]); p3[j] = lo(row.v[NR3 + j]); }
 #pragma unroll
-    for (int j = 0; j < NR1; ++j) {
-        const int k = tid + j * NT;
-        int ly = 0, lx = 0;
-        ring_cell<TX, TY, 1>(k < Ring<TX, TY, 1>::N ? k : 0, ly, lx);
-        m4[j] = k < Ring<TX, TY, 1>::N ? i4(ly, lx) : -1;
-        m3[j] = i3(ly, lx);
+        for (int j = 0; j < NR1; ++j) {
+            m4[j] = hi(row.v[NR3 + NR2 + j]);
+            m3[j] = lo(row.v[NR3 + NR2 + j]);
+        }
     }
-    auto ps = [&](int r4, int r3) { return SPLIT ? sP0[r4] : sPs[r3]; };   // P*
+    auto ps = [&](int r4, int r3) { return SPLIT ? sP0[r4] : sm.ps[r3]; };   // P*
 
     __builtin_amdgcn_s_waitcnt(0);   // this thread's copies have landed
     __syncthreads();                 // ... everyone's
@@ -1451,7 +1529,7 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
             const T y = DornicLive<T>::step(di.seed, di.inst_id, (uint32_t)((int64_t)gy * L + gx), c.step, sP0[r],
                                             sP0[r - W4], sP0[r + W4], sP0[r - 1], sP0[r + 1], c.k, c.compat != 0, n);
             *n_out = n;
-            sPs[i3(ly, lx)] = y;
+            sm.ps[i3(ly, lx)] = y;
         };
         for (int j = 0; j < NOWN; ++j) {
             int ly, lx;
@@ -1474,34 +1552,44 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
     }
 
     // ---- stage 1: K1 from (P*, W, O) on the radius-3 frame ----
-    T accP[NOWN], accW[NOWN], accO[NOWN];
-    auto stage1 = [&](int r4, int r3, T& kP, T& kW, T& kO) {
-        const T P = ps(r4, r3), W = sW0[r4], O = sO0[r4];
+    // The tile's own y values are read here once and kept in registers.
+    T accP[NOWN], accW[NOWN], accO[NOWN], yP[NOWN], yW[NOWN], yO[NOWN];
+    auto put = [&](int r3, T P, T W, T O) {
+        sm.wo[r3] = WOPair<T>{W, O};
+        sm.p[r3] = P;
+    };
+    auto stage1 = [&](int r4, int r3, T& P, T& W, T& O, T& kP, T& kW, T& kO) {
+        P = ps(r4, r3);
+        W = sW0[r4];
+        O = sO0[r4];
         const T nW = sW0[r4 - W4] + sW0[r4 + W4] + sW0[r4 - 1] + sW0[r4 + 1];
         const T nO = sO0[r4 - W4] + sO0[r4 + W4] + sO0[r4 - 1] + sO0[r4 + 1];
         rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
-        sS[0][r3] = P + c.dt2 * kP;
-        sS[1][r3] = W + c.dt2 * kW;
-        sS[2][r3] = O + c.dt2 * kO;
+        put(r3, P + c.dt2 * kP, W + c.dt2 * kW, O + c.dt2 * kO);
     };
 #pragma unroll
-    for (int j = 0; j < NOWN; ++j) stage1(o4[j], o3[j], accP[j], accW[j], accO[j]);
+    for (int j = 0; j < NOWN; ++j) stage1(o4[j], o3[j], yP[j], yW[j], yO[j], accP[j], accW[j], accO[j]);
 #pragma unroll
     for (int j = 0; j < NR3; ++j) {
         if (q4[j] >= 0) {
-            T kP, kW, kO;
-            stage1(q4[j], q3[j], kP, kW, kO);
+            T P, W, O, kP, kW, kO;
+            stage1(q4[j], q3[j], P, W, O, kP, kW, kO);
         }
     }
     __syncthreads();
 
-    // ---- stages 2 and 3: K from the stage state in sS, new state held in
-    //      registers until every thread has read the old one ----
+    // ---- stages 2 and 3: K from the stage state, new state held in registers
+    //      until every thread has read the old one.  The cells evaluated here lie
+    //      within radius 2, so b = r - W3 - 1 >= 0 and the reads are at b + 1 (up),
+    //      b + W3 (left), b + W3 + 1 (the cell), b + W3 + 2 (right), b + 2 W3 + 1
+    //      (down); sums in the order up, down, left, right ----
     auto kstage = [&](int r, T& kP, T& kW, T& kO) {
-        const T P = sS[0][r], W = sS[1][r], O = sS[2][r];
-        const T nW = sS[1][r - W3] + sS[1][r + W3] + sS[1][r - 1] + sS[1][r + 1];
-        const T nO = sS[2][r - W3] + sS[2][r + W3] + sS[2][r - 1] + sS[2][r + 1];
-        rk_rhs<T>(c, P, W, O, nW, nO, kP, kW, kO);
+        const WOPair<T>* bp = sm.wo + (r - W3 - 1);
+        const WOPair<T> up = bp[1], lf = bp[W3], ce = bp[W3 + 1], rt = bp[W3 + 2], dn = bp[2 * W3 + 1];
+        const T P = sm.p[r];
+        const T nW = up.w + dn.w + lf.w + rt.w;
+        const T nO = up.o + dn.o + lf.o + rt.o;
+        rk_rhs<T>(c, P, ce.w, ce.o, nW, nO, kP, kW, kO);
     };
     {   // stage 2: K2 on radius 2, ys3 = y + dt/2 K2
         T nP[NOWN], nW[NOWN], nO[NOWN];
@@ -1510,13 +1598,12 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         for (int j = 0; j < NOWN; ++j) {
             T kP, kW, kO;
             kstage(o3[j], kP, kW, kO);
-            const T P = ps(o4[j], o3[j]), W = sW0[o4[j]], O = sO0[o4[j]];
             accP[j] = accP[j] + (T)2.0 * kP;
             accW[j] = accW[j] + (T)2.0 * kW;
             accO[j] = accO[j] + (T)2.0 * kO;
-            nP[j] = P + c.dt2 * kP;
-            nW[j] = W + c.dt2 * kW;
-            nO[j] = O + c.dt2 * kO;
+            nP[j] = yP[j] + c.dt2 * kP;
+            nW[j] = yW[j] + c.dt2 * kW;
+            nO[j] = yO[j] + c.dt2 * kO;
         }
 #pragma unroll
         for (int j = 0; j < NR2; ++j) {
@@ -1531,14 +1618,10 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NOWN; ++j) {
-            sS[0][o3[j]] = nP[j]; sS[1][o3[j]] = nW[j]; sS[2][o3[j]] = nO[j];
-        }
+        for (int j = 0; j < NOWN; ++j) put(o3[j], nP[j], nW[j], nO[j]);
 #pragma unroll
         for (int j = 0; j < NR2; ++j) {
-            if (p4[j] >= 0) {
-                sS[0][p3[j]] = rP[j]; sS[1][p3[j]] = rW[j]; sS[2][p3[j]] = rO[j];
-            }
+            if (p4[j] >= 0) put(p3[j], rP[j], rW[j], rO[j]);
         }
         __syncthreads();
     }
@@ -1549,13 +1632,12 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         for (int j = 0; j < NOWN; ++j) {
             T kP, kW, kO;
             kstage(o3[j], kP, kW, kO);
-            const T P = ps(o4[j], o3[j]), W = sW0[o4[j]], O = sO0[o4[j]];
             accP[j] = accP[j] + (T)2.0 * kP;
             accW[j] = accW[j] + (T)2.0 * kW;
             accO[j] = accO[j] + (T)2.0 * kO;
-            nP[j] = P + c.dt * kP;
-            nW[j] = W + c.dt * kW;
-            nO[j] = O + c.dt * kO;
+            nP[j] = yP[j] + c.dt * kP;
+            nW[j] = yW[j] + c.dt * kW;
+            nO[j] = yO[j] + c.dt * kO;
         }
 #pragma unroll
         for (int j = 0; j < NR1; ++j) {
@@ -1570,14 +1652,10 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NOWN; ++j) {
-            sS[0][o3[j]] = nP[j]; sS[1][o3[j]] = nW[j]; sS[2][o3[j]] = nO[j];
-        }
+        for (int j = 0; j < NOWN; ++j) put(o3[j], nP[j], nW[j], nO[j]);
 #pragma unroll
         for (int j = 0; j < NR1; ++j) {
-            if (m4[j] >= 0) {
-                sS[0][m3[j]] = rP[j]; sS[1][m3[j]] = rW[j]; sS[2][m3[j]] = rO[j];
-            }
+            if (m4[j] >= 0) put(m3[j], rP[j], rW[j], rO[j]);
         }
         __syncthreads();
     }
@@ -1591,11 +1669,10 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         own(j, ly, lx);
         T kP, kW, kO;
         kstage(o3[j], kP, kW, kO);
-        const T P = ps(o4[j], o3[j]), W = sW0[o4[j]], O = sO0[o4[j]];
         T y[3];
-        y[0] = P + c.dt6 * (accP[j] + kP);
-        y[1] = W + c.dt6 * (accW[j] + kW);
-        y[2] = O + c.dt6 * (accO[j] + kO);
+        y[0] = yP[j] + c.dt6 * (accP[j] + kP);
+        y[1] = yW[j] + c.dt6 * (accW[j] + kW);
+        y[2] = yO[j] + c.dt6 * (accO[j] + kO);
         const int gy = tl.y0 + ly, gx = tl.x0 + lx;
         const bool in = gy < L && gx < L;
         const int64_t cell = (int64_t)gy * L + gx;
@@ -1609,11 +1686,11 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
                 fout[f * N + cell] = v;
                 sum[f] += (double)v;
             }
-            wcnt[f] += (uint32_t)__popcll(__ballot(in && v > (T)0));
+            wcnt[f] += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in && v > (T)0));
         }
     }
     double2* dst = a.partials + (slot * a.ntiles + tile) * 3;
-    block_partials3<NT>(sum, wcnt, dst, s_red);
+    block_partials3<NT>(sum, wcnt, dst, sm.red);
 }
 
 // ---------------------------------------------------------------------------

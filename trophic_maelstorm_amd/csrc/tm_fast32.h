@@ -146,11 +146,21 @@ __device__ __forceinline__ PtrsF ptrs_setup(double zd)
     return p;
 }
 
+// A trial's candidate k = floor((2a/us + b) U + z + 0.43), us = 0.5 - |U|, and
+// its fast acceptance (Hormann's box, us >= 0.07 and V <= v_r), which needs no
+// logarithm.
+__device__ __forceinline__ double ptrs_k(const PtrsF& p, float U, float us)
+{
+    return __builtin_floor((double)((2.0f * p.a * rcpf(us) + p.b) * U) + p.zd + 0.43);
+}
+
+__device__ __forceinline__ bool ptrs_in_box(const PtrsF& p, float us, float V) { return us >= 0.07f && V <= p.vr; }
+
 __device__ __forceinline__ bool ptrs_trial(const PtrsF& p, float U, float V, double& kd)
 {
     const float us = 0.5f - __builtin_fabsf(U);
-    kd = __builtin_floor((double)((2.0f * p.a * rcpf(us) + p.b) * U) + p.zd + 0.43);
-    if (us >= 0.07f && V <= p.vr) return true;
+    kd = ptrs_k(p, U, us);
+    if (ptrs_in_box(p, us, V)) return true;
     if (kd < 0.0 || (us < 0.013f && V > us)) return false;
     const float k = (float)kd, z = p.z;
     // log p(k; z) = -stirlerr(k) - bd0(k, z) - log(2 pi k)/2, bd0 = k log(k/z) + z - k

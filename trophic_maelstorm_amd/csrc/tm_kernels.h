@@ -287,10 +287,10 @@ __device__ __forceinline__ double dornic_cell(Stream& s, double alpha_i, double 
 // the Poisson and Gamma deviates from the FP32 samplers of tm_fast32.h.
 // ONE = true is the first pass of the deferring step (dp_tile_deferred): it
 // returns false, leaving n and y unspecified, where the transition needs more
-// inversion terms than the counted search holds, a second PTRS trial, the
-// Gamma trial's logarithmic test (its squeeze failed) or a second Gamma trial,
-// or a fifth word; otherwise it draws the same words and computes the same
-// value as ONE = false.
+// inversion terms than the counted search holds, the PTRS trial's logarithmic
+// test (outside its box) or a second PTRS trial, the Gamma trial's logarithmic
+// test (its squeeze failed) or a second Gamma trial, or a fifth word; otherwise
+// it draws the same words and computes the same value as ONE = false.
 template <bool ONE, class St>
 __device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, const DornicConst& k, bool compat,
                                               int64_t& n, float& y)
@@ -311,11 +311,15 @@ __device__ __forceinline__ bool dornic_cell_f(St& s, float shape0, double zd, co
             n = ni;
             nf = (float)ni;
         } else {
+            // one PTRS trial, settled by its box alone: a trial outside it (about a
+            // fifth at large means) is finished by the full sampler in the second
+            // pass, so no lane of the first pass waits on its logarithms
             const PtrsF pt = ptrs_setup(zd);
             const float U = s.u32() - 0.5f;
             const float V = s.u32();
-            double kd;
-            if (!ptrs_trial(pt, U, V, kd)) return false;
+            const float us = 0.5f - __builtin_fabsf(U);
+            if (!ptrs_in_box(pt, us, V)) return false;
+            const double kd = ptrs_k(pt, U, us);
             n = (int64_t)kd;
             nf = (float)kd;
             big = true;

@@ -84,8 +84,9 @@ static const Variant kRK[] = {
      &launch_dp<float, 128, 32, 256, 1>},
     {32, 16, 256, 0, 1, "rk_f32_split_32x16_t256_w6", &launch_rk<float, 32, 16, 256, true>, 128, 32,
      &launch_dp<float, 128, 32, 256, 1, 6>},
-    // _q: the Dornic kernel in two passes (one PTRS and one Gamma trial first;
-    // cells needing more go to a queue in LDS, dp_tile_deferred)
+    // _q: the Dornic kernel in two passes (one PTRS trial settled by its box and
+    // one Gamma trial by its squeeze first; cells needing more go to a queue in
+    // LDS, dp_tile_deferred)
     {32, 16, 256, 0, 1, "rk_f32_split_32x16_t256_q", &launch_rk<float, 32, 16, 256, true>, 128, 32,
      &launch_dp<float, 128, 32, 256, 1, 1, 1>},
     {32, 16, 256, 0, 1, "rk_f32_split_32x16_t256_q24", &launch_rk<float, 32, 16, 256, true>, 128, 24,

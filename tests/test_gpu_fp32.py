@@ -192,6 +192,36 @@ def test_fp32_deferring_step_is_identical(tm, hi, sampler):
     np.testing.assert_allclose(r_a, r_b, rtol=1e-12)
 
 
+@pytest.mark.parametrize("prec, one_pass, two_pass, sampler", [
+    ("F32", "dp_f32_128x32_t256_w6", "dp_f32_128x24_t256_w6q", 1),
+    ("F64", "dp_f64_64x32_t256", "dp_f64_64x32_t256_x8aq_w6", 0),
+])
+def test_deferring_step_with_every_cell_deferred(tm, prec, one_pass, two_pass, sampler):
+    """A full queue: with densities in [20, 30) every Poisson mean is above 100
+    (about 8.5 per unit density here), so the first pass defers every cell -- FP64
+    every PTRS mean, FP32 under the compat sampler every mean above 100 -- and the
+    queue holds the whole tile.  The field, the draws and the rho(t) rows are
+    still the one-pass step's."""
+    P = getattr(tm, prec)
+    names = tm.variant_names(tm.MODEL_DP, P)
+    insts = [tm.Instance(L, 0, 9, k) for k, L in enumerate([96, 130, 64, 257])]
+    out = {}
+    for want in (one_pass, two_pass):
+        with tm.Context() as ctx:
+            ctx.configure(tm.MODEL_DP, P, tm.DPParams(a=0.9, scheme=tm.DP_EXACT, sampler=sampler), insts)
+            ctx.set_launch(names.index(want), 1)
+            ctx.init_uniform([20.0], [30.0])
+            ctx.record_draws(True)
+            rows = ctx.run(1)
+            out[want] = (np.concatenate([ctx.download(k).ravel() for k in range(len(insts))]),
+                         np.concatenate([ctx.draws(k).ravel() for k in range(len(insts))]), rows)
+    (f_a, d_a, r_a), (f_b, d_b, r_b) = out.values()
+    assert (d_a >= 100).mean() > 0.99            # every cell sampled, nearly all at means above 100
+    assert np.array_equal(f_a, f_b)
+    assert np.array_equal(d_a, d_b)
+    assert np.array_equal(r_a, r_b)
+
+
 @pytest.mark.parametrize("variant", ["dp_f32_128x32_t256_w6", "dp_f32_128x24_t256_w6q"])
 def test_fp32_numeric_fault_is_reported(tm, variant):
     """The first fault (instance, cell, step) is reported by the FP32 step as by the

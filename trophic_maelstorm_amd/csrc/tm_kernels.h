@@ -510,8 +510,7 @@ __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0,
 #pragma unroll 1
         for (int j = 0; j < NLD; ++j) {
             const int q = j * NT + tid;
-            uint32_t off = 0;
-            if (q < NC) {
+            if (q < NC) {   // lanes past the image copy nothing, so the buffer can end with it
                 const int ry = q / CPR, cx = q - (q / CPR) * CPR;
                 int gy = y0 + ry - H, gx = x0 - HC + cx * V;
                 gy = gy < 0 ? gy + L : (gy >= L ? gy - L : gy);   // one wrap: L >= RH, L >= RWV
@@ -520,9 +519,9 @@ __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0,
                     gy = wrap(y0 + ry - H, L);
                     gx = wrap(x0 - HC + cx * V, L);
                 }
-                off = (uint32_t)gy * (uint32_t)L + (uint32_t)gx;
+                const uint32_t off = (uint32_t)gy * (uint32_t)L + (uint32_t)gx;
+                __builtin_amdgcn_global_load_lds(fin + off, lds + 4 * (j * NT + wave_base), 16, 0, 0);
             }
-            __builtin_amdgcn_global_load_lds(fin + off, lds + 4 * (j * NT + wave_base), 16, 0, 0);
         }
     } else {
         constexpr int DW = sizeof(T) / 4;             // dwords per element
@@ -532,14 +531,13 @@ __device__ __forceinline__ void tile_to_lds(const T* fin, int L, int x0, int y0,
 #pragma unroll 1
         for (int j = 0; j < NLD; ++j) {
             const int q = j * NT + tid;
-            uint32_t off = 0;
             if (q < NQ) {
                 const int e = q / DW, w = q - (q / DW) * DW;
                 const int ry = e / RWV, rx = e - (e / RWV) * RWV;
                 const int gy = wrap(y0 + ry - H, L), gx = wrap(x0 - HC + rx, L);
-                off = ((uint32_t)gy * (uint32_t)L + (uint32_t)gx) * DW + w;
+                const uint32_t off = ((uint32_t)gy * (uint32_t)L + (uint32_t)gx) * DW + w;
+                __builtin_amdgcn_global_load_lds(src0 + off, lds + j * NT + wave_base, 4, 0, 0);
             }
-            __builtin_amdgcn_global_load_lds(src0 + off, lds + j * NT + wave_base, 4, 0, 0);
         }
     }
 #endif
@@ -630,7 +628,11 @@ struct DPShape {
     static constexpr int G = TY / R4;                        // groups per thread per tile
     static constexpr int NW = NT / 64;                       // waves = partials per tile
     static constexpr int NQ = RH * RWV * (int)(sizeof(T) / 4);
-    static constexpr int NBUF = ((NQ + 4 * NT - 1) / (4 * NT)) * 4 * NT;   // dwords per image
+    // dwords per image: the image itself (tile_to_lds copies nothing past it).
+    // Rounded up to whole DMA rounds it cost the two-pass 128 x 24 FP32 kernel a
+    // sixth workgroup per CU (28.7 KB of LDS against 26.5 KB).
+    static constexpr int NBUF = NQ;
+    static_assert(NQ % 4 == 0, "whole 16-byte chunks");
     static_assert(NT % CG == 0 && TY % R4 == 0 && TX % 64 == 0, "tile shape");
 };
 

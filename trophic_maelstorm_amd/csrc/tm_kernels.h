@@ -1684,14 +1684,16 @@ __global__ __launch_bounds__(NT) void rk_step_kernel(StepArgs a)
         const int64_t cell = (int64_t)gy * L + gx;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
-            T v = y[f];
+            // ok fails for negative, NaN or infinite values (finite >= 0 values pass
+            // both comparisons); a flushed value (0 <= v < eps) is ok.  Formed in
+            // every lane, so the ballot below needs no value carried out of a branch.
+            const bool ok = y[f] >= (T)0 && y[f] <= Arith<T>::largest();
+            const T v = ok && y[f] < c.eps ? (T)0 : y[f];
             if (in) {
-                // negative, NaN or infinite (finite >= 0 values pass both comparisons)
-                if (!(v >= (T)0 && v <= Arith<T>::largest())) record_fault(a.state, tl.inst, f * N + cell, counter);
-                if (v < c.eps && v >= (T)0) v = (T)0;
+                if (!ok) record_fault(a.state, tl.inst, f * N + cell, counter);
                 fout[f * N + cell] = v;
-                sum[f] += (double)v;
             }
+            sum[f] += (double)(in ? v : (T)0);       // + 0 for cells past the lattice
             wcnt[f] += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in && v > (T)0));
         }
     }

@@ -1216,6 +1216,7 @@ template <typename T>
 struct RKConst {
     DornicConst k;          // P Dornic constants (beta = -d - 4 D_P/dx^2)
     T R, cgmax, gmax, alpha, rW, K1, K2, K2W0, dw, dO, dt, dt2, dt6, eps;
+    T aK2W0;                // alpha k2 W0 (the FP32 form of f_2Dor)
     int compat;
     int64_t step;
 };
@@ -1228,7 +1229,7 @@ struct RKConstD {
     double R, cgmax, gmax, alpha, rW, K1, K2, K2W0, dw, dO, dt, dt2, dt6, eps;
     int compat, pad;
     int64_t step;
-    float fR, fcgmax, fgmax, falpha, frW, fK1, fK2, fK2W0, fdw, fdO, fdt, fdt2, fdt6, feps;
+    float fR, fcgmax, fgmax, falpha, frW, fK1, fK2, fK2W0, fdw, fdO, fdt, fdt2, fdt6, feps, faK2W0;
 };
 
 __device__ __forceinline__ RKConstD rk_constants(const tm_rietkerk_params& p, int64_t step)
@@ -1258,6 +1259,7 @@ __device__ __forceinline__ RKConstD rk_constants(const tm_rietkerk_params& p, in
     c.frW = (float)c.rW; c.fK1 = (float)c.K1; c.fK2 = (float)c.K2; c.fK2W0 = (float)c.K2W0;
     c.fdw = (float)c.dw; c.fdO = (float)c.dO; c.fdt = (float)c.dt; c.fdt2 = (float)c.dt2;
     c.fdt6 = (float)c.dt6; c.feps = (float)c.eps;
+    c.faK2W0 = (float)(c.alpha * c.K2W0);
     return c;
 }
 
@@ -1269,6 +1271,7 @@ __device__ __forceinline__ RKConst<T> rk_cast(const RKConstD& d)
     c.R = d.R; c.cgmax = d.cgmax; c.gmax = d.gmax; c.alpha = d.alpha; c.rW = d.rW;
     c.K1 = d.K1; c.K2 = d.K2; c.K2W0 = d.K2W0; c.dw = d.dw; c.dO = d.dO;
     c.dt = d.dt; c.dt2 = d.dt2; c.dt6 = d.dt6; c.eps = d.eps;
+    c.aK2W0 = d.alpha * d.K2W0;
     c.compat = d.compat;
     c.step = d.step;
     return c;
@@ -1282,6 +1285,7 @@ __device__ __forceinline__ RKConst<float> rk_cast<float>(const RKConstD& d)
     c.R = d.fR; c.cgmax = d.fcgmax; c.gmax = d.fgmax; c.alpha = d.falpha; c.rW = d.frW;
     c.K1 = d.fK1; c.K2 = d.fK2; c.K2W0 = d.fK2W0; c.dw = d.fdw; c.dO = d.fdO;
     c.dt = d.fdt; c.dt2 = d.fdt2; c.dt6 = d.fdt6; c.eps = d.feps;
+    c.aK2W0 = d.faK2W0;
     c.compat = d.compat;
     c.step = d.step;
     return c;
@@ -1300,19 +1304,20 @@ __device__ __forceinline__ void rk_rhs(const RKConst<T>& c, T P, T W, T O, T nW,
     fO = c.R - A + c.dO * (nO - (T)4 * O);
 }
 
-// FP32: the same terms with the hardware reciprocal shared by the two W/(W + k1)
-// quotients and explicit fused multiply-adds (no rounding order to keep: the
+// FP32: the same terms with the hardware reciprocal, the quotient t = W P / (W + k1)
+// shared by the two uptake terms, alpha (P + k2 W0) as one fused multiply-add
+// and explicit fused multiply-adds elsewhere (no rounding order to keep: the
 // FP32 path is checked against FP64 statistically, and every FP32 kernel uses
-// this one form).
+// this one form).  The diffusion terms keep d (n - 4 x), whose difference is
+// formed before the large factor d_O multiplies it.
 template <>
 __device__ __forceinline__ void rk_rhs<float>(const RKConst<float>& c, float P, float W, float O, float nW,
                                               float nO, float& fP, float& fW, float& fO)
 {
-    const float rw = __builtin_amdgcn_rcpf(W + c.K1);
-    const float wp = W * P;
-    fP = (c.cgmax * wp) * rw;
-    const float A = (c.alpha * (P + c.K2W0)) * __builtin_amdgcn_rcpf(P + c.K2) * O;
-    fW = __builtin_fmaf(c.dw, __builtin_fmaf(-4.0f, W, nW), __builtin_fmaf(-c.rW, W, A - (c.gmax * wp) * rw));
+    const float t = (W * P) * __builtin_amdgcn_rcpf(W + c.K1);
+    fP = c.cgmax * t;
+    const float A = __builtin_fmaf(c.alpha, P, c.aK2W0) * __builtin_amdgcn_rcpf(P + c.K2) * O;
+    fW = __builtin_fmaf(c.dw, __builtin_fmaf(-4.0f, W, nW), __builtin_fmaf(-c.rW, W, __builtin_fmaf(-c.gmax, t, A)));
     fO = __builtin_fmaf(c.dO, __builtin_fmaf(-4.0f, O, nO), c.R - A);
 }
 

@@ -3,9 +3,12 @@
 // A context owns: the instance descriptor and tile tables, the double-buffered
 // fields of every instance (one allocation, instance blocks [nf][L*L] back to
 // back), the tile partials of the per-step rho(t) sums, the device step state,
-// the rho(t) rows, one HIP stream, and up to two instantiated hipGraphs (one
-// per field-buffer parity) each replaying `graph_steps` step launches plus the
-// rows fold.  The host tracks the same step counter the kernels advance.
+// the rho(t) rows and one HIP stream.  tm_run enqueues blocks of `graph_steps`
+// steps (constants, step launches, rows fold, counter bump) directly on the
+// stream: replaying the same block as a hipGraph measured 2-8% slower per step
+// on MI355X (ROCm 7.2) for every model, from the 20 us C2 step to the 1 ms C3
+// step, so graphs only serve tm_time_steps.  The host tracks the same step
+// counter the kernels advance.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,7 +77,6 @@ struct tm_ctx {
     int parity = 0;
     int64_t counter = 0;
 
-    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [blocked][parity]
     hipGraphExec_t time_graph[2] = {nullptr, nullptr};   // tm_time_steps blocks (no rows fold), [parity]
     int time_graph_steps = 0;                            // ... of this many steps
     hipEvent_t time_ev[2] = {nullptr, nullptr};
@@ -212,11 +214,6 @@ double perception_rmax(const tm_dpc_params& p, int ns)
 
 void drop_graphs(tm_ctx* c)
 {
-    for (auto& gp : c->graph)
-        for (auto& g : gp) {
-            if (g) hipGraphExecDestroy(g);
-            g = nullptr;
-        }
     for (auto& g : c->time_graph) {
         if (g) hipGraphExecDestroy(g);
         g = nullptr;
@@ -438,7 +435,8 @@ void enqueue_block(tm_ctx* c, int parity, int n, bool fold)
     launch_bump(c->d_state, n, c->stream);
 }
 
-// Capture one block of graph_steps steps (with the rows fold) for a parity.
+// Capture one block of n steps (with or without the rows fold) for a parity:
+// the graphs tm_time_steps replays.
 tm_status capture_block(tm_ctx* c, int parity, int n, bool fold, hipGraphExec_t* out)
 {
     hipGraph_t g = nullptr;
@@ -453,32 +451,6 @@ tm_status capture_block(tm_ctx* c, int parity, int n, bool fold, hipGraphExec_t*
         return fail(c, TM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
     return TM_OK;
-}
-
-tm_status capture(tm_ctx* c, int parity)
-{
-    hipGraphExec_t& g = c->graph[c->blocked][parity];
-    if (g) return TM_OK;
-    return capture_block(c, parity, c->graph_steps, true, &g);
-}
-
-// Every graph the current variant can replay (both parities, and both modes of
-// an adaptive variant), captured and instantiated before a run's first launch:
-// once per context and launch shape, never in the middle of a run.
-tm_status capture_all(tm_ctx* c)
-{
-    if (c->graph_steps <= 0) return TM_OK;
-    const Variant& v = cur_variant(c);
-    const bool both = v.spl > 1 && v.adaptive;
-    const bool keep = c->blocked;
-    tm_status s = TM_OK;
-    for (int m = 0; m < 2 && s == TM_OK; ++m) {
-        if (!both && (m != 0) != keep) continue;
-        c->blocked = m != 0;
-        for (int par = 0; par < 2 && s == TM_OK; ++par) s = capture(c, par);
-    }
-    c->blocked = keep;
-    return s;
 }
 
 tm_status ensure_rows(tm_ctx* c, int64_t n_steps)
@@ -599,7 +571,6 @@ tm_status advance(tm_ctx* c, int64_t n_steps)
     if ((s = set_run_base(c)) != TM_OK) return s;
     const bool adaptive = cur_variant(c).spl > 1 && cur_variant(c).adaptive;
     if (adaptive && (s = ensure_live(c)) != TM_OK) return s;
-    if (n_steps >= c->graph_steps && (s = capture_all(c)) != TM_OK) return s;
     int64_t left = n_steps;
     const int B = c->graph_steps;
     volatile double* h_sum = (volatile double*)c->h_pinned + 2;   // [2], after the run-base staging word
@@ -629,8 +600,8 @@ tm_status advance(tm_ctx* c, int64_t n_steps)
         return update_mode(c);
     };
     while (B > 0 && left >= B) {
-        if ((s = capture(c, c->parity)) != TM_OK) return s;
-        HIPCHK(c, hipGraphLaunch(c->graph[c->blocked][c->parity], c->stream));
+        enqueue_block(c, c->parity, B, true);
+        HIPCHK(c, hipGetLastError());
         count_steps(c, B);
         c->counter += B;
         c->parity ^= (launches_of(c, B) & 1);
@@ -1434,7 +1405,7 @@ tm_status tm_autotune(tm_ctx* c, const char* cache_path, int* out_variant, int* 
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    // Graph block: short steps are launch-bound and want long graphs.
+    // Block length: short steps want long blocks (fewer host round trips).
     const double per_step_ms = best;
     int g = per_step_ms < 0.05 ? 200 : (graph_steps > 0 ? graph_steps : 100);
     c->variant = best_v;

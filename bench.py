@@ -10,7 +10,7 @@ host.  Those steps are the live start of the quench: every cell is occupied and
 takes a Poisson and a Gamma draw each step.
 
 Reported beside it:
-  * the whole 10^4-step C4 run in FP32 and FP64, timed per graph block, split
+  * the whole 10^4-step C4 run in FP32 and FP64, timed per launch block, split
     into its live phase (single-step launches) and its absorbing phase (eight
     steps per launch), each with its own rate and HBM fraction;
   * the plain FP64 single-instance Rietkerk step (C2: 256^2, 2x10^4 steps);
@@ -180,7 +180,7 @@ def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=N
                  keep_open=False):
     """Warm up, then time `steps` steps of the real trajectory from t = 0: wall
     clock around the whole job, and the device time of every step launch (HIP
-    events on the context stream inside each run() call).  blocks: run one graph
+    events on the context stream inside each run() call).  blocks: run one launch
     block per call, and split the device time by launch kind (single-step
     blocks, K-step blocks)."""
     ctx, vname, gsteps = _open(tm, b, variant_name, tune)
@@ -234,7 +234,7 @@ def run_workload(tm, b, steps, warmup, want_rows=True, tune=True, variant_name=N
 
 def phase_split(r, b, k_per_launch=8):
     """Live (single-step launches) and absorbing (K-step launches) phases of a
-    run timed per graph block: steps, device ms per step, cell-updates/s and the
+    run timed per launch block: steps, device ms per step, cell-updates/s and the
     HBM fraction of each kind's algorithmic bytes per launch."""
     cells, bpc = cells_of(b), bytes_per_cell(b)
     out = {}

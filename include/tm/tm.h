@@ -271,22 +271,24 @@ tm_status tm_get_step(tm_ctx* ctx, int inst, int64_t* step);
 
 /* Rewind every instance to its step0 and clear the fault record; fields are
  * left as they are (call tm_init_uniform / tm_upload_fields next).  Launch
- * shape, graphs and tuning are kept.  Starts a fresh replicate of the batch,
+ * shape and tuning are kept.  Starts a fresh replicate of the batch,
  * the role of the replicate loop head rietkerk_bjork_FKE.cpp:3887-3899. */
 tm_status tm_reset(tm_ctx* ctx);
 
 /* Launch shape.  variant: index into the kernel table for the configured
- * model/precision (-1 = default); graph_steps: steps per captured hipGraph
- * block (0 = eager launches), rounded up to a whole number of launches for a
- * variant that advances several steps per launch.  tm_variant_count /
- * tm_variant_name describe the table. */
+ * model/precision (-1 = default); graph_steps: steps per launch block (the
+ * per-step constants, the step launches, the rho(t) fold and the counter bump,
+ * enqueued together; 0 = blocks of at most 64 steps), rounded up to a whole
+ * number of launches for a variant that advances several steps per launch.
+ * (The name is kept from when blocks were replayed as hipGraphs.)
+ * tm_variant_count / tm_variant_name describe the table. */
 tm_status tm_set_launch(tm_ctx* ctx, int variant, int graph_steps);
-/* The launch shape in use: variant index and graph block size (either nullable). */
+/* The launch shape in use: variant index and block length (either nullable). */
 tm_status tm_get_launch(const tm_ctx* ctx, int* variant, int* graph_steps);
 int tm_variant_count(int model, int precision);
 const char* tm_variant_name(int model, int precision, int variant);
 
-/* Autotuner: time each kernel variant (and graph block sizes) on the configured
+/* Autotuner: time each kernel variant (and pick a block length) on the configured
  * batch and its current fields without advancing it, and pick the fastest; the
  * variant in place when tuning starts is kept unless another is at least 5%
  * faster.  The choice persists in cache_path (NULL = $TM_TUNE_CACHE or

@@ -764,7 +764,7 @@ class Context:
         self._check(self._lib.tm_set_launch(self._h, variant, graph_steps))
 
     def launch(self):
-        """(variant index, graph block steps) in use."""
+        """(variant index, steps per launch block) in use."""
         v, g = C.c_int(), C.c_int()
         self._check(self._lib.tm_get_launch(self._h, C.byref(v), C.byref(g)))
         return v.value, g.value
